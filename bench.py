@@ -1,0 +1,208 @@
+"""Benchmark of the MLS-MPM step path on one MI355X.
+
+Workload: the 1,000,000-particle water/oil column (1000x1000 cells of size 1,
+60% water / 40% oil, Gate 4 liquids, velocities U(+-2), gravity 90), stepped
+at the CFL microstep of a 1/240 s macrostep (the plan of its first
+macrostep) with the CFL guard on, one batch per macrostep.  A "step" is one
+microstep over all particles; value = particles x committed microsteps / s,
+inputs resident in HBM, the timed region bracketed by stream synchronisation.
+
+Printed JSON line (one): metric/value/unit of the float32 gather path plus
+the fixed-point path's rate, the roofline of the dominant kernel (HIP events
+on the solver's stream; algorithmic bytes per launch), and the CPU baseline
+(the FP64 oracle step, single thread, on a bounded sample of the same scene).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s @1M particles + %HBM peak; batched & adjoint vs plain step"
+# Algorithmic bytes (float32 SoA): per particle-microstep, read x, v, C, F,
+# volume, material id (56 B) and write x, v, C, F (48 B); per grid node and
+# liquid phase present, 56 B.
+PARTICLE_BYTES = 104
+NODE_PHASE_BYTES = 56
+# P2G + grid-update kernel: particle reads of P2G (56 B) and the node
+# outputs it writes (velocity 8 B per phase + liquid mass/volume 8 B).
+P2G_PARTICLE_BYTES = 56
+G2P_PARTICLE_BYTES = 48 + 32  # writes x, v, C, F; reads x, F, volume, material id, density
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_solver(pkg, arr, config, mode, timing=False):
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    s = factory(arr, config, device=0)
+    if timing:
+        s.ctx.set_timing(True)
+    return s
+
+
+def plan_for(pkg, solver, config):
+    b = solver.cfl_bounds
+    macro = pkg.scenes.WATER_OIL_MACRO_DT
+    bound = pkg.cfl.calculate_velocity_bound(b.velocityBound, macro, 90.0)
+    plan = pkg.cfl.calculate_microsteps(macro, config["grid"]["cellSize"], b.waveSpeedBound, bound, 1024)
+    limit = pkg.cfl.calculate_velocity_limit(config["grid"]["cellSize"], b.waveSpeedBound, plan.microDt)
+    return plan, limit
+
+
+def run_steps(pkg, solver, plan, limit, steps):
+    """Queue `steps` guarded microsteps in batches of one macrostep."""
+    params = pkg.layout.pack_step_parameters(solver.state, plan.microDt, pkg.scenes.WATER_OIL_GRAVITY,
+                                             solver.allocated_particle_count, 0,
+                                             {"waveSpeedBound": solver.cfl_bounds.waveSpeedBound,
+                                              "velocityBound": limit})
+    left = steps
+    while left > 0:
+        k = min(left, plan.stepCount)
+        solver.ctx.step_batch(params, k, True)
+        left -= k
+
+
+def timed(pkg, solver, plan, limit, steps, warmup):
+    run_steps(pkg, solver, plan, limit, warmup)
+    solver.ctx.read_compact()  # synchronise
+    t0 = time.perf_counter()
+    run_steps(pkg, solver, plan, limit, steps)
+    speed, done, status = solver.ctx.read_compact()
+    dt = time.perf_counter() - t0
+    if status[0] != 0:
+        raise RuntimeError(f"benchmark step failed with status {list(status)}")
+    return dt, speed
+
+
+def cpu_baseline(pkg, arr, config, budget_s=12.0):
+    from oracle import oracle as orc
+    state = pkg.layout.validate_input(arr, config)
+    parts = orc.particles_from_packed(state.packed)
+    cfg = orc.make_config(1.0 / 9600.0, config["grid"]["cellsX"], config["grid"]["cellsY"], 1.0, 0.02, 1.0,
+                          (0.0, 90.0))
+    orc.run_f64(parts, cfg, 1)  # warm-up step
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        parts = orc.run_f64(parts, cfg, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if (steps >= 3 and el > budget_s * 0.5) or el > budget_s:
+            break
+    return len(arr) * steps / el, steps, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--cells", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["both", "gather", "fixed"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (each GPU box has one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pkg = __graft_entry__.load_package()
+    log(f"[bench] building scene: {args.cells}x{args.cells} cells")
+    arr, config = pkg.scenes.water_oil_column(cells=args.cells)
+    n = len(arr)
+    nodes = (args.cells + 3) ** 2
+
+    results = {}
+    kernel_times = {}
+    microdt = None
+    for mode in (["gather", "fixed"] if args.mode == "both" else [args.mode]):
+        solver = build_solver(pkg, arr, config, mode)
+        plan, limit = plan_for(pkg, solver, config)
+        microdt = plan.microDt
+        log(f"[bench] {mode}: plan {plan.stepCount} microsteps of {plan.microDt:.4e} s; timing {args.steps} steps")
+        el, speed = timed(pkg, solver, plan, limit, args.steps, args.warmup)
+        results[mode] = {"seconds": el, "steps": args.steps, "rate": n * args.steps / el,
+                         "ms_per_step": 1e3 * el / args.steps, "max_speed": speed}
+        log(f"[bench] {mode}: {1e3 * el / args.steps:.4f} ms/step, {n * args.steps / el:.4e} particle-steps/s")
+        solver.dispose()
+        # per-kernel times with HIP events on the solver's stream (separate run)
+        solver = build_solver(pkg, arr, config, mode, timing=True)
+        run_steps(pkg, solver, plan, limit, min(args.steps, 2 * plan.stepCount))
+        solver.ctx.read_compact()
+        kt = {}
+        for k in ("bin", "scan_reduce", "scan_blocks", "scan_final", "scatter", "p2g_grid", "p2g_grid_fixed", "g2p",
+                  "measure_speed", "fix_rollback", "commit_last", "batch_begin"):
+            mean, launches = solver.ctx.read_timing(k)
+            if launches:
+                kt[k] = {"mean_us": round(mean, 3), "launches": launches}
+        kernel_times[mode] = kt
+        log(f"[bench] {mode} kernel means (us): " + ", ".join(f"{k}={v['mean_us']}" for k, v in kt.items()))
+        solver.dispose()
+
+    main_mode = "gather" if "gather" in results else "fixed"
+    r = results[main_mode]
+    kt = kernel_times[main_mode]
+    p2g_name = "p2g_grid" if main_mode == "gather" else "p2g_grid_fixed"
+    phases = 2  # water and oil
+    p2g_bytes = P2G_PARTICLE_BYTES * n + (8 * phases + 8) * nodes
+    g2p_bytes = G2P_PARTICLE_BYTES * n + (8 * phases + 8) * nodes
+    cand = {p2g_name: (p2g_bytes, kt.get(p2g_name, {}).get("mean_us")),
+            "g2p": (g2p_bytes, kt.get("g2p", {}).get("mean_us"))}
+    dom = max(cand, key=lambda k: cand[k][1] or 0.0)
+    dom_bytes, dom_us = cand[dom]
+    achieved = dom_bytes / (dom_us * 1e-6) / 1e9 if dom_us else None
+    step_bytes = PARTICLE_BYTES * n + NODE_PHASE_BYTES * phases * nodes
+    step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline: FP64 oracle step, one thread")
+        rate, steps, el = cpu_baseline(pkg, arr, config)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} FP64 oracle microsteps of the full {n}-particle scene ({el:.1f} s, after 1 "
+                         "warm-up step)"}
+
+    line = {
+        "metric": METRIC,
+        "value": r["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"water/oil column, {n} particles (60% water / 40% oil), {args.cells}x{args.cells} "
+                               "cells of size 1, CFL microsteps of a 1/240 s macrostep, float32 gather P2G",
+                   "particles": n, "grid_nodes": nodes, "microdt": microdt},
+        "fixed_point": None if "fixed" not in results else {
+            "value": results["fixed"]["rate"], "ms_per_step": results["fixed"]["ms_per_step"]},
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_gbs,
+                     "frac": step_gbs / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_us": dom_us},
+        "kernels_us": kernel_times,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * mpm355 — MI355X (gfx950) 2D MLS-MPM step path behind a plain C ABI.
+ *
+ * Drop-in boundary for the WebGPU runtime below `MpmMlsGpuSolver`
+ * (src/lib/mpm/kernels/solver.ts:106-144).  Every entry point takes plain
+ * pointers and sizes; no GPU or framework types cross the boundary.  Wire
+ * formats are the reference's own:
+ *   - particle record: 20 x 32-bit words (layout.ts:385-437)
+ *       [0]x [1]y [2]vx [3]vy [4..7]C(m00,m01,m10,m11) [8..11]F(m00,m01,m10,m11)
+ *       [12]plasticVolume [13]model(u32) [14]lambda [15]mu [16]density
+ *       [17]volume [18]param (Tait exponent / powder friction / NH damage)
+ *       [19]bondedTriangleVertex+1 (u32, 0 = none)
+ *   - step parameters: the 144-byte block of packMpmMlsGpuStepParameters
+ *       (layout.ts:896-997)
+ *   - numerical status: 8 x u32 (shaders.ts:157-172, solver.ts:889-893)
+ *   - compact readback: f32 maxSpeed | u32 completed | 8 x u32 status
+ *       (solver.ts:2396-2441)
+ *
+ * Conventions (solver.ts:1414-1441, kernels/solver.spec.ts:1324-1337):
+ * arguments are validated before any device work and an invalid call returns
+ * MPM_EINVAL with no state change; numerical failures inside a step are not
+ * error returns — they set persistent status words and roll the particles
+ * back to the last committed microstep.  Calls on one context must be
+ * serialized by the caller (the reference allows one batch in flight).
+ */
+#ifndef MPM355_H
+#define MPM355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_EINVAL 1    /* invalid argument: maps to RangeError/TypeError */
+#define MPM_EDEVICE 2   /* HIP runtime failure: maps to Error with cause */
+#define MPM_ESTATE 3    /* call not valid in the context's current state */
+#define MPM_ECAPACITY 4 /* particle capacity exceeded */
+
+#define MPM_P2G_GATHER 0u      /* float32 P2G (createMpmMlsGpuSolver) */
+#define MPM_P2G_FIXED_POINT 1u /* int32 2^24 fixed-point P2G
+                                  (createExperimentalMpmMlsAtomicGpuSolver) */
+
+#define MPM_FLAG_KEEP_FIXED_GRID 1u /* keep the last step's 11-word int32
+                                       accumulators readable through
+                                       mpm_read_fixed_grid */
+
+#define MPM_PARAMS_BYTES 144
+#define MPM_STATUS_WORDS 8
+#define MPM_PARTICLE_WORDS 20
+
+typedef struct mpm_ctx mpm_ctx;
+
+/* Solver configuration: the GPU-relevant part of MpmMlsGpuConfig
+ * (layout.ts:17-19) after float32 packing (layout.ts:242-270). */
+typedef struct mpm_config {
+    uint32_t cells_x, cells_y;
+    float cell_size;
+    float particle_mass;
+    float particle_volume;
+    float wall_friction;
+    uint32_t capacity;   /* particle slots; >= initial count */
+    uint32_t p2g_mode;   /* MPM_P2G_GATHER or MPM_P2G_FIXED_POINT */
+    uint32_t flags;      /* MPM_FLAG_* */
+    int32_t device;      /* HIP device ordinal; -1 = current */
+} mpm_config;
+
+/* Context creation and upload of the initial packed particles.
+ * Replaces createGpuSolver/createRuntime (solver.ts:408-469, :605-1230).
+ * packed20: n records of 20 words; object_ids: n words (may be NULL).
+ * Bonded triangles (word 19 != 0) are rejected with MPM_EINVAL. */
+int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* object_ids,
+               uint32_t n, mpm_ctx** out);
+
+/* Enqueue `step_count` microsteps with one parameter block.
+ * Replaces stepBatch encode+submit (solver.ts:1407-1599): per microstep
+ * prepare+bin -> P2G -> grid update -> G2P (+CFL guard when params enable it)
+ * -> commit/rollback; the maximum particle speed of the final state is
+ * measured when measure_speed != 0 (or the CFL guard is on).  Asynchronous:
+ * returns once the work is queued. */
+int mpm_step_batch(mpm_ctx* ctx, const void* params144, uint32_t step_count, uint32_t measure_speed);
+
+/* Wait for queued work and read the 40-byte compact snapshot.
+ * Replaces measureMaximumParticleSpeed's copy + mapAsync (solver.ts:2388-2512). */
+int mpm_read_compact(mpm_ctx* ctx, float* max_speed, uint32_t* completed, uint32_t status[8]);
+
+/* checkNumericalStatus (solver.ts:2514-2580) */
+int mpm_read_status(mpm_ctx* ctx, uint32_t status[8]);
+
+/* writeBuffer(stateError, zeros) (solver.ts:1878-1882): resets status to
+ * {0, 0xFFFFFFFF, 0, 0, 0, 0, 0, 0}. */
+int mpm_clear_status(mpm_ctx* ctx);
+
+/* readParticles (solver.ts:2594-2731): indices == NULL reads slots [0, n).
+ * out20 receives n records of 20 words, object_ids n words (may be NULL). */
+int mpm_read_particles(mpm_ctx* ctx, const uint32_t* indices, uint32_t n, uint32_t* out20,
+                       uint32_t* object_ids);
+
+/* appendParticles device writes (solver.ts:2216-2242): appends n records
+ * after the active slots; MPM_ECAPACITY if they do not fit. */
+int mpm_append_particles(mpm_ctx* ctx, const uint32_t* packed20, const uint32_t* object_ids, uint32_t n);
+
+/* Drag interaction uniform (solver.ts:1389-1401):
+ * p[0..3] = target x, target y, radius (0 = off), stiffness; p[4] = damping. */
+int mpm_set_interaction(mpm_ctx* ctx, const float p[8]);
+
+/* Number of active particle slots and the capacity. */
+int mpm_get_counts(mpm_ctx* ctx, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count);
+
+/* Debug/parity: the int32 fixed-point accumulators of the last committed P2G
+ * (node_count*11 words, order [solid m,qx,qy | water m,qx,qy,V | oil m,qx,qy,V]).
+ * Requires MPM_P2G_FIXED_POINT and MPM_FLAG_KEEP_FIXED_GRID. */
+int mpm_read_fixed_grid(mpm_ctx* ctx, int32_t* out, size_t words);
+
+/* Debug/parity: per-node grid state of the last step (node_count*8 floats:
+ * solid v(2), water v(2), oil v(2), liquid mass, liquid volume). */
+int mpm_read_grid(mpm_ctx* ctx, float* out, size_t floats);
+
+/* Debug/parity: particle bin keys (gridIndex of the stencil base node,
+ * 0xFFFFFFFF for removed particles) and the node_count+1 bin starts from the
+ * most recent binning pass. */
+int mpm_read_bins(mpm_ctx* ctx, uint32_t* keys, uint32_t n, uint32_t* starts, size_t start_words);
+
+/* Mean kernel time in microseconds of the named kernel over the last
+ * mpm_step_batch calls made while timing was enabled (HIP events on the
+ * context's stream). enable != 0 turns timing on and resets counters. */
+int mpm_set_timing(mpm_ctx* ctx, uint32_t enable);
+int mpm_read_timing(mpm_ctx* ctx, const char* kernel, double* mean_us, uint64_t* launches);
+
+/* dispose (solver.ts:2950-2964). Safe on NULL. */
+void mpm_destroy(mpm_ctx* ctx);
+
+/* Message of the last failed call on ctx (or of the last failed mpm_create
+ * when ctx is NULL). Never NULL. */
+const char* mpm_last_error(mpm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

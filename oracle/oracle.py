@@ -1,0 +1,191 @@
+"""ctypes front end of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg may
+import this module, and only as the checker.  The product (the HIP library and
+its host layer under ``powder-play_amd/``) never imports it.
+
+The C sources (``mpm_oracle.c``) restate ``stepMpm``
+(src/lib/mpm/reference/solver.ts:396-824) in FP64 and the WGSL fixed-point
+P2G (src/lib/mpm/kernels/shaders.ts:808-1070) as an ordered f32 sequence.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmpm_oracle.so"
+
+PARTICLE_DTYPE = np.dtype(
+    [
+        ("x", "f8"), ("y", "f8"), ("vx", "f8"), ("vy", "f8"),
+        ("c00", "f8"), ("c01", "f8"), ("c10", "f8"), ("c11", "f8"),
+        ("f00", "f8"), ("f01", "f8"), ("f10", "f8"), ("f11", "f8"),
+        ("plastic_volume", "f8"), ("lambda_", "f8"), ("mu", "f8"),
+        ("density", "f8"), ("volume", "f8"), ("param", "f8"),
+        ("model", "i4"), ("bonded_vertex", "i4"),
+    ],
+    align=True,
+)
+assert PARTICLE_DTYPE.itemsize == 152
+
+MODEL_NEO_HOOKEAN, MODEL_SNOW, MODEL_WATER, MODEL_OIL, MODEL_POWDER = 0, 1, 2, 3, 4
+MODEL_REMOVED = -1
+
+ERRORS = {0: "ok", -1: "invalid input", -2: "non-finite", -3: "non-positive determinant",
+          -4: "substep limit", -5: "unsupported"}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("cells_x", ctypes.c_int32), ("cells_y", ctypes.c_int32),
+        ("cell_size", ctypes.c_double), ("particle_mass", ctypes.c_double),
+        ("particle_volume", ctypes.c_double), ("gravity_x", ctypes.c_double),
+        ("gravity_y", ctypes.c_double), ("wall_friction", ctypes.c_double),
+    ]
+
+
+class Diagnostics(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_double) for name in (
+        "input_mass", "grid_mass", "output_mass", "input_px", "input_py",
+        "grid_px", "grid_py", "output_px", "output_py")]
+
+
+class ParamsF32(ctypes.Structure):
+    _fields_ = [
+        ("cells_x", ctypes.c_uint32), ("cells_y", ctypes.c_uint32),
+        ("node_width", ctypes.c_uint32), ("node_height", ctypes.c_uint32),
+        ("particle_count", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
+        ("cell_size", ctypes.c_float), ("particle_mass", ctypes.c_float),
+        ("particle_volume", ctypes.c_float), ("dt", ctypes.c_float),
+        ("gradient_scale", ctypes.c_float),
+    ]
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int):
+        super().__init__(f"oracle step failed: {ERRORS.get(code, code)}")
+        self.code = code
+
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.c_void_p
+        _lib.orc_step_f64.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Config), P, ctypes.POINTER(Diagnostics)]
+        _lib.orc_run_f64.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Config), ctypes.c_int32]
+        _lib.orc_grid_f64.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Config), P]
+        _lib.orc_prepare_f32.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ParamsF32), P]
+        _lib.orc_prepare_f32.restype = ctypes.c_uint32
+        _lib.orc_fixed_p2g_f32.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ParamsF32), P]
+        _lib.orc_fixed_p2g_f32.restype = ctypes.c_uint32
+        _lib.orc_decode_fixed_f32.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ParamsF32), P]
+        _lib.orc_bin.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ParamsF32), P, P, P]
+        _lib.orc_bin.restype = ctypes.c_uint32
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_config(dt, cells_x, cells_y, cell_size, particle_mass, particle_volume,
+                gravity=(0.0, 0.0), wall_friction=0.0) -> Config:
+    return Config(dt, cells_x, cells_y, cell_size, particle_mass, particle_volume,
+                  gravity[0], gravity[1], wall_friction)
+
+
+def particles_from_packed(packed20: np.ndarray) -> np.ndarray:
+    """20-word f32 AoS records (layout.ts:385-437) -> FP64 oracle particles."""
+    f = packed20.reshape(-1, 20).view(np.float32).astype(np.float64)
+    u = packed20.reshape(-1, 20).view(np.uint32)
+    p = np.zeros(f.shape[0], PARTICLE_DTYPE)
+    names = ["x", "y", "vx", "vy", "c00", "c01", "c10", "c11", "f00", "f01", "f10", "f11", "plastic_volume"]
+    for k, name in enumerate(names):
+        p[name] = f[:, k]
+    p["lambda_"], p["mu"], p["density"], p["volume"], p["param"] = f[:, 14], f[:, 15], f[:, 16], f[:, 17], f[:, 18]
+    p["model"] = u[:, 13].astype(np.int64).astype(np.int32)
+    p["bonded_vertex"] = u[:, 19].astype(np.int32)
+    return p
+
+
+def step_f64(particles: np.ndarray, cfg: Config):
+    out = np.zeros_like(particles)
+    diag = Diagnostics()
+    code = lib().orc_step_f64(_ptr(particles), len(particles), ctypes.byref(cfg), _ptr(out), ctypes.byref(diag))
+    if code:
+        raise OracleError(code)
+    return out, diag
+
+
+def run_f64(particles: np.ndarray, cfg: Config, steps: int) -> np.ndarray:
+    p = particles.copy()
+    code = lib().orc_run_f64(_ptr(p), len(p), ctypes.byref(cfg), steps)
+    if code:
+        raise OracleError(code)
+    return p
+
+
+GRID_FIELDS = ["m", "qx", "qy", "vx", "vy", "wm", "wqx", "wqy", "wvx", "wvy", "wV",
+               "om", "oqx", "oqy", "ovx", "ovy", "oV"]
+
+
+def grid_f64(particles: np.ndarray, cfg: Config) -> np.ndarray:
+    nodes = (cfg.cells_x + 3) * (cfg.cells_y + 3)
+    out = np.zeros((nodes, len(GRID_FIELDS)), np.float64)
+    code = lib().orc_grid_f64(_ptr(particles), len(particles), ctypes.byref(cfg), _ptr(out))
+    if code:
+        raise OracleError(code)
+    return out
+
+
+def params_f32(cells_x, cells_y, cell_size, particle_mass, particle_volume, dt, n) -> ParamsF32:
+    cs = np.float32(cell_size)
+    return ParamsF32(cells_x, cells_y, cells_x + 3, cells_y + 3, n, (cells_x + 3) * (cells_y + 3),
+                     cs, np.float32(particle_mass), np.float32(particle_volume), np.float32(dt),
+                     np.float32(4.0 / (float(cell_size) * float(cell_size))))
+
+
+def prepare_f32(packed20: np.ndarray, prm: ParamsF32):
+    n = packed20.size // 20
+    a = np.zeros((n, 4), np.float32)
+    flags = lib().orc_prepare_f32(_ptr(np.ascontiguousarray(packed20, np.uint32)), n, ctypes.byref(prm), _ptr(a))
+    return a, flags
+
+
+def fixed_p2g_f32(packed20: np.ndarray, prm: ParamsF32):
+    n = packed20.size // 20
+    g = np.zeros((prm.node_count, 11), np.int32)
+    flags = lib().orc_fixed_p2g_f32(_ptr(np.ascontiguousarray(packed20, np.uint32)), n, ctypes.byref(prm), _ptr(g))
+    return g, flags
+
+
+def decode_fixed_f32(grid: np.ndarray, prm: ParamsF32) -> np.ndarray:
+    out = np.zeros((prm.node_count, 11), np.float32)
+    lib().orc_decode_fixed_f32(_ptr(np.ascontiguousarray(grid, np.int32)), prm.node_count, ctypes.byref(prm), _ptr(out))
+    return out
+
+
+def bin_particles(packed20: np.ndarray, prm: ParamsF32):
+    n = packed20.size // 20
+    key = np.zeros(n, np.uint32)
+    start = np.zeros(prm.node_count + 1, np.uint32)
+    perm = np.zeros(n, np.uint32)
+    flags = lib().orc_bin(_ptr(np.ascontiguousarray(packed20, np.uint32)), n, ctypes.byref(prm),
+                          _ptr(key), _ptr(start), _ptr(perm))
+    return key, start, perm, flags

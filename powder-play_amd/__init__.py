@@ -1,0 +1,21 @@
+"""mpm355 — MI355X-native 2D MLS-MPM step path (host side).
+
+The compute path is ``libmpm355.so`` (hand-written gfx950 kernels behind the
+C-ABI in include/mpm355.h); this package is the host mirror of the reference's
+solver API (src/lib/mpm/kernels/solver.ts:106-144) plus its layout, CFL
+scheduling and scene generators.  Import it as ``powder_play_amd`` (the
+directory name carries a hyphen; see ``load_package`` in ``__graft_entry__``).
+"""
+from . import cfl, layout, native, scenes  # noqa: F401
+from .layout import ParticleArrays, RangeError  # noqa: F401
+from .solver import (  # noqa: F401
+    MpmMlsGpuNumericalError,
+    MpmMlsGpuSolver,
+    create_experimental_mpm_mls_atomic_gpu_solver,
+    create_mpm_mls_gpu_solver,
+)
+
+__all__ = [
+    "cfl", "layout", "native", "scenes", "ParticleArrays", "RangeError", "MpmMlsGpuNumericalError",
+    "MpmMlsGpuSolver", "create_mpm_mls_gpu_solver", "create_experimental_mpm_mls_atomic_gpu_solver",
+]

@@ -1,0 +1,185 @@
+"""ctypes binding of ``libmpm355.so`` (declared in include/mpm355.h).
+
+The library is the product: every solver call in this package goes through it.
+There is no CPU fallback — if the library is missing or cannot load, the
+import of the solver raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmpm355.so"
+
+MPM_OK, MPM_EINVAL, MPM_EDEVICE, MPM_ESTATE, MPM_ECAPACITY = 0, 1, 2, 3, 4
+MPM_P2G_GATHER, MPM_P2G_FIXED_POINT = 0, 1
+MPM_FLAG_KEEP_FIXED_GRID = 1
+
+# Every symbol include/mpm355.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+SIGNATURES = {
+    "mpm_create": (ctypes.c_int, [_P, _P, _P, _U32, ctypes.POINTER(_P)]),
+    "mpm_step_batch": (ctypes.c_int, [_P, _P, _U32, _U32]),
+    "mpm_read_compact": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mpm_read_status": (ctypes.c_int, [_P, _P]),
+    "mpm_clear_status": (ctypes.c_int, [_P]),
+    "mpm_read_particles": (ctypes.c_int, [_P, _P, _U32, _P, _P]),
+    "mpm_append_particles": (ctypes.c_int, [_P, _P, _P, _U32]),
+    "mpm_set_interaction": (ctypes.c_int, [_P, _P]),
+    "mpm_get_counts": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mpm_read_fixed_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "mpm_read_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "mpm_read_bins": (ctypes.c_int, [_P, _P, _U32, _P, ctypes.c_size_t]),
+    "mpm_set_timing": (ctypes.c_int, [_P, _U32]),
+    "mpm_read_timing": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_uint64)]),
+    "mpm_destroy": (None, [_P]),
+    "mpm_last_error": (ctypes.c_char_p, [_P]),
+}
+
+
+class MpmConfig(ctypes.Structure):
+    _fields_ = [
+        ("cells_x", _U32), ("cells_y", _U32), ("cell_size", ctypes.c_float),
+        ("particle_mass", ctypes.c_float), ("particle_volume", ctypes.c_float),
+        ("wall_friction", ctypes.c_float), ("capacity", _U32), ("p2g_mode", _U32),
+        ("flags", _U32), ("device", ctypes.c_int32),
+    ]
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call failed; ``code`` is the MPM_E* value."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """Owns one ``mpm_ctx``; thin, checked wrappers around the C-ABI."""
+
+    def __init__(self, cfg: MpmConfig, packed: np.ndarray, object_ids: np.ndarray | None):
+        self.lib = load()
+        packed = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1, 20)
+        ids = None if object_ids is None else np.ascontiguousarray(object_ids, dtype=np.uint32)
+        handle = ctypes.c_void_p()
+        code = self.lib.mpm_create(ctypes.byref(cfg), _ptr(packed), _ptr(ids), packed.shape[0], ctypes.byref(handle))
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_last_error(None).decode())
+        self.handle = handle
+        self.cfg = cfg
+
+    def _check(self, code: int):
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_last_error(self.handle).decode())
+
+    def step_batch(self, params: bytes, steps: int, measure: bool):
+        buf = (ctypes.c_uint8 * 144).from_buffer_copy(params)
+        self._check(self.lib.mpm_step_batch(self.handle, ctypes.addressof(buf), steps, 1 if measure else 0))
+
+    def read_compact(self):
+        speed = np.zeros(1, np.float32)
+        done = np.zeros(1, np.uint32)
+        status = np.zeros(8, np.uint32)
+        self._check(self.lib.mpm_read_compact(self.handle, _ptr(speed), _ptr(done), _ptr(status)))
+        return float(speed[0]), int(done[0]), status
+
+    def read_status(self) -> np.ndarray:
+        status = np.zeros(8, np.uint32)
+        self._check(self.lib.mpm_read_status(self.handle, _ptr(status)))
+        return status
+
+    def clear_status(self):
+        self._check(self.lib.mpm_clear_status(self.handle))
+
+    def counts(self):
+        n, cap, nodes = (ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32())
+        self._check(self.lib.mpm_get_counts(self.handle, ctypes.byref(n), ctypes.byref(cap), ctypes.byref(nodes)))
+        return n.value, cap.value, nodes.value
+
+    def read_particles(self, indices=None):
+        if indices is None:
+            n = self.counts()[0]
+            idx = None
+        else:
+            idx = np.ascontiguousarray(indices, dtype=np.uint32)
+            n = idx.size
+        out = np.zeros((n, 20), np.uint32)
+        ids = np.zeros(n, np.uint32)
+        self._check(self.lib.mpm_read_particles(self.handle, _ptr(idx), n, _ptr(out), _ptr(ids)))
+        return out, ids
+
+    def append(self, packed: np.ndarray, object_ids: np.ndarray | None):
+        packed = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1, 20)
+        ids = None if object_ids is None else np.ascontiguousarray(object_ids, dtype=np.uint32)
+        self._check(self.lib.mpm_append_particles(self.handle, _ptr(packed), _ptr(ids), packed.shape[0]))
+
+    def set_interaction(self, values):
+        arr = np.zeros(8, np.float32)
+        arr[: len(values)] = values
+        self._check(self.lib.mpm_set_interaction(self.handle, _ptr(arr)))
+
+    def read_fixed_grid(self) -> np.ndarray:
+        nodes = self.counts()[2]
+        out = np.zeros((nodes, 11), np.int32)
+        self._check(self.lib.mpm_read_fixed_grid(self.handle, _ptr(out), out.size))
+        return out
+
+    def read_grid(self) -> np.ndarray:
+        nodes = self.counts()[2]
+        out = np.zeros((nodes, 8), np.float32)
+        self._check(self.lib.mpm_read_grid(self.handle, _ptr(out), out.size))
+        return out
+
+    def read_bins(self):
+        n, _, nodes = self.counts()
+        keys = np.zeros(n, np.uint32)
+        starts = np.zeros(nodes + 1, np.uint32)
+        self._check(self.lib.mpm_read_bins(self.handle, _ptr(keys), n, _ptr(starts), starts.size))
+        return keys, starts
+
+    def set_timing(self, enable: bool):
+        self._check(self.lib.mpm_set_timing(self.handle, 1 if enable else 0))
+
+    def read_timing(self, kernel: str):
+        mean = ctypes.c_double()
+        launches = ctypes.c_uint64()
+        self._check(self.lib.mpm_read_timing(self.handle, kernel.encode(), ctypes.byref(mean), ctypes.byref(launches)))
+        return mean.value, launches.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.mpm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,136 @@
+"""Synthetic scenes of the step path's workloads.
+
+* ``lcg_uniform`` — the Gate 4 scene generator's LCG
+  (src/lib/mpm/scenes/gate4.ts:134-144): state = state*1664525 + 1013904223
+  mod 2^32, value = state / 2^32; evaluated in bulk by jump-ahead.
+* ``tait_tank`` — SI Tait water tank: 40x40 particles at 2.5 mm filling the
+  lower-left 100x100 mm of a 200x200 mm box, 80x80 cells, rho0 1000, n 7,
+  c 20 m/s (Tait material mapping: reference/tait-eos.ts:26-86; SI box:
+  si-domain.ts:1-12; wall friction 1 as scenes/si-fluid.ts:21).
+* ``water_oil_column`` — 1,000,000 particles, 60% water / 40% oil of the Gate 4
+  liquids (scenes/gate4.ts:82-93), one per cell of a 1000x1000 grid, jittered,
+  velocities U(+-2).
+* ``five_model_block`` — the 45-particle five-material fixture of
+  experiments/transfer-regression.ts:34-53.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .layout import ParticleArrays
+
+LCG_A = 1664525
+LCG_C = 1013904223
+MASK32 = 0xFFFFFFFF
+GATE4_SCENE_SEED = 0x4D504D34          # gate4.ts:10
+GATE4_WATER = {"model": "water", "lambda": 160000.0, "mu": 0.0, "density": 0.02}   # gate4.ts:82-87
+GATE4_OIL = {"model": "oil", "lambda": 120000.0, "mu": 0.0, "density": 0.016}      # gate4.ts:88-93
+
+
+def lcg_uniform(seed: int, n: int) -> np.ndarray:
+    """n successive draws of the Gate 4 LCG starting from `seed`, as float64 in [0, 1)."""
+    if n <= 0:
+        return np.zeros(0)
+    A = np.zeros(n + 1, np.uint64)
+    C = np.zeros(n + 1, np.uint64)
+    A[1], C[1] = LCG_A, LCG_C
+    length = 1
+    while length < n:
+        m = min(length, n - length)
+        j = np.arange(1, m + 1)
+        A[length + j] = (A[j] * A[length]) & MASK32
+        C[length + j] = (A[j] * C[length] + C[j]) & MASK32
+        length += m
+    states = (A[1:] * np.uint64(seed & MASK32) + C[1:]) & MASK32
+    return states.astype(np.float64) / 4294967296.0
+
+
+def tait_material(reference_density=1000.0, exponent=7.0, sound_speed=20.0, extrusion=0.02, model="water") -> dict:
+    """createMpmTaitLiquidMaterial + deriveTaitPressureCoefficientPa
+    (tait-eos.ts:26-44, :73-86): B = rho0 c^2 / n mapped to areal units."""
+    b = reference_density * sound_speed * sound_speed / exponent
+    return {"model": model, "lambda": b * extrusion, "mu": 0.0, "density": reference_density * extrusion,
+            "equationOfState": {"model": "tait", "exponent": exponent}}
+
+
+def tait_tank():
+    """SI Tait tank: returns (particles as dicts, config, macro dt, macro count)."""
+    spacing = 0.0025
+    material = tait_material()
+    area = spacing * spacing
+    particles = []
+    for j in range(40):
+        for i in range(40):
+            particles.append({
+                "position": {"x": (i + 0.5) * spacing, "y": (j + 0.5) * spacing},
+                "velocity": {"x": 0.0, "y": 0.0},
+                "C": {"m00": 0.0, "m01": 0.0, "m10": 0.0, "m11": 0.0},
+                "F": {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0},
+                "volume": area,
+            })
+    config = {"grid": {"cellsX": 80, "cellsY": 80, "cellSize": spacing},
+              "particleMass": material["density"] * area, "particleVolume": area, "material": material,
+              "wallFriction": 1.0}
+    return particles, config, 1.0 / 120.0, 240
+
+
+TAIT_TANK_GRAVITY = {"x": 0.0, "y": -9.81}
+
+
+def water_oil_column(cells: int = 1000, seed: int = GATE4_SCENE_SEED, water_fraction: float = 0.6,
+                     speed: float = 2.0):
+    """One particle per cell of a cells x cells grid (cell size 1), jittered
+    uniformly inside its cell; water for the first water_fraction of an LCG
+    ordering, oil for the rest; velocities U(-speed, speed)^2.
+    Returns (ParticleArrays, config)."""
+    n = cells * cells
+    u = lcg_uniform(seed, 5 * n)
+    jx, jy, vx, vy, order_key = (u[k::5] for k in range(5))
+    cx = np.tile(np.arange(cells, dtype=np.float64), cells)
+    cy = np.repeat(np.arange(cells, dtype=np.float64), cells)
+    x = cx + 0.5 + (jx - 0.5)
+    y = cy + 0.5 + (jy - 0.5)
+    order = np.argsort(order_key, kind="stable")
+    is_water = np.zeros(n, bool)
+    is_water[order[: int(round(water_fraction * n))]] = True
+    model = np.where(is_water, 2, 3).astype(np.uint32)
+    arr = ParticleArrays(
+        x=x, y=y, vx=(2 * vx - 1) * speed, vy=(2 * vy - 1) * speed, model=model,
+        lambda_=np.where(is_water, GATE4_WATER["lambda"], GATE4_OIL["lambda"]),
+        mu=np.zeros(n), density=np.where(is_water, GATE4_WATER["density"], GATE4_OIL["density"]),
+        volume=np.ones(n), param=np.zeros(n), object_id=np.zeros(n, np.uint32))
+    config = {"grid": {"cellsX": cells, "cellsY": cells, "cellSize": 1.0}, "particleMass": 0.02,
+              "particleVolume": 1.0, "material": dict(GATE4_WATER), "capacity": n}
+    return arr, config
+
+
+WATER_OIL_GRAVITY = {"x": 0.0, "y": 90.0}
+WATER_OIL_MACRO_DT = 1.0 / 240.0
+
+FIVE_MODEL_MATERIALS = [  # transfer-regression.ts:34-40
+    {"model": "neo-hookean", "lambda": 20.0, "mu": 10.0},
+    {"model": "snow", "lambda": 20.0, "mu": 10.0},
+    {"model": "water", "lambda": 8.0, "mu": 0.0, "density": 1.0},
+    {"model": "oil", "lambda": 6.0, "mu": 0.0, "density": 0.8},
+    {"model": "powder", "lambda": 20.0, "mu": 10.0, "frictionCoefficient": 0.55},
+]
+
+
+def five_model_block(models=None):
+    """9 particles per material on a 3x3 patch (transfer-regression.ts:41-53)."""
+    mats = FIVE_MODEL_MATERIALS if models is None else [m for m in FIVE_MODEL_MATERIALS if m["model"] in models]
+    particles = []
+    for group, material in enumerate(FIVE_MODEL_MATERIALS):
+        if material not in mats:
+            continue
+        for local in range(9):
+            particles.append({
+                "position": {"x": 4 + group * 8 + local % 3, "y": 4 + local // 3},
+                "velocity": {"x": 0.03 * group, "y": -0.02},
+                "C": {"m00": 0.0, "m01": 0.001, "m10": 0.0, "m11": 0.0},
+                "F": {"m00": 0.98, "m01": 0.01, "m10": 0.0, "m11": 0.99},
+                "material": material, "objectId": group + 1, "plasticVolume": 1.0,
+            })
+    config = {"grid": {"cellsX": 48, "cellsY": 16, "cellSize": 1.0}, "particleMass": 0.1,
+              "particleVolume": 0.05, "material": FIVE_MODEL_MATERIALS[0], "capacity": 54}
+    return particles, config

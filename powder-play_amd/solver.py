@@ -1,0 +1,493 @@
+"""``MpmMlsGpuSolver`` over the native MI355X step (libmpm355.so).
+
+Mirrors the reference's solver host API (src/lib/mpm/kernels/solver.ts:106-144)
+for the step/batch path: ``step``, ``step_batch``, ``step_cfl_batch``,
+``step_cfl_batch_adaptive``, ``read_particles``,
+``measure_maximum_particle_speed``, ``check_numerical_status``,
+``append_particles``, drag interaction and ``dispose``.  Promises of the TS API
+are synchronous calls here; validation raises ``RangeError``/``TypeError``
+before any device work, numerical failures surface as
+:class:`MpmMlsGpuNumericalError` after a whole-step rollback.
+"""
+from __future__ import annotations
+
+import math
+import struct
+import time
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from . import cfl as cflmod
+from . import layout
+from .layout import RangeError
+from .native import MPM_FLAG_KEEP_FIXED_GRID, MPM_P2G_FIXED_POINT, MPM_P2G_GATHER, Context, MpmConfig
+
+ERROR_DEFORMATION = 2
+ERROR_DEFORMATION_SUBSTEP_LIMIT = 32
+ERROR_CFL_BOUND = 16384
+CFL_RECOVERY_BOUND_SAFETY_FACTOR = 1.25   # solver.ts:73
+MAX_U32 = 0xFFFFFFFF
+
+_FLAG_MESSAGES = [  # solver.ts:161-178
+    (1, "invalid particle stencil or linked-list state"),
+    (2, "non-positive trial deformation determinant"),
+    (4, "P2G validation failed"),
+    (8, "non-finite grid state"),
+    (16, "non-finite G2P particle state"),
+    (32, "deformation update exceeded the adaptive substep limit"),
+    (64, "non-positive projected deformation determinant"),
+    (128, "invalid P2G material parameters"),
+    (256, "non-finite particle stress or momentum"),
+    (1024, "non-finite P2G node accumulation"),
+    (2048, "non-finite P2G node output"),
+    (4096, "invalid triangle contact field or reaction"),
+    (8192, "invalid bonded triangle state"),
+    (16384, "particle exceeded the enforced CFL wave-speed or velocity bound"),
+    (32768, "invalid persistent solid-fracture bond state"),
+    (65536, "experimental fixed-point P2G contribution or accumulator overflow"),
+]
+
+
+def _u2f(u: int) -> float:
+    return struct.unpack("<f", struct.pack("<I", int(u) & 0xFFFFFFFF))[0]
+
+
+class MpmMlsGpuNumericalError(RuntimeError):
+    """MpmMlsGpuNumericalError (solver.ts:146-206)."""
+
+    def __init__(self, flags, particle_index=None, diagnostics=None, projected_deformation=None, cfl_details=None,
+                 completed_microsteps=None, measured_maximum_speed=None):
+        msg = "; ".join(d for f, d in _FLAG_MESSAGES if flags & f) or f"status 0x{flags:x}"
+        ctx = "" if particle_index is None else f" at particle {particle_index}"
+        super().__init__(f"GPU MLS-MPM rejected a step ({msg}{ctx}); all particles were rolled back to the last "
+                         "valid state.")
+        self.flags = flags
+        self.particle_index = particle_index
+        self.diagnostics = diagnostics
+        self.projected_deformation = projected_deformation
+        self.cfl_details = cfl_details
+        self.completed_microsteps = completed_microsteps
+        self.measured_maximum_speed = measured_maximum_speed
+
+
+def decode_numerical_error(status, completed=None, max_speed=None):
+    """decodeNumericalError (solver.ts:232-284)."""
+    status = [int(s) for s in status]
+    if status[0] == 0:
+        return None
+    idx = None if status[1] == MAX_U32 else status[1]
+    vals = [_u2f(status[k]) for k in range(2, 6)]
+    diag = None if status[6] != 1 else {"determinant": vals[0], "deformationIncrement": vals[1],
+                                         "precedingDeterminant": vals[2]}
+    proj = None if status[6] != 2 else {"m00": vals[0], "m01": vals[1], "m10": vals[2], "m11": vals[3]}
+    cfl = None if status[6] != 3 else {"particleSpeed": vals[0], "waveSpeed": vals[1], "velocityLimit": vals[2],
+                                        "waveSpeedLimit": vals[3]}
+    return MpmMlsGpuNumericalError(status[0], idx, diag, proj, cfl, completed, max_speed)
+
+
+def is_recoverable_cfl_error(error: MpmMlsGpuNumericalError) -> bool:
+    """isRecoverableCflError (solver.ts:296-312)."""
+    if error.flags == ERROR_CFL_BOUND:
+        d = error.cfl_details
+        return d is not None and all(math.isfinite(d[k]) for k in d)
+    return error.flags in (ERROR_DEFORMATION, ERROR_DEFORMATION_SUBSTEP_LIMIT) and \
+        error.measured_maximum_speed is not None and math.isfinite(error.measured_maximum_speed)
+
+
+@dataclass(frozen=True)
+class SpeedMeasurement:
+    maximumSpeed: float
+    sequence: int
+
+
+@dataclass(frozen=True)
+class CflBatchMeasurement:
+    maximumSpeed: float
+    sequence: int
+    recommendedVelocityBound: float
+    attemptedMicroStepCount: int
+    retryCount: int
+    retrySubmissionMilliseconds: float
+    readbackMilliseconds: float
+    simulatedDuration: float
+    remainingDuration: float
+
+
+MPM_MLS_GPU_DRAG_DAMPING = 12.0
+
+
+class MpmMlsGpuSolver:
+    """Solver host over one native context (one HIP stream)."""
+
+    def __init__(self, particles, config: dict, cfl_bounds: cflmod.CflBounds | None = None,
+                 p2g_mode: int = MPM_P2G_GATHER, keep_fixed_grid: bool = False, device: int = -1):
+        self.state = layout.validate_input(particles, config)
+        derived = cfl_bounds
+        if derived is None:
+            derived = cflmod.derive_cfl_bounds_packed(self.state.packed, self.state.particle_mass_f32)
+        self._validate_cfl_bounds(derived)
+        self.cfl_bounds = derived
+        cfg = MpmConfig(self.state.config["grid"]["cellsX"], self.state.config["grid"]["cellsY"],
+                        self.state.cell_size_f32, self.state.particle_mass_f32, self.state.particle_volume_f32,
+                        self.state.wall_friction_f32, self.state.particle_capacity, p2g_mode,
+                        MPM_FLAG_KEEP_FIXED_GRID if keep_fixed_grid else 0, device)
+        self.ctx = Context(cfg, self.state.packed, self.state.object_ids)
+        self.p2g_mode = p2g_mode
+        self._disposed = False
+        self._cfl_state_usable = True
+        self._speed_measurement_submitted = False
+        self._last_scheduled_velocity_bound = None
+        self._latest_sequence = 0
+        self._latest_microsteps = 0
+        self._drag = None
+
+    # ----------------------------------------------------------- properties
+    @property
+    def particle_count(self) -> int:
+        return self.ctx.counts()[0]
+
+    @property
+    def allocated_particle_count(self) -> int:
+        return self.ctx.counts()[0]
+
+    @property
+    def particle_capacity(self) -> int:
+        return self.ctx.counts()[1]
+
+    def _assert_active(self):
+        if self._disposed:
+            raise RuntimeError("MLS-MPM GPU solver has been disposed.")
+
+    @staticmethod
+    def _validate_cfl_bounds(bounds):
+        """validateCflBounds (solver.ts:471-500)."""
+        if bounds is None:
+            return
+        for name in ("waveSpeedBound", "velocityBound"):
+            v = getattr(bounds, name)
+            if not math.isfinite(v) or v < 0:
+                raise RangeError(f"cflBounds.{name} must be finite and non-negative.")
+        if not math.isfinite(bounds.courantLimit) or bounds.courantLimit <= 0:
+            raise RangeError("cflBounds.courantLimit must be positive and finite.")
+
+    # -------------------------------------------------------------- stepping
+    def step(self, dt: float, gravity: dict):
+        self.step_batch(1, dt, gravity)
+
+    def step_batch(self, step_count: int, dt: float, gravity: dict, cfl_velocity_bound: float | None = None,
+                   bond_reference_dt: float | None = None, measure_speed: bool = False):
+        """stepBatch (solver.ts:1407-1599)."""
+        self._assert_active()
+        if not isinstance(step_count, int) or isinstance(step_count, bool) or step_count < 1:
+            raise RangeError("stepCount must be a positive safe integer.")
+        guard = None
+        if cfl_velocity_bound is not None:
+            if self.cfl_bounds is None:
+                raise RangeError("this solver configuration has no safely derived CFL bounds.")
+            if not math.isfinite(cfl_velocity_bound) or cfl_velocity_bound < 0:
+                raise RangeError("velocityBound must be finite and non-negative.")
+            guard = {"waveSpeedBound": self.cfl_bounds.waveSpeedBound, "velocityBound": cfl_velocity_bound}
+        params = layout.pack_step_parameters(self.state, dt, gravity, self.allocated_particle_count, 0, guard,
+                                             bond_reference_dt)
+        try:
+            self.ctx.step_batch(params, step_count, measure_speed or guard is not None)
+        except Exception as cause:
+            self._cfl_state_usable = False
+            self._speed_measurement_submitted = False
+            raise RuntimeError(f"Failed to encode or submit an MLS-MPM batch of {step_count} step(s).") from cause
+        self._speed_measurement_submitted = guard is not None
+        if guard is None:
+            self._cfl_state_usable = False
+
+    def step_cfl_batch(self, macro_step_count: int, macro_dt: float, microstep_plan: cflmod.MicrostepPlan,
+                       velocity_bound: float, max_microsteps: int, gravity: dict,
+                       bond_reference_dt: float | None = None):
+        """stepCflBatch (solver.ts:1601-1722): certify the plan, then one batch."""
+        self._assert_active()
+        if self.cfl_bounds is None:
+            raise RangeError("CFL stepping requires a material-derived wave-speed bound; this solver configuration "
+                             "has no safely derived CFL bounds.")
+        if not self._cfl_state_usable:
+            raise RuntimeError("a direct non-CFL GPU step or a failed CFL measurement invalidated the tracked "
+                               "velocity bound; recreate the solver before CFL stepping.")
+        if self._speed_measurement_submitted:
+            raise RuntimeError("queue measureMaximumParticleSpeed() after the previous CFL batch before submitting "
+                               "another one.")
+        if self._last_scheduled_velocity_bound is not None and velocity_bound < self._last_scheduled_velocity_bound:
+            raise RangeError("velocityBound must cover the last submitted GPU maximum-particle envelope.")
+        if not isinstance(macro_step_count, int) or macro_step_count < 1:
+            raise RangeError("macroStepCount must be a positive safe integer.")
+        if not isinstance(max_microsteps, int) or max_microsteps < 1:
+            raise RangeError("maxMicrosteps must be a positive safe integer.")
+        cell = self.state.config["grid"]["cellSize"]
+        b = self.cfl_bounds
+        expected = cflmod.calculate_microsteps(macro_dt, cell, b.waveSpeedBound, velocity_bound, max_microsteps,
+                                               courantLimit=b.courantLimit)
+        certified = None
+        if isinstance(microstep_plan, cflmod.MicrostepPlan) and microstep_plan.stepCount >= expected.stepCount:
+            certified = cflmod.calculate_microsteps(macro_dt, cell, b.waveSpeedBound, velocity_bound,
+                                                    max_microsteps, microstep_plan.stepCount, b.courantLimit)
+        if certified is None or certified != microstep_plan:
+            raise RangeError("microstepPlan does not match this solver configuration or is not a certified CFL "
+                             "refinement.")
+        micro_count = macro_step_count * certified.stepCount
+        if micro_count > max_microsteps:
+            raise RangeError(f"requested microstep count ({micro_count}) exceeds maxMicrosteps ({max_microsteps}).")
+        represented = certified.microDt * certified.stepCount
+        tol = max(macro_dt, 5e-324) * 2.220446049250313e-16 * 4
+        if abs(represented - macro_dt) > tol:
+            raise RangeError("microstepPlan does not preserve the requested macro dt.")
+        gdt = cflmod.fround(certified.microDt)
+        if self._drag is not None and (not math.isfinite(gdt) or gdt <= 0 or gdt * MPM_MLS_GPU_DRAG_DAMPING > 1):
+            raise RangeError("CFL microDt is too large for the explicit drag damping update to remain dissipative.")
+        limit = cflmod.calculate_velocity_limit(cell, b.waveSpeedBound, certified.microDt, b.courantLimit)
+        if limit < velocity_bound:
+            raise RangeError("float32 microDt does not preserve the planned CFL velocity bound; increase the "
+                             "microstep count.")
+        ref = bond_reference_dt if bond_reference_dt is not None else (
+            self.state.bond_reference_dt_f32 if self.state.bond_reference_dt_f32 is not None else macro_dt)
+        self.step_batch(micro_count, certified.microDt, gravity, limit, ref)
+        self._latest_sequence += 1
+        self._latest_microsteps = micro_count
+        self._last_scheduled_velocity_bound = velocity_bound
+
+    def step_cfl_batch_adaptive(self, macro_step_count, macro_dt, microstep_plan, velocity_bound, max_microsteps,
+                                gravity, gravity_for_duration=None) -> CflBatchMeasurement:
+        """stepCflBatchAdaptive (solver.ts:1724-2021): retry the failed
+        interval with at least two finer microsteps and raised bounds."""
+        if self.cfl_bounds is None:
+            raise RangeError("CFL stepping requires a material-derived wave-speed bound; this solver configuration "
+                             "has no safely derived CFL bounds.")
+        if not isinstance(macro_step_count, int) or macro_step_count < 1:
+            raise RangeError("macroStepCount must be a positive safe integer.")
+        if not math.isfinite(macro_dt) or macro_dt <= 0:
+            raise RangeError("macroDt must be positive and finite.")
+        total = macro_step_count * macro_dt
+        if not math.isfinite(total):
+            raise RangeError("CFL batch duration is outside the supported range.")
+        bond_ref = self.state.bond_reference_dt_f32 if self.state.bond_reference_dt_f32 is not None else macro_dt
+        cell = self.state.config["grid"]["cellSize"]
+        remaining = total
+        a_count, a_dt, a_plan, a_bound = macro_step_count, macro_dt, microstep_plan, velocity_bound
+        consumed = 0
+        retries = 0
+        submissions = 0
+        retry_ms = 0.0
+        readback_ms = 0.0
+        tol = max(total, 5e-324) * 2.220446049250313e-16 * 4
+
+        def result(measurement: SpeedMeasurement, recommended=None):
+            return CflBatchMeasurement(measurement.maximumSpeed, measurement.sequence,
+                                       measurement.maximumSpeed if recommended is None else recommended, consumed,
+                                       retries, retry_ms, readback_ms, max(0.0, total - remaining), remaining)
+
+        while True:
+            a_micro = a_count * a_plan.stepCount
+            a_duration = a_count * a_dt
+            budget = max_microsteps - consumed
+            if a_micro < 1 or not math.isfinite(a_duration) or a_duration <= 0 or a_micro > budget:
+                raise RangeError(f"adaptive CFL retry requires {a_micro} microsteps with {budget} remaining.")
+            t0 = time.perf_counter()
+            g = gravity if gravity_for_duration is None else gravity_for_duration(total - remaining, a_duration)
+            self.step_cfl_batch(a_count, a_dt, a_plan, a_bound, budget, g, bond_ref)
+            if submissions > 0:
+                retry_ms += (time.perf_counter() - t0) * 1e3
+            submissions += 1
+            t1 = time.perf_counter()
+            try:
+                measurement = self.measure_maximum_particle_speed()
+            except MpmMlsGpuNumericalError as error:
+                readback_ms += (time.perf_counter() - t1) * 1e3
+                if not is_recoverable_cfl_error(error):
+                    raise
+                done = error.completed_microsteps
+                if done is None or done < 0 or done >= a_micro:
+                    raise RuntimeError(f"Adaptive CFL recovery could not determine completed microsteps: {error}") \
+                        from error
+                done_duration = done * a_plan.microDt
+                if not math.isfinite(done_duration) or done_duration >= remaining:
+                    raise RuntimeError(f"Adaptive CFL recovery received invalid simulation progress: {error}") \
+                        from error
+                remaining -= done_duration
+                consumed += a_micro
+                observed = max(error.measured_maximum_speed or 0.0,
+                               (error.cfl_details or {}).get("particleSpeed", 0.0) or 0.0)
+                retry_bound = max(a_bound, self._last_scheduled_velocity_bound or 0.0,
+                                  observed * CFL_RECOVERY_BOUND_SAFETY_FACTOR)
+                if not math.isfinite(retry_bound):
+                    raise RuntimeError(f"Adaptive CFL recovery stopped after {retries} retry attempt(s) (non-finite "
+                                       f"velocity bound); last GPU error: {error}") from error
+                self._last_scheduled_velocity_bound = retry_bound
+                if error.cfl_details is not None:
+                    retry_wave = error.cfl_details["waveSpeed"] * CFL_RECOVERY_BOUND_SAFETY_FACTOR
+                    if not math.isfinite(retry_wave):
+                        raise RuntimeError("Adaptive CFL recovery stopped (non-finite wave-speed bound)") from error
+                    self.cfl_bounds = replace(self.cfl_bounds,
+                                              waveSpeedBound=max(self.cfl_bounds.waveSpeedBound, retry_wave))
+                retry_budget = max_microsteps - consumed
+                if retry_budget < 2:
+                    self.ctx.clear_status()
+                    self._cfl_state_usable = True
+                    return result(SpeedMeasurement(observed, self._latest_sequence), retry_bound)
+                failed = min(remaining, a_plan.microDt)
+                try:
+                    a_plan = cflmod.calculate_microsteps(failed, cell, self.cfl_bounds.waveSpeedBound, retry_bound,
+                                                         retry_budget, 2, self.cfl_bounds.courantLimit)
+                except RangeError as cause:
+                    if not cflmod.is_microstep_capacity_error(cause):
+                        raise RuntimeError(f"Adaptive CFL recovery could not schedule a finer step ({cause}): "
+                                           f"{error}") from cause
+                    self.ctx.clear_status()
+                    self._cfl_state_usable = True
+                    return result(SpeedMeasurement(observed, self._latest_sequence), retry_bound)
+                self.ctx.clear_status()
+                self._cfl_state_usable = True
+                a_count, a_dt, a_bound = 1, failed, retry_bound
+                retries += 1
+                continue
+            readback_ms += (time.perf_counter() - t1) * 1e3
+            consumed += a_micro
+            remaining = max(0.0, remaining - a_duration)
+            if remaining <= tol:
+                remaining = 0.0
+                return result(measurement, max(a_bound, measurement.maximumSpeed) if retries > 0
+                              else measurement.maximumSpeed)
+            cont_budget = max_microsteps - consumed
+            if cont_budget < 2:
+                return result(measurement, max(a_bound, measurement.maximumSpeed))
+            a_bound = max(a_bound, measurement.maximumSpeed)
+            safe = cflmod.calculate_safe_dt(cell, self.cfl_bounds.waveSpeedBound, a_bound,
+                                            self.cfl_bounds.courantLimit)
+            max_step = min(a_plan.microDt, safe)
+            if not math.isfinite(max_step) or max_step <= 0:
+                raise RangeError(f"Adaptive CFL recovery produced an invalid continuation timestep ({max_step}).")
+            required = math.ceil(remaining / max_step)
+            cont_duration = remaining
+            min_count = max(1, required)
+            if required >= cont_budget:
+                reserved = cont_budget - 1
+                cont_duration = min(remaining, max_step * reserved * (1 - 2.220446049250313e-16 * 8))
+                min_count = max(1, math.ceil(cont_duration / max_step))
+            if not math.isfinite(cont_duration) or cont_duration <= 0:
+                return result(measurement, max(a_bound, measurement.maximumSpeed))
+            a_plan = cflmod.calculate_microsteps(cont_duration, cell, self.cfl_bounds.waveSpeedBound, a_bound,
+                                                 cont_budget, min_count, self.cfl_bounds.courantLimit)
+            a_count, a_dt = 1, cont_duration
+
+    # ------------------------------------------------------------- readback
+    def measure_maximum_particle_speed(self) -> SpeedMeasurement:
+        """measureMaximumParticleSpeed (solver.ts:2388-2512): one 40-byte
+        compact snapshot; raises the decoded numerical error if any."""
+        self._assert_active()
+        was_submitted = self._speed_measurement_submitted
+        self._speed_measurement_submitted = False
+        speed, done, status = self.ctx.read_compact()
+        err = decode_numerical_error(status, done, speed)
+        if err is not None:
+            self._cfl_state_usable = False
+            raise err
+        if not was_submitted:
+            # no guarded batch queued a measurement: measure the current state
+            recs, _ = self.ctx.read_particles()
+            v = recs.view(np.float32)[:, 2:4]
+            live = recs[:, 13] != layout.MODEL_REMOVED
+            sp = np.sqrt(v[live, 0] * v[live, 0] + v[live, 1] * v[live, 1])
+            speed = float(np.max(np.where(np.isfinite(sp), sp, np.float32(3.4028234663852886e38)))) if sp.size else 0.0
+        return SpeedMeasurement(float(speed), self._latest_sequence)
+
+    def check_numerical_status(self):
+        """checkNumericalStatus (solver.ts:2514-2580)."""
+        self._assert_active()
+        err = decode_numerical_error(self.ctx.read_status())
+        if err is not None:
+            raise err
+
+    def clear_numerical_status(self):
+        self._assert_active()
+        self.ctx.clear_status()
+
+    def read_particles(self, particle_indices=None, packed: bool = False):
+        """readParticles (solver.ts:2594-2731)."""
+        self._assert_active()
+        if particle_indices is not None:
+            idx = list(particle_indices)
+            n = self.allocated_particle_count
+            for i in idx:
+                if not isinstance(i, (int, np.integer)) or i < 0 or i >= n:
+                    raise RangeError(f"particle index {i} is outside the active particle range.")
+        recs, ids = self.ctx.read_particles(particle_indices)
+        err = decode_numerical_error(self.ctx.read_status())
+        if err is not None:
+            raise err
+        if packed:
+            return recs, ids
+        return layout.unpack_particles(recs, ids)
+
+    def append_particles(self, particles) -> dict:
+        """appendParticles (solver.ts:2108-2278): validated, capacity-bounded."""
+        self._assert_active()
+        requested = len(particles)
+        if requested == 0:
+            return {"requested": 0, "accepted": 0, "refused": 0, "dropped": 0}
+        cfg = dict(self.state.config)
+        cfg["capacity"] = max(requested, 1)
+        add = layout.validate_input(particles, cfg)
+        free = self.particle_capacity - self.allocated_particle_count
+        accepted = min(free, requested)
+        if accepted > 0:
+            self.ctx.append(add.packed[:accepted], add.object_ids[:accepted])
+            if self.cfl_bounds is not None:
+                extra = cflmod.derive_cfl_bounds_packed(add.packed[:accepted], self.state.particle_mass_f32)
+                if extra is None:
+                    self.cfl_bounds = None
+                else:
+                    self.cfl_bounds = replace(self.cfl_bounds,
+                                              waveSpeedBound=max(self.cfl_bounds.waveSpeedBound, extra.waveSpeedBound),
+                                              velocityBound=max(self.cfl_bounds.velocityBound, extra.velocityBound))
+        return {"requested": requested, "accepted": accepted, "refused": requested - accepted, "dropped": 0,
+                "activeParticleCount": self.allocated_particle_count, "capacity": self.particle_capacity,
+                **({} if accepted == requested else {"reason": "capacity"})}
+
+    # ------------------------------------------------------------ interaction
+    def begin_drag(self, target: dict, radius: float, stiffness: float = 40.0):
+        if not math.isfinite(radius) or radius <= 0:
+            raise RangeError("drag radius must be positive and finite.")
+        self._drag = (target["x"], target["y"], radius, stiffness)
+        self.ctx.set_interaction([target["x"], target["y"], radius, stiffness, MPM_MLS_GPU_DRAG_DAMPING])
+
+    def update_drag(self, target: dict):
+        if self._drag is None:
+            return
+        _, _, r, k = self._drag
+        self.begin_drag(target, r, k)
+
+    def end_drag(self):
+        self._drag = None
+        self.ctx.set_interaction([0.0] * 8)
+
+    def dispose(self):
+        if not self._disposed:
+            self._disposed = True
+            self.ctx.close()
+
+    # ------------------------------------------------------- parity helpers
+    def read_fixed_grid(self) -> np.ndarray:
+        return self.ctx.read_fixed_grid()
+
+    def read_grid(self) -> np.ndarray:
+        return self.ctx.read_grid()
+
+    def read_bins(self):
+        return self.ctx.read_bins()
+
+
+def create_mpm_mls_gpu_solver(particles, config: dict, cfl_bounds=None, **kw) -> MpmMlsGpuSolver:
+    """createMpmMlsGpuSolver (solver.ts:390-397): float32 gather P2G."""
+    return MpmMlsGpuSolver(particles, config, cfl_bounds, MPM_P2G_GATHER, **kw)
+
+
+def create_experimental_mpm_mls_atomic_gpu_solver(particles, config: dict, cfl_bounds=None,
+                                                  **kw) -> MpmMlsGpuSolver:
+    """createExperimentalMpmMlsAtomicGpuSolver (solver.ts:399-406): int32 fixed-point P2G."""
+    return MpmMlsGpuSolver(particles, config, cfl_bounds, MPM_P2G_FIXED_POINT, **kw)

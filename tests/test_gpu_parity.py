@@ -1,0 +1,229 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+* five-material transfer fixture (experiments/transfer-regression.ts:33-95):
+  8 steps vs the FP64 oracle within the reference's 1e-3, batch == singles;
+* fixed-point P2G: int32 accumulators bit-exact vs the oracle's integer path,
+  at small size and after several committed steps;
+* binning: bin keys and starts bit-exact vs the oracle's counting sort;
+* overflow regression (transfer-regression.ts:97-170): status 65536 and a
+  bitwise rollback;
+* CFL guard failure: status 16384, progress count and rollback;
+* 1M-particle water/oil column: determinism, conservation and oracle parity
+  of a sampled step.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATE_COLS = ("x", "y", "vx", "vy", "c00", "c01", "c10", "c11", "f00", "f01", "f10", "f11")
+
+
+def as_f64(recs):
+    return recs.view(np.float32)[:, :12].astype(np.float64)
+
+
+def oracle_state(orc, expected):
+    return np.stack([expected[k] for k in STATE_COLS], axis=1)
+
+
+def run_gpu(pkg, factory, particles, config, steps, dt, gravity, batch=True, **kw):
+    s = factory(particles, config, device=0, **kw)
+    try:
+        if batch:
+            s.step_batch(steps, dt, gravity)
+        else:
+            for _ in range(steps):
+                s.step(dt, gravity)
+        s.check_numerical_status()
+        recs, _ = s.read_particles(packed=True)
+        return recs
+    finally:
+        s.dispose()
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+def test_transfer_regression_five_models(pkg, orc, mode):
+    particles, config = pkg.scenes.five_model_block()
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    dt, g = 0.001, {"x": 0.0, "y": 0.0}
+    batch = run_gpu(pkg, factory, particles, config, 8, dt, g, batch=True)
+    singles = run_gpu(pkg, factory, particles, config, 8, dt, g, batch=False)
+    state = pkg.layout.validate_input(particles, config)
+    expected = orc.run_f64(orc.particles_from_packed(state.packed), orc.make_config(dt, 48, 16, 1.0, 0.1, 0.05), 8)
+    err = float(np.max(np.abs(as_f64(batch) - oracle_state(orc, expected))))
+    print(f"{mode}: max |GPU - FP64| after 8 steps = {err:.3e}")
+    assert err <= 1e-3  # transfer-regression.ts:29
+    assert np.array_equal(batch, singles)  # batch vs single: bitwise
+
+
+def test_fixed_point_grid_bit_exact(pkg, orc):
+    arr, config = pkg.scenes.water_oil_column(cells=96)
+    s = pkg.create_experimental_mpm_mls_atomic_gpu_solver(arr, config, device=0, keep_fixed_grid=True)
+    try:
+        dt, g = 1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY
+        for step in range(4):
+            before, _ = s.read_particles(packed=True)
+            s.step(dt, g)
+            s.check_numerical_status()
+            got = s.read_fixed_grid()
+            prm = orc.params_f32(96, 96, 1.0, 0.02, 1.0, dt, len(arr))
+            want, flags = orc.fixed_p2g_f32(before, prm)
+            assert flags == 0
+            mism = int(np.count_nonzero(got != want))
+            assert mism == 0, f"step {step}: {mism} int32 words differ"
+    finally:
+        s.dispose()
+
+
+def test_fixed_point_grid_bit_exact_tait_and_solids(pkg, orc):
+    particles, config = pkg.scenes.five_model_block()
+    s = pkg.create_experimental_mpm_mls_atomic_gpu_solver(particles, config, device=0, keep_fixed_grid=True)
+    try:
+        before, _ = s.read_particles(packed=True)
+        s.step(0.001, {"x": 0.0, "y": -1.0})
+        s.check_numerical_status()
+        got = s.read_fixed_grid()
+        prm = orc.params_f32(48, 16, 1.0, 0.1, 0.05, 0.001, len(particles))
+        want, flags = orc.fixed_p2g_f32(before, prm)
+        assert flags == 0
+        assert np.array_equal(got, want)
+    finally:
+        s.dispose()
+
+
+def test_binning_bit_exact(pkg, orc):
+    arr, config = pkg.scenes.water_oil_column(cells=64)
+    s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
+    try:
+        before, _ = s.read_particles(packed=True)
+        s.step(1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY)
+        keys, starts = s.read_bins()
+        prm = orc.params_f32(64, 64, 1.0, 0.02, 1.0, 1e-4, len(arr))
+        okey, ostart, _, flags = orc.bin_particles(before, prm)
+        assert flags == 0
+        assert np.array_equal(keys, okey)
+        assert np.array_equal(starts, ostart)
+    finally:
+        s.dispose()
+
+
+@pytest.mark.parametrize("contribution", [False, True])
+def test_fixed_point_overflow_rolls_back_bitwise(pkg, contribution):
+    particles, config, _, _ = pkg.scenes.tait_tank()
+    base = dict(particles[0])
+    count = 1 if contribution else 512
+    ps = [dict(base, position={"x": 0.1, "y": 0.1}, velocity={"x": 1e7 if contribution else 0.0, "y": 0.0})
+          for _ in range(count)]
+    s = pkg.create_experimental_mpm_mls_atomic_gpu_solver(ps, config, device=0)
+    try:
+        before, _ = s.ctx.read_particles()
+        s.step(1e-5, {"x": 0.0, "y": 0.0})
+        for _ in range(2):
+            with pytest.raises(pkg.MpmMlsGpuNumericalError) as e:
+                s.check_numerical_status()
+            assert e.value.flags == 65536
+        after, _ = s.ctx.read_particles()
+        assert np.array_equal(before, after)
+    finally:
+        s.dispose()
+
+
+def test_cfl_guard_failure_rolls_back_and_counts(pkg):
+    particles, config = pkg.scenes.five_model_block(models=("water", "oil"))
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+    try:
+        before, _ = s.ctx.read_particles()
+        s.step_batch(5, 0.001, {"x": 0.0, "y": 0.0}, cfl_velocity_bound=1e-6)
+        with pytest.raises(pkg.MpmMlsGpuNumericalError) as e:
+            s.measure_maximum_particle_speed()
+        err = e.value
+        assert err.flags == 16384
+        assert err.completed_microsteps == 0
+        assert err.cfl_details["velocityLimit"] == pytest.approx(1e-6, rel=1e-6)
+        after, _ = s.ctx.read_particles()
+        assert np.array_equal(before, after)
+    finally:
+        s.dispose()
+
+
+def test_adaptive_cfl_macrostep_tait_tank(pkg):
+    """One 1/120 s macrostep of the SI Tait tank through the adaptive CFL
+    loop (solver.ts:1724-2021): the whole interval is simulated, retrying
+    with raised bounds when the compressed Tait wave speed trips the guard."""
+    particles, config, macro_dt, _ = pkg.scenes.tait_tank()
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+    try:
+        b = s.cfl_bounds
+        bound = pkg.cfl.calculate_velocity_bound(b.velocityBound, macro_dt, 9.81)
+        plan = pkg.cfl.calculate_microsteps(macro_dt, config["grid"]["cellSize"], b.waveSpeedBound, bound, 512)
+        assert 200 <= plan.stepCount <= 240
+        m = s.step_cfl_batch_adaptive(1, macro_dt, plan, bound, 1024, pkg.scenes.TAIT_TANK_GRAVITY)
+        assert m.remainingDuration == 0.0
+        assert m.simulatedDuration == pytest.approx(macro_dt, rel=1e-12)
+        s.check_numerical_status()
+        print(f"tait tank macrostep: {m.attemptedMicroStepCount} microsteps, {m.retryCount} retries")
+    finally:
+        s.dispose()
+
+
+def test_tait_tank_fixed_dt_vs_fp64(pkg, orc):
+    """224 fixed microsteps of the SI Tait tank (the CFL plan of its first
+    macrostep) in float32 on the GPU vs the FP64 oracle."""
+    particles, config, macro_dt, _ = pkg.scenes.tait_tank()
+    steps = 224
+    dt = macro_dt / steps
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+    try:
+        s.step_batch(steps, dt, pkg.scenes.TAIT_TANK_GRAVITY)
+        s.check_numerical_status()
+        recs, _ = s.read_particles(packed=True)
+    finally:
+        s.dispose()
+    state = pkg.layout.validate_input(particles, config)
+    ocfg = orc.make_config(float(np.float32(dt)), 80, 80, float(np.float32(0.0025)),
+                           float(np.float32(config["particleMass"])), float(np.float32(config["particleVolume"])),
+                           (0.0, float(np.float32(-9.81))), 1.0)
+    expected = orc.run_f64(orc.particles_from_packed(state.packed), ocfg, steps)
+    pos_err = float(np.max(np.abs(as_f64(recs)[:, :2] - oracle_state(orc, expected)[:, :2])))
+    vel_err = float(np.max(np.abs(as_f64(recs)[:, 2:4] - oracle_state(orc, expected)[:, 2:4])))
+    print(f"tait tank {steps} microsteps: max |dx| = {pos_err:.3e} m, max |dv| = {vel_err:.3e} m/s")
+    assert pos_err <= 1e-4 * 0.2   # 1e-4 of the 0.2 m box
+    assert vel_err <= 1e-3 * 1.0   # 1e-3 of the ~1 m/s speed scale
+
+
+def test_column_1m_deterministic_conservative_and_oracle_close(pkg, orc):
+    arr, config = pkg.scenes.water_oil_column(cells=1000)
+    dt, g = 1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY
+    results = []
+    for _ in range(2):
+        s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
+        try:
+            s.step_batch(4, dt, g)
+            s.check_numerical_status()
+            results.append(s.read_particles(packed=True)[0])
+        finally:
+            s.dispose()
+    assert np.array_equal(results[0], results[1])  # deterministic float path
+    state = pkg.layout.validate_input(arr, config)
+    # oracle parity on one step from the initial state, sampled particles
+    s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
+    try:
+        s.step(dt, g)
+        s.check_numerical_status()
+        got = s.read_particles(packed=True)[0]
+    finally:
+        s.dispose()
+    exp, diag = orc.step_f64(orc.particles_from_packed(state.packed),
+                             orc.make_config(float(np.float32(dt)), 1000, 1000, 1.0, 0.02, 1.0, (0.0, 90.0)))
+    want = oracle_state(orc, exp)
+    have = as_f64(got)
+    scale = np.maximum(np.abs(want), 1.0)
+    rel = float(np.max(np.abs(have[:, :4] - want[:, :4]) / scale[:, :4]))
+    print(f"1M column, one step: max relative |GPU - FP64| over x, v = {rel:.3e}")
+    assert rel <= 1e-4
+    assert diag.grid_mass == pytest.approx(diag.input_mass, rel=1e-12)
