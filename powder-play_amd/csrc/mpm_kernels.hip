@@ -61,6 +61,14 @@ __device__ __forceinline__ bool finite4(float4 v) { return finitef(v.x) && finit
 __device__ __forceinline__ uint32_t load_status(const Control* c) {
     return __atomic_load_n(&c->status[0], __ATOMIC_RELAXED);
 }
+// Status read shared by the whole block, for kernels that synchronise: every
+// thread takes the same early exit, so no barrier waits on exited waves.
+__device__ __forceinline__ bool block_status_set(const Control* c) {
+    __shared__ uint32_t s_status;
+    if (threadIdx.x == 0) s_status = load_status(c);
+    __syncthreads();
+    return s_status != 0u;
+}
 __device__ __forceinline__ void raise_flags(Control* c, uint32_t bits) { atomicOr(&c->status[0], bits); }
 __device__ __forceinline__ void raise_at(Control* c, uint32_t bits, uint32_t index) {
     atomicMin(&c->status[1], index);
@@ -284,6 +292,7 @@ __global__ __launch_bounds__(256) void k_bin(const Params* __restrict__ Pp, Cont
     const uint32_t pf = prepare_particle<GENERAL>(P, model, in.C[i], in.F[i], st.lambda[i], st.mu[i], st.density[i],
                                                   st.volume[i], st.param[i], in.plastic[i], A, aux);
     if (pf) {
+        bins.key[i] = INVALID_KEY;
         if (pf & ERR_P2G) raise_at(ctl, pf, i); else raise_flags(ctl, pf);
         return;
     }
@@ -318,8 +327,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Control* ctl, cons
                                                                uint32_t* __restrict__ block_sums, uint32_t nodes,
                                                                uint32_t commit_prev) {
     __shared__ uint32_t s_w[SCAN_THREADS / 64];
-    const uint32_t st = load_status(ctl);
-    if (st) return;
+    if (block_status_set(ctl)) return;
     if (commit_prev && blockIdx.x == 0 && threadIdx.x == 0) {
         // the preceding microstep committed; failures its G2P found while
         // preparing this microstep's particles belong to this microstep
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Control* ctl, cons
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(Control* ctl, uint32_t* block_sums, uint32_t nb) {
     __shared__ uint32_t s_w[SCAN_THREADS / 64];
     __shared__ uint32_t s_carry;
-    if (load_status(ctl)) return;
+    if (block_status_set(ctl)) return;
     if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
     for (uint32_t base = 0; base < nb; base += SCAN_THREADS) {
@@ -379,7 +387,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(Control* ctl, uint3
                                                               const uint32_t* __restrict__ block_sums,
                                                               uint32_t* __restrict__ start, uint32_t nodes) {
     __shared__ uint32_t s_w[SCAN_THREADS / 64];
-    if (load_status(ctl)) return;
+    if (block_status_set(ctl)) return;
     uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t c[SCAN_ITEMS];
     uint32_t local = 0;
@@ -477,7 +485,7 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_grid(const Params* __restri
     __shared__ float s_vo[TS * TS];
     __shared__ uint32_t s_cut;
 
-    if (load_status(ctl)) return;
+    if (block_status_set(ctl)) return;
     const Params& P = *Pp;
     const int tid = threadIdx.x;
     const int lx = tid % TS, ly = tid / TS;
@@ -553,18 +561,23 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_grid(const Params* __restri
             if (!make_stencil(P, p, sc) || sc.by + 1 - ry0 != r || sc.bx + 1 - rx0 < 0 || sc.bx + 1 - rx0 >= RC) {
                 flags |= ERR_STENCIL;
             } else {
-                const int c = sc.bx + 1 - rx0;
+                int c = sc.bx + 1 - rx0;
                 if (!split) {
                     const uint32_t key = (uint32_t)(sc.by + 1) * (uint32_t)W + (uint32_t)(sc.bx + 1);
                     const uint32_t cs = bins.start[key], ce = bins.start[key + 1];
                     uint32_t lr = 0;
                     for (uint32_t q = cs; q < ce; ++q) lr += bins.perm[q] < idx ? 1u : 0u;
                     li = s_rowoff[r] + (cs - s_start[r][0]) + lr - c0;
+                    if (li >= c1 - c0) {  // bin and position disagree: never expected
+                        flags |= ERR_STENCIL;
+                        li = j - c0;
+                        c = 0xff;
+                    }
                 }
                 const float2 v = in.vel[idx];
                 const float4 A = bins.affine[idx];
                 const float4 aux = bins.aux[idx];
-                tag = (uint32_t)c | (__float_as_uint(aux.w) << 8);
+                tag = c == 0xff ? 0xffffu : ((uint32_t)c | (__float_as_uint(aux.w) << 8));
                 s.vx[li] = v.x;
                 s.vy[li] = v.y;
                 s.m[li] = aux.x;
@@ -990,6 +1003,7 @@ __global__ __launch_bounds__(256) void k_g2p(const Params* __restrict__ Pp, Cont
         const uint32_t pf = prepare_particle<GENERAL>(P, model, C, Fn, st.lambda[i], st.mu[i], st.density[i],
                                                       st.volume[i], st.param[i], pvn, A, aux);
         if (pf) {
+            bins.key[i] = INVALID_KEY;
             if (pf & ERR_P2G) atomicMin(&ctl->deferred_index, i);
             atomicOr(&ctl->deferred, pf);
             return;
@@ -997,7 +1011,11 @@ __global__ __launch_bounds__(256) void k_g2p(const Params* __restrict__ Pp, Cont
         bins.affine[i] = A;
         bins.aux[i] = aux;
         Stencil ns;
-        if (!make_stencil(P, fp, ns)) { raise_flags(ctl, ERR_STENCIL); return; }
+        if (!make_stencil(P, fp, ns)) {
+            bins.key[i] = INVALID_KEY;
+            raise_flags(ctl, ERR_STENCIL);
+            return;
+        }
         const uint32_t k = (uint32_t)(ns.by + 1) * W + (uint32_t)(ns.bx + 1);
         bins.key[i] = k;
         bins.rank[i] = atomicAdd(&bins.count[k], 1u);
@@ -1035,12 +1053,13 @@ __global__ __launch_bounds__(256) void k_fix_rollback(const Control* ctl, Partic
 __global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, const float2* __restrict__ vel,
                                                        const uint32_t* __restrict__ model, uint32_t n) {
     __shared__ float s_w[4];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
-    if (i < n && model[i] != MODEL_REMOVED) {
-        const float2 v = vel[i];
-        const float sp = sqrtf(v.x * v.x + v.y * v.y);
-        m = finitef(sp) ? sp : FLT_MAX;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (model[i] != MODEL_REMOVED) {
+            const float2 v = vel[i];
+            const float sp = sqrtf(v.x * v.x + v.y * v.y);
+            m = fmaxf(m, finitef(sp) ? sp : FLT_MAX);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
@@ -1118,9 +1137,11 @@ hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool
         const uint32_t fb = min(pb, 1024u);
         MPM_LAUNCH("fix_rollback", fb, 256, k_fix_rollback, dim3(fb), dim3(256), 0, lc.stream, lc.ctl, lc.buf[0],
                    lc.buf[1], n, (uint32_t)start_buf, steps);
-        if (measure)
-            MPM_LAUNCH("measure_speed", pb, 256, k_measure_speed, dim3(pb), dim3(256), 0, lc.stream, lc.ctl,
+        if (measure) {
+            const uint32_t mb = min(pb, 512u);
+            MPM_LAUNCH("measure_speed", mb, 256, k_measure_speed, dim3(mb), dim3(256), 0, lc.stream, lc.ctl,
                        lc.buf[cur].vel, lc.st.model, n);
+        }
     }
     return hipSuccess;
 }

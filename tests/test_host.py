@@ -1,0 +1,205 @@
+"""Host-side logic (CPU): C-ABI exports and argument validation, layout
+packing, CFL planning and scheduling.  Known answers are the reference's own
+spec cases (kernels/cfl.spec.ts, cfl-bounds.spec.ts, step-scheduler.spec.ts,
+layout.spec.ts, reference/tait-eos.spec.ts)."""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_symbols():
+    text = (ROOT / "include" / "mpm355.h").read_text()
+    return sorted(set(re.findall(r"\b(mpm_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    lib = pkg.native.load()
+    declared = header_symbols()
+    assert len(declared) >= 16
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(pkg.native.SIGNATURES)
+
+
+def test_create_rejects_invalid_arguments_before_device_work(pkg):
+    lib = pkg.native.load()
+    handle = ctypes.c_void_p()
+    bad = pkg.native.MpmConfig(0, 8, 1.0, 1.0, 1.0, 0.0, 0, 0, 0, -1)
+    assert lib.mpm_create(ctypes.byref(bad), None, None, 0, ctypes.byref(handle)) == pkg.native.MPM_EINVAL
+    assert b"cellsX" in lib.mpm_last_error(None)
+    cfg = pkg.native.MpmConfig(8, 8, 1.0, 1.0, 1.0, 0.0, 4, 0, 0, -1)
+    rec = np.zeros((1, 20), np.uint32)
+    rec.view(np.float32)[0, [0, 1, 8, 11, 12, 17]] = [4.0, 4.0, 1.0, -1.0, 1.0, 1.0]  # det < 0
+    code = lib.mpm_create(ctypes.byref(cfg), rec.ctypes.data_as(ctypes.c_void_p), None, 1, ctypes.byref(handle))
+    assert code == pkg.native.MPM_EINVAL
+    assert b"determinant" in lib.mpm_last_error(None)
+    rec.view(np.float32)[0, 11] = 1.0
+    rec[0, 19] = 1  # bonded triangle vertex
+    code = lib.mpm_create(ctypes.byref(cfg), rec.ctypes.data_as(ctypes.c_void_p), None, 1, ctypes.byref(handle))
+    assert code == pkg.native.MPM_EINVAL
+    assert lib.mpm_step_batch(None, None, 1, 0) == pkg.native.MPM_EINVAL
+
+
+def make_particle(x, y, **kw):
+    p = {"position": {"x": x, "y": y}, "velocity": {"x": 0.0, "y": 0.0},
+         "C": {"m00": 0.0, "m01": 0.0, "m10": 0.0, "m11": 0.0},
+         "F": {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0}}
+    p.update(kw)
+    return p
+
+
+LAYOUT_CONFIG = {"grid": {"cellsX": 16, "cellsY": 8, "cellSize": 1}, "particleMass": 1, "particleVolume": 1,
+                 "material": {"lambda": 20, "mu": 10}}
+
+
+def test_layout_packs_reference_words(pkg):  # layout.spec.ts:67-91
+    st = pkg.layout.validate_input([make_particle(0, 0), make_particle(16, 8)], LAYOUT_CONFIG)
+    f = st.packed.view(np.float32)
+    assert list(f[0, :12]) == [0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1]
+    assert f[0, 12] == 1 and st.packed[0, 13] == 0 and list(f[0, 14:16]) == [20, 10]
+    assert st.particle_count == 2 and st.particle_capacity == 2
+    assert list(st.object_ids) == [0, 0]
+    assert (st.domain_width, st.domain_height, st.domain_width_f32, st.domain_height_f32) == (16, 8, 16, 8)
+
+
+def test_layout_rejects_out_of_domain_and_bad_determinant(pkg):
+    with pytest.raises(pkg.RangeError):
+        pkg.layout.validate_input([make_particle(16.5, 0)], LAYOUT_CONFIG)
+    with pytest.raises(pkg.RangeError):
+        pkg.layout.validate_input([make_particle(1, 1, F={"m00": 1, "m01": 0, "m10": 0, "m11": -1})], LAYOUT_CONFIG)
+    with pytest.raises(pkg.RangeError):
+        pkg.layout.validate_input([make_particle(1, 1, damage=2.0)], LAYOUT_CONFIG)
+
+
+def test_step_parameters_block(pkg):  # layout.ts:896-997
+    st = pkg.layout.validate_input([make_particle(1, 1)], LAYOUT_CONFIG)
+    raw = pkg.layout.pack_step_parameters(st, 0.001, {"x": 0, "y": 9.8},
+                                          cfl_guard={"waveSpeedBound": 31.5, "velocityBound": 2.0})
+    assert len(raw) == 144
+    ints = struct.unpack("<8I", raw[:32])
+    assert ints == (16, 8, 19, 11, 1, 209, 9, 0)
+    f = struct.unpack("<20f", raw[32:112])
+    assert f[0:4] == (1.0, 1.0, 1.0, np.float32(0.001))
+    assert f[6] == pytest.approx(-0.001 * 1 * 4)  # stress scale
+    assert f[7] == 4.0 and f[9] == np.float32(9.8) and f[10:12] == (16.0, 8.0)
+    assert f[14] == 2.0 and f[15] == pytest.approx(math.sqrt(1 / math.pi) + 0.75, rel=1e-6)
+    assert f[16:19] == (31.5, 2.0, 1.0)
+
+
+def test_cfl_known_answers(pkg):  # cfl.spec.ts:55-69, :142-147, :158-179
+    cfl = pkg.cfl
+    base = dict(macroDt=0.025, cellSize=0.1, waveSpeedBound=2, velocityBound=1, maxMicrosteps=3)
+    plan = cfl.calculate_microsteps(**base)
+    assert cfl.DEFAULT_MPM_COURANT_LIMIT == 0.3
+    assert plan.stepCount == 3 and plan.microDt <= plan.safeDt
+    assert plan.safeDt == pytest.approx(0.03 / 3)
+    assert cfl.calculate_microsteps(0.25, 1, 2, 3, 3, courantLimit=0.5) == cfl.MicrostepPlan(3, 0.25 / 3, 0.1)
+    assert cfl.calculate_microsteps(0.1, 0.1, 0, 0, 1) == cfl.MicrostepPlan(1, 0.1, math.inf)
+    with pytest.raises(pkg.RangeError, match="exceeds maxMicrosteps"):
+        cfl.calculate_microsteps(**dict(base, maxMicrosteps=2))
+    five = cfl.calculate_microsteps(**dict(base, maxMicrosteps=6), minimumStepCount=5)
+    assert five.stepCount == 5
+    eleven = cfl.calculate_microsteps(1, 1, 4, 6, 11, courantLimit=1)
+    assert eleven.stepCount == 11 and float(np.float32(eleven.microDt)) <= eleven.safeDt
+    big = 1.7976931348623157e308
+    assert cfl.calculate_microsteps(0.1, big, big, big, 1).safeDt == pytest.approx(0.15, abs=1e-14)
+    assert cfl.calculate_microsteps(1e30, big, 1, 1, 1, courantLimit=2).safeDt == big
+    with pytest.raises(pkg.RangeError, match="float32 microstep count"):
+        cfl.calculate_microsteps(big, 1, 0, 0, 2 ** 53 - 1)
+    for bad in (dict(macroDt=0), dict(cellSize=0), dict(waveSpeedBound=-1), dict(velocityBound=float("nan")),
+                dict(maxMicrosteps=0)):
+        with pytest.raises(pkg.RangeError):
+            cfl.calculate_microsteps(**dict(base, **bad))
+
+
+def test_cfl_velocity_limit(pkg):  # cfl.spec.ts:86-100, :102-130
+    v = pkg.cfl.calculate_velocity_limit(10, 1400, 0.001, 0.3)
+    assert float(np.float32(v)) == v
+    assert (v + float(np.float32(1400))) * float(np.float32(0.001)) <= 0.3 * float(np.float32(10))
+    assert v > 1599
+    with pytest.raises(pkg.RangeError, match="no finite non-negative"):
+        pkg.cfl.calculate_velocity_limit(10, 400, 0.01)
+
+
+def test_cfl_bounds_known_answers(pkg):  # cfl-bounds.spec.ts:42-82
+    gate4 = {"particleMass": 4, "particleVolume": 100, "material": {"lambda": 20, "mu": 10}}
+    b = pkg.cfl.derive_cfl_bounds([make_particle(1, 1)], gate4)
+    assert b.waveSpeedBound == pytest.approx(math.sqrt(40 / 0.04), abs=1e-4)
+    stretched = pkg.cfl.derive_cfl_bounds([make_particle(1, 1, F={"m00": 2, "m01": 0, "m10": 0, "m11": 1})], gate4)
+    assert stretched.waveSpeedBound > b.waveSpeedBound
+    tait = {"lambda": 998.2 * 400 / 7.15 * 0.02, "mu": 0, "model": "water", "density": 998.2 * 0.02,
+            "equationOfState": {"model": "tait", "exponent": 7.15}}
+    t = pkg.cfl.derive_cfl_bounds([make_particle(1, 1, material=tait)], gate4)
+    assert t.waveSpeedBound == pytest.approx(20, abs=5e-3)
+    tc = pkg.cfl.derive_cfl_bounds([make_particle(1, 1, material=tait, F={"m00": 0.9, "m01": 0, "m10": 0,
+                                                                          "m11": 1})], gate4)
+    assert tc.waveSpeedBound > t.waveSpeedBound
+    powder = dict(gate4, material={"lambda": 4, "mu": 2, "model": "powder", "frictionCoefficient": 0.5})
+    assert pkg.cfl.derive_cfl_bounds([], powder) == pkg.cfl.CflBounds(0.0, 0.0)
+    assert pkg.cfl.derive_cfl_bounds([make_particle(0, 0)], powder) is None
+    # the packed form agrees with the object form
+    st = pkg.layout.validate_input([make_particle(1, 1, material=tait)], {**gate4, "grid": {"cellsX": 4, "cellsY": 4,
+                                                                                             "cellSize": 1}})
+    assert pkg.cfl.derive_cfl_bounds_packed(st.packed, 4).waveSpeedBound == pytest.approx(t.waveSpeedBound, rel=1e-6)
+
+
+def test_velocity_bound_rounds_up_eight_ulps(pkg):
+    v = pkg.cfl.calculate_velocity_bound(1.0, 0.5, 9.81)
+    assert v > 1.0 + 0.5 * 9.81
+    assert v < 1.0 + 0.5 * 9.81 + 1e-5
+
+
+@pytest.mark.parametrize("speed,expected", [(0.5, 2), (1, 4), (2, 8), (4, 16)])
+def test_step_scheduler_speeds(pkg, speed, expected):  # step-scheduler.spec.ts:33-54
+    b = pkg.cfl.calculate_step_batch(0, 1 / 60, speed, 1 / 240, 0.1, 20, 1, 64)
+    assert b.macroStepCount == expected and b.microstepPlan.stepCount == 3
+    assert b.microStepCount == expected * 3
+    assert b.simulatedSeconds == pytest.approx(speed / 60)
+
+
+def test_step_scheduler_batches(pkg):  # step-scheduler.spec.ts:12-31, :56-114
+    cfl = pkg.cfl
+    b = cfl.calculate_step_batch(0, 0.05, 1, 0.025, 0.1, 2, 1, 6)
+    assert (b.macroStepCount, b.microstepPlan.stepCount, b.microStepCount) == (2, 3, 6)
+    assert b.simulatedSeconds == 0.05 and b.remainderSeconds == pytest.approx(0)
+    first = cfl.calculate_step_batch(0, 0.0125, 1, 0.025, 0.1, 2, 1, 6)
+    second = cfl.calculate_step_batch(first.remainderSeconds, 0.0125, 1, 0.025, 0.1, 2, 1, 6)
+    assert (first.macroStepCount, first.microStepCount) == (0, 0)
+    assert (second.macroStepCount, second.microStepCount) == (1, 3)
+    with pytest.raises(pkg.RangeError, match="requested microstep count"):
+        cfl.calculate_step_batch(0, 0.05, 1, 0.025, 0.1, 2, 1, 5)
+    part = cfl.calculate_step_batch(0, 0.025 * 4.5, 1, 0.025, 0.1, 2, 1, 6, allowPartialBatch=True)
+    assert (part.macroStepCount, part.microStepCount) == (2, 6)
+    assert part.remainderSeconds == pytest.approx(0.025 * 2.5)
+
+
+def test_tait_pressure_coefficient(pkg):  # tait-eos.spec.ts:17-38
+    m = pkg.scenes.tait_material(reference_density=998.2, exponent=7.15, sound_speed=1482.0, extrusion=1.0)
+    assert m["lambda"] == pytest.approx(306_625_260, rel=1e-8)
+
+
+def test_lcg_matches_sequential_recurrence(pkg):  # gate4.ts:134-144
+    seed = 0x4D504D34
+    state, seq = seed, []
+    for _ in range(1000):
+        state = (state * 1664525 + 1013904223) & 0xFFFFFFFF
+        seq.append(state / 2 ** 32)
+    assert np.array_equal(pkg.scenes.lcg_uniform(seed, 1000), np.array(seq))
+
+
+def test_numerical_error_decode(pkg):  # solver.ts:232-284
+    from powder_play_amd.solver import decode_numerical_error
+    status = [16384, 7] + [struct.unpack("<I", struct.pack("<f", v))[0] for v in (3.0, 20.0, 2.5, 20.5)] + [3, 0]
+    err = decode_numerical_error(status, 4, 3.0)
+    assert err.flags == 16384 and err.particle_index == 7 and err.completed_microsteps == 4
+    assert err.cfl_details == {"particleSpeed": 3.0, "waveSpeed": 20.0, "velocityLimit": 2.5, "waveSpeedLimit": 20.5}
+    assert decode_numerical_error([0] * 8) is None
+    assert decode_numerical_error([2, 0xFFFFFFFF, 0, 0, 0, 0, 1, 0]).particle_index is None
