@@ -38,7 +38,7 @@ NODE_PHASE_BYTES = 56
 # P2G + grid-update kernel: particle reads of P2G (56 B) and the node
 # outputs it writes (velocity 8 B per phase + liquid mass/volume 8 B).
 P2G_PARTICLE_BYTES = 56
-G2P_PARTICLE_BYTES = 48 + 32  # writes x, v, C, F; reads x, F, volume, material id, density
+G2P_PARTICLE_BYTES = 48 + 32  # writes x, v, C, F; reads x, F, volume and material id
 
 
 def log(msg: str):
@@ -143,8 +143,8 @@ def main():
         run_steps(pkg, solver, plan, limit, min(args.steps, 2 * plan.stepCount))
         solver.ctx.read_compact()
         kt = {}
-        for k in ("bin", "scan_reduce", "scan_blocks", "scan_final", "scatter", "p2g_grid", "p2g_grid_fixed", "g2p",
-                  "measure_speed", "fix_rollback", "commit_last", "batch_begin"):
+        for k in ("p2g_grid", "p2g_grid_fixed", "g2p", "measure_speed", "fix_rollback", "commit_last",
+                  "batch_begin"):
             mean, launches = solver.ctx.read_timing(k)
             if launches:
                 kt[k] = {"mean_us": round(mean, 3), "launches": launches}

@@ -65,6 +65,9 @@ typedef struct mpm_config {
     uint32_t p2g_mode;   /* MPM_P2G_GATHER or MPM_P2G_FIXED_POINT */
     uint32_t flags;      /* MPM_FLAG_* */
     int32_t device;      /* HIP device ordinal; -1 = current */
+    uint32_t bucket_slots; /* initial slots per 4x4-cell bucket; 0 = twice the
+                              densest bucket of the initial state (>= 32).
+                              Buckets grow on demand either way. */
 } mpm_config;
 
 /* Context creation and upload of the initial packed particles.
@@ -118,10 +121,15 @@ int mpm_read_fixed_grid(mpm_ctx* ctx, int32_t* out, size_t words);
  * solid v(2), water v(2), oil v(2), liquid mass, liquid volume). */
 int mpm_read_grid(mpm_ctx* ctx, float* out, size_t floats);
 
-/* Debug/parity: particle bin keys (gridIndex of the stencil base node,
- * 0xFFFFFFFF for removed particles) and the node_count+1 bin starts from the
- * most recent binning pass. */
-int mpm_read_bins(mpm_ctx* ctx, uint32_t* keys, uint32_t n, uint32_t* starts, size_t start_words);
+/* Debug/parity: stencil-base keys of the committed particles
+ * (gridIndex(base), shaders.ts:518-520; 0xFFFFFFFF for removed particles) and
+ * the particle count of every 4x4-cell bucket (bucket_counts may be NULL;
+ * count_words must equal the bucket count of mpm_get_buckets). */
+int mpm_read_bins(mpm_ctx* ctx, uint32_t* keys, uint32_t n, uint32_t* bucket_counts, size_t count_words);
+
+/* Bucket grid (buckets_x * buckets_y buckets of 4x4 stencil-base cells) and
+ * the current slots per bucket (grows on demand). */
+int mpm_get_buckets(mpm_ctx* ctx, uint32_t* buckets_x, uint32_t* buckets_y, uint32_t* slots_per_bucket);
 
 /* Mean kernel time in microseconds of the named kernel over the last
  * mpm_step_batch calls made while timing was enabled (HIP events on the

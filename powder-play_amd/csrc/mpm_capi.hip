@@ -6,8 +6,11 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <map>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/mpm355.h"
@@ -19,10 +22,16 @@ using namespace mpm;
 namespace {
 
 constexpr int PARAM_RING = 16;
+constexpr uint32_t MIN_BUCKET_CAP = 32;
+constexpr uint32_t MAX_BUCKET_CAP = 1024;  // a bucket must fit one LDS chunk of the P2G tile
+constexpr uint32_t MAX_PENDING = CONTROL_HISTORY;
 
 thread_local std::string g_create_error = "no error";
 
-struct Allocation {
+uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+struct DeviceArrays {
     std::vector<void*> ptrs;
     hipError_t alloc(void** p, size_t bytes) {
         *p = nullptr;
@@ -37,8 +46,19 @@ struct Allocation {
     }
 };
 
-uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
-float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+struct Pending {
+    Params params;
+    uint32_t steps;
+    bool measure;
+    uint32_t seq;
+    uint32_t done_before;  // microsteps of the caller's batch committed before a replay
+};
+
+// Host copy of the per-particle record fields that never change on the GPU.
+struct HostStatic {
+    std::vector<uint32_t> model, object_id;
+    std::vector<float> lambda, mu, density, volume, param;
+};
 
 }  // namespace
 
@@ -46,20 +66,27 @@ struct mpm_ctx {
     mpm_config cfg{};
     int device = 0;
     hipStream_t stream = nullptr;
-    Allocation mem;
+    DeviceArrays mem;       // home arrays, grid, control, params, materials
+    DeviceArrays slot_mem;  // bucket slot buffers (reallocated when they grow)
     LaunchCtx lc{};
     Params* d_params = nullptr;
     Control* d_ctl = nullptr;
-    Params* h_param_ring = nullptr;       // pinned
+    Material* d_mats = nullptr;
+    uint32_t mat_capacity = 0;
+    Params* h_param_ring = nullptr;  // pinned
     hipEvent_t ring_events[PARAM_RING] = {};
     bool ring_used[PARAM_RING] = {};
     int ring_next = 0;
-    Control* h_ctl = nullptr;             // pinned readback
-    uint32_t* h_object_ids = nullptr;     // host copy of object ids
+    Control* h_ctl = nullptr;        // pinned readback
+    HostStatic st;
+    std::vector<Material> mats;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, uint32_t> mat_index;
     float interaction[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t n = 0;
-    uint32_t node_w = 0, node_h = 0, node_count = 0;
-    int cur = 0;                          // buffer holding the committed state
+    int cur = 0;                     // slot buffer holding the committed state
+    uint32_t seq = 0;                // batch sequence number
+    std::vector<Pending> pending;    // batches queued since the last settle
+    uint32_t last_done_before = 0;   // done_before of the most recently launched batch
     std::string error = "no error";
     KernelTimes* times = nullptr;
 };
@@ -80,9 +107,9 @@ int hip_fail(mpm_ctx* c, hipError_t e, const char* what) {
     return fail(c, MPM_EDEVICE, "%s failed: %s", what, hipGetErrorString(e));
 }
 
-#define HIP_TRY(ctx, expr)                                   \
-    do {                                                     \
-        hipError_t _e = (expr);                              \
+#define HIP_TRY(ctx, expr)                                     \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
         if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
     } while (0)
 
@@ -106,7 +133,8 @@ int validate_records(mpm_ctx* c, const mpm_config& cfg, const uint32_t* rec, uin
         const float x = u2f(r[0]), y = u2f(r[1]);
         if (x < 0.0f || x > dom_w || y < 0.0f || y > dom_h)
             return fail(c, MPM_EINVAL, "particles[%u].position is outside the float32 GPU grid domain", i);
-        const float det = u2f(f2u(u2f(r[8]) * u2f(r[11]))) - u2f(r[9]) * u2f(r[10]);
+        const float p1 = u2f(r[8]) * u2f(r[11]), p2 = u2f(r[9]) * u2f(r[10]);
+        const float det = p1 - p2;
         if (!(det > 0.0f))
             return fail(c, MPM_EINVAL, "particles[%u].F must retain a finite positive determinant in float32", i);
         const float pv = u2f(r[12]), volume = u2f(r[17]), density = u2f(r[16]), param = u2f(r[18]);
@@ -115,18 +143,95 @@ int validate_records(mpm_ctx* c, const mpm_config& cfg, const uint32_t* rec, uin
         if (!(volume > 0.0f)) return fail(c, MPM_EINVAL, "particles[%u].volume must be greater than zero", i);
         if ((model == 2u || model == 3u) && !(density > 0.0f))
             return fail(c, MPM_EINVAL, "particles[%u].material.density must be greater than zero", i);
+        if ((model == 2u || model == 3u) && !(u2f(r[14]) > 0.0f))
+            return fail(c, MPM_EINVAL, "particles[%u].material.lambda must be greater than zero", i);
         if (model == 0u && (param < 0.0f || param > 1.0f))
             return fail(c, MPM_EINVAL, "particles[%u].damage must be between zero and one", i);
+        if (param < 0.0f) return fail(c, MPM_EINVAL, "particles[%u] material parameter must be non-negative", i);
     }
     return MPM_OK;
 }
 
-void unpack_records(const uint32_t* rec, uint32_t n, std::vector<float2>& pos, std::vector<float2>& vel,
-                    std::vector<float4>& C, std::vector<float4>& F, std::vector<float>& pl, std::vector<uint32_t>& model,
-                    std::vector<float>& lambda, std::vector<float>& mu, std::vector<float>& density,
-                    std::vector<float>& volume, std::vector<float>& param) {
-    pos.resize(n); vel.resize(n); C.resize(n); F.resize(n); pl.resize(n); model.resize(n);
-    lambda.resize(n); mu.resize(n); density.resize(n); volume.resize(n); param.resize(n);
+uint32_t material_row(mpm_ctx* c, const uint32_t* r) {
+    auto key = std::make_tuple(r[13], r[14], r[15], r[16], r[17], r[18]);
+    auto it = c->mat_index.find(key);
+    if (it != c->mat_index.end()) return it->second;
+    Material m{};
+    m.model = r[13];
+    m.lambda = u2f(r[14]);
+    m.mu = u2f(r[15]);
+    m.density = u2f(r[16]);
+    m.volume = u2f(r[17]);
+    m.param = u2f(r[18]);
+    const bool liquid = m.model == MODEL_WATER || m.model == MODEL_OIL;
+    m.mass = liquid ? m.density * m.volume : c->cfg.particle_mass;
+    m.phase = m.model == MODEL_WATER ? 1u : (m.model == MODEL_OIL ? 2u : 0u);
+    const uint32_t id = (uint32_t)c->mats.size();
+    c->mats.push_back(m);
+    c->mat_index.emplace(key, id);
+    return id;
+}
+
+// Bucket of a record with the kernels' float32 stencil (shaders.ts:667-676).
+bool host_bucket(const mpm_ctx* c, const uint32_t* r, uint32_t* bucket) {
+    const float gx = u2f(r[0]) / c->cfg.cell_size, gy = u2f(r[1]) / c->cfg.cell_size;
+    if (!isfinite(gx) || !isfinite(gy)) return false;
+    const int bx = (int)floorf(gx - 0.5f), by = (int)floorf(gy - 0.5f);
+    if (bx < -1 || by < -1 || bx + 2 > (int)c->cfg.cells_x + 1 || by + 2 > (int)c->cfg.cells_y + 1) return false;
+    *bucket = (uint32_t)((by + 1) >> 2) * c->lc.bw + (uint32_t)((bx + 1) >> 2);
+    return true;
+}
+
+int alloc_slots(mpm_ctx* c, uint32_t cap) {
+    c->slot_mem.release();
+    c->lc.cap = cap;
+    const size_t slots = (size_t)c->lc.nb * cap;
+    const bool general = scene_general(c->lc.scene_flags);
+    hipError_t e;
+    for (int b = 0; b < 2; ++b) {
+        SlotBuf& s = c->lc.buf[b];
+        s = SlotBuf{};
+#define SALLOC(ptr, bytes)                                                                               \
+    if ((e = c->slot_mem.alloc((void**)&(ptr), (bytes))) != hipSuccess)                                  \
+        return fail(c, MPM_EDEVICE, "hipMalloc(%zu bytes) for bucket slots failed: %s", (size_t)(bytes), \
+                    hipGetErrorString(e));
+        SALLOC(s.pos, slots * sizeof(float2));
+        SALLOC(s.vel, slots * sizeof(float2));
+        SALLOC(s.C, slots * sizeof(float4));
+        SALLOC(s.F, slots * sizeof(float4));
+        SALLOC(s.pid, slots * sizeof(uint32_t));
+        SALLOC(s.mat, slots * sizeof(uint32_t));
+        SALLOC(s.cnt, (size_t)c->lc.nb * sizeof(uint32_t));
+        if (general) {
+            SALLOC(s.S, slots * sizeof(float4));
+            SALLOC(s.plastic, slots * sizeof(float));
+        }
+#undef SALLOC
+    }
+    return MPM_OK;
+}
+
+int upload_materials(mpm_ctx* c) {
+    if (c->mats.size() > c->mat_capacity) {
+        uint32_t capm = std::max<uint32_t>(16u, c->mat_capacity);
+        while (capm < c->mats.size()) capm *= 2;
+        Material* d = nullptr;
+        HIP_TRY(c, hipMalloc((void**)&d, capm * sizeof(Material)));
+        if (c->d_mats) HIP_TRY(c, hipFree(c->d_mats));
+        c->d_mats = d;
+        c->mat_capacity = capm;
+        c->lc.mats = d;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_mats, c->mats.data(), c->mats.size() * sizeof(Material), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+// Write n records into the caller-order arrays at `offset`.
+int write_home(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n, uint32_t offset) {
+    std::vector<float2> pos(n), vel(n);
+    std::vector<float4> C(n), F(n);
+    std::vector<float> pl(n);
+    std::vector<uint32_t> model(n), mat(n);
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t* r = rec + 20ull * i;
         pos[i] = make_float2(u2f(r[0]), u2f(r[1]));
@@ -135,53 +240,136 @@ void unpack_records(const uint32_t* rec, uint32_t n, std::vector<float2>& pos, s
         F[i] = make_float4(u2f(r[8]), u2f(r[9]), u2f(r[10]), u2f(r[11]));
         pl[i] = u2f(r[12]);
         model[i] = r[13];
-        lambda[i] = u2f(r[14]);
-        mu[i] = u2f(r[15]);
-        density[i] = u2f(r[16]);
-        volume[i] = u2f(r[17]);
-        param[i] = u2f(r[18]);
-    }
-}
-
-template <class T>
-hipError_t upload(T* dst, const std::vector<T>& src, uint32_t offset, hipStream_t s) {
-    if (src.empty()) return hipSuccess;
-    return hipMemcpyAsync(dst + offset, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
-}
-
-int write_particles(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n, uint32_t offset) {
-    std::vector<float2> pos, vel;
-    std::vector<float4> C, F;
-    std::vector<float> pl, lambda, mu, density, volume, param;
-    std::vector<uint32_t> model;
-    unpack_records(rec, n, pos, vel, C, F, pl, model, lambda, mu, density, volume, param);
-    const ParticleBuf& b = c->lc.buf[c->cur];
-    HIP_TRY(c, upload(b.pos, pos, offset, c->stream));
-    HIP_TRY(c, upload(b.vel, vel, offset, c->stream));
-    HIP_TRY(c, upload(b.C, C, offset, c->stream));
-    HIP_TRY(c, upload(b.F, F, offset, c->stream));
-    HIP_TRY(c, upload(b.plastic, pl, offset, c->stream));
-    HIP_TRY(c, upload(c->lc.st.model, model, offset, c->stream));
-    HIP_TRY(c, upload(c->lc.st.lambda, lambda, offset, c->stream));
-    HIP_TRY(c, upload(c->lc.st.mu, mu, offset, c->stream));
-    HIP_TRY(c, upload(c->lc.st.density, density, offset, c->stream));
-    HIP_TRY(c, upload(c->lc.st.volume, volume, offset, c->stream));
-    HIP_TRY(c, upload(c->lc.st.param, param, offset, c->stream));
-    for (uint32_t i = 0; i < n; ++i) {
-        c->h_object_ids[offset + i] = ids ? ids[i] : 0u;
-        const uint32_t m = model[i];
+        mat[i] = material_row(c, r);
+        const uint32_t j = offset + i;
+        c->st.model[j] = r[13];
+        c->st.lambda[j] = u2f(r[14]);
+        c->st.mu[j] = u2f(r[15]);
+        c->st.density[j] = u2f(r[16]);
+        c->st.volume[j] = u2f(r[17]);
+        c->st.param[j] = u2f(r[18]);
+        c->st.object_id[j] = ids ? ids[i] : 0u;
+        const uint32_t m = r[13];
         c->lc.scene_flags |= m == 2u ? SCENE_WATER : (m == 3u ? SCENE_OIL : SCENE_SOLID);
-        if ((m == 2u || m == 3u) && param[i] > 0.0f) c->lc.scene_flags |= SCENE_TAIT;
+        if ((m == 2u || m == 3u) && u2f(r[18]) > 0.0f) c->lc.scene_flags |= SCENE_TAIT;
     }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
+    const Home& h = c->lc.home;
+    if (n) {
+        HIP_TRY(c, hipMemcpy(h.pos + offset, pos.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(h.vel + offset, vel.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(h.C + offset, C.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(h.F + offset, F.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(h.plastic + offset, pl.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(h.model + offset, model.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(h.mat + offset, mat.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    return upload_materials(c);
+}
+
+// Bucket every live particle of the caller-order arrays into buffer cur,
+// growing the bucket capacity until it fits.
+int rebucket_all(mpm_ctx* c, uint32_t min_cap) {
+    uint32_t cap = std::max(min_cap, c->lc.cap);
+    for (;;) {
+        if (cap > MAX_BUCKET_CAP)
+            return fail(c, MPM_ECAPACITY, "particle density exceeds %u particles per 4x4-cell bucket",
+                        MAX_BUCKET_CAP);
+        if (cap != c->lc.cap || !c->lc.buf[0].pos) {
+            int a = alloc_slots(c, cap);
+            if (a) return a;
+        }
+        HIP_TRY(c, launch_clear_counts(c->lc, c->cur));
+        // keep the status: a flag raised before the rebucket stays raised
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipMemcpy(c->h_ctl, c->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+        const uint32_t before = c->h_ctl->status[0];
+        HIP_TRY(c, launch_bucket(c->lc, c->cur, 0, c->n));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipMemcpy(c->h_ctl, c->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+        if (!(c->h_ctl->status[0] & ERR_INTERNAL_REBUCKET)) return MPM_OK;
+        c->h_ctl->status[0] = before;
+        HIP_TRY(c, hipMemcpy(&c->d_ctl->status[0], &before, sizeof(uint32_t), hipMemcpyHostToDevice));
+        cap *= 2;
+    }
+}
+
+int initial_cap(mpm_ctx* c, const uint32_t* rec, uint32_t n) {
+    std::vector<uint32_t> occ(c->lc.nb, 0u);
+    uint32_t mx = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t b;
+        if (rec[20ull * i + 13] == MODEL_REMOVED || !host_bucket(c, rec + 20ull * i, &b)) continue;
+        mx = std::max(mx, ++occ[b]);
+    }
+    uint32_t cap = std::max(MIN_BUCKET_CAP, ((2 * mx + 15) / 16) * 16);
+    return (int)std::min(cap, MAX_BUCKET_CAP);
+}
+
+int settle(mpm_ctx* c);
+
+int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t done_before) {
+    if (c->pending.size() >= MAX_PENDING) {
+        // keep the replay list within the control block's history
+        int s = settle(c);
+        if (s) return s;
+    }
+    const int slot = c->ring_next;
+    c->ring_next = (c->ring_next + 1) % PARAM_RING;
+    if (c->ring_used[slot]) HIP_TRY(c, hipEventSynchronize(c->ring_events[slot]));
+    c->h_param_ring[slot] = p;
+    HIP_TRY(c, hipMemcpyAsync(c->d_params, &c->h_param_ring[slot], sizeof(Params), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ring_events[slot], c->stream));
+    c->ring_used[slot] = true;
+    LaunchCtx lc = c->lc;
+    lc.timing = c->times;
+    const uint32_t seq = c->seq++;
+    HIP_TRY(c, launch_batch(lc, c->cur, steps, measure, seq));
+    c->pending.push_back({p, steps, measure, seq, done_before});
+    c->last_done_before = done_before;
+    c->cur = (c->cur + (int)(steps & 1u)) & 1;
     return MPM_OK;
 }
 
-int sync(mpm_ctx* c) {
+// Wait for queued work; replay the microsteps a full bucket interrupted.
+int settle(mpm_ctx* c) {
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->times) c->times->resolve();
-    return MPM_OK;
+    for (;;) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (c->times) c->times->resolve();
+        HIP_TRY(c, hipMemcpy(c->h_ctl, c->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+        const uint32_t st = c->h_ctl->status[0];
+        if (!(st & ERR_INTERNAL_REBUCKET)) {
+            c->pending.clear();
+            return MPM_OK;
+        }
+        // find the batch that ran out of slots; later ones did not run
+        size_t k = 0;
+        while (k < c->pending.size() &&
+               c->h_ctl->history[c->pending[k].seq % CONTROL_HISTORY] >= c->pending[k].steps)
+            ++k;
+        std::vector<Pending> replay;
+        if (k < c->pending.size()) {
+            Pending first = c->pending[k];
+            const uint32_t done = c->h_ctl->history[first.seq % CONTROL_HISTORY];
+            first.steps -= done;
+            first.done_before += done;
+            replay.push_back(first);
+            for (size_t q = k + 1; q < c->pending.size(); ++q) replay.push_back(c->pending[q]);
+        }
+        c->pending.clear();
+        const uint32_t rest = st & ~ERR_INTERNAL_REBUCKET;
+        HIP_TRY(c, hipMemcpy(&c->d_ctl->status[0], &rest, sizeof(uint32_t), hipMemcpyHostToDevice));
+        // committed state -> caller-order arrays -> larger buckets
+        HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+        int r = rebucket_all(c, c->lc.cap * 2);
+        if (r) return r;
+        if (rest != 0u) return MPM_OK;  // a real flag stops the replay
+        for (const Pending& p : replay) {
+            if (p.steps == 0) continue;
+            r = launch(c, p.params, p.steps, p.measure, p.done_before);
+            if (r) return r;
+        }
+    }
 }
 
 }  // namespace
@@ -210,7 +398,6 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     if (capacity < n) return fail(nullptr, MPM_EINVAL, "capacity must be at least the initial particle count");
     if (capacity > (1u << 30)) return fail(nullptr, MPM_EINVAL, "capacity exceeds the supported 2^30 particle slots");
     const uint64_t nodes64 = (uint64_t)(c.cells_x + 3) * (uint64_t)(c.cells_y + 3);
-    if (nodes64 >= 0xffffffffull) return fail(nullptr, MPM_EINVAL, "halo grid dimensions exceed the GPU index range");
     {
         int v = validate_records(nullptr, c, packed20, n);
         if (v) return v;
@@ -220,52 +407,50 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     if (!ctx) return fail(nullptr, MPM_EDEVICE, "out of host memory");
     ctx->cfg = c;
     ctx->cfg.capacity = capacity;
-    ctx->node_w = c.cells_x + 3;
-    ctx->node_h = c.cells_y + 3;
-    ctx->node_count = (uint32_t)nodes64;
     auto bail = [&](int code) {
         g_create_error = ctx->error;
         mpm_destroy(ctx);
         return code;
     };
-    hipError_t e;
-    if (c.device >= 0) {
-        if ((e = hipSetDevice(c.device)) != hipSuccess) return bail(hip_fail(ctx, e, "hipSetDevice"));
+    LaunchCtx& lc = ctx->lc;
+    lc.node_w = c.cells_x + 3;
+    lc.node_h = c.cells_y + 3;
+    lc.node_count = (uint32_t)nodes64;
+    lc.cells_x = c.cells_x;
+    lc.cells_y = c.cells_y;
+    lc.cell_size = c.cell_size;
+    lc.bw = (lc.node_w - 3) / BUCKET_CELLS + 1;  // stencil bases span storage cells [0, cells]
+    lc.bh = (lc.node_h - 3) / BUCKET_CELLS + 1;
+    lc.nb = lc.bw * lc.bh;
+    try {
+        HostStatic& s = ctx->st;
+        for (auto* v : {&s.model, &s.object_id}) v->assign(capacity ? capacity : 1, 0u);
+        for (auto* v : {&s.lambda, &s.mu, &s.density, &s.volume, &s.param}) v->assign(capacity ? capacity : 1, 0.0f);
+    } catch (...) {
+        return bail(fail(ctx, MPM_EDEVICE, "out of host memory"));
     }
+    hipError_t e;
+    if (c.device >= 0 && (e = hipSetDevice(c.device)) != hipSuccess) return bail(hip_fail(ctx, e, "hipSetDevice"));
     if ((e = hipGetDevice(&ctx->device)) != hipSuccess) return bail(hip_fail(ctx, e, "hipGetDevice"));
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(hip_fail(ctx, e, "hipStreamCreateWithFlags"));
 
     const size_t cap = capacity ? capacity : 1;
-    const size_t nodes = ctx->node_count;
-    LaunchCtx& lc = ctx->lc;
-    Allocation& m = ctx->mem;
-#define ALLOC(ptr, bytes)                                                                \
-    do {                                                                                 \
-        if ((e = m.alloc((void**)&(ptr), (bytes))) != hipSuccess)                        \
-            return bail(fail(ctx, MPM_EDEVICE, "hipMalloc(%zu bytes) failed: %s", (size_t)(bytes), hipGetErrorString(e))); \
+    const size_t nodes = lc.node_count;
+    DeviceArrays& m = ctx->mem;
+#define ALLOC(ptr, bytes)                                                                                    \
+    do {                                                                                                     \
+        if ((e = m.alloc((void**)&(ptr), (bytes))) != hipSuccess)                                            \
+            return bail(fail(ctx, MPM_EDEVICE, "hipMalloc(%zu bytes) failed: %s", (size_t)(bytes),           \
+                             hipGetErrorString(e)));                                                         \
     } while (0)
-    for (int b = 0; b < 2; ++b) {
-        ALLOC(lc.buf[b].pos, cap * sizeof(float2));
-        ALLOC(lc.buf[b].vel, cap * sizeof(float2));
-        ALLOC(lc.buf[b].C, cap * sizeof(float4));
-        ALLOC(lc.buf[b].F, cap * sizeof(float4));
-        ALLOC(lc.buf[b].plastic, cap * sizeof(float));
-    }
-    ALLOC(lc.st.model, cap * sizeof(uint32_t));
-    ALLOC(lc.st.lambda, cap * sizeof(float));
-    ALLOC(lc.st.mu, cap * sizeof(float));
-    ALLOC(lc.st.density, cap * sizeof(float));
-    ALLOC(lc.st.volume, cap * sizeof(float));
-    ALLOC(lc.st.param, cap * sizeof(float));
-    ALLOC(lc.bins.key, cap * sizeof(uint32_t));
-    ALLOC(lc.bins.rank, cap * sizeof(uint32_t));
-    ALLOC(lc.bins.perm, cap * sizeof(uint32_t));
-    ALLOC(lc.bins.count, nodes * sizeof(uint32_t));
-    ALLOC(lc.bins.start, (nodes + 1) * sizeof(uint32_t));
-    ALLOC(lc.bins.block_sums, scan_block_count(ctx->node_count) * sizeof(uint32_t));
-    ALLOC(lc.bins.affine, cap * sizeof(float4));
-    ALLOC(lc.bins.aux, cap * sizeof(float4));
+    ALLOC(lc.home.pos, cap * sizeof(float2));
+    ALLOC(lc.home.vel, cap * sizeof(float2));
+    ALLOC(lc.home.C, cap * sizeof(float4));
+    ALLOC(lc.home.F, cap * sizeof(float4));
+    ALLOC(lc.home.plastic, cap * sizeof(float));
+    ALLOC(lc.home.model, cap * sizeof(uint32_t));
+    ALLOC(lc.home.mat, cap * sizeof(uint32_t));
     ALLOC(lc.grid.v_solid, nodes * sizeof(float2));
     ALLOC(lc.grid.v_water, nodes * sizeof(float2));
     ALLOC(lc.grid.v_oil, nodes * sizeof(float2));
@@ -282,45 +467,35 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     for (int k = 0; k < PARAM_RING; ++k)
         if ((e = hipEventCreateWithFlags(&ctx->ring_events[k], hipEventDisableTiming)) != hipSuccess)
             return bail(hip_fail(ctx, e, "hipEventCreate"));
-    ctx->h_object_ids = new (std::nothrow) uint32_t[cap];
-    if (!ctx->h_object_ids) return bail(fail(ctx, MPM_EDEVICE, "out of host memory"));
-
-    // grid arrays start at zero so debug reads of absent phases are defined
     for (void* p : {(void*)lc.grid.v_solid, (void*)lc.grid.v_water, (void*)lc.grid.v_oil, (void*)lc.grid.liquid})
-        if ((e = hipMemsetAsync(p, 0, nodes * sizeof(float2), ctx->stream)) != hipSuccess)
-            return bail(hip_fail(ctx, e, "hipMemsetAsync(grid)"));
-    if (keep_fixed && (e = hipMemsetAsync(lc.grid.fixed, 0, nodes * 11 * sizeof(int32_t), ctx->stream)) != hipSuccess)
-        return bail(hip_fail(ctx, e, "hipMemsetAsync(fixed grid)"));
-    if ((e = hipMemsetAsync(lc.bins.count, 0, nodes * sizeof(uint32_t), ctx->stream)) != hipSuccess)
-        return bail(hip_fail(ctx, e, "hipMemsetAsync(count)"));
-    if ((e = hipMemsetAsync(lc.bins.start, 0, (nodes + 1) * sizeof(uint32_t), ctx->stream)) != hipSuccess)
-        return bail(hip_fail(ctx, e, "hipMemsetAsync(start)"));
-    if ((e = hipMemsetAsync(lc.bins.key, 0xff, cap * sizeof(uint32_t), ctx->stream)) != hipSuccess)
-        return bail(hip_fail(ctx, e, "hipMemsetAsync(key)"));
+        if ((e = hipMemset(p, 0, nodes * sizeof(float2))) != hipSuccess) return bail(hip_fail(ctx, e, "hipMemset"));
+    if (keep_fixed && (e = hipMemset(lc.grid.fixed, 0, nodes * 11 * sizeof(int32_t))) != hipSuccess)
+        return bail(hip_fail(ctx, e, "hipMemset(fixed grid)"));
     Control init{};
     init.status[1] = 0xffffffffu;  // solver.ts:889-893
+    init.deferred_index[0] = init.deferred_index[1] = 0xffffffffu;
+    init.fail_step = 0xffffffffu;
     *ctx->h_ctl = init;
-    if ((e = hipMemcpyAsync(ctx->d_ctl, ctx->h_ctl, sizeof(Control), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
-        return bail(hip_fail(ctx, e, "hipMemcpyAsync(control)"));
+    if ((e = hipMemcpy(ctx->d_ctl, ctx->h_ctl, sizeof(Control), hipMemcpyHostToDevice)) != hipSuccess)
+        return bail(hip_fail(ctx, e, "hipMemcpy(control)"));
 
     lc.stream = ctx->stream;
     lc.params = ctx->d_params;
     lc.ctl = ctx->d_ctl;
-    lc.node_w = ctx->node_w;
-    lc.node_h = ctx->node_h;
-    lc.node_count = ctx->node_count;
     lc.fixed_point = c.p2g_mode == MPM_P2G_FIXED_POINT ? 1u : 0u;
     lc.keep_fixed = keep_fixed ? 1u : 0u;
     lc.scene_flags = 0u;
-    lc.scan_blocks = (uint32_t)scan_block_count(ctx->node_count);
     ctx->cur = 0;
-    if (n > 0) {
-        int w = write_particles(ctx, packed20, object_ids, n, 0);
-        if (w) return bail(w);
-    }
+    int w = write_home(ctx, packed20, object_ids, n, 0);
+    if (w) return bail(w);
     ctx->n = n;
     lc.n = n;
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(hip_fail(ctx, e, "hipStreamSynchronize"));
+    const uint32_t cap0 = c.bucket_slots ? std::min<uint32_t>(c.bucket_slots, MAX_BUCKET_CAP)
+                                         : (uint32_t)initial_cap(ctx, packed20, n);
+    int a = alloc_slots(ctx, cap0);
+    if (a) return bail(a);
+    int r = rebucket_all(ctx, cap0);
+    if (r) return bail(r);
     *out = ctx;
     return MPM_OK;
 }
@@ -331,19 +506,20 @@ int mpm_step_batch(mpm_ctx* c, const void* params144, uint32_t step_count, uint3
     if (step_count < 1) return fail(c, MPM_EINVAL, "stepCount must be a positive safe integer.");
     Params p{};
     memcpy(&p, params144, MPM_PARAMS_BYTES);
-    if (p.cells_x != c->cfg.cells_x || p.cells_y != c->cfg.cells_y || p.node_w != c->node_w || p.node_h != c->node_h)
+    const LaunchCtx& lc = c->lc;
+    if (p.cells_x != c->cfg.cells_x || p.cells_y != c->cfg.cells_y || p.node_w != lc.node_w || p.node_h != lc.node_h)
         return fail(c, MPM_EINVAL, "params grid does not match the solver grid");
-    if (p.node_count != c->node_count) return fail(c, MPM_EINVAL, "params nodeCount does not match the solver grid");
+    if (p.node_count != lc.node_count) return fail(c, MPM_EINVAL, "params nodeCount does not match the solver grid");
     if (p.particle_count != c->n)
         return fail(c, MPM_EINVAL, "params particleCount %u does not match the %u allocated particle slots",
                     p.particle_count, c->n);
     if (p.body_count != 0u) return fail(c, MPM_EINVAL, "bonded triangle bodies are not supported");
     if (!(p.dt > 0.0f) || !isfinite(p.dt)) return fail(c, MPM_EINVAL, "dt must be greater than zero");
     if (p.dx != c->cfg.cell_size) return fail(c, MPM_EINVAL, "params cellSize does not match the solver grid");
+    if (p.particle_mass != c->cfg.particle_mass || p.particle_volume != c->cfg.particle_volume)
+        return fail(c, MPM_EINVAL, "params particleMass/particleVolume do not match the solver configuration");
     if (!isfinite(p.gx) || !isfinite(p.gy)) return fail(c, MPM_EINVAL, "gravity must be finite");
-    if (!isfinite(p.particle_mass) || !(p.particle_mass > 0.0f) || !isfinite(p.particle_volume) ||
-        !(p.particle_volume > 0.0f) || !isfinite(p.grad_scale) || !(p.grad_scale > 0.0f))
-        return fail(c, MPM_EINVAL, "params physical block is invalid");
+    if (!isfinite(p.grad_scale) || !(p.grad_scale > 0.0f)) return fail(c, MPM_EINVAL, "params gradient scale is invalid");
     if (p.cfl_on >= 0.5f && (!isfinite(p.wave_bound) || !isfinite(p.vel_bound) || p.vel_bound < 0.0f))
         return fail(c, MPM_EINVAL, "CFL bounds must be finite and non-negative");
     p.drag_target_x = c->interaction[0];
@@ -352,28 +528,15 @@ int mpm_step_batch(mpm_ctx* c, const void* params144, uint32_t step_count, uint3
     p.drag_stiffness = c->interaction[3];
     p.drag_damping = c->interaction[4];
     HIP_TRY(c, hipSetDevice(c->device));
-    const int slot = c->ring_next;
-    c->ring_next = (c->ring_next + 1) % PARAM_RING;
-    if (c->ring_used[slot]) HIP_TRY(c, hipEventSynchronize(c->ring_events[slot]));
-    c->h_param_ring[slot] = p;
-    HIP_TRY(c, hipMemcpyAsync(c->d_params, &c->h_param_ring[slot], sizeof(Params), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ring_events[slot], c->stream));
-    c->ring_used[slot] = true;
-    LaunchCtx lc = c->lc;
-    lc.timing = c->times;
-    const bool measure = measure_speed != 0u || p.cfl_on >= 0.5f;
-    HIP_TRY(c, launch_batch(lc, c->cur, step_count, measure));
-    c->cur = (c->cur + (int)(step_count & 1u)) & 1;
-    return MPM_OK;
+    return launch(c, p, step_count, measure_speed != 0u || p.cfl_on >= 0.5f, 0u);
 }
 
 int mpm_read_compact(mpm_ctx* c, float* max_speed, uint32_t* completed, uint32_t status[8]) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
-    int s = sync(c);
+    int s = settle(c);
     if (s) return s;
-    HIP_TRY(c, hipMemcpy(c->h_ctl, c->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
     if (max_speed) *max_speed = u2f(c->h_ctl->max_speed_bits);
-    if (completed) *completed = c->h_ctl->completed;
+    if (completed) *completed = c->h_ctl->completed + c->last_done_before;
     if (status) memcpy(status, c->h_ctl->status, sizeof(uint32_t) * 8);
     return MPM_OK;
 }
@@ -382,7 +545,7 @@ int mpm_read_status(mpm_ctx* c, uint32_t status[8]) { return mpm_read_compact(c,
 
 int mpm_clear_status(mpm_ctx* c) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
-    int s = sync(c);
+    int s = settle(c);
     if (s) return s;
     uint32_t init[8] = {0u, 0xffffffffu, 0u, 0u, 0u, 0u, 0u, 0u};
     HIP_TRY(c, hipMemcpy(c->d_ctl->status, init, sizeof init, hipMemcpyHostToDevice));
@@ -396,36 +559,32 @@ int mpm_read_particles(mpm_ctx* c, const uint32_t* indices, uint32_t n, uint32_t
     if (indices)
         for (uint32_t k = 0; k < n; ++k)
             if (indices[k] >= c->n) return fail(c, MPM_EINVAL, "particle index %u is out of range", indices[k]);
-    int s = sync(c);
+    int s = settle(c);
     if (s) return s;
+    HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t total = indices ? c->n : n;
     std::vector<float2> pos(total), vel(total);
     std::vector<float4> C(total), F(total);
-    std::vector<float> pl(total), lambda(total), mu(total), density(total), volume(total), param(total);
-    std::vector<uint32_t> model(total);
-    const ParticleBuf& b = c->lc.buf[c->cur];
+    std::vector<float> pl(total);
+    const Home& h = c->lc.home;
     if (total > 0) {
-        HIP_TRY(c, hipMemcpy(pos.data(), b.pos, total * sizeof(float2), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(vel.data(), b.vel, total * sizeof(float2), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(C.data(), b.C, total * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(F.data(), b.F, total * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(pl.data(), b.plastic, total * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(model.data(), c->lc.st.model, total * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(lambda.data(), c->lc.st.lambda, total * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(mu.data(), c->lc.st.mu, total * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(density.data(), c->lc.st.density, total * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(volume.data(), c->lc.st.volume, total * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(c, hipMemcpy(param.data(), c->lc.st.param, total * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(pos.data(), h.pos, total * sizeof(float2), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(vel.data(), h.vel, total * sizeof(float2), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(C.data(), h.C, total * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(F.data(), h.F, total * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(pl.data(), h.plastic, total * sizeof(float), hipMemcpyDeviceToHost));
     }
+    const HostStatic& st = c->st;
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t i = indices ? indices[k] : k;
         uint32_t* r = out20 + 20ull * k;
         r[0] = f2u(pos[i].x); r[1] = f2u(pos[i].y); r[2] = f2u(vel[i].x); r[3] = f2u(vel[i].y);
         r[4] = f2u(C[i].x); r[5] = f2u(C[i].y); r[6] = f2u(C[i].z); r[7] = f2u(C[i].w);
         r[8] = f2u(F[i].x); r[9] = f2u(F[i].y); r[10] = f2u(F[i].z); r[11] = f2u(F[i].w);
-        r[12] = f2u(pl[i]); r[13] = model[i]; r[14] = f2u(lambda[i]); r[15] = f2u(mu[i]);
-        r[16] = f2u(density[i]); r[17] = f2u(volume[i]); r[18] = f2u(param[i]); r[19] = 0u;
-        if (object_ids) object_ids[k] = c->h_object_ids[i];
+        r[12] = f2u(pl[i]); r[13] = st.model[i]; r[14] = f2u(st.lambda[i]); r[15] = f2u(st.mu[i]);
+        r[16] = f2u(st.density[i]); r[17] = f2u(st.volume[i]); r[18] = f2u(st.param[i]); r[19] = 0u;
+        if (object_ids) object_ids[k] = st.object_id[i];
     }
     return MPM_OK;
 }
@@ -438,13 +597,21 @@ int mpm_append_particles(mpm_ctx* c, const uint32_t* packed20, const uint32_t* o
         return fail(c, MPM_ECAPACITY, "appending %u particles exceeds the capacity of %u", n, c->cfg.capacity);
     int v = validate_records(c, c->cfg, packed20, n);
     if (v) return v;
-    int s = sync(c);
+    int s = settle(c);
     if (s) return s;
-    int w = write_particles(c, packed20, object_ids, n, c->n);
+    // committed state -> caller-order arrays, append, bucket everything again
+    HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const bool was_general = scene_general(c->lc.scene_flags);
+    int w = write_home(c, packed20, object_ids, n, c->n);
     if (w) return w;
     c->n += n;
     c->lc.n = c->n;
-    return MPM_OK;
+    if (scene_general(c->lc.scene_flags) != was_general) {
+        int a = alloc_slots(c, c->lc.cap);
+        if (a) return a;
+    }
+    return rebucket_all(c, c->lc.cap);
 }
 
 int mpm_set_interaction(mpm_ctx* c, const float p[8]) {
@@ -462,15 +629,15 @@ int mpm_get_counts(mpm_ctx* c, uint32_t* particle_count, uint32_t* capacity, uin
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     if (particle_count) *particle_count = c->n;
     if (capacity) *capacity = c->cfg.capacity;
-    if (node_count) *node_count = c->node_count;
+    if (node_count) *node_count = c->lc.node_count;
     return MPM_OK;
 }
 
 int mpm_read_fixed_grid(mpm_ctx* c, int32_t* out, size_t words) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     if (!c->lc.grid.fixed) return fail(c, MPM_ESTATE, "fixed-point grid retention is not enabled on this solver");
-    if (!out || words != (size_t)c->node_count * 11) return fail(c, MPM_EINVAL, "output must hold node_count*11 words");
-    int s = sync(c);
+    if (!out || words != (size_t)c->lc.node_count * 11) return fail(c, MPM_EINVAL, "output must hold node_count*11 words");
+    int s = settle(c);
     if (s) return s;
     HIP_TRY(c, hipMemcpy(out, c->lc.grid.fixed, words * sizeof(int32_t), hipMemcpyDeviceToHost));
     return MPM_OK;
@@ -478,14 +645,14 @@ int mpm_read_fixed_grid(mpm_ctx* c, int32_t* out, size_t words) {
 
 int mpm_read_grid(mpm_ctx* c, float* out, size_t floats) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
-    if (!out || floats != (size_t)c->node_count * 8) return fail(c, MPM_EINVAL, "output must hold node_count*8 floats");
-    int s = sync(c);
+    if (!out || floats != (size_t)c->lc.node_count * 8) return fail(c, MPM_EINVAL, "output must hold node_count*8 floats");
+    int s = settle(c);
     if (s) return s;
-    std::vector<float2> a(c->node_count);
+    std::vector<float2> a(c->lc.node_count);
     const float2* src[4] = {c->lc.grid.v_solid, c->lc.grid.v_water, c->lc.grid.v_oil, c->lc.grid.liquid};
     for (int k = 0; k < 4; ++k) {
         HIP_TRY(c, hipMemcpy(a.data(), src[k], a.size() * sizeof(float2), hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < c->node_count; ++i) {
+        for (uint32_t i = 0; i < c->lc.node_count; ++i) {
             out[8ull * i + 2 * k] = a[i].x;
             out[8ull * i + 2 * k + 1] = a[i].y;
         }
@@ -493,21 +660,40 @@ int mpm_read_grid(mpm_ctx* c, float* out, size_t floats) {
     return MPM_OK;
 }
 
-int mpm_read_bins(mpm_ctx* c, uint32_t* keys, uint32_t n, uint32_t* starts, size_t start_words) {
+int mpm_read_bins(mpm_ctx* c, uint32_t* keys, uint32_t n, uint32_t* bucket_counts, size_t count_words) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     if (n > c->n) return fail(c, MPM_EINVAL, "cannot read %u keys from %u slots", n, c->n);
-    if (starts && start_words != (size_t)c->node_count + 1)
-        return fail(c, MPM_EINVAL, "starts must hold node_count+1 words");
-    int s = sync(c);
+    if (bucket_counts && count_words != (size_t)c->lc.nb)
+        return fail(c, MPM_EINVAL, "bucket_counts must hold %u words", c->lc.nb);
+    int s = settle(c);
     if (s) return s;
-    if (keys && n) HIP_TRY(c, hipMemcpy(keys, c->lc.bins.key, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (starts) HIP_TRY(c, hipMemcpy(starts, c->lc.bins.start, start_words * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+    if (keys && n) {
+        uint32_t* d = nullptr;
+        HIP_TRY(c, hipMalloc((void**)&d, (size_t)c->n * sizeof(uint32_t)));
+        hipError_t e = launch_keys(c->lc, d);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(keys, d, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        if (e != hipSuccess) return hip_fail(c, e, "keys");
+    }
+    if (bucket_counts)
+        HIP_TRY(c, hipMemcpy(bucket_counts, c->lc.buf[c->cur].cnt, count_words * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_get_buckets(mpm_ctx* c, uint32_t* buckets_x, uint32_t* buckets_y, uint32_t* slots_per_bucket) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (buckets_x) *buckets_x = c->lc.bw;
+    if (buckets_y) *buckets_y = c->lc.bh;
+    if (slots_per_bucket) *slots_per_bucket = c->lc.cap;
     return MPM_OK;
 }
 
 int mpm_set_timing(mpm_ctx* c, uint32_t enable) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
-    int s = sync(c);
+    int s = settle(c);
     if (s) return s;
     if (enable) {
         if (!c->times) {
@@ -526,7 +712,7 @@ int mpm_read_timing(mpm_ctx* c, const char* kernel, double* mean_us, uint64_t* l
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     if (!kernel) return fail(c, MPM_EINVAL, "kernel name must not be NULL");
     if (!c->times) return fail(c, MPM_ESTATE, "timing is not enabled");
-    int s = sync(c);
+    int s = settle(c);
     if (s) return s;
     auto it = c->times->totals.find(kernel);
     if (it == c->times->totals.end() || it->second.second == 0) {
@@ -544,12 +730,13 @@ void mpm_destroy(mpm_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     delete c->times;
+    c->slot_mem.release();
     c->mem.release();
+    if (c->d_mats) (void)hipFree(c->d_mats);
     for (int k = 0; k < PARAM_RING; ++k)
         if (c->ring_events[k]) (void)hipEventDestroy(c->ring_events[k]);
     if (c->h_param_ring) (void)hipHostFree(c->h_param_ring);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
-    delete[] c->h_object_ids;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

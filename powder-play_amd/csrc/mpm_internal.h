@@ -25,6 +25,10 @@ enum : uint32_t {
     ERR_CFL_BOUND = 16384u,
     ERR_FRACTURE_BOND = 32768u,
     ERR_FIXED_POINT = 65536u,
+    // Not a reference flag: a bucket ran out of slots.  The step rolls back
+    // like any failure and the host grows the buckets and replays the
+    // remaining microsteps; callers never see this bit.
+    ERR_INTERNAL_REBUCKET = 0x80000000u,
 };
 
 enum : uint32_t {
@@ -37,6 +41,7 @@ enum : uint32_t {
 };
 
 constexpr uint32_t INVALID_KEY = 0xffffffffu;
+constexpr int BUCKET_CELLS = 4;  // a bucket holds the particles of 4x4 stencil-base cells
 
 // Decoded step-parameter block: the 144 bytes of packMpmMlsGpuStepParameters
 // (layout.ts:896-997) followed by the 32-byte drag interaction uniform
@@ -62,39 +67,46 @@ struct Control {
     uint32_t status[8];
     uint32_t completed;       // committed microsteps of the current batch
     uint32_t max_speed_bits;  // f32 bits of the measured maximum speed
-    uint32_t deferred;        // prepare failures found by G2P for the next microstep
-    uint32_t deferred_index;  // lowest particle index among them
-    uint32_t pad[4];
+    uint32_t deferred[2];       // per slot buffer: stress-cache failures of the state it holds
+    uint32_t deferred_index[2]; // lowest particle index among them
+    uint32_t fail_step;         // first microstep of the batch that raised a flag
+    uint32_t pad;
+    uint32_t history[64];       // committed microsteps per batch, by sequence number
+};
+constexpr uint32_t CONTROL_HISTORY = 64;
+
+// One material row: particles reference it by index (material id).  Rows are
+// the distinct (model, lambda, mu, density, volume, param) of the scene.
+struct Material {
+    uint32_t model;
+    float lambda, mu, density, volume, param;
+    float mass;     // density*volume for liquids, particleMass otherwise
+    uint32_t phase; // 0 solid, 1 water, 2 oil
+};
+static_assert(sizeof(Material) == 32, "Material row is 32 bytes");
+
+// Particle state stored by bucket: slot = bucket * cap + k, k < cnt[bucket].
+struct SlotBuf {
+    float2* pos;
+    float2* vel;
+    float4* C;
+    float4* F;
+    float4* S;        // P*F^T stress cache (scenes with solids or Tait liquids)
+    float* plastic;   // snow plastic volume (scenes with solids)
+    uint32_t* pid;    // particle index in the caller's order
+    uint32_t* mat;    // material row
+    uint32_t* cnt;    // per bucket
 };
 
-// Dynamic particle state of one ping-pong buffer (SoA).
-struct ParticleBuf {
+// Particle arrays in the caller's index order (upload, readback, removed).
+struct Home {
     float2* pos;
     float2* vel;
     float4* C;
     float4* F;
     float* plastic;
-};
-
-// Particle fields that do not change within a step.
-struct ParticleStatic {
     uint32_t* model;
-    float* lambda;
-    float* mu;
-    float* density;
-    float* volume;
-    float* param;
-};
-
-struct Bins {
-    uint32_t* key;    // per particle: gridIndex(base) or INVALID_KEY
-    uint32_t* rank;   // per particle: slot inside its bin (arrival order)
-    uint32_t* count;  // per node; zero between uses
-    uint32_t* start;  // node_count + 1
-    uint32_t* perm;   // particle indices grouped by bin
-    uint32_t* block_sums;
-    float4* affine;   // per particle: A = m*C + (-dt*V*4/dx^2) P F^T (preparedAffines)
-    float4* aux;      // per particle: (mass, volume, det F, phase bits)
+    uint32_t* mat;
 };
 
 struct Grid {
@@ -107,30 +119,38 @@ struct Grid {
 
 // Scene content flags, host-derived from the particle models.
 enum : uint32_t { SCENE_SOLID = 1u, SCENE_WATER = 2u, SCENE_OIL = 4u, SCENE_TAIT = 8u };
-// Scenes with solids or Tait liquids need the FP64 stress path.
+// Scenes with solids or Tait liquids carry the FP64 stress cache.
 inline bool scene_general(uint32_t flags) { return (flags & (SCENE_SOLID | SCENE_TAIT)) != 0u; }
 
-struct KernelTimes;  // host-side timing (mpm_capi.hip)
+struct KernelTimes;  // host-side timing (mpm_timing.h)
 
-// Launchers (mpm_kernels.hip).  All work is enqueued on `stream`.
 struct LaunchCtx {
     hipStream_t stream;
     const Params* params;  // device
     Control* ctl;          // device
-    ParticleBuf buf[2];
-    ParticleStatic st;
-    Bins bins;
+    SlotBuf buf[2];
+    Home home;
+    const Material* mats;  // device
     Grid grid;
-    uint32_t n;            // active particle slots
+    uint32_t n;            // active particle slots (caller indices)
     uint32_t node_w, node_h, node_count;
+    uint32_t cells_x, cells_y;
+    float cell_size;
+    uint32_t bw, bh, nb, cap;  // buckets: bw x bh, nb total, cap slots each
     uint32_t fixed_point;  // 0 = float gather, 1 = int32 fixed point
     uint32_t keep_fixed;
     uint32_t scene_flags;
-    uint32_t scan_blocks;
     KernelTimes* timing;   // nullptr when timing is off
 };
 
-hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure);
-size_t scan_block_count(uint32_t node_count);
+hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure, uint32_t seq);
+// Bucket particles [first, first+count) of the home arrays into buffer b
+// (counts must already hold the particles bucketed so far).
+hipError_t launch_bucket(const LaunchCtx& lc, int b, uint32_t first, uint32_t count);
+hipError_t launch_clear_counts(const LaunchCtx& lc, int b);
+// Write buffer b's particles back to the home arrays.
+hipError_t launch_unbucket(const LaunchCtx& lc, int b);
+// Stencil-base keys of the home positions (debug/parity).
+hipError_t launch_keys(const LaunchCtx& lc, uint32_t* keys);
 
 }  // namespace mpm

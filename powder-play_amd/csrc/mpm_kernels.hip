@@ -1,18 +1,24 @@
 // gfx950 kernels of the 2D MLS-MPM microstep.
 //
-// One microstep is six launches on one stream:
-//   scan_reduce -> scan_blocks -> scan_final   bin starts from per-node counts
-//   scatter                                    particle indices grouped by bin
-//   p2g_grid<FIXED>                            LDS-staged gather P2G + grid update
-//   g2p                                        G2P, F update, projections, CFL guard,
-//                                              and the next microstep's bin counts
-// framed per batch by batch_begin + bin (first counts) and commit_last +
-// fix_rollback + measure_speed.  Numerical semantics follow the WGSL kernels
-// of src/lib/mpm/kernels/shaders.ts (cited per function); the fixed-point
-// path reproduces their f32 operation order exactly (the library is built
-// with -ffp-contract=off) so its int32 accumulators are bit-exact.
+// Particles live in buckets of 4x4 stencil-base cells (double-buffered slot
+// arrays, SlotBuf).  One microstep is two launches on one stream:
+//   p2g_grid<FIXED, GENERAL>  per 28x28-node tile: stage the particles of the
+//                             9x9 surrounding buckets in LDS (ordered by cell,
+//                             then particle index), gather the quadratic
+//                             B-spline P2G for a 30x30 node patch, run the grid
+//                             update on the 28x28 interior, write node velocities
+//   g2p<GENERAL>              per 8x8-bucket tile: stage the 34x34 node patch in
+//                             LDS, G2P, F update, projections, CFL guard, and
+//                             write each particle into its next bucket
+// framed per batch by batch_begin and commit_last + fix_rollback +
+// measure_speed.  Numerical semantics follow the WGSL kernels of
+// src/lib/mpm/kernels/shaders.ts (cited per function); the fixed-point path
+// reproduces their f32 operation order exactly (the library is built with
+// -ffp-contract=off) so its int32 accumulators are bit-exact.
 #include <float.h>
 #include <math.h>
+
+#include <algorithm>
 
 #include "mpm_internal.h"
 #include "mpm_timing.h"
@@ -41,19 +47,24 @@ constexpr double MAX_NEO_HOOKEAN_STRETCH = 20.0;
 constexpr float MAX_DEFORMATION_INCREMENT = 0.25f;
 constexpr uint32_t MAX_DEFORMATION_SUBSTEPS = 64u;
 
-// P2G tile geometry: a 1024-thread block computes node sums for a 32x32
-// patch (a one-node ring around its 30x30 output tile, needed by the liquid
-// contact's volume gradient) from the particles binned to the 34x34 base
-// cells that reach it, staged through LDS in chunks of CAP particles.
-constexpr int TS = 32;
-constexpr int TI = TS - 2;
-constexpr int RC = TS + 2;
-constexpr int CAP = 1280;
-constexpr int P2G_THREADS = TS * TS;
-
-constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_ITEMS = 4;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+// P2G tile: 28x28 interior nodes (7x7 buckets), a one-node ring for the
+// liquid-contact volume gradient (30x30 computed), fed by 9x9 buckets.
+constexpr int P_TI = 28;                        // interior nodes per tile side
+constexpr int P_TC = P_TI + 2;                  // with the one-node ring
+constexpr int P_RB = P_TI / BUCKET_CELLS + 2;   // region buckets per side
+constexpr int P_NB = P_RB * P_RB;
+constexpr int P_RC = P_RB * BUCKET_CELLS;       // region cells per side
+constexpr int P_CELLS = P_RC * P_RC;
+constexpr int P_THREADS = 1024;
+constexpr int P_CAP = 2 * P_THREADS;            // staged particles per gather chunk: two per thread
+static_assert(P_TC <= 32 && P_TC * 32 <= P_THREADS, "one node thread per tile node, 32 to a row");
+static_assert(P_CELLS <= 2 * P_THREADS, "the cell scan takes two cells a thread");
+static_assert(P_NB <= 128, "the bucket scan takes two buckets a lane of one wave");
+// G2P tile: 8x8 buckets (32x32 cells) and their 34x34 node patch.
+constexpr int G_TB = 8;
+constexpr int G_TC = G_TB * BUCKET_CELLS;
+constexpr int G_TN = G_TC + 2;
+constexpr int G_THREADS = 256;
 
 __device__ __forceinline__ bool finitef(float v) { return (__float_as_uint(v) & 0x7f800000u) != 0x7f800000u; }
 __device__ __forceinline__ bool finite2(float2 v) { return finitef(v.x) && finitef(v.y); }
@@ -69,12 +80,6 @@ __device__ __forceinline__ bool block_status_set(const Control* c) {
     __syncthreads();
     return s_status != 0u;
 }
-__device__ __forceinline__ void raise_flags(Control* c, uint32_t bits) { atomicOr(&c->status[0], bits); }
-__device__ __forceinline__ void raise_at(Control* c, uint32_t bits, uint32_t index) {
-    atomicMin(&c->status[1], index);
-    atomicOr(&c->status[0], bits);
-}
-
 // determinant2x2 (shaders.ts:261-266): keeps the residual of the second
 // product when the two terms nearly cancel.
 __device__ __forceinline__ float det_comp(float4 m) {
@@ -185,32 +190,32 @@ __device__ bool solid_stress(uint32_t model, float4 Ff, float lambda, float mu, 
     return true;
 }
 
-// prepareParticle (shaders.ts:808-963): the per-particle affine momentum
-// A = m*C + (-dt*V*4/dx^2) * P*F^T plus (mass, volume, det F, phase) for P2G.
-// GENERAL = false handles scenes of tension-free liquids only (pure f32);
-// GENERAL = true adds Tait liquids (FP64 pow) and the solid models (FP64).
+// Stress P*F^T of prepareParticle (shaders.ts:808-963) with its material
+// checks.  Liquids in f32 (tension-free linear EOS; Tait via FP64 pow);
+// solids in FP64, rounded once.  GENERAL = false admits linear liquids only.
 // Returns the status bits to raise.
 template <bool GENERAL>
-__device__ uint32_t prepare_particle(const Params& P, uint32_t model, float4 C, float4 F, float lambda, float mu,
-                                     float density, float volume, float param, float pv, float4& A, float4& aux) {
-    float det = det_comp(F);
-    if (!finitef(det) || det <= 0.0f || !finitef(volume) || volume <= 0.0f) return ERR_DEFORMATION;
-    float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
-    const bool liquid = model == MODEL_WATER || model == MODEL_OIL;
-    if (liquid) {
-        if (!finitef(density) || density <= 0.0f || !finitef(lambda) || lambda <= 0.0f || !finitef(param) ||
-            param < 0.0f)
+__device__ uint32_t particle_stress(const Material& mt, float4 F, float pv, float det, float4& S) {
+    S = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!finitef(det) || det <= 0.0f || !finitef(mt.volume) || mt.volume <= 0.0f) return ERR_DEFORMATION;
+    const uint32_t model = mt.model;
+    if (model == MODEL_WATER || model == MODEL_OIL) {
+        if (!finitef(mt.density) || mt.density <= 0.0f || !finitef(mt.lambda) || mt.lambda <= 0.0f ||
+            !finitef(mt.param) || mt.param < 0.0f)
             return ERR_P2G | ERR_INVALID_MATERIAL;
-        if (param > 0.0f) {  // Tait: -J * B * (J^-n - 1)
-            if constexpr (!GENERAL) return ERR_P2G | ERR_INVALID_MATERIAL;
-            float inv = 1.0f / det;
-            float pw = (float)pow((double)inv, (double)param);
-            float pressure = lambda * (pw - 1.0f);
-            float stress = -det * pressure;
-            S.x = stress; S.w = stress;
-        } else {             // tension-free linear: lambda * min(J - 1, 0)
-            float jm1 = det - 1.0f;
-            float pressure = lambda * (jm1 < 0.0f ? jm1 : 0.0f);
+        if (mt.param > 0.0f) {  // Tait: -J * B * (J^-n - 1)
+            if constexpr (!GENERAL) {
+                return ERR_P2G | ERR_INVALID_MATERIAL;
+            } else {
+                const float inv = 1.0f / det;
+                const float pw = (float)pow((double)inv, (double)mt.param);
+                const float pressure = mt.lambda * (pw - 1.0f);
+                const float stress = -det * pressure;
+                S.x = stress; S.w = stress;
+            }
+        } else {  // tension-free linear: lambda * min(J - 1, 0)
+            const float jm1 = det - 1.0f;
+            const float pressure = mt.lambda * (jm1 < 0.0f ? jm1 : 0.0f);
             S.x = pressure; S.w = pressure;
         }
     } else {
@@ -218,41 +223,40 @@ __device__ uint32_t prepare_particle(const Params& P, uint32_t model, float4 C, 
             return ERR_P2G | ERR_INVALID_MATERIAL;
         } else {
             if (model == MODEL_NEO_HOOKEAN) {
-                if (!finitef(param) || param < 0.0f || param > 1.0f) return ERR_P2G | ERR_INVALID_MATERIAL;
-                if (!solid_stress(model, F, lambda, mu, param, pv, S)) return ERR_P2G | ERR_NONFINITE_PARTICLE;
+                if (!finitef(mt.param) || mt.param < 0.0f || mt.param > 1.0f) return ERR_P2G | ERR_INVALID_MATERIAL;
+                if (!solid_stress(model, F, mt.lambda, mt.mu, mt.param, pv, S)) return ERR_P2G | ERR_NONFINITE_PARTICLE;
             } else if (model == MODEL_SNOW) {
                 if (!finitef(pv) || pv < 0.6f || pv > 20.0f) return ERR_P2G | ERR_INVALID_MATERIAL;
-                if (!solid_stress(model, F, lambda, mu, param, pv, S)) return ERR_P2G | ERR_INVALID_MATERIAL;
+                if (!solid_stress(model, F, mt.lambda, mt.mu, mt.param, pv, S)) return ERR_P2G | ERR_INVALID_MATERIAL;
             } else if (model == MODEL_POWDER) {
-                if (!finitef(lambda) || !finitef(mu) || mu <= 0.0f || !finitef(param) || param < 0.0f)
+                if (!finitef(mt.lambda) || !finitef(mt.mu) || mt.mu <= 0.0f || !finitef(mt.param) || mt.param < 0.0f)
                     return ERR_P2G | ERR_INVALID_MATERIAL;
-                if (!solid_stress(model, F, lambda, mu, param, pv, S)) return ERR_P2G | ERR_NONFINITE_PARTICLE;
+                if (!solid_stress(model, F, mt.lambda, mt.mu, mt.param, pv, S)) return ERR_P2G | ERR_NONFINITE_PARTICLE;
             } else {
                 return ERR_P2G | ERR_INVALID_MATERIAL;
             }
         }
     }
-    const float mass = liquid ? density * volume : P.particle_mass;
-    const float ss = (-P.dt * volume) * P.grad_scale;
-    A.x = mass * C.x + ss * S.x;
-    A.y = mass * C.y + ss * S.y;
-    A.z = mass * C.z + ss * S.z;
-    A.w = mass * C.w + ss * S.w;
-    if (!finite4(S) || !finite4(A)) return ERR_P2G | ERR_NONFINITE_PARTICLE;
-    const uint32_t phase = model == MODEL_WATER ? 1u : (model == MODEL_OIL ? 2u : 0u);
-    aux = make_float4(mass, volume, det, __uint_as_float(phase));
+    if (!finite4(S)) return ERR_P2G | ERR_NONFINITE_PARTICLE;
     return 0;
+}
+
+// A = m*C + (-dt*V*4/dx^2) * P*F^T (shaders.ts:954-958).
+__device__ __forceinline__ float4 affine_of(const Params& P, const Material& mt, float4 C, float4 S) {
+    const float ss = (-P.dt * mt.volume) * P.grad_scale;
+    return make_float4(mt.mass * C.x + ss * S.x, mt.mass * C.y + ss * S.y, mt.mass * C.z + ss * S.z,
+                       mt.mass * C.w + ss * S.w);
 }
 
 // Stencil base and fraction (shaders.ts:667-676, :1023-1028).
 struct Stencil { int bx, by; float fx, fy; };
-__device__ __forceinline__ bool make_stencil(const Params& P, float2 p, Stencil& s) {
-    float gx = p.x / P.dx, gy = p.y / P.dx;
+__device__ __forceinline__ bool make_stencil(float dx, uint32_t cells_x, uint32_t cells_y, float2 p, Stencil& s) {
+    const float gx = p.x / dx, gy = p.y / dx;
     if (!finitef(gx) || !finitef(gy)) return false;
     s.bx = (int)floorf(gx - 0.5f);
     s.by = (int)floorf(gy - 0.5f);
     // stencilFitsHalo (shaders.ts:511-516)
-    if (s.bx < -1 || s.by < -1 || s.bx + 2 > (int)P.cells_x + 1 || s.by + 2 > (int)P.cells_y + 1) return false;
+    if (s.bx < -1 || s.by < -1 || s.bx + 2 > (int)cells_x + 1 || s.by + 2 > (int)cells_y + 1) return false;
     s.fx = gx - (float)s.bx;
     s.fy = gy - (float)s.by;
     return true;
@@ -264,168 +268,137 @@ __device__ __forceinline__ float qweight(float f, int o) {
     return o == 0 ? w0 : (o == 1 ? w1 : w2);
 }
 
+// Raise status bits from microstep `step` of the batch; the batch commits
+// the microsteps before the first one that raised (commitParticles,
+// shaders.ts:3067-3080).
+__device__ __forceinline__ void raise_step(Control* c, uint32_t bits, uint32_t step) {
+    atomicMin(&c->fail_step, step);
+    atomicOr(&c->status[0], bits);
+}
+__device__ __forceinline__ void raise_step_at(Control* c, uint32_t bits, uint32_t step, uint32_t index) {
+    atomicMin(&c->status[1], index);
+    raise_step(c, bits, step);
+}
+
+// Slot in bucket `b` for each active lane.  Lanes are grouped by bucket
+// with ballots first; then every group's first lane reserves the group's
+// slots with one atomic, all groups in the same instruction, so the wave
+// waits for a single round trip.  Every lane of the wave must call it
+// (converged control flow).
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* cnt, uint32_t b, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    unsigned long long group = 0ull;
+    int leader = lane;
+    while (pending) {
+        const int first = __ffsll((long long)pending) - 1;
+        const uint32_t fb = __shfl(b, first, 64);
+        const unsigned long long same = __ballot(active && b == fb);
+        if (active && b == fb) {
+            group = same;
+            leader = first;
+        }
+        pending &= ~same;
+    }
+    uint32_t base = 0u;
+    if (active && lane == leader) base = atomicAdd(&cnt[b], (uint32_t)__popcll(group));
+    base = __shfl(base, leader, 64);
+    return active ? base + (uint32_t)__popcll(group & ((1ull << lane) - 1ull)) : 0xffffffffu;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(const Stencil& s, uint32_t bw) {
+    return (uint32_t)((s.by + 1) >> 2) * bw + (uint32_t)((s.bx + 1) >> 2);
+}
+
 // ------------------------------------------------------------ batch frame
-__global__ void k_batch_begin(Control* ctl, uint32_t* count, uint32_t nodes) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t k = i; k < nodes; k += gridDim.x * blockDim.x) count[k] = 0u;
-    if (i == 0) {
+__global__ void k_batch_begin(Control* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->completed = 0u;
         ctl->max_speed_bits = 0u;
-        ctl->deferred = 0u;
-        ctl->deferred_index = 0xffffffffu;
+        // a flag already raised stops the batch before its first microstep
+        ctl->fail_step = ctl->status[0] != 0u ? 0u : 0xffffffffu;
     }
 }
 
-// insertParticleBucket binning (shaders.ts:657-693) as a counting sort:
-// each live particle counts itself into the bin of its stencil base node and
-// fills its prepared affine (prepareAndBuildBuckets, shaders.ts:965-969).
+__global__ void k_clear_counts(uint32_t* cnt, uint32_t nb) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) cnt[k] = 0u;
+}
+
+// Bucket particles [first, first+count) of the caller-order arrays
+// (insertParticleBucket, shaders.ts:657-693, as slot allocation) and fill
+// the stress cache of scenes that carry it.
 template <bool GENERAL>
-__global__ __launch_bounds__(256) void k_bin(const Params* __restrict__ Pp, Control* ctl, ParticleBuf in,
-                                             ParticleStatic st, Bins bins, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (load_status(ctl)) return;
-    const Params& P = *Pp;
-    const uint32_t model = st.model[i];
-    if (model == MODEL_REMOVED) { bins.key[i] = INVALID_KEY; return; }
-    float4 A, aux;
-    const uint32_t pf = prepare_particle<GENERAL>(P, model, in.C[i], in.F[i], st.lambda[i], st.mu[i], st.density[i],
-                                                  st.volume[i], st.param[i], in.plastic[i], A, aux);
-    if (pf) {
-        bins.key[i] = INVALID_KEY;
-        if (pf & ERR_P2G) raise_at(ctl, pf, i); else raise_flags(ctl, pf);
+__global__ __launch_bounds__(256) void k_bucket(Control* ctl, Home home, const Material* __restrict__ mats,
+                                                SlotBuf out, uint32_t first, uint32_t count, float dx,
+                                                uint32_t cells_x, uint32_t cells_y, uint32_t bw, uint32_t cap) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = first + t;
+    bool active = t < count;
+    const uint32_t model = active ? home.model[i] : MODEL_REMOVED;
+    active = active && model != MODEL_REMOVED;
+    Stencil s = {0, 0, 0.0f, 0.0f};
+    float2 p = make_float2(0.f, 0.f);
+    if (active) {
+        p = home.pos[i];
+        if (!make_stencil(dx, cells_x, cells_y, p, s)) {
+            raise_step(ctl, ERR_STENCIL, 0u);
+            active = false;
+        }
+    }
+    const uint32_t b = active ? bucket_of(s, bw) : 0u;
+    const uint32_t k = wave_alloc(out.cnt, b, active);
+    if (!active) return;
+    if (k >= cap) {
+        raise_step(ctl, ERR_INTERNAL_REBUCKET, 0u);
         return;
     }
-    bins.affine[i] = A;
-    bins.aux[i] = aux;
-    Stencil s;
-    if (!make_stencil(P, in.pos[i], s)) { bins.key[i] = INVALID_KEY; raise_flags(ctl, ERR_STENCIL); return; }
-    uint32_t k = (uint32_t)(s.by + 1) * P.node_w + (uint32_t)(s.bx + 1);
-    bins.key[i] = k;
-    bins.rank[i] = atomicAdd(&bins.count[k], 1u);
-}
-
-// ------------------------------------------------------------------- scan
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-    }
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Per-tile totals of the bin counts; block 0 also commits the preceding
-// microstep (commitParticles' progress counter, shaders.ts:3067-3080).
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Control* ctl, const uint32_t* __restrict__ count,
-                                                               uint32_t* __restrict__ block_sums, uint32_t nodes,
-                                                               uint32_t commit_prev) {
-    __shared__ uint32_t s_w[SCAN_THREADS / 64];
-    if (block_status_set(ctl)) return;
-    if (commit_prev && blockIdx.x == 0 && threadIdx.x == 0) {
-        // the preceding microstep committed; failures its G2P found while
-        // preparing this microstep's particles belong to this microstep
-        ctl->completed += 1u;
-        const uint32_t d = ctl->deferred;
-        if (d) {
-            if (d & ERR_P2G) atomicMin(&ctl->status[1], ctl->deferred_index);
-            atomicOr(&ctl->status[0], d);
+    const size_t g = (size_t)b * cap + k;
+    const float4 F = home.F[i];
+    const uint32_t m = home.mat[i];
+    out.pos[g] = p;
+    out.vel[g] = home.vel[i];
+    out.C[g] = home.C[i];
+    out.F[g] = F;
+    out.pid[g] = i;
+    out.mat[g] = m;
+    if constexpr (GENERAL) {
+        const float pv = home.plastic[i];
+        out.plastic[g] = pv;
+        float4 S;
+        const uint32_t pf = particle_stress<true>(mats[m], F, pv, det_comp(F), S);
+        out.S[g] = S;
+        if (pf) {
+            if (pf & ERR_P2G) raise_step_at(ctl, pf, 0u, i); else raise_step(ctl, pf, 0u);
         }
     }
-    uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k)
-        if (base + k < nodes) v += count[base + k];
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        uint32_t t = threadIdx.x < SCAN_THREADS / 64 ? s_w[threadIdx.x] : 0u;
-        t = wave_sum(t);
-        if (threadIdx.x == 0) block_sums[blockIdx.x] = t;
+}
+
+// Buffer -> caller-order arrays (readParticles, solver.ts:2594-2731).
+__global__ __launch_bounds__(256) void k_unbucket(SlotBuf in, Home home, uint32_t nb, uint32_t cap,
+                                                  uint32_t general) {
+    const size_t total = (size_t)nb * cap;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)(g / cap), k = (uint32_t)(g % cap);
+        if (k >= in.cnt[b]) continue;
+        const uint32_t i = in.pid[g];
+        home.pos[i] = in.pos[g];
+        home.vel[i] = in.vel[g];
+        home.C[i] = in.C[g];
+        home.F[i] = in.F[g];
+        if (general) home.plastic[i] = in.plastic[g];
     }
 }
 
-// Exclusive scan of the tile totals (one block).
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(Control* ctl, uint32_t* block_sums, uint32_t nb) {
-    __shared__ uint32_t s_w[SCAN_THREADS / 64];
-    __shared__ uint32_t s_carry;
-    if (block_status_set(ctl)) return;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < nb; base += SCAN_THREADS) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < nb ? block_sums[i] : 0u;
-        uint32_t inc = wave_inclusive_scan(v);
-        if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = inc;
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            uint32_t t = threadIdx.x < SCAN_THREADS / 64 ? s_w[threadIdx.x] : 0u;
-            uint32_t ti = wave_inclusive_scan(t);
-            if (threadIdx.x < SCAN_THREADS / 64) s_w[threadIdx.x] = ti - t;
-        }
-        __syncthreads();
-        uint32_t carry = s_carry;
-        uint32_t excl = carry + s_w[threadIdx.x >> 6] + inc - v;
-        if (i < nb) block_sums[i] = excl;
-        __syncthreads();
-        if (threadIdx.x == SCAN_THREADS - 1) s_carry = excl + v;
-        __syncthreads();
-    }
-}
-
-// Bin starts: exclusive scan inside each tile plus the tile offset; clears
-// the counts for the next binning pass.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(Control* ctl, uint32_t* __restrict__ count,
-                                                              const uint32_t* __restrict__ block_sums,
-                                                              uint32_t* __restrict__ start, uint32_t nodes) {
-    __shared__ uint32_t s_w[SCAN_THREADS / 64];
-    if (block_status_set(ctl)) return;
-    uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    uint32_t c[SCAN_ITEMS];
-    uint32_t local = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        c[k] = base + k < nodes ? count[base + k] : 0u;
-        local += c[k];
-    }
-    uint32_t inc = wave_inclusive_scan(local);
-    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = inc;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        uint32_t t = threadIdx.x < SCAN_THREADS / 64 ? s_w[threadIdx.x] : 0u;
-        uint32_t ti = wave_inclusive_scan(t);
-        if (threadIdx.x < SCAN_THREADS / 64) s_w[threadIdx.x] = ti - t;
-    }
-    __syncthreads();
-    uint32_t run = block_sums[blockIdx.x] + s_w[threadIdx.x >> 6] + inc - local;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        if (base + k < nodes) {
-            start[base + k] = run;
-            count[base + k] = 0u;
-        }
-        run += c[k];
-    }
-    // the thread covering the last node also writes the end sentinel
-    if (nodes > 0 && base <= nodes - 1 && nodes - 1 < base + SCAN_ITEMS) start[nodes] = run;
-}
-
-__global__ __launch_bounds__(256) void k_scatter(Control* ctl, const uint32_t* __restrict__ key,
-                                                 const uint32_t* __restrict__ rank, const uint32_t* __restrict__ start,
-                                                 uint32_t* __restrict__ perm, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_keys(Home home, uint32_t* keys, uint32_t n, float dx, uint32_t cells_x,
+                                              uint32_t cells_y, uint32_t node_w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (load_status(ctl)) return;
-    uint32_t k = key[i];
-    if (k != INVALID_KEY) perm[start[k] + rank[i]] = i;
+    Stencil s;
+    if (home.model[i] == MODEL_REMOVED || !make_stencil(dx, cells_x, cells_y, home.pos[i], s)) {
+        keys[i] = INVALID_KEY;
+        return;
+    }
+    keys[i] = (uint32_t)(s.by + 1) * node_w + (uint32_t)(s.bx + 1);
 }
 
 // ------------------------------------------------------------ P2G + grid
@@ -469,282 +442,542 @@ __device__ __forceinline__ uint32_t fixed_round(float value, int32_t& out) {
     return 0u;
 }
 
-struct Staged {
-    float fx[CAP], fy[CAP], m[CAP], vx[CAP], vy[CAP], a0[CAP], a1[CAP], a2[CAP], a3[CAP], vol[CAP], det[CAP];
-    uint32_t tag[CAP];  // region column | phase << 8 (phase 0 solid, 1 water, 2 oil)
-};
-
-template <bool FIXED>
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g_grid(const Params* __restrict__ Pp, Control* ctl, ParticleBuf in,
-                                                        ParticleStatic st, Bins bins, Grid grid, uint32_t tiles_x,
-                                                        uint32_t keep_fixed, uint32_t scene_flags) {
-    __shared__ uint32_t s_start[RC][RC + 1];
-    __shared__ uint32_t s_rowoff[RC + 1];
-    __shared__ Staged s;
-    __shared__ float s_vw[TS * TS];
-    __shared__ float s_vo[TS * TS];
-    __shared__ uint32_t s_cut;
-
-    if (block_status_set(ctl)) return;
-    const Params& P = *Pp;
-    const int tid = threadIdx.x;
-    const int lx = tid % TS, ly = tid / TS;
-    const int X0 = (int)(blockIdx.x % tiles_x) * TI, Y0 = (int)(blockIdx.x / tiles_x) * TI;
-    const int W = (int)P.node_w, H = (int)P.node_h;
-    const int rx0 = X0 - 3, ry0 = Y0 - 3;
-    const int cx_lo = max(rx0, 0), cx_hi = min(rx0 + RC, W);
-    const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
-    const bool node_ok = sx >= 0 && sx < W && sy >= 0 && sy < H;
-
-    for (int e = tid; e < RC * (RC + 1); e += P2G_THREADS) {
-        int r = e / (RC + 1), c = e % (RC + 1);
-        int cy = ry0 + r;
-        uint32_t v = 0;
-        if (cy >= 0 && cy < H && cx_lo < cx_hi) {
-            int cx = min(max(rx0 + c, cx_lo), cx_hi);
-            v = bins.start[(uint32_t)cy * (uint32_t)W + (uint32_t)cx];
+// updateGrid (shaders.ts:1485-1746) of one interior node from its summed
+// channels f = {solid m, qx, qy | water m, qx, qy, V | oil m, qx, qy, V}.
+// `nvol(k, ox, oy)` returns the summed water (k = 0) or oil (k = 1) volume of
+// the neighbour at offset (ox, oy).  Returns the status bits to raise.
+template <typename NVol>
+__device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& grid, uint32_t scene_flags,
+                                                uint32_t node, int nx, int ny, const float (&f)[11],
+                                                uint32_t nonfinite_bits, NVol nvol) {
+    const float f_sm = f[0], f_sqx = f[1], f_sqy = f[2], f_wm = f[3], f_wqx = f[4], f_wqy = f[5], f_wv = f[6];
+    const float f_om = f[7], f_oqx = f[8], f_oqy = f[9], f_ov = f[10];
+    uint32_t flags = 0;
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) finite = finite && finitef(f[k]);
+    if (!finite) return nonfinite_bits;
+    if (f_sm < 0.0f || f_wm < 0.0f || f_wv < 0.0f || f_om < 0.0f || f_ov < 0.0f) return ERR_GRID;
+    float2 vs = finish_velocity(P, f_sm, f_sqx, f_sqy, nx, ny);
+    float2 vw = finish_velocity(P, f_wm, f_wqx, f_wqy, nx, ny);
+    float2 vo = finish_velocity(P, f_om, f_oqx, f_oqy, nx, ny);
+    if (nx >= 0 && ny >= 0 && nx <= (int)P.cells_x && ny <= (int)P.cells_y && f_wm > 0.0f && f_om > 0.0f &&
+        f_wv > 0.0f && f_ov > 0.0f) {
+        // solveLiquidGridContact with phaseVolumeGradient (shaders.ts:1527-1594, :1779-1799)
+        const float wgx = nvol(0, 1, 0) - nvol(0, -1, 0), wgy = nvol(0, 0, 1) - nvol(0, 0, -1);
+        const float ogx = nvol(1, 1, 0) - nvol(1, -1, 0), ogy = nvol(1, 0, 1) - nvol(1, 0, -1);
+        const float sepx = ogx - wgx, sepy = ogy - wgy;
+        const float len = sqrtf(sepx * sepx + sepy * sepy);
+        if (finitef(len) && len > 1e-8f) {
+            float nrx = sepx / len, nry = sepy / len;
+            const float wd = f_wm / f_wv, od = f_om / f_ov;
+            const float dscale = fmaxf(wd, od);
+            const float contrast = (od - wd) / dscale;
+            const float gmag = sqrtf(P.gx * P.gx + P.gy * P.gy);
+            float align = 0.0f;
+            if (!finitef(wd) || wd <= 0.0f || !finitef(od) || od <= 0.0f || !finitef(contrast) || !finitef(gmag)) {
+                flags |= ERR_GRID;
+            } else {
+                if (gmag > 0.0f) align = (nrx * P.gx + nry * P.gy) / gmag;
+                if (!finitef(align)) flags |= ERR_GRID;
+                if (contrast * align < -LIQUID_STABLE_ORDER_EPSILON) { nrx = -nrx; nry = -nry; }
+                const float rel = (vo.x - vw.x) * nrx + (vo.y - vw.y) * nry;
+                const float overlap = fminf(fmaxf(2.0f * fminf(f_wv, f_ov) / (f_wv + f_ov), 0.0f), 1.0f);
+                const float minsep =
+                    fmaxf(LIQUID_CONTACT_SEPARATION_FRACTION * overlap * P.dx / LIQUID_CONTACT_RELAXATION_SECONDS,
+                          fmaxf(-rel, 0.0f) * LIQUID_CONTACT_APPROACH_SPEED_MULTIPLIER);
+                if (rel < minsep) {
+                    const float imp = (minsep - rel) / (1.0f / f_wm + 1.0f / f_om);
+                    const float ix = imp * nrx, iy = imp * nry;
+                    vw.x = vw.x - ix / f_wm; vw.y = vw.y - iy / f_wm;
+                    vo.x = vo.x + ix / f_om; vo.y = vo.y + iy / f_om;
+                }
+            }
         }
-        s_start[r][c] = v;
     }
+    if (!finite2(vs) || !finite2(vw) || !finite2(vo)) return flags | ERR_GRID | ERR_TRIANGLE_CONTACT;
+    const int cxn = (int)P.cells_x, cyn = (int)P.cells_y;
+    vs.x = wall_cushion(vs.x, nx, cxn); vs.y = wall_cushion(vs.y, ny, cyn);
+    vw.x = wall_cushion(vw.x, nx, cxn); vw.y = wall_cushion(vw.y, ny, cyn);
+    vo.x = wall_cushion(vo.x, nx, cxn); vo.y = wall_cushion(vo.y, ny, cyn);
+    const float fr = P.wall_friction;
+    vs.x = wall_friction(vs.x, ny, cyn, fr); vs.y = wall_friction(vs.y, nx, cxn, fr);
+    vw.x = wall_friction(vw.x, ny, cyn, fr); vw.y = wall_friction(vw.y, nx, cxn, fr);
+    vo.x = wall_friction(vo.x, ny, cyn, fr); vo.y = wall_friction(vo.y, nx, cxn, fr);
+    if (scene_flags & SCENE_SOLID) grid.v_solid[node] = vs;
+    if (scene_flags & SCENE_WATER) grid.v_water[node] = vw;
+    if (scene_flags & SCENE_OIL) grid.v_oil[node] = vo;
+    if (scene_flags & (SCENE_WATER | SCENE_OIL)) grid.liquid[node] = make_float2(f_wm + f_om, f_wv + f_ov);
+    return flags;
+}
+
+// Shared head of both P2G kernels: status gate, deferred stress-cache flags,
+// clearing the next bucket counts this tile owns, and the staged offsets of
+// the 9x9 region buckets (s_boff, one wave, two buckets a lane).  Returns
+// false when a flag is already raised (the whole block returns).
+__device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32_t* cnt_next, uint32_t tiles_x,
+                                         uint32_t bw, uint32_t bh, uint32_t cap, uint32_t step, int in_buf,
+                                         uint32_t* s_boff) {
+    const int tid = threadIdx.x;
+    const int out_buf = in_buf ^ 1;
+    // After a failed microstep the output buffer of this one holds the
+    // committed state: leave its counts and deferred flags untouched.
+    if (block_status_set(ctl)) return false;
+    if (blockIdx.x == 0 && tid == 0) {
+        // stress-cache failures of the state this microstep starts from
+        const uint32_t d = ctl->deferred[in_buf];
+        if (d) {
+            if (d & ERR_P2G) atomicMin(&ctl->status[1], ctl->deferred_index[in_buf]);
+            raise_step(ctl, d, step);
+        }
+        ctl->deferred[out_buf] = 0u;
+        ctl->deferred_index[out_buf] = 0xffffffffu;
+    }
+    const int tx = (int)(blockIdx.x % tiles_x), ty = (int)(blockIdx.x / tiles_x);
+    // this tile owns buckets [7tx, 7tx+7) x [7ty, 7ty+7): clear their next counts
+    if (tid < 49) {
+        const int obx = 7 * tx + tid % 7, oby = 7 * ty + tid / 7;
+        if (obx < (int)bw && oby < (int)bh) cnt_next[(uint32_t)oby * bw + (uint32_t)obx] = 0u;
+    }
+    const int rbx0 = tx * P_TI / BUCKET_CELLS - 1, rby0 = ty * P_TI / BUCKET_CELLS - 1;
+    if (tid < 64) {
+        uint32_t c[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int b = tid + 64 * k;
+            const int gbx = rbx0 + b % P_RB, gby = rby0 + b / P_RB;
+            c[k] = 0u;
+            if (b < P_NB && gbx >= 0 && gby >= 0 && gbx < (int)bw && gby < (int)bh)
+                c[k] = min(in.cnt[(uint32_t)gby * bw + (uint32_t)gbx], cap);
+        }
+        uint32_t i0 = c[0], i1 = c[1];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t0 = __shfl_up(i0, off, 64), t1 = __shfl_up(i1, off, 64);
+            if (tid >= off) { i0 += t0; i1 += t1; }
+        }
+        const uint32_t tot0 = __shfl(i0, 63, 64), tot1 = __shfl(i1, 63, 64);
+        s_boff[tid] = i0 - c[0];
+        if (tid + 64 < P_NB) s_boff[tid + 64] = tot0 + i1 - c[1];
+        if (tid == 0) s_boff[P_NB] = tot0 + tot1;
+    }
+    return true;
+}
+
+// Region bucket holding staged particle j of [s_boff[b0], s_boff[b1]).
+__device__ __forceinline__ int region_bucket(const uint32_t* s_boff, int b0, int b1, uint32_t j) {
+    int lo = b0, hi = b1 - 1;
+    while (lo < hi) {  // last region bucket with s_boff <= j
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_boff[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Exclusive scan of s[0, n) in place (n <= 2 * P_THREADS); s[n] receives the
+// total.  Every thread of the block must call it; it ends with a barrier.
+__device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = 2 * tid;
+    const uint32_t a = i0 < n ? s[i0] : 0u, b = i0 + 1 < n ? s[i0 + 1] : 0u;
+    const uint32_t pair = a + b;
+    uint32_t incl = pair;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        constexpr int WAVES = P_THREADS / 64;
+        const uint32_t w = lane < WAVES ? s_wave[lane] : 0u;
+        uint32_t wi = w;
+#pragma unroll
+        for (int off = 1; off < WAVES; off <<= 1) {
+            const uint32_t t = __shfl_up(wi, off, 64);
+            if (lane >= off) wi += t;
+        }
+        if (lane < WAVES) s_wave[lane] = wi - w;
+        if (lane == WAVES - 1) s_wave[WAVES] = wi;
+    }
+    __syncthreads();
+    const uint32_t base = s_wave[wave] + incl - pair;
+    if (i0 < n) s[i0] = base;
+    if (i0 + 1 < n) s[i0 + 1] = base + a;
+    if (tid == 0) s[n] = s_wave[P_THREADS / 64];
+    __syncthreads();
+}
+
+// Float P2G: LDS-staged gather (p2g, shaders.ts:1072-1211) fused with
+// updateGrid (:1485-1799).  One 1024-thread block per 28x28-node tile: it
+// gathers the tile and a one-node ring of complete sums, then updates the
+// interior (the ring supplies the volume gradient).
+//
+// The tile's particles are those of the 9x9 buckets around it (a 36x36-cell
+// region).  They are staged in LDS in region-row-major cell order, ascending
+// particle index inside a cell, so each node sums its three stencil rows as
+// three contiguous ranges in a fixed order (rows, then cells, then particle
+// index): the float sums are reproducible run to run.  A region holding more
+// than P_CAP particles is processed in chunks of whole bucket rows (single
+// buckets only when one bucket row exceeds P_CAP); the sums carry across
+// chunks.
+template <bool GENERAL>
+__global__ __launch_bounds__(P_THREADS) void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
+                                                        uint32_t* __restrict__ cnt_next,
+                                                        const Material* __restrict__ mats, Grid grid,
+                                                        uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
+                                                        uint32_t scene_flags, uint32_t step, int in_buf) {
+    __shared__ uint32_t s_boff[P_NB + 1];    // staged offset of each region bucket
+    __shared__ uint32_t s_chunk[P_NB + 1];   // chunk boundaries (region bucket indices)
+    __shared__ uint32_t s_nchunks;
+    __shared__ uint32_t s_wave[P_THREADS / 64 + 1];
+    __shared__ uint32_t s_cell[P_CELLS + 1]; // per region cell: count, then start
+    __shared__ uint32_t s_pid[P_CAP];        // particle indices at provisional places
+    __shared__ float s_fx[P_CAP], s_fy[P_CAP], s_m[P_CAP], s_vx[P_CAP], s_vy[P_CAP];
+    __shared__ float s_a0[P_CAP], s_a1[P_CAP], s_a2[P_CAP], s_a3[P_CAP], s_vol[P_CAP], s_det[P_CAP];
+    __shared__ uint32_t s_tag[P_CAP];        // region column | phase << 8; 0xffff = contributes nothing
+
+    if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    const int tid = threadIdx.x;
+    const Params& P = *Pp;
+    const int tx = (int)(blockIdx.x % tiles_x), ty = (int)(blockIdx.x / tiles_x);
+    const int lx = tid % 32, ly = tid / 32;
+    const int X0 = tx * P_TI, Y0 = ty * P_TI;
+    const int W = (int)P.node_w, H = (int)P.node_h;
+    const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
+    const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
+    const bool node_ok = lx < P_TC && ly < P_TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
     __syncthreads();
     if (tid == 0) {
-        uint32_t acc = 0;
-        for (int r = 0; r < RC; ++r) {
-            s_rowoff[r] = acc;
-            acc += s_start[r][RC] - s_start[r][0];
+        uint32_t n = 0;
+        s_chunk[0] = 0u;
+        if (s_boff[P_NB] > (uint32_t)P_CAP) {
+            for (int r = 0; r < P_RB; ++r) {
+                const int lo = r * P_RB, hi = lo + P_RB;
+                if (s_boff[hi] - s_boff[s_chunk[n]] <= (uint32_t)P_CAP) continue;
+                if ((int)s_chunk[n] < lo) s_chunk[++n] = (uint32_t)lo;
+                if (s_boff[hi] - s_boff[lo] > (uint32_t)P_CAP) {
+                    for (int b = lo; b < hi; ++b)
+                        if (s_boff[b + 1] - s_boff[s_chunk[n]] > (uint32_t)P_CAP && b > (int)s_chunk[n])
+                            s_chunk[++n] = (uint32_t)b;
+                }
+            }
         }
-        s_rowoff[RC] = acc;
+        s_chunk[++n] = P_NB;
+        s_nchunks = n;
     }
     __syncthreads();
-    const uint32_t total = s_rowoff[RC];
 
-    // accumulators: solid (m, qx, qy), water (m, qx, qy, V), oil (m, qx, qy, V)
-    float f_sm = 0, f_sqx = 0, f_sqy = 0, f_wm = 0, f_wqx = 0, f_wqy = 0, f_wv = 0, f_om = 0, f_oqx = 0, f_oqy = 0,
-          f_ov = 0;
-    long long i_acc[11];
+    float acc[11];
 #pragma unroll
-    for (int k = 0; k < 11; ++k) i_acc[k] = 0;
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0f;
     uint32_t flags = 0;
-    const float mref = P.particle_mass, vref = P.particle_volume, dx = P.dx;
+    const float dx = P.dx;
+    const uint32_t nchunks = s_nchunks;
+    const int cx0 = rbx0 * BUCKET_CELLS, cy0 = rby0 * BUCKET_CELLS;  // first region cell (stencil-base coords)
 
-    for (uint32_t c0 = 0; c0 < total;) {
-        uint32_t c1 = total;
-        bool split = false;
-        if (total - c0 > (uint32_t)CAP) {
-            if (tid == 0) s_cut = c0;
-            __syncthreads();
-            for (int e = tid; e < RC * RC; e += P2G_THREADS) {
-                int r = e / RC, c = e % RC;
-                uint32_t bnd = s_rowoff[r] + (s_start[r][c + 1] - s_start[r][0]);
-                if (bnd > c0 && bnd <= c0 + (uint32_t)CAP) atomicMax(&s_cut, bnd);
+    for (uint32_t ch = 0; ch < nchunks; ++ch) {
+        const int b0 = (int)s_chunk[ch], b1 = (int)s_chunk[ch + 1];
+        const uint32_t c0 = s_boff[b0], c1 = s_boff[b1];
+        for (int e = tid; e <= P_CELLS; e += P_THREADS) s_cell[e] = 0u;
+        __syncthreads();
+        // (A) load up to two particles a thread, count them per region cell
+        uint32_t rc[2], tmp[2], pid[2], mat[2];
+        float fx[2], fy[2];
+        float2 v[2];
+        float4 C[2], F[2], S[2];
+        float pv[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t j = c0 + (uint32_t)tid + (uint32_t)(k * P_THREADS);
+            rc[k] = 0xffffffffu;
+            tmp[k] = 0u; pid[k] = 0u; mat[k] = 0u;
+            fx[k] = 0.f; fy[k] = 0.f;
+            v[k] = make_float2(0.f, 0.f);
+            C[k] = make_float4(0.f, 0.f, 0.f, 0.f); F[k] = C[k]; S[k] = C[k];
+            pv[k] = 1.0f;
+            if (j >= c1) continue;
+            const int rb = region_bucket(s_boff, b0, b1, j);
+            const int gbx = rbx0 + rb % P_RB, gby = rby0 + rb / P_RB;
+            const size_t g = ((size_t)gby * bw + (size_t)gbx) * cap + (j - s_boff[rb]);
+            const float2 p = in.pos[g];
+            pid[k] = in.pid[g];
+            mat[k] = in.mat[g];
+            v[k] = in.vel[g];
+            C[k] = in.C[g];
+            F[k] = in.F[g];
+            if constexpr (GENERAL) {
+                S[k] = in.S[g];
+                pv[k] = in.plastic[g];
             }
-            __syncthreads();
-            c1 = s_cut;
-            if (c1 == c0) { c1 = c0 + (uint32_t)CAP; split = true; }
-            __syncthreads();
-        }
-        // stage [c0, c1): each live particle lands at its bin's LDS range in
-        // ascending particle-index order, which fixes the summation order.
-        for (uint32_t j = c0 + tid; j < c1; j += P2G_THREADS) {
-            int lo = 0, hi = RC - 1;
-            while (lo < hi) {  // last row r with s_rowoff[r] <= j
-                int mid = (lo + hi + 1) >> 1;
-                if (s_rowoff[mid] <= j) lo = mid; else hi = mid - 1;
-            }
-            const int r = lo;
-            const uint32_t slot = s_start[r][0] + (j - s_rowoff[r]);
-            const uint32_t idx = bins.perm[slot];
-            const float2 p = in.pos[idx];
-            Stencil sc = {0, 0, 0.0f, 0.0f};
-            uint32_t li = j - c0;
-            uint32_t tag = 0xffffu;  // contributes nothing unless staged below
-            if (!make_stencil(P, p, sc) || sc.by + 1 - ry0 != r || sc.bx + 1 - rx0 < 0 || sc.bx + 1 - rx0 >= RC) {
-                flags |= ERR_STENCIL;
+            Stencil sc;
+            if (make_stencil(P.dx, P.cells_x, P.cells_y, p, sc) && ((sc.bx + 1) >> 2) == gbx &&
+                ((sc.by + 1) >> 2) == gby) {
+                fx[k] = sc.fx;
+                fy[k] = sc.fy;
+                rc[k] = (uint32_t)((sc.by + 1 - cy0) * P_RC + (sc.bx + 1 - cx0));
+                tmp[k] = atomicAdd(&s_cell[rc[k]], 1u);
             } else {
-                int c = sc.bx + 1 - rx0;
-                if (!split) {
-                    const uint32_t key = (uint32_t)(sc.by + 1) * (uint32_t)W + (uint32_t)(sc.bx + 1);
-                    const uint32_t cs = bins.start[key], ce = bins.start[key + 1];
-                    uint32_t lr = 0;
-                    for (uint32_t q = cs; q < ce; ++q) lr += bins.perm[q] < idx ? 1u : 0u;
-                    li = s_rowoff[r] + (cs - s_start[r][0]) + lr - c0;
-                    if (li >= c1 - c0) {  // bin and position disagree: never expected
-                        flags |= ERR_STENCIL;
-                        li = j - c0;
-                        c = 0xff;
-                    }
-                }
-                const float2 v = in.vel[idx];
-                const float4 A = bins.affine[idx];
-                const float4 aux = bins.aux[idx];
-                tag = c == 0xff ? 0xffffu : ((uint32_t)c | (__float_as_uint(aux.w) << 8));
-                s.vx[li] = v.x;
-                s.vy[li] = v.y;
-                s.m[li] = aux.x;
-                s.vol[li] = aux.y;
-                s.det[li] = aux.z;
-                s.a0[li] = A.x; s.a1[li] = A.y; s.a2[li] = A.z; s.a3[li] = A.w;
+                flags |= ERR_STENCIL;
             }
-            s.fx[li] = sc.fx;
-            s.fy[li] = sc.fy;
-            s.tag[li] = tag;
         }
         __syncthreads();
-
+        // (B) cell starts
+        block_exclusive_scan(s_cell, P_CELLS, s_wave);
+        // (C) ascending particle index inside each cell fixes the summation order
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (rc[k] != 0xffffffffu) s_pid[s_cell[rc[k]] + tmp[k]] = pid[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (rc[k] == 0xffffffffu) continue;
+            const uint32_t q0 = s_cell[rc[k]], q1 = s_cell[rc[k] + 1];
+            uint32_t li = q0;
+            for (uint32_t q = q0; q < q1; ++q) li += s_pid[q] < pid[k] ? 1u : 0u;
+            const Material mt = mats[mat[k]];
+            const float det = det_comp(F[k]);
+            uint32_t pf = 0;
+            float4 Sk = S[k];
+            if constexpr (GENERAL) {
+                if (!finitef(det) || det <= 0.0f || !finitef(mt.volume) || mt.volume <= 0.0f) pf = ERR_DEFORMATION;
+            } else {
+                pf = particle_stress<false>(mt, F[k], pv[k], det, Sk);
+            }
+            const float4 A = affine_of(P, mt, C[k], Sk);
+            if (!pf && !finite4(A)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
+            if (pf) {
+                if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
+                flags |= pf;
+            }
+            s_fx[li] = fx[k];
+            s_fy[li] = fy[k];
+            s_m[li] = mt.mass;
+            s_vx[li] = v[k].x;
+            s_vy[li] = v[k].y;
+            s_a0[li] = A.x; s_a1[li] = A.y; s_a2[li] = A.z; s_a3[li] = A.w;
+            s_vol[li] = mt.volume;
+            s_det[li] = det;
+            s_tag[li] = pf ? 0xffffu : ((rc[k] % (uint32_t)P_RC) | (mt.phase << 8));
+        }
+        __syncthreads();
+        // (D) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region
         if (node_ok) {
 #pragma unroll 1
             for (int dy = 0; dy < 3; ++dy) {
-                const int r = ly + dy;
-                const uint32_t rb = s_rowoff[r] - s_start[r][0];
-                uint32_t b = rb + s_start[r][lx], e = rb + s_start[r][lx + 3];
-                b = max(b, c0);
-                e = min(e, c1);
+                const int row = (ly + 1 + dy) * P_RC;
+                const uint32_t qa = s_cell[row + lx + 1], qb = s_cell[row + lx + 4];
                 const int oy = 2 - dy;
 #pragma unroll 1
-                for (uint32_t q = b; q < e; ++q) {
-                    const uint32_t li = q - c0;
-                    const uint32_t tag = s.tag[li];
-                    const int c = (int)(tag & 0xffu);
-                    if (c == 0xff) continue;
+                for (uint32_t q = qa; q < qb; ++q) {
+                    const uint32_t tag = s_tag[q];
+                    if (tag == 0xffffu) continue;
                     const uint32_t ph = tag >> 8;
-                    const int ox = lx - c + 2;
-                    const float fx = s.fx[li], fy = s.fy[li];
+                    const int ox = lx + 3 - (int)(tag & 0xffu);
+                    const float fx = s_fx[q], fy = s_fy[q];
                     const float w = qweight(fx, ox) * qweight(fy, oy);
                     const float dpx = ((float)ox - fx) * dx;
                     const float dpy = ((float)oy - fy) * dx;
-                    const float m = s.m[li];
-                    const float qx = (m * s.vx[li] + s.a0[li] * dpx) + s.a1[li] * dpy;
-                    const float qy = (m * s.vy[li] + s.a2[li] * dpx) + s.a3[li] * dpy;
+                    const float m = s_m[q];
+                    const float qx = (m * s_vx[q] + s_a0[q] * dpx) + s_a1[q] * dpy;
+                    const float qy = (m * s_vy[q] + s_a2[q] * dpx) + s_a3[q] * dpy;
                     if (!finitef(w) || w < 0.0f || !finitef(qx) || !finitef(qy)) {
                         flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
                         continue;
                     }
-                    if constexpr (FIXED) {
-                        int32_t im, iqx, iqy, iv = 0;
-                        flags |= fixed_round((w * m) / mref, im);
-                        flags |= fixed_round((w * qx) / mref, iqx);
-                        flags |= fixed_round((w * qy) / mref, iqy);
-                        if (ph != 0u) flags |= fixed_round((w * (s.vol[li] * s.det[li])) / vref, iv);
-                        const int base = ph == 0u ? 0 : (ph == 1u ? 3 : 7);
-#pragma unroll
-                        for (int k = 0; k < 11; ++k) {
-                            const int rel = k - base;
-                            long long add = rel == 0 ? im : rel == 1 ? iqx : rel == 2 ? iqy : rel == 3 ? iv : 0;
-                            if (rel < 0 || rel > 3 || (ph == 0u && rel == 3)) add = 0;
-                            i_acc[k] += add;
-                        }
+                    const float wm = w * m, wqx = w * qx, wqy = w * qy;
+                    if (ph == 0u) {
+                        acc[0] += wm; acc[1] += wqx; acc[2] += wqy;
                     } else {
-                        const float wm = w * m, wqx = w * qx, wqy = w * qy;
-                        if (ph == 0u) {
-                            f_sm += wm; f_sqx += wqx; f_sqy += wqy;
-                        } else {
-                            const float wv = (w * s.vol[li]) * s.det[li];
-                            if (ph == 1u) { f_wm += wm; f_wqx += wqx; f_wqy += wqy; f_wv += wv; }
-                            else { f_om += wm; f_oqx += wqx; f_oqy += wqy; f_ov += wv; }
-                        }
+                        const float wv = (w * s_vol[q]) * s_det[q];
+                        if (ph == 1u) { acc[3] += wm; acc[4] += wqx; acc[5] += wqy; acc[6] += wv; }
+                        else { acc[7] += wm; acc[8] += wqx; acc[9] += wqy; acc[10] += wv; }
                     }
                 }
             }
         }
         __syncthreads();
-        c0 = c1;
     }
 
-    if constexpr (FIXED) {
-        // gatherAtomicGrid decode (shaders.ts:1045-1070) of the wrapped words;
-        // an exact sum outside int32 is the accumulator overflow of 65536.
-        float vals[11];
-#pragma unroll
-        for (int k = 0; k < 11; ++k) {
-            if (i_acc[k] > 2147483647LL || i_acc[k] < -2147483648LL) flags |= ERR_FIXED_POINT;
-            const int32_t word = (int32_t)(uint32_t)(unsigned long long)i_acc[k];
-            vals[k] = (float)word / FIXED_POINT_SCALE;
-        }
-        f_sm = vals[0] * mref; f_sqx = vals[1] * mref; f_sqy = vals[2] * mref;
-        f_wm = vals[3] * mref; f_wqx = vals[4] * mref; f_wqy = vals[5] * mref; f_wv = vals[6] * vref;
-        f_om = vals[7] * mref; f_oqx = vals[8] * mref; f_oqy = vals[9] * mref; f_ov = vals[10] * vref;
+    // ring volumes for the gradient (the staging arrays are free now)
+    float* s_vw = s_fx;
+    float* s_vo = s_fy;
+    const int ring = ly * P_TC + lx;
+    if (lx < P_TC && ly < P_TC) {
+        s_vw[ring] = node_ok ? acc[6] : 0.0f;
+        s_vo[ring] = node_ok ? acc[10] : 0.0f;
     }
-    s_vw[tid] = node_ok ? f_wv : 0.0f;
-    s_vo[tid] = node_ok ? f_ov : 0.0f;
     __syncthreads();
-
-    const bool interior = node_ok && lx >= 1 && lx <= TI && ly >= 1 && ly <= TI;
-    if (interior) {
+    if (node_ok && lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI) {
         const uint32_t node = (uint32_t)sy * (uint32_t)W + (uint32_t)sx;
-        if constexpr (FIXED) {
-            if (keep_fixed) {
+        flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, acc, ERR_P2G | ERR_NONFINITE_ACCUMULATION,
+                             [&](int k, int ox, int oy) {
+                                 const float* a = k == 0 ? s_vw : s_vo;
+                                 return a[ring + oy * P_TC + ox];
+                             });
+    }
+    if (flags) raise_step(ctl, flags, step);
+}
+
+// Fixed-point P2G (prepareAndScatterParticles + gatherAtomicGrid,
+// shaders.ts:971-1070) fused with updateGrid.  Same tiles as the float
+// gather, but every particle of the region adds its rounded int32
+// contributions that land on the tile or its ring into 64-bit LDS words with
+// integer atomics: integer sums do not depend on the order, and an exact sum
+// outside int32 is the accumulator overflow of 65536.
+template <bool GENERAL>
+__global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
+                                                       uint32_t* __restrict__ cnt_next,
+                                                       const Material* __restrict__ mats, Grid grid,
+                                                       uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
+                                                       uint32_t keep_fixed, uint32_t scene_flags, uint32_t step,
+                                                       int in_buf) {
+    // channels: solid m, qx, qy | water m, qx, qy, V | oil m, qx, qy, V
+    __shared__ unsigned long long s_acc[11][P_TC * P_TC];
+    __shared__ uint32_t s_boff[P_NB + 1];
+
+    for (int e = threadIdx.x; e < 11 * P_TC * P_TC; e += P_THREADS) (&s_acc[0][0])[e] = 0ull;
+    if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const Params& P = *Pp;
+    const int tx = (int)(blockIdx.x % tiles_x), ty = (int)(blockIdx.x / tiles_x);
+    const int X0 = tx * P_TI, Y0 = ty * P_TI;
+    const int W = (int)P.node_w, H = (int)P.node_h;
+    const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
+
+    uint32_t flags = 0;
+    const float mref = P.particle_mass, vref = P.particle_volume, dx = P.dx;
+    const uint32_t total = s_boff[P_NB];
+    for (uint32_t base = 0; base < total; base += 2u * P_THREADS) {
+        // load two particles a thread first, so both loads are in flight
+        uint32_t pid[2], mat[2];
+        int gbx[2], gby[2];
+        float2 p[2], v[2];
+        float4 C[2], F[2], S[2];
+        float pv[2];
+        bool have[2];
 #pragma unroll
-                for (int k = 0; k < 11; ++k)
-                    grid.fixed[(size_t)node * 11 + k] = (int32_t)(uint32_t)(unsigned long long)i_acc[k];
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t j = base + (uint32_t)tid + (uint32_t)(k * P_THREADS);
+            have[k] = j < total;
+            pid[k] = 0u; mat[k] = 0u; gbx[k] = 0; gby[k] = 0;
+            p[k] = make_float2(0.f, 0.f); v[k] = p[k];
+            C[k] = make_float4(0.f, 0.f, 0.f, 0.f); F[k] = C[k]; S[k] = C[k];
+            pv[k] = 1.0f;
+            if (!have[k]) continue;
+            const int rb = region_bucket(s_boff, 0, P_NB, j);
+            gbx[k] = rbx0 + rb % P_RB;
+            gby[k] = rby0 + rb / P_RB;
+            const size_t g = ((size_t)gby[k] * bw + (size_t)gbx[k]) * cap + (j - s_boff[rb]);
+            p[k] = in.pos[g];
+            pid[k] = in.pid[g];
+            mat[k] = in.mat[g];
+            v[k] = in.vel[g];
+            C[k] = in.C[g];
+            F[k] = in.F[g];
+            if constexpr (GENERAL) {
+                S[k] = in.S[g];
+                pv[k] = in.plastic[g];
             }
         }
-        if (!finitef(f_sm) || !finitef(f_sqx) || !finitef(f_sqy) || !finitef(f_wm) || !finitef(f_wqx) ||
-            !finitef(f_wqy) || !finitef(f_wv) || !finitef(f_om) || !finitef(f_oqx) || !finitef(f_oqy) ||
-            !finitef(f_ov)) {
-            flags |= FIXED ? ERR_GRID : (ERR_P2G | ERR_NONFINITE_ACCUMULATION);
-        } else if (f_sm < 0.0f || f_wm < 0.0f || f_wv < 0.0f || f_om < 0.0f || f_ov < 0.0f) {
-            flags |= ERR_GRID;
-        } else {
-            // updateGrid (shaders.ts:1485-1746)
-            const int nx = sx - 1, ny = sy - 1;
-            float2 vs = finish_velocity(P, f_sm, f_sqx, f_sqy, nx, ny);
-            float2 vw = finish_velocity(P, f_wm, f_wqx, f_wqy, nx, ny);
-            float2 vo = finish_velocity(P, f_om, f_oqx, f_oqy, nx, ny);
-            if (nx >= 0 && ny >= 0 && nx <= (int)P.cells_x && ny <= (int)P.cells_y && f_wm > 0.0f && f_om > 0.0f &&
-                f_wv > 0.0f && f_ov > 0.0f) {
-                // phaseVolumeGradient (shaders.ts:1779-1799) from the LDS ring
-                const float wgx = s_vw[tid + 1] - s_vw[tid - 1], wgy = s_vw[tid + TS] - s_vw[tid - TS];
-                const float ogx = s_vo[tid + 1] - s_vo[tid - 1], ogy = s_vo[tid + TS] - s_vo[tid - TS];
-                const float sepx = ogx - wgx, sepy = ogy - wgy;
-                const float len = sqrtf(sepx * sepx + sepy * sepy);
-                if (finitef(len) && len > 1e-8f) {
-                    float nrx = sepx / len, nry = sepy / len;
-                    const float wd = f_wm / f_wv, od = f_om / f_ov;
-                    const float dscale = fmaxf(wd, od);
-                    const float contrast = (od - wd) / dscale;
-                    const float gmag = sqrtf(P.gx * P.gx + P.gy * P.gy);
-                    float align = 0.0f;
-                    if (!finitef(wd) || wd <= 0.0f || !finitef(od) || od <= 0.0f || !finitef(contrast) ||
-                        !finitef(gmag)) {
-                        flags |= ERR_GRID;
-                    } else {
-                        if (gmag > 0.0f) align = (nrx * P.gx + nry * P.gy) / gmag;
-                        if (!finitef(align)) flags |= ERR_GRID;
-                        if (contrast * align < -LIQUID_STABLE_ORDER_EPSILON) { nrx = -nrx; nry = -nry; }
-                        const float rel = (vo.x - vw.x) * nrx + (vo.y - vw.y) * nry;
-                        const float overlap =
-                            fminf(fmaxf(2.0f * fminf(f_wv, f_ov) / (f_wv + f_ov), 0.0f), 1.0f);
-                        const float minsep = fmaxf(
-                            LIQUID_CONTACT_SEPARATION_FRACTION * overlap * dx / LIQUID_CONTACT_RELAXATION_SECONDS,
-                            fmaxf(-rel, 0.0f) * LIQUID_CONTACT_APPROACH_SPEED_MULTIPLIER);
-                        if (rel < minsep) {
-                            const float imp = (minsep - rel) / (1.0f / f_wm + 1.0f / f_om);
-                            const float ix = imp * nrx, iy = imp * nry;
-                            vw.x = vw.x - ix / f_wm; vw.y = vw.y - iy / f_wm;
-                            vo.x = vo.x + ix / f_om; vo.y = vo.y + iy / f_om;
-                        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!have[k]) continue;
+            Stencil sc;
+            if (!make_stencil(P.dx, P.cells_x, P.cells_y, p[k], sc) || ((sc.bx + 1) >> 2) != gbx[k] ||
+                ((sc.by + 1) >> 2) != gby[k]) {
+                flags |= ERR_STENCIL;
+                continue;
+            }
+            // tile-local node of stencil offset (0, 0); the tile's nodes are
+            // storage nodes [X0-1, X0+P_TI] x [Y0-1, Y0+P_TI]
+            const int nx0 = sc.bx + 2 - X0, ny0 = sc.by + 2 - Y0;
+            if (nx0 + 2 < 0 || ny0 + 2 < 0 || nx0 >= P_TC || ny0 >= P_TC) continue;
+            const Material mt = mats[mat[k]];
+            const float det = det_comp(F[k]);
+            uint32_t pf = 0;
+            float4 Sk = S[k];
+            if constexpr (GENERAL) {
+                if (!finitef(det) || det <= 0.0f || !finitef(mt.volume) || mt.volume <= 0.0f) pf = ERR_DEFORMATION;
+            } else {
+                pf = particle_stress<false>(mt, F[k], pv[k], det, Sk);
+            }
+            const float4 A = affine_of(P, mt, C[k], Sk);
+            if (!pf && !finite4(A)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
+            if (pf) {
+                if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
+                flags |= pf;
+                continue;
+            }
+            const uint32_t ph = mt.phase;
+            const int c0 = ph == 0u ? 0 : (ph == 1u ? 3 : 7);
+            const float m = mt.mass;
+            const float mvx = m * v[k].x, mvy = m * v[k].y;
+            const float vd = mt.volume * det;
+            float wx[3], wy[3];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                wx[o] = qweight(sc.fx, o);
+                wy[o] = qweight(sc.fy, o);
+            }
+#pragma unroll
+            for (int oy = 0; oy < 3; ++oy) {
+                const int ly = ny0 + oy;
+                if (ly < 0 || ly >= P_TC) continue;
+                const float dpy = ((float)oy - sc.fy) * dx;
+#pragma unroll
+                for (int ox = 0; ox < 3; ++ox) {
+                    const int lx = nx0 + ox;
+                    if (lx < 0 || lx >= P_TC) continue;
+                    const float dpx = ((float)ox - sc.fx) * dx;
+                    const float w = wx[ox] * wy[oy];
+                    const float qx = (mvx + A.x * dpx) + A.y * dpy;
+                    const float qy = (mvy + A.z * dpx) + A.w * dpy;
+                    if (!finitef(w) || w < 0.0f || !finitef(qx) || !finitef(qy)) {
+                        flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
+                        continue;
                     }
+                    const int node = ly * P_TC + lx;
+                    int32_t im, iqx, iqy, iv = 0;
+                    flags |= fixed_round((w * m) / mref, im);
+                    flags |= fixed_round((w * qx) / mref, iqx);
+                    flags |= fixed_round((w * qy) / mref, iqy);
+                    if (ph != 0u) flags |= fixed_round((w * vd) / vref, iv);
+                    atomicAdd(&s_acc[c0][node], (unsigned long long)(long long)im);
+                    atomicAdd(&s_acc[c0 + 1][node], (unsigned long long)(long long)iqx);
+                    atomicAdd(&s_acc[c0 + 2][node], (unsigned long long)(long long)iqy);
+                    if (ph != 0u) atomicAdd(&s_acc[c0 + 3][node], (unsigned long long)(long long)iv);
                 }
             }
-            if (!finite2(vs) || !finite2(vw) || !finite2(vo)) {
-                flags |= ERR_GRID | ERR_TRIANGLE_CONTACT;
-            } else {
-                const int cxn = (int)P.cells_x, cyn = (int)P.cells_y;
-                vs.x = wall_cushion(vs.x, nx, cxn); vs.y = wall_cushion(vs.y, ny, cyn);
-                vw.x = wall_cushion(vw.x, nx, cxn); vw.y = wall_cushion(vw.y, ny, cyn);
-                vo.x = wall_cushion(vo.x, nx, cxn); vo.y = wall_cushion(vo.y, ny, cyn);
-                const float fr = P.wall_friction;
-                vs.x = wall_friction(vs.x, ny, cyn, fr); vs.y = wall_friction(vs.y, nx, cxn, fr);
-                vw.x = wall_friction(vw.x, ny, cyn, fr); vw.y = wall_friction(vw.y, nx, cxn, fr);
-                vo.x = wall_friction(vo.x, ny, cyn, fr); vo.y = wall_friction(vo.y, nx, cxn, fr);
-                if (scene_flags & SCENE_SOLID) grid.v_solid[node] = vs;
-                if (scene_flags & SCENE_WATER) grid.v_water[node] = vw;
-                if (scene_flags & SCENE_OIL) grid.v_oil[node] = vo;
-                if (scene_flags & (SCENE_WATER | SCENE_OIL)) grid.liquid[node] = make_float2(f_wm + f_om, f_wv + f_ov);
-            }
         }
     }
-    if (flags) raise_flags(ctl, flags);
+    __syncthreads();
+
+    // gatherAtomicGrid decode (shaders.ts:1045-1070) of the wrapped words
+    auto decoded = [&](int k, int node) {
+        const int32_t word = (int32_t)(uint32_t)s_acc[k][node];
+        return ((float)word / FIXED_POINT_SCALE) * (k == 6 || k == 10 ? vref : mref);
+    };
+    const int lx = tid % 32, ly = tid / 32;
+    const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
+    if (lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI && sx < W && sy < H) {
+        const int ring = ly * P_TC + lx;
+        const uint32_t node = (uint32_t)sy * (uint32_t)W + (uint32_t)sx;
+        float f[11];
+#pragma unroll
+        for (int k = 0; k < 11; ++k) {
+            const long long sum = (long long)s_acc[k][ring];
+            if (sum > 2147483647LL || sum < -2147483648LL) flags |= ERR_FIXED_POINT;
+            if (keep_fixed) grid.fixed[(size_t)node * 11 + k] = (int32_t)(uint32_t)(unsigned long long)sum;
+            f[k] = decoded(k, ring);
+        }
+        flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, f, ERR_GRID, [&](int k, int ox, int oy) {
+            return decoded(k == 0 ? 6 : 10, ring + oy * P_TC + ox);
+        });
+    }
+    if (flags) raise_step(ctl, flags, step);
 }
 
 // -------------------------------------------------------------------- G2P
@@ -813,57 +1046,40 @@ __device__ __forceinline__ float2 confine_axis(float x, float v, float maximum) 
     return make_float2(x, v);
 }
 
-// g2p (shaders.ts:2114-2421) + validateCflParticle (:2423-2454), then the
-// bin count of the updated particle for the next microstep.
+// g2p (shaders.ts:2114-2421) for one particle with the grid patch in LDS,
+// plus validateCflParticle (:2423-2454).  Returns false when the particle
+// raised a flag (the microstep then rolls back).
+struct G2POut { float2 pos, vel; float4 C, F; float plastic; };
+
 template <bool GENERAL>
-__global__ __launch_bounds__(256) void k_g2p(const Params* __restrict__ Pp, Control* ctl, ParticleBuf in,
-                                             ParticleBuf out, ParticleStatic st, Grid grid, Bins bins, uint32_t n,
-                                             uint32_t bin_next) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (load_status(ctl)) return;
-    const Params& P = *Pp;
-    const uint32_t model = st.model[i];
-    if (model == MODEL_REMOVED) {
-        out.pos[i] = in.pos[i];
-        out.vel[i] = in.vel[i];
-        out.C[i] = in.C[i];
-        out.F[i] = in.F[i];
-        out.plastic[i] = in.plastic[i];
-        if (bin_next) bins.key[i] = INVALID_KEY;
-        return;
-    }
-    const float2 p = in.pos[i];
-    const float4 F = in.F[i];
-    Stencil s;
-    if (!make_stencil(P, p, s)) { raise_flags(ctl, ERR_STENCIL); return; }
+__device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
+                             float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
+                             uint32_t step, G2POut& o) {
+    const uint32_t model = mt.model;
     const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
-    const float2* __restrict__ gv = water ? grid.v_water : (oil ? grid.v_oil : grid.v_solid);
-    const uint32_t W = P.node_w;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
     const float dx = P.dx;
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy) {
         const float wy = qweight(s.fy, oy);
         const float dpy = ((float)oy - s.fy) * dx;
-        const uint32_t row = (uint32_t)(s.by + oy + 1) * W + (uint32_t)(s.bx + 1);
+        const int row = (s.by + 1 + oy - ny0) * G_TN + (s.bx + 1 - nx0);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
-            const uint32_t node = row + (uint32_t)ox;
             const float w = qweight(s.fx, ox) * wy;
             const float dpx = ((float)ox - s.fx) * dx;
-            const float2 g = gv[node];
+            const float2 g = lv[row + ox];
             if (!finitef(w) || w < 0.0f || !finitef(dpx) || !finitef(dpy) || !finite2(g)) {
-                raise_flags(ctl, ERR_G2P);
-                return;
+                raise_step(ctl, ERR_G2P, step);
+                return false;
             }
             vx = vx + w * g.x;
             vy = vy + w * g.y;
             if (liquid) {
-                const float2 lq = grid.liquid[node];
+                const float2 lq = llq[row + ox];
                 if (!finitef(lq.x) || lq.x < 0.0f || !finitef(lq.y) || lq.y < 0.0f) {
-                    raise_at(ctl, ERR_G2P, i);
-                    return;
+                    raise_step_at(ctl, ERR_G2P, step, pid);
+                    return false;
                 }
                 pm = pm + w * lq.x;
                 pvol = pvol + w * lq.y;
@@ -877,189 +1093,326 @@ __global__ __launch_bounds__(256) void k_g2p(const Params* __restrict__ Pp, Cont
     if (liquid) {
         const float pdet = det_comp(F);
         if (!finitef(pm) || pm <= 0.0f || !finitef(pvol) || pvol <= 0.0f || !finitef(pdet) || pdet <= 0.0f) {
-            raise_at(ctl, ERR_G2P, i);
-            return;
+            raise_step_at(ctl, ERR_G2P, step, pid);
+            return false;
         }
-        const float pd = st.density[i] / pdet;
+        const float pd = mt.density / pdet;
         const float ld = pm / pvol;
         const float rel = (pd - ld) / pd;
         vx = vx + P.dt * (rel * P.gx);
         vy = vy + P.dt * (rel * P.gy);
         if (!finitef(pd) || pd <= 0.0f || !finitef(ld) || ld <= 0.0f || !finitef(rel) || !finitef(vx) ||
             !finitef(vy)) {
-            raise_at(ctl, ERR_G2P, i);
-            return;
+            raise_step_at(ctl, ERR_G2P, step, pid);
+            return false;
         }
     }
     const float gs = P.grad_scale, dt = P.dt;
     const float4 C = make_float4(a0 * gs, a1 * gs, a2 * gs, a3 * gs);
     const float2 np = make_float2(p.x + dt * vx, p.y + dt * vy);
-    if (!finitef(vx) || !finitef(vy) || !finite4(C) || !finite2(np)) { raise_flags(ctl, ERR_G2P); return; }
+    if (!finitef(vx) || !finitef(vy) || !finite4(C) || !finite2(np)) {
+        raise_step(ctl, ERR_G2P, step);
+        return false;
+    }
     const float rowsum = fmaxf(fabsf(C.x) + fabsf(C.y), fabsf(C.z) + fabsf(C.w));
     const float increment = dt * rowsum;
     const float required = ceilf(increment / MAX_DEFORMATION_INCREMENT);
     if (!finitef(required) || required > (float)MAX_DEFORMATION_SUBSTEPS) {
-        raise_at(ctl, ERR_SUBSTEP_LIMIT, i);
-        return;
+        raise_step_at(ctl, ERR_SUBSTEP_LIMIT, step, pid);
+        return false;
     }
     uint32_t substeps = required < 1.0f ? 1u : (uint32_t)required;
     float4 trial = integrate_deformation(F, C, dt, substeps);
-    if (!finite4(trial)) { raise_flags(ctl, ERR_G2P); return; }
+    if (!finite4(trial)) {
+        raise_step(ctl, ERR_G2P, step);
+        return false;
+    }
     double tdet = det_f64(trial);
     // A near-singular F can lose orientation to f32 rounding; retry that update
     // with doubled substeps (shaders.ts:2274-2286).  The sign test is FP64.
     while (isfinite(tdet) && tdet <= 0.0 && substeps < MAX_DEFORMATION_SUBSTEPS) {
         substeps = min(substeps * 2u, MAX_DEFORMATION_SUBSTEPS);
         trial = integrate_deformation(F, C, dt, substeps);
-        if (!finite4(trial)) { raise_flags(ctl, ERR_G2P); return; }
+        if (!finite4(trial)) {
+            raise_step(ctl, ERR_G2P, step);
+            return false;
+        }
         tdet = det_f64(trial);
     }
     if (!isfinite(tdet) || tdet <= 0.0) {
-        record_failure(ctl, 1u, i, (float)tdet, increment, (float)det_f64(F), 0.0f);
-        raise_flags(ctl, ERR_DEFORMATION);
-        return;
+        record_failure(ctl, 1u, pid, (float)tdet, increment, (float)det_f64(F), 0.0f);
+        raise_step(ctl, ERR_DEFORMATION, step);
+        return false;
     }
     float4 Fn = trial;
-    float pvn = in.plastic[i];
+    float pvn = pv;
     if (liquid) {
         Fn = project_liquid(trial);
     } else if constexpr (!GENERAL) {
-        raise_flags(ctl, ERR_G2P);
-        return;
+        raise_step(ctl, ERR_G2P, step);
+        return false;
     } else if (model == MODEL_SNOW) {
         Decomp d;
-        if (!decompose({trial.x, trial.y, trial.z, trial.w}, d)) { raise_flags(ctl, ERR_G2P); return; }
+        if (!decompose({trial.x, trial.y, trial.z, trial.w}, d)) {
+            raise_step(ctl, ERR_G2P, step);
+            return false;
+        }
         const double lo = 1 - SNOW_COMPRESSION_YIELD, hi = 1 + SNOW_STRETCH_YIELD;
         const double s1 = fmax(lo, fmin(hi, d.s1)), s2 = fmax(lo, fmin(hi, d.s2));
-        const double upv = fmax(SNOW_MIN_PLASTIC_VOLUME, fmin(SNOW_MAX_PLASTIC_VOLUME, ((double)pvn * tdet) / (s1 * s2)));
+        const double upv = fmax(SNOW_MIN_PLASTIC_VOLUME, fmin(SNOW_MAX_PLASTIC_VOLUME, ((double)pv * tdet) / (s1 * s2)));
         const M2 r = reconstruct(d, s1, s2);
         Fn = make_float4((float)r.a, (float)r.b, (float)r.c, (float)r.d);
         pvn = (float)upv;
     } else if (model == MODEL_POWDER) {
-        const float lambda = st.lambda[i], mu = st.mu[i], fric = st.param[i];
-        if (!finitef(lambda) || !finitef(mu) || mu <= 0.0f || !finitef(fric) || fric < 0.0f) {
-            raise_flags(ctl, ERR_G2P);
-            return;
+        if (!finitef(mt.lambda) || !finitef(mt.mu) || mt.mu <= 0.0f || !finitef(mt.param) || mt.param < 0.0f) {
+            raise_step(ctl, ERR_G2P, step);
+            return false;
         }
         Decomp d;
-        if (!decompose({trial.x, trial.y, trial.z, trial.w}, d)) { raise_flags(ctl, ERR_G2P); return; }
-        const double l1 = log(d.s1), l2 = log(d.s2), lv = l1 + l2;
+        if (!decompose({trial.x, trial.y, trial.z, trial.w}, d)) {
+            raise_step(ctl, ERR_G2P, step);
+            return false;
+        }
+        const double l1 = log(d.s1), l2 = log(d.s2), lv2 = l1 + l2;
         const double minlv = log(MIN_POWDER_DETERMINANT);
-        const double blv = fmax(minlv, lv);
-        const double pressure = fmax(0.0, -((double)lambda + (double)mu) * blv);
+        const double blv = fmax(minlv, lv2);
+        const double pressure = fmax(0.0, -((double)mt.lambda + (double)mt.mu) * blv);
         const double dev = 0.5 * (l1 - l2);
-        const double maxdev = ((double)fric * pressure) / (2 * (double)mu);
-        if (!(dev <= maxdev && lv >= minlv)) {
+        const double maxdev = ((double)mt.param * pressure) / (2 * (double)mt.mu);
+        if (!(dev <= maxdev && lv2 >= minlv)) {
             const double pdev = fmin(dev, maxdev), mean = 0.5 * blv;
             const M2 r = reconstruct(d, exp(mean + pdev), exp(mean - pdev));
             Fn = make_float4((float)r.a, (float)r.b, (float)r.c, (float)r.d);
         }
     } else if (model == MODEL_NEO_HOOKEAN) {
         const float tr = trial.x * trial.x + trial.y * trial.y + trial.z * trial.z + trial.w * trial.w;
-        if (!finitef(tr)) { raise_flags(ctl, ERR_G2P); return; }
+        if (!finitef(tr)) {
+            raise_step(ctl, ERR_G2P, step);
+            return false;
+        }
         if (tdet < MIN_NEO_HOOKEAN_STRETCH * sqrt((double)tr) ||
             (double)tr > MAX_NEO_HOOKEAN_STRETCH * MAX_NEO_HOOKEAN_STRETCH) {
             Decomp d;
-            if (!decompose({trial.x, trial.y, trial.z, trial.w}, d)) { raise_flags(ctl, ERR_G2P); return; }
+            if (!decompose({trial.x, trial.y, trial.z, trial.w}, d)) {
+                raise_step(ctl, ERR_G2P, step);
+                return false;
+            }
             const double s1 = fmax(MIN_NEO_HOOKEAN_STRETCH, fmin(MAX_NEO_HOOKEAN_STRETCH, d.s1));
             const double s2 = fmax(MIN_NEO_HOOKEAN_STRETCH, fmin(MAX_NEO_HOOKEAN_STRETCH, d.s2));
             const M2 r = reconstruct(d, s1, s2);
             Fn = make_float4((float)r.a, (float)r.b, (float)r.c, (float)r.d);
         }
     } else {
-        raise_flags(ctl, ERR_G2P);
-        return;
+        raise_step(ctl, ERR_G2P, step);
+        return false;
     }
     const double ndet = det_f64(Fn);
-    if (!finite4(Fn) || !finitef(pvn) || pvn < 0.6f || pvn > 20.0f) { raise_flags(ctl, ERR_G2P); return; }
+    if (!finite4(Fn) || !finitef(pvn) || pvn < 0.6f || pvn > 20.0f) {
+        raise_step(ctl, ERR_G2P, step);
+        return false;
+    }
     if (!isfinite(ndet) || ndet <= 0.0) {
-        record_failure(ctl, 2u, i, Fn.x, Fn.y, Fn.z, Fn.w);
-        raise_flags(ctl, ERR_PROJECTED_DEFORMATION);
-        return;
+        record_failure(ctl, 2u, pid, Fn.x, Fn.y, Fn.z, Fn.w);
+        raise_step(ctl, ERR_PROJECTED_DEFORMATION, step);
+        return false;
     }
     // projectLiquidFromTriangles is the identity without bonded bodies; the
     // second confineAxis pass is then idempotent (shaders.ts:2376-2408).
     const float2 bxv = confine_axis(np.x, vx, P.dom_w);
     const float2 byv = confine_axis(np.y, vy, P.dom_h);
-    const float2 fp = make_float2(bxv.x, byv.x), fv = make_float2(bxv.y, byv.y);
-    out.pos[i] = fp;
-    out.vel[i] = fv;
-    out.C[i] = C;
-    out.F[i] = Fn;
-    out.plastic[i] = pvn;
+    o.pos = make_float2(bxv.x, byv.x);
+    o.vel = make_float2(bxv.y, byv.y);
+    o.C = C;
+    o.F = Fn;
+    o.plastic = pvn;
     if (P.cfl_on >= 0.5f && P.body_count == 0u) {
-        const float speed = sqrtf(fv.x * fv.x + fv.y * fv.y);
-        const float ws = wave_speed_bound<GENERAL>(P, model, Fn, st.lambda[i], st.mu[i], st.density[i], st.volume[i], st.param[i]);
-        if (!finite2(fv) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
-            record_failure(ctl, 3u, i, speed, ws, P.vel_bound, P.wave_bound);
-            raise_flags(ctl, ERR_CFL_BOUND);
-            return;
+        const float speed = sqrtf(o.vel.x * o.vel.x + o.vel.y * o.vel.y);
+        const float ws = wave_speed_bound<GENERAL>(P, model, Fn, mt.lambda, mt.mu, mt.density, mt.volume, mt.param);
+        if (!finite2(o.vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
+            record_failure(ctl, 3u, pid, speed, ws, P.vel_bound, P.wave_bound);
+            raise_step(ctl, ERR_CFL_BOUND, step);
+            return false;
         }
     }
-    if (bin_next) {
-        // prepare + bin the committed candidate for the next microstep; its
-        // failures are deferred so this microstep can still commit
-        float4 A, aux;
-        const uint32_t pf = prepare_particle<GENERAL>(P, model, C, Fn, st.lambda[i], st.mu[i], st.density[i],
-                                                      st.volume[i], st.param[i], pvn, A, aux);
-        if (pf) {
-            bins.key[i] = INVALID_KEY;
-            if (pf & ERR_P2G) atomicMin(&ctl->deferred_index, i);
-            atomicOr(&ctl->deferred, pf);
-            return;
+    return true;
+}
+
+// One 256-thread block per 8x8-bucket tile: stage the 34x34 node patch, run
+// G2P for the tile's particles and write each into its next bucket.
+template <bool GENERAL>
+__global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
+                                                 SlotBuf out, const Material* __restrict__ mats, Grid grid,
+                                                 uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
+                                                 uint32_t scene_flags, uint32_t step, int out_buf) {
+    __shared__ float2 s_vs[G_TN * G_TN];
+    __shared__ float2 s_vw[G_TN * G_TN];
+    __shared__ float2 s_vo[G_TN * G_TN];
+    __shared__ float2 s_lq[G_TN * G_TN];
+    __shared__ uint32_t s_boff[G_TB * G_TB + 1];
+    if (block_status_set(ctl)) return;
+    const Params& P = *Pp;
+    const int tid = threadIdx.x;
+    const int tbx = (int)(blockIdx.x % tiles_x) * G_TB, tby = (int)(blockIdx.x / tiles_x) * G_TB;
+    const int nx0 = tbx * BUCKET_CELLS, ny0 = tby * BUCKET_CELLS;  // first node (= first cell) of the patch
+    const int W = (int)P.node_w, H = (int)P.node_h;
+    if (tid < 64) {
+        const int gbx = tbx + tid % G_TB, gby = tby + tid / G_TB;
+        uint32_t c = 0u;
+        if (gbx < (int)bw && gby < (int)bh) c = min(in.cnt[(uint32_t)gby * bw + (uint32_t)gbx], cap);
+        // exclusive scan of the 64 counts in one wave
+        uint32_t incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += t;
         }
-        bins.affine[i] = A;
-        bins.aux[i] = aux;
-        Stencil ns;
-        if (!make_stencil(P, fp, ns)) {
-            bins.key[i] = INVALID_KEY;
-            raise_flags(ctl, ERR_STENCIL);
-            return;
+        s_boff[tid] = incl - c;
+        if (tid == 63) s_boff[64] = incl;
+    }
+    for (int e = tid; e < G_TN * G_TN; e += G_THREADS) {
+        const int nx = nx0 + e % G_TN, ny = ny0 + e / G_TN;
+        float2 a = make_float2(0.f, 0.f), bwv = a, c = a, d = a;
+        if (nx < W && ny < H) {
+            const uint32_t node = (uint32_t)ny * (uint32_t)W + (uint32_t)nx;
+            if (scene_flags & SCENE_SOLID) a = grid.v_solid[node];
+            if (scene_flags & SCENE_WATER) bwv = grid.v_water[node];
+            if (scene_flags & SCENE_OIL) c = grid.v_oil[node];
+            if (scene_flags & (SCENE_WATER | SCENE_OIL)) d = grid.liquid[node];
         }
-        const uint32_t k = (uint32_t)(ns.by + 1) * W + (uint32_t)(ns.bx + 1);
-        bins.key[i] = k;
-        bins.rank[i] = atomicAdd(&bins.count[k], 1u);
+        s_vs[e] = a;
+        s_vw[e] = bwv;
+        s_vo[e] = c;
+        s_lq[e] = d;
+    }
+    __syncthreads();
+    const uint32_t total = s_boff[64];
+    for (uint32_t base = 0; base < total; base += G_THREADS) {
+        const uint32_t j = base + (uint32_t)tid;
+        bool ok = j < total;
+        G2POut o;
+        uint32_t pid = 0, mat = 0, dest = 0;
+        float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) {
+            int lo = 0, hi = 63;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_boff[mid] <= j) lo = mid; else hi = mid - 1;
+            }
+            const int gbx = tbx + lo % G_TB, gby = tby + lo / G_TB;
+            const size_t g = ((size_t)gby * bw + (size_t)gbx) * cap + (j - s_boff[lo]);
+            const float2 p = in.pos[g];
+            const float4 F = in.F[g];
+            pid = in.pid[g];
+            mat = in.mat[g];
+            const float pv = GENERAL ? in.plastic[g] : 1.0f;
+            const Material mt = mats[mat];
+            Stencil s;
+            if (!make_stencil(P.dx, P.cells_x, P.cells_y, p, s) || ((s.bx + 1) >> 2) != gbx ||
+                ((s.by + 1) >> 2) != gby) {
+                raise_step(ctl, ERR_STENCIL, step);
+                ok = false;
+            } else {
+                const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u ? s_vo : s_vs);
+                ok = g2p_particle<GENERAL>(P, ctl, mt, pid, p, F, pv, s, lv, s_lq, nx0, ny0, step, o);
+            }
+            if (ok) {
+                Stencil ns;
+                if (!make_stencil(P.dx, P.cells_x, P.cells_y, o.pos, ns)) {
+                    raise_step(ctl, ERR_STENCIL, step);
+                    ok = false;
+                } else {
+                    dest = bucket_of(ns, bw);
+                }
+            }
+            if constexpr (GENERAL) {
+                if (ok) {
+                    // stress cache of the new state; failures belong to the
+                    // microstep that will start from it
+                    const uint32_t pf = particle_stress<true>(mt, o.F, o.plastic, det_comp(o.F), S);
+                    if (pf) {
+                        if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], pid);
+                        atomicOr(&ctl->deferred[out_buf], pf);
+                    }
+                }
+            }
+        }
+        const uint32_t k = wave_alloc(out.cnt, dest, ok);
+        if (ok) {
+            if (k >= cap) {
+                raise_step(ctl, ERR_INTERNAL_REBUCKET, step);
+            } else {
+                const size_t g = (size_t)dest * cap + k;
+                out.pos[g] = o.pos;
+                out.vel[g] = o.vel;
+                out.C[g] = o.C;
+                out.F[g] = o.F;
+                out.pid[g] = pid;
+                out.mat[g] = mat;
+                if constexpr (GENERAL) {
+                    out.plastic[g] = o.plastic;
+                    out.S[g] = S;
+                }
+            }
+        }
     }
 }
 
 // ------------------------------------------------------------- batch end
-__global__ void k_commit_last(Control* ctl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && ctl->status[0] == 0u) ctl->completed += 1u;
+__global__ void k_commit_last(Control* ctl, uint32_t steps, uint32_t seq) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t done = min(ctl->fail_step, steps);
+        ctl->completed = done;
+        ctl->history[seq % CONTROL_HISTORY] = done;
+    }
 }
 
 // On a failed microstep the committed state sits in the buffer of the last
-// committed step; copy it to the batch's output buffer when their parity
-// differs (commitParticles' whole-step rollback, shaders.ts:3067-3080).
-__global__ __launch_bounds__(256) void k_fix_rollback(const Control* ctl, ParticleBuf b0, ParticleBuf b1, uint32_t n,
-                                                      uint32_t start_buf, uint32_t steps) {
+// committed microstep; copy it to the batch's output buffer when their
+// parity differs (commitParticles' whole-step rollback, shaders.ts:3067-3080).
+__global__ __launch_bounds__(256) void k_fix_rollback(Control* ctl, SlotBuf b0, SlotBuf b1, uint32_t nb,
+                                                      uint32_t cap, uint32_t start_buf, uint32_t steps,
+                                                      uint32_t general) {
     if (ctl->status[0] == 0u) return;
     const uint32_t committed = (start_buf + ctl->completed) & 1u;
     const uint32_t final_buf = (start_buf + steps) & 1u;
     if (committed == final_buf) return;
-    const ParticleBuf& src = committed ? b1 : b0;
-    const ParticleBuf& dst = committed ? b0 : b1;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        dst.pos[i] = src.pos[i];
-        dst.vel[i] = src.vel[i];
-        dst.C[i] = src.C[i];
-        dst.F[i] = src.F[i];
-        dst.plastic[i] = src.plastic[i];
+    const SlotBuf& src = committed ? b1 : b0;
+    const SlotBuf& dst = committed ? b0 : b1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->deferred[final_buf] = ctl->deferred[committed];
+        ctl->deferred_index[final_buf] = ctl->deferred_index[committed];
+    }
+    const size_t total = (size_t)nb * cap;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)(g / cap), k = (uint32_t)(g % cap);
+        const uint32_t c = src.cnt[b];
+        if (k == 0) dst.cnt[b] = c;
+        if (k >= c) continue;
+        dst.pos[g] = src.pos[g];
+        dst.vel[g] = src.vel[g];
+        dst.C[g] = src.C[g];
+        dst.F[g] = src.F[g];
+        dst.pid[g] = src.pid[g];
+        dst.mat[g] = src.mat[g];
+        if (general) {
+            dst.S[g] = src.S[g];
+            dst.plastic[g] = src.plastic[g];
+        }
     }
 }
 
 // measureParticleSpeedPartials + reduceParticleSpeedPartials
-// (shaders.ts:2463-2533) as one pass: wave max, then one atomic per block on
-// the non-negative float's bit pattern.
-__global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, const float2* __restrict__ vel,
-                                                       const uint32_t* __restrict__ model, uint32_t n) {
+// (shaders.ts:2463-2533): wave max, then one atomic per block on the
+// non-negative float's bit pattern.
+__global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, SlotBuf buf, uint32_t nb, uint32_t cap) {
     __shared__ float s_w[4];
     float m = 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (model[i] != MODEL_REMOVED) {
-            const float2 v = vel[i];
-            const float sp = sqrtf(v.x * v.x + v.y * v.y);
-            m = fmaxf(m, finitef(sp) ? sp : FLT_MAX);
-        }
+    const size_t total = (size_t)nb * cap;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        if ((uint32_t)(g % cap) >= buf.cnt[g / cap]) continue;
+        const float2 v = buf.vel[g];
+        const float sp = sqrtf(v.x * v.x + v.y * v.y);
+        m = fmaxf(m, finitef(sp) ? sp : FLT_MAX);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
@@ -1071,13 +1424,11 @@ __global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, const float
     }
 }
 
-inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
+inline uint32_t blocks_for(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
 }  // namespace
 
-size_t scan_block_count(uint32_t node_count) { return (node_count + SCAN_TILE - 1) / SCAN_TILE; }
-
-#define MPM_LAUNCH(name, grid, block, ...)                                        \
+#define MPM_LAUNCH(name, ...)                                                     \
     do {                                                                          \
         TimedRegion _tr(lc.timing, name, lc.stream);                              \
         hipLaunchKernelGGL(__VA_ARGS__);                                          \
@@ -1085,64 +1436,66 @@ size_t scan_block_count(uint32_t node_count) { return (node_count + SCAN_TILE - 
         if (_e != hipSuccess) return _e;                                          \
     } while (0)
 
-hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure) {
-    const uint32_t n = lc.n, nodes = lc.node_count;
-    const uint32_t pb = blocks_for(n > 0 ? n : 1, 256);
-    const uint32_t nb = (uint32_t)scan_block_count(nodes);
-    const uint32_t tiles_x = (lc.node_w + TI - 1) / TI, tiles_y = (lc.node_h + TI - 1) / TI;
-    const uint32_t zb = min(blocks_for(nodes, 256), 2048u);
-    MPM_LAUNCH("batch_begin", zb, 256, k_batch_begin, dim3(zb), dim3(256), 0, lc.stream, lc.ctl, lc.bins.count, nodes);
+hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure, uint32_t seq) {
     const bool general = scene_general(lc.scene_flags);
-    if (n > 0) {
-        if (general)
-            MPM_LAUNCH("bin", pb, 256, k_bin<true>, dim3(pb), dim3(256), 0, lc.stream, lc.params, lc.ctl,
-                       lc.buf[start_buf], lc.st, lc.bins, n);
-        else
-            MPM_LAUNCH("bin", pb, 256, k_bin<false>, dim3(pb), dim3(256), 0, lc.stream, lc.params, lc.ctl,
-                       lc.buf[start_buf], lc.st, lc.bins, n);
-    }
+    const uint32_t ptx = (lc.node_w + P_TI - 1) / P_TI, pty = (lc.node_h + P_TI - 1) / P_TI;
+    const uint32_t gtx = (lc.bw + G_TB - 1) / G_TB, gty = (lc.bh + G_TB - 1) / G_TB;
+    const size_t slots = (size_t)lc.nb * lc.cap;
+    MPM_LAUNCH("batch_begin", k_batch_begin, dim3(1), dim3(64), 0, lc.stream, lc.ctl);
     int cur = start_buf;
     for (uint32_t s = 0; s < steps; ++s) {
-        const ParticleBuf& in = lc.buf[cur];
-        const ParticleBuf& out = lc.buf[cur ^ 1];
-        MPM_LAUNCH("scan_reduce", nb, SCAN_THREADS, k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, lc.stream, lc.ctl,
-                   lc.bins.count, lc.bins.block_sums, nodes, s > 0 ? 1u : 0u);
-        MPM_LAUNCH("scan_blocks", 1, SCAN_THREADS, k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, lc.stream, lc.ctl,
-                   lc.bins.block_sums, nb);
-        MPM_LAUNCH("scan_final", nb, SCAN_THREADS, k_scan_final, dim3(nb), dim3(SCAN_THREADS), 0, lc.stream, lc.ctl,
-                   lc.bins.count, lc.bins.block_sums, lc.bins.start, nodes);
-        if (n > 0)
-            MPM_LAUNCH("scatter", pb, 256, k_scatter, dim3(pb), dim3(256), 0, lc.stream, lc.ctl, lc.bins.key,
-                       lc.bins.rank, lc.bins.start, lc.bins.perm, n);
-        if (lc.fixed_point)
-            MPM_LAUNCH("p2g_grid_fixed", tiles_x * tiles_y, P2G_THREADS, k_p2g_grid<true>, dim3(tiles_x * tiles_y),
-                       dim3(P2G_THREADS), 0, lc.stream, lc.params, lc.ctl, in, lc.st, lc.bins, lc.grid, tiles_x,
-                       lc.keep_fixed, lc.scene_flags);
-        else
-            MPM_LAUNCH("p2g_grid", tiles_x * tiles_y, P2G_THREADS, k_p2g_grid<false>, dim3(tiles_x * tiles_y),
-                       dim3(P2G_THREADS), 0, lc.stream, lc.params, lc.ctl, in, lc.st, lc.bins, lc.grid, tiles_x, 0u,
-                       lc.scene_flags);
-        if (n > 0) {
-            if (general)
-                MPM_LAUNCH("g2p", pb, 256, k_g2p<true>, dim3(pb), dim3(256), 0, lc.stream, lc.params, lc.ctl, in, out,
-                           lc.st, lc.grid, lc.bins, n, s + 1 < steps ? 1u : 0u);
-            else
-                MPM_LAUNCH("g2p", pb, 256, k_g2p<false>, dim3(pb), dim3(256), 0, lc.stream, lc.params, lc.ctl, in, out,
-                           lc.st, lc.grid, lc.bins, n, s + 1 < steps ? 1u : 0u);
+        const SlotBuf& in = lc.buf[cur];
+        const SlotBuf& out = lc.buf[cur ^ 1];
+        if (lc.fixed_point) {
+            auto p2g = general ? k_p2g_fixed<true> : k_p2g_fixed<false>;
+            MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
+                       out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.keep_fixed, lc.scene_flags, s, cur);
+        } else {
+            auto p2g = general ? k_p2g_gather<true> : k_p2g_gather<false>;
+            MPM_LAUNCH("p2g_grid", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
+                       out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur);
         }
+        auto g2p = general ? k_g2p<true> : k_g2p<false>;
+        MPM_LAUNCH("g2p", g2p, dim3(gtx * gty), dim3(G_THREADS), 0, lc.stream, lc.params, lc.ctl, in, out, lc.mats,
+                   lc.grid, gtx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur ^ 1);
         cur ^= 1;
     }
-    if (steps > 0) MPM_LAUNCH("commit_last", 1, 64, k_commit_last, dim3(1), dim3(64), 0, lc.stream, lc.ctl);
-    if (n > 0) {
-        const uint32_t fb = min(pb, 1024u);
-        MPM_LAUNCH("fix_rollback", fb, 256, k_fix_rollback, dim3(fb), dim3(256), 0, lc.stream, lc.ctl, lc.buf[0],
-                   lc.buf[1], n, (uint32_t)start_buf, steps);
-        if (measure) {
-            const uint32_t mb = min(pb, 512u);
-            MPM_LAUNCH("measure_speed", mb, 256, k_measure_speed, dim3(mb), dim3(256), 0, lc.stream, lc.ctl,
-                       lc.buf[cur].vel, lc.st.model, n);
-        }
-    }
+    MPM_LAUNCH("commit_last", k_commit_last, dim3(1), dim3(64), 0, lc.stream, lc.ctl, steps, seq);
+    const uint32_t fb = std::min<uint32_t>(blocks_for(slots, 256), 2048u);
+    MPM_LAUNCH("fix_rollback", k_fix_rollback, dim3(fb), dim3(256), 0, lc.stream, lc.ctl, lc.buf[0], lc.buf[1], lc.nb,
+               lc.cap, (uint32_t)start_buf, steps, general ? 1u : 0u);
+    if (measure)
+        MPM_LAUNCH("measure_speed", k_measure_speed, dim3(std::min<uint32_t>(blocks_for(slots, 256), 1024u)),
+                   dim3(256), 0, lc.stream, lc.ctl, lc.buf[cur], lc.nb, lc.cap);
+    return hipSuccess;
+}
+
+hipError_t launch_clear_counts(const LaunchCtx& lc, int b) {
+    MPM_LAUNCH("clear_counts", k_clear_counts, dim3(std::min<uint32_t>(blocks_for(lc.nb, 256), 1024u)), dim3(256), 0,
+               lc.stream, lc.buf[b].cnt, lc.nb);
+    return hipSuccess;
+}
+
+hipError_t launch_bucket(const LaunchCtx& lc, int b, uint32_t first, uint32_t count) {
+    if (count == 0) return hipSuccess;
+    const float dx = lc.cell_size;
+    auto kern = scene_general(lc.scene_flags) ? k_bucket<true> : k_bucket<false>;
+    MPM_LAUNCH("bucket", kern, dim3(blocks_for(count, 256)), dim3(256), 0, lc.stream, lc.ctl, lc.home, lc.mats,
+               lc.buf[b], first, count, dx, lc.cells_x, lc.cells_y, lc.bw, lc.cap);
+    return hipSuccess;
+}
+
+hipError_t launch_unbucket(const LaunchCtx& lc, int b) {
+    const size_t slots = (size_t)lc.nb * lc.cap;
+    MPM_LAUNCH("unbucket", k_unbucket, dim3(std::min<uint32_t>(blocks_for(slots, 256), 2048u)), dim3(256), 0,
+               lc.stream, lc.buf[b], lc.home, lc.nb, lc.cap, scene_general(lc.scene_flags) ? 1u : 0u);
+    return hipSuccess;
+}
+
+hipError_t launch_keys(const LaunchCtx& lc, uint32_t* keys) {
+    if (lc.n == 0) return hipSuccess;
+    MPM_LAUNCH("keys", k_keys, dim3(blocks_for(lc.n, 256)), dim3(256), 0, lc.stream, lc.home, keys, lc.n,
+               lc.cell_size, lc.cells_x, lc.cells_y, lc.node_w);
     return hipSuccess;
 }
 

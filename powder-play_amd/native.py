@@ -34,6 +34,7 @@ SIGNATURES = {
     "mpm_read_fixed_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_read_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_read_bins": (ctypes.c_int, [_P, _P, _U32, _P, ctypes.c_size_t]),
+    "mpm_get_buckets": (ctypes.c_int, [_P, _P, _P, _P]),
     "mpm_set_timing": (ctypes.c_int, [_P, _U32]),
     "mpm_read_timing": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_uint64)]),
@@ -47,7 +48,7 @@ class MpmConfig(ctypes.Structure):
         ("cells_x", _U32), ("cells_y", _U32), ("cell_size", ctypes.c_float),
         ("particle_mass", ctypes.c_float), ("particle_volume", ctypes.c_float),
         ("wall_friction", ctypes.c_float), ("capacity", _U32), ("p2g_mode", _U32),
-        ("flags", _U32), ("device", ctypes.c_int32),
+        ("flags", _U32), ("device", ctypes.c_int32), ("bucket_slots", _U32),
     ]
 
 
@@ -157,12 +158,18 @@ class Context:
         self._check(self.lib.mpm_read_grid(self.handle, _ptr(out), out.size))
         return out
 
+    def buckets(self):
+        bx, by, cap = (ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32())
+        self._check(self.lib.mpm_get_buckets(self.handle, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(cap)))
+        return bx.value, by.value, cap.value
+
     def read_bins(self):
-        n, _, nodes = self.counts()
+        n = self.counts()[0]
+        bx, by, _ = self.buckets()
         keys = np.zeros(n, np.uint32)
-        starts = np.zeros(nodes + 1, np.uint32)
-        self._check(self.lib.mpm_read_bins(self.handle, _ptr(keys), n, _ptr(starts), starts.size))
-        return keys, starts
+        counts = np.zeros(bx * by, np.uint32)
+        self._check(self.lib.mpm_read_bins(self.handle, _ptr(keys), n, _ptr(counts), counts.size))
+        return keys, counts
 
     def set_timing(self, enable: bool):
         self._check(self.lib.mpm_set_timing(self.handle, 1 if enable else 0))

@@ -121,7 +121,8 @@ class MpmMlsGpuSolver:
     """Solver host over one native context (one HIP stream)."""
 
     def __init__(self, particles, config: dict, cfl_bounds: cflmod.CflBounds | None = None,
-                 p2g_mode: int = MPM_P2G_GATHER, keep_fixed_grid: bool = False, device: int = -1):
+                 p2g_mode: int = MPM_P2G_GATHER, keep_fixed_grid: bool = False, device: int = -1,
+                 bucket_slots: int = 0):
         self.state = layout.validate_input(particles, config)
         derived = cfl_bounds
         if derived is None:
@@ -131,7 +132,7 @@ class MpmMlsGpuSolver:
         cfg = MpmConfig(self.state.config["grid"]["cellsX"], self.state.config["grid"]["cellsY"],
                         self.state.cell_size_f32, self.state.particle_mass_f32, self.state.particle_volume_f32,
                         self.state.wall_friction_f32, self.state.particle_capacity, p2g_mode,
-                        MPM_FLAG_KEEP_FIXED_GRID if keep_fixed_grid else 0, device)
+                        MPM_FLAG_KEEP_FIXED_GRID if keep_fixed_grid else 0, device, bucket_slots)
         self.ctx = Context(cfg, self.state.packed, self.state.object_ids)
         self.p2g_mode = p2g_mode
         self._disposed = False

@@ -1,15 +1,16 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle.
 
 * five-material transfer fixture (experiments/transfer-regression.ts:33-95):
-  8 steps vs the FP64 oracle within the reference's 1e-3, batch == singles;
+  8 steps vs the FP64 oracle within the reference's 1e-3, batch == singles
+  bitwise (both P2G paths sum in a fixed order);
 * fixed-point P2G: int32 accumulators bit-exact vs the oracle's integer path,
   at small size and after several committed steps;
 * binning: bin keys and starts bit-exact vs the oracle's counting sort;
 * overflow regression (transfer-regression.ts:97-170): status 65536 and a
   bitwise rollback;
 * CFL guard failure: status 16384, progress count and rollback;
-* 1M-particle water/oil column: determinism, conservation and oracle parity
-  of a sampled step.
+* 1M-particle water/oil column: determinism of both paths, conservation and
+  oracle parity of a step.
 """
 from __future__ import annotations
 
@@ -97,19 +98,61 @@ def test_fixed_point_grid_bit_exact_tait_and_solids(pkg, orc):
 
 
 def test_binning_bit_exact(pkg, orc):
+    """Stencil-base keys of every particle and the per-bucket particle counts
+    (4x4-cell buckets) against the oracle's counting sort, before and after
+    steps that move particles across buckets."""
     arr, config = pkg.scenes.water_oil_column(cells=64)
     s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
     try:
-        before, _ = s.read_particles(packed=True)
-        s.step(1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY)
-        keys, starts = s.read_bins()
-        prm = orc.params_f32(64, 64, 1.0, 0.02, 1.0, 1e-4, len(arr))
-        okey, ostart, _, flags = orc.bin_particles(before, prm)
-        assert flags == 0
-        assert np.array_equal(keys, okey)
-        assert np.array_equal(starts, ostart)
+        W = 67
+        bx, by, cap = s.ctx.buckets()
+        assert (bx, by) == (17, 17) and cap >= 32
+        for step in range(3):
+            state, _ = s.read_particles(packed=True)
+            keys, counts = s.read_bins()
+            prm = orc.params_f32(64, 64, 1.0, 0.02, 1.0, 1e-4, len(arr))
+            okey, ostart, _, flags = orc.bin_particles(state, prm)
+            assert flags == 0
+            assert np.array_equal(keys, okey)
+            obucket = ((okey // W) >> 2) * bx + ((okey % W) >> 2)
+            assert np.array_equal(counts, np.bincount(obucket, minlength=bx * by).astype(np.uint32))
+            s.step_batch(40, 1.0 / 2400.0, pkg.scenes.WATER_OIL_GRAVITY)
     finally:
         s.dispose()
+
+
+def test_bucket_overflow_replays_transparently(pkg, orc):
+    """Liquid driven into a neighbouring bucket overfills its 64 slots (70 particles by microstep 50, 80 by 80): the
+    step rolls back, the buckets grow and the interrupted microsteps replay,
+    invisibly to the caller; the state still matches the FP64 oracle."""
+    water = pkg.scenes.FIVE_MODEL_MATERIALS[2]
+
+    def p(x, y, vx):
+        return {"position": {"x": x, "y": y}, "velocity": {"x": vx, "y": 0.0},
+                "C": {"m00": 0.0, "m01": 0.0, "m10": 0.0, "m11": 0.0},
+                "F": {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0}, "material": water}
+    still = [p(5.0 + 0.4 * (i % 6), 4.6 + 0.25 * (i // 6), 0.0) for i in range(60)]
+    moving = [p(8.0 + 0.4 * (i % 6), 4.6 + 0.25 * (i // 6), -20.0) for i in range(60)]
+    particles = still + moving
+    config = {"grid": {"cellsX": 48, "cellsY": 16, "cellSize": 1.0}, "particleMass": 0.1, "particleVolume": 0.05,
+              "material": water}
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0, bucket_slots=64)
+    try:
+        _, _, cap0 = s.ctx.buckets()
+        s.step_batch(80, 1e-3, {"x": 0.0, "y": 0.0})
+        s.check_numerical_status()
+        recs, _ = s.read_particles(packed=True)
+        _, _, cap1 = s.ctx.buckets()
+    finally:
+        s.dispose()
+    state = pkg.layout.validate_input(particles, config)
+    exp = orc.run_f64(orc.particles_from_packed(state.packed), orc.make_config(1e-3, 48, 16, 1.0, 0.1, 0.05), 80)
+    got, want = as_f64(recs), oracle_state(orc, exp)
+    pos_err = float(np.max(np.abs(got[:, :2] - want[:, :2])))
+    vel_err = float(np.max(np.abs(got[:, 2:4] - want[:, 2:4])))
+    print(f"bucket growth {cap0} -> {cap1} slots; max |dx| {pos_err:.3e}, max |dv| {vel_err:.3e}")
+    assert cap0 == 64 and cap1 > cap0
+    assert pos_err <= 1e-3 and vel_err <= 1e-2
 
 
 @pytest.mark.parametrize("contribution", [False, True])
@@ -199,16 +242,17 @@ def test_tait_tank_fixed_dt_vs_fp64(pkg, orc):
 def test_column_1m_deterministic_conservative_and_oracle_close(pkg, orc):
     arr, config = pkg.scenes.water_oil_column(cells=1000)
     dt, g = 1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY
-    results = []
-    for _ in range(2):
-        s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
-        try:
-            s.step_batch(4, dt, g)
-            s.check_numerical_status()
-            results.append(s.read_particles(packed=True)[0])
-        finally:
-            s.dispose()
-    assert np.array_equal(results[0], results[1])  # deterministic float path
+    for factory in (pkg.create_mpm_mls_gpu_solver, pkg.create_experimental_mpm_mls_atomic_gpu_solver):
+        results = []
+        for _ in range(2):
+            s = factory(arr, config, device=0)
+            try:
+                s.step_batch(4, dt, g)
+                s.check_numerical_status()
+                results.append(s.read_particles(packed=True)[0])
+            finally:
+                s.dispose()
+        assert np.array_equal(results[0], results[1])  # both paths reproducible bit for bit
     state = pkg.layout.validate_input(arr, config)
     # oracle parity on one step from the initial state, sampled particles
     s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
