@@ -223,6 +223,7 @@ int upload_materials(mpm_ctx* c) {
         c->lc.mats = d;
     }
     HIP_TRY(c, hipMemcpy(c->d_mats, c->mats.data(), c->mats.size() * sizeof(Material), hipMemcpyHostToDevice));
+    c->lc.nmat = (uint32_t)c->mats.size();
     return MPM_OK;
 }
 

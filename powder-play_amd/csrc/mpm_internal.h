@@ -131,6 +131,7 @@ struct LaunchCtx {
     SlotBuf buf[2];
     Home home;
     const Material* mats;  // device
+    uint32_t nmat;         // material rows
     Grid grid;
     uint32_t n;            // active particle slots (caller indices)
     uint32_t node_w, node_h, node_count;

@@ -44,7 +44,8 @@ constexpr float MAX_LIQUID_DETERMINANT = 2.0f;
 constexpr double MIN_POWDER_DETERMINANT = 0.5;
 constexpr double MIN_NEO_HOOKEAN_STRETCH = 0.05;
 constexpr double MAX_NEO_HOOKEAN_STRETCH = 20.0;
-constexpr float MAX_DEFORMATION_INCREMENT = 0.25f;
+constexpr float MAX_DEFORMATION_INCREMENT = 0.25f;  // a power of two (dividing by it is exact scaling)
+static_assert(MAX_DEFORMATION_INCREMENT == 0.25f, "the substep count scales by the exact reciprocal");
 constexpr uint32_t MAX_DEFORMATION_SUBSTEPS = 64u;
 
 // P2G tile: 28x28 interior nodes (7x7 buckets), a one-node ring for the
@@ -56,7 +57,10 @@ constexpr int P_NB = P_RB * P_RB;
 constexpr int P_RC = P_RB * BUCKET_CELLS;       // region cells per side
 constexpr int P_CELLS = P_RC * P_RC;
 constexpr int P_THREADS = 1024;
-constexpr int P_CAP = 2 * P_THREADS;            // staged particles per gather chunk: two per thread
+constexpr int G_CAP = 1536;                     // staged particles per gather chunk
+static_assert(G_CAP <= 2 * P_THREADS, "the gather loads at most two particles a thread");
+static_assert(P_TC * P_TC <= G_CAP, "the ring volumes reuse two staging arrays");
+constexpr int LDS_MATS = 64;                    // material rows cached in LDS
 static_assert(P_TC <= 32 && P_TC * 32 <= P_THREADS, "one node thread per tile node, 32 to a row");
 static_assert(P_CELLS <= 2 * P_THREADS, "the cell scan takes two cells a thread");
 static_assert(P_NB <= 128, "the bucket scan takes two buckets a lane of one wave");
@@ -442,6 +446,46 @@ __device__ __forceinline__ uint32_t fixed_round(float value, int32_t& out) {
     return 0u;
 }
 
+// Copy the first LDS_MATS material rows into LDS (all threads of the block;
+// a barrier must follow before material_at is used).
+__device__ __forceinline__ void stage_materials(Material* s_mat, const Material* mats, uint32_t nmat) {
+    const uint32_t words = min(nmat, (uint32_t)LDS_MATS) * (uint32_t)(sizeof(Material) / 4);
+    for (uint32_t e = threadIdx.x; e < words; e += blockDim.x)
+        reinterpret_cast<uint32_t*>(s_mat)[e] = reinterpret_cast<const uint32_t*>(mats)[e];
+}
+__device__ __forceinline__ Material material_at(const Material* s_mat, const Material* mats, uint32_t m) {
+    uint4 lo, hi;  // the row as two 16-byte words, from LDS or from global memory
+    if (m < (uint32_t)LDS_MATS) {
+        lo = reinterpret_cast<const uint4*>(s_mat)[2 * m];
+        hi = reinterpret_cast<const uint4*>(s_mat)[2 * m + 1];
+    } else {
+        lo = reinterpret_cast<const uint4*>(mats)[2 * m];
+        hi = reinterpret_cast<const uint4*>(mats)[2 * m + 1];
+    }
+    Material r;
+    r.model = lo.x;
+    r.lambda = __uint_as_float(lo.y);
+    r.mu = __uint_as_float(lo.z);
+    r.density = __uint_as_float(lo.w);
+    r.volume = __uint_as_float(hi.x);
+    r.param = __uint_as_float(hi.y);
+    r.mass = __uint_as_float(hi.z);
+    r.phase = hi.w;
+    return r;
+}
+
+// All six quadratic weights of a stencil fraction finite and non-negative
+// (the weight test of p2g, shaders.ts:1153-1157, for every support node).
+__device__ __forceinline__ bool weights_valid(float fx, float fy) {
+    bool ok = true;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        const float a = qweight(fx, o), b = qweight(fy, o);
+        ok = ok && finitef(a) && a >= 0.0f && finitef(b) && b >= 0.0f;
+    }
+    return ok;
+}
+
 // updateGrid (shaders.ts:1485-1746) of one interior node from its summed
 // channels f = {solid m, qx, qy | water m, qx, qy, V | oil m, qx, qy, V}.
 // `nvol(k, ox, oy)` returns the summed water (k = 0) or oil (k = 1) volume of
@@ -619,25 +663,27 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 // particle index inside a cell, so each node sums its three stencil rows as
 // three contiguous ranges in a fixed order (rows, then cells, then particle
 // index): the float sums are reproducible run to run.  A region holding more
-// than P_CAP particles is processed in chunks of whole bucket rows (single
-// buckets only when one bucket row exceeds P_CAP); the sums carry across
-// chunks.
+// than G_CAP particles is processed in chunks of whole bucket rows (single
+// buckets only when one bucket row exceeds G_CAP); the sums carry across
+// chunks.  The LDS footprint stays under half a CU so two tiles share one.
 template <bool GENERAL>
-__global__ __launch_bounds__(P_THREADS) void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
-                                                        uint32_t* __restrict__ cnt_next,
-                                                        const Material* __restrict__ mats, Grid grid,
-                                                        uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
-                                                        uint32_t scene_flags, uint32_t step, int in_buf) {
+__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint32_t* __restrict__ cnt_next,
+                  const Material* __restrict__ mats, uint32_t nmat, Grid grid, uint32_t tiles_x, uint32_t bw,
+                  uint32_t bh, uint32_t cap, uint32_t scene_flags, uint32_t step, int in_buf) {
     __shared__ uint32_t s_boff[P_NB + 1];    // staged offset of each region bucket
     __shared__ uint32_t s_chunk[P_NB + 1];   // chunk boundaries (region bucket indices)
     __shared__ uint32_t s_nchunks;
     __shared__ uint32_t s_wave[P_THREADS / 64 + 1];
+    __shared__ Material s_mat[LDS_MATS];
     __shared__ uint32_t s_cell[P_CELLS + 1]; // per region cell: count, then start
-    __shared__ uint32_t s_pid[P_CAP];        // particle indices at provisional places
-    __shared__ float s_fx[P_CAP], s_fy[P_CAP], s_m[P_CAP], s_vx[P_CAP], s_vy[P_CAP];
-    __shared__ float s_a0[P_CAP], s_a1[P_CAP], s_a2[P_CAP], s_a3[P_CAP], s_vol[P_CAP], s_det[P_CAP];
-    __shared__ uint32_t s_tag[P_CAP];        // region column | phase << 8; 0xffff = contributes nothing
+    // staged particle records in gather order
+    __shared__ float s_fx[G_CAP], s_fy[G_CAP], s_m[G_CAP], s_mvx[G_CAP], s_mvy[G_CAP];
+    __shared__ float s_a0[G_CAP], s_a1[G_CAP], s_a2[G_CAP], s_a3[G_CAP], s_vd[G_CAP];
+    __shared__ uint32_t s_tag[G_CAP];  // region column | phase << 8; 0xffff = contributes nothing
+    uint32_t* s_pid = s_tag;           // particle indices at provisional places, before staging
 
+    stage_materials(s_mat, mats, nmat);
     if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     const int tid = threadIdx.x;
     const Params& P = *Pp;
@@ -652,14 +698,14 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_gather(const Params* __restri
     if (tid == 0) {
         uint32_t n = 0;
         s_chunk[0] = 0u;
-        if (s_boff[P_NB] > (uint32_t)P_CAP) {
+        if (s_boff[P_NB] > (uint32_t)G_CAP) {
             for (int r = 0; r < P_RB; ++r) {
                 const int lo = r * P_RB, hi = lo + P_RB;
-                if (s_boff[hi] - s_boff[s_chunk[n]] <= (uint32_t)P_CAP) continue;
+                if (s_boff[hi] - s_boff[s_chunk[n]] <= (uint32_t)G_CAP) continue;
                 if ((int)s_chunk[n] < lo) s_chunk[++n] = (uint32_t)lo;
-                if (s_boff[hi] - s_boff[lo] > (uint32_t)P_CAP) {
+                if (s_boff[hi] - s_boff[lo] > (uint32_t)G_CAP) {
                     for (int b = lo; b < hi; ++b)
-                        if (s_boff[b + 1] - s_boff[s_chunk[n]] > (uint32_t)P_CAP && b > (int)s_chunk[n])
+                        if (s_boff[b + 1] - s_boff[s_chunk[n]] > (uint32_t)G_CAP && b > (int)s_chunk[n])
                             s_chunk[++n] = (uint32_t)b;
                 }
             }
@@ -676,41 +722,28 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_gather(const Params* __restri
     const float dx = P.dx;
     const uint32_t nchunks = s_nchunks;
     const int cx0 = rbx0 * BUCKET_CELLS, cy0 = rby0 * BUCKET_CELLS;  // first region cell (stencil-base coords)
+    constexpr int PER = G_CAP / P_THREADS + (G_CAP % P_THREADS ? 1 : 0);  // staged particles a thread
 
     for (uint32_t ch = 0; ch < nchunks; ++ch) {
         const int b0 = (int)s_chunk[ch], b1 = (int)s_chunk[ch + 1];
         const uint32_t c0 = s_boff[b0], c1 = s_boff[b1];
         for (int e = tid; e <= P_CELLS; e += P_THREADS) s_cell[e] = 0u;
         __syncthreads();
-        // (A) load up to two particles a thread, count them per region cell
-        uint32_t rc[2], tmp[2], pid[2], mat[2];
-        float fx[2], fy[2];
-        float2 v[2];
-        float4 C[2], F[2], S[2];
-        float pv[2];
+        // (A) positions: region cell of each particle, counted per cell
+        uint32_t rc[PER], tmp[PER], pid[PER];
+        size_t gi[PER];
+        float fx[PER], fy[PER];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const uint32_t j = c0 + (uint32_t)tid + (uint32_t)(k * P_THREADS);
             rc[k] = 0xffffffffu;
-            tmp[k] = 0u; pid[k] = 0u; mat[k] = 0u;
-            fx[k] = 0.f; fy[k] = 0.f;
-            v[k] = make_float2(0.f, 0.f);
-            C[k] = make_float4(0.f, 0.f, 0.f, 0.f); F[k] = C[k]; S[k] = C[k];
-            pv[k] = 1.0f;
+            tmp[k] = 0u; pid[k] = 0u; gi[k] = 0; fx[k] = 0.f; fy[k] = 0.f;
             if (j >= c1) continue;
             const int rb = region_bucket(s_boff, b0, b1, j);
             const int gbx = rbx0 + rb % P_RB, gby = rby0 + rb / P_RB;
-            const size_t g = ((size_t)gby * bw + (size_t)gbx) * cap + (j - s_boff[rb]);
-            const float2 p = in.pos[g];
-            pid[k] = in.pid[g];
-            mat[k] = in.mat[g];
-            v[k] = in.vel[g];
-            C[k] = in.C[g];
-            F[k] = in.F[g];
-            if constexpr (GENERAL) {
-                S[k] = in.S[g];
-                pv[k] = in.plastic[g];
-            }
+            gi[k] = ((size_t)gby * bw + (size_t)gbx) * cap + (j - s_boff[rb]);
+            const float2 p = in.pos[gi[k]];
+            pid[k] = in.pid[gi[k]];
             Stencil sc;
             if (make_stencil(P.dx, P.cells_x, P.cells_y, p, sc) && ((sc.bx + 1) >> 2) == gbx &&
                 ((sc.by + 1) >> 2) == gby) {
@@ -727,73 +760,102 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_gather(const Params* __restri
         block_exclusive_scan(s_cell, P_CELLS, s_wave);
         // (C) ascending particle index inside each cell fixes the summation order
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < PER; ++k)
             if (rc[k] != 0xffffffffu) s_pid[s_cell[rc[k]] + tmp[k]] = pid[k];
         __syncthreads();
+        uint32_t li[PER];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < PER; ++k) {
+            li[k] = 0xffffffffu;
             if (rc[k] == 0xffffffffu) continue;
             const uint32_t q0 = s_cell[rc[k]], q1 = s_cell[rc[k] + 1];
-            uint32_t li = q0;
-            for (uint32_t q = q0; q < q1; ++q) li += s_pid[q] < pid[k] ? 1u : 0u;
-            const Material mt = mats[mat[k]];
-            const float det = det_comp(F[k]);
+            uint32_t r = q0;
+            for (uint32_t q = q0; q < q1; ++q) r += s_pid[q] < pid[k] ? 1u : 0u;
+            li[k] = r;
+        }
+        __syncthreads();
+        // (D) stage: prepared affine (prepareParticle, shaders.ts:808-963) and
+        // the per-particle factors of the contributions
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (li[k] == 0xffffffffu) continue;
+            const size_t g = gi[k];
+            const uint32_t mat = in.mat[g];
+            const float2 v = in.vel[g];
+            const float4 C = in.C[g];
+            const float4 F = in.F[g];
+            float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
+            float pv = 1.0f;
+            if constexpr (GENERAL) {
+                S = in.S[g];
+                pv = in.plastic[g];
+            }
+            const Material mt = material_at(s_mat, mats, mat);
+            const float det = det_comp(F);
             uint32_t pf = 0;
-            float4 Sk = S[k];
             if constexpr (GENERAL) {
                 if (!finitef(det) || det <= 0.0f || !finitef(mt.volume) || mt.volume <= 0.0f) pf = ERR_DEFORMATION;
             } else {
-                pf = particle_stress<false>(mt, F[k], pv[k], det, Sk);
+                pf = particle_stress<false>(mt, F, pv, det, S);
             }
-            const float4 A = affine_of(P, mt, C[k], Sk);
+            const float4 A = affine_of(P, mt, C, S);
             if (!pf && !finite4(A)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
+            // every stencil weight of the particle is used by some node: the
+            // per-contribution weight test of the reference, done once
+            if (!pf && !weights_valid(fx[k], fy[k])) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
             if (pf) {
                 if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
                 flags |= pf;
             }
-            s_fx[li] = fx[k];
-            s_fy[li] = fy[k];
-            s_m[li] = mt.mass;
-            s_vx[li] = v[k].x;
-            s_vy[li] = v[k].y;
-            s_a0[li] = A.x; s_a1[li] = A.y; s_a2[li] = A.z; s_a3[li] = A.w;
-            s_vol[li] = mt.volume;
-            s_det[li] = det;
-            s_tag[li] = pf ? 0xffffu : ((rc[k] % (uint32_t)P_RC) | (mt.phase << 8));
+            const uint32_t q = li[k];
+            s_fx[q] = fx[k];
+            s_fy[q] = fy[k];
+            s_m[q] = mt.mass;
+            s_mvx[q] = mt.mass * v.x;
+            s_mvy[q] = mt.mass * v.y;
+            s_a0[q] = A.x; s_a1[q] = A.y; s_a2[q] = A.z; s_a3[q] = A.w;
+            s_vd[q] = mt.volume * det;
+            s_tag[q] = pf ? 0xffffu : ((rc[k] % (uint32_t)P_RC) | (mt.phase << 8));
         }
         __syncthreads();
-        // (D) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region
+        // (E) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region.
+        // The three row ranges run as one loop, so a wave iterates over the
+        // largest per-node pair count rather than the largest per cell.
         if (node_ok) {
-#pragma unroll 1
+            uint32_t q0[3], n[3];
+#pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
-                const int row = (ly + 1 + dy) * P_RC;
-                const uint32_t qa = s_cell[row + lx + 1], qb = s_cell[row + lx + 4];
-                const int oy = 2 - dy;
+                const int row = (ly + 1 + dy) * P_RC + lx + 1;
+                q0[dy] = s_cell[row];
+                n[dy] = s_cell[row + 3] - q0[dy];
+            }
+            const uint32_t n01 = n[0] + n[1], total = n01 + n[2];
 #pragma unroll 1
-                for (uint32_t q = qa; q < qb; ++q) {
-                    const uint32_t tag = s_tag[q];
-                    if (tag == 0xffffu) continue;
-                    const uint32_t ph = tag >> 8;
-                    const int ox = lx + 3 - (int)(tag & 0xffu);
-                    const float fx = s_fx[q], fy = s_fy[q];
-                    const float w = qweight(fx, ox) * qweight(fy, oy);
-                    const float dpx = ((float)ox - fx) * dx;
-                    const float dpy = ((float)oy - fy) * dx;
-                    const float m = s_m[q];
-                    const float qx = (m * s_vx[q] + s_a0[q] * dpx) + s_a1[q] * dpy;
-                    const float qy = (m * s_vy[q] + s_a2[q] * dpx) + s_a3[q] * dpy;
-                    if (!finitef(w) || w < 0.0f || !finitef(qx) || !finitef(qy)) {
-                        flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
-                        continue;
-                    }
-                    const float wm = w * m, wqx = w * qx, wqy = w * qy;
-                    if (ph == 0u) {
-                        acc[0] += wm; acc[1] += wqx; acc[2] += wqy;
-                    } else {
-                        const float wv = (w * s_vol[q]) * s_det[q];
-                        if (ph == 1u) { acc[3] += wm; acc[4] += wqx; acc[5] += wqy; acc[6] += wv; }
-                        else { acc[7] += wm; acc[8] += wqx; acc[9] += wqy; acc[10] += wv; }
-                    }
+            for (uint32_t k = 0; k < total; ++k) {
+                const int dy = k < n[0] ? 0 : (k < n01 ? 1 : 2);
+                const uint32_t q = dy == 0 ? q0[0] + k : (dy == 1 ? q0[1] + (k - n[0]) : q0[2] + (k - n01));
+                const uint32_t tag = s_tag[q];
+                if (tag == 0xffffu) continue;
+                const uint32_t ph = tag >> 8;
+                const int ox = lx + 3 - (int)(tag & 0xffu);
+                const int oy = 2 - dy;
+                const float fx = s_fx[q], fy = s_fy[q];
+                const float w = qweight(fx, ox) * qweight(fy, oy);
+                const float dpx = ((float)ox - fx) * dx;
+                const float dpy = ((float)oy - fy) * dx;
+                const float qx = (s_mvx[q] + s_a0[q] * dpx) + s_a1[q] * dpy;
+                const float qy = (s_mvy[q] + s_a2[q] * dpx) + s_a3[q] * dpy;
+                if (!finitef(qx) || !finitef(qy)) {
+                    flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
+                    continue;
+                }
+                const float wm = w * s_m[q], wqx = w * qx, wqy = w * qy;
+                if (ph == 0u) {
+                    acc[0] += wm; acc[1] += wqx; acc[2] += wqy;
+                } else {
+                    const float wv = w * s_vd[q];
+                    if (ph == 1u) { acc[3] += wm; acc[4] += wqx; acc[5] += wqy; acc[6] += wv; }
+                    else { acc[7] += wm; acc[8] += wqx; acc[9] += wqy; acc[10] += wv; }
                 }
             }
         }
@@ -983,7 +1045,7 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restric
 // -------------------------------------------------------------------- G2P
 // integrateDeformation (shaders.ts:1459-1483)
 __device__ __forceinline__ float4 integrate_deformation(float4 F, float4 C, float dt, uint32_t substeps) {
-    const float h = dt / (float)substeps;
+    const float h = substeps == 1u ? dt : dt / (float)substeps;  // dt / 1 is dt
     const float4 u = make_float4(1.0f + h * C.x, h * C.y, h * C.z, 1.0f + h * C.w);
     float4 r = F;
     for (uint32_t k = 0; k < substeps; ++k) {
@@ -1059,35 +1121,46 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
     const float dx = P.dx;
+    // The reference tests every weight, offset and node value for
+    // finiteness here (shaders.ts:2165-2192).  When this kernel runs, the
+    // same step's P2G has accepted every weight of the particle and the grid
+    // update has accepted every node value it wrote (finite velocities,
+    // finite non-negative liquid mass and volume); nodes outside the grid are
+    // staged as zero.  The tests cannot fail, so they are not repeated.
+    float wx[3], wy[3], dpx[3], dpy[3];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        wx[o] = qweight(s.fx, o);
+        wy[o] = qweight(s.fy, o);
+        dpx[o] = ((float)o - s.fx) * dx;
+        dpy[o] = ((float)o - s.fy) * dx;
+    }
+    const int row0 = (s.by + 1 - ny0) * G_TN + (s.bx + 1 - nx0);
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy) {
-        const float wy = qweight(s.fy, oy);
-        const float dpy = ((float)oy - s.fy) * dx;
-        const int row = (s.by + 1 + oy - ny0) * G_TN + (s.bx + 1 - nx0);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
-            const float w = qweight(s.fx, ox) * wy;
-            const float dpx = ((float)ox - s.fx) * dx;
-            const float2 g = lv[row + ox];
-            if (!finitef(w) || w < 0.0f || !finitef(dpx) || !finitef(dpy) || !finite2(g)) {
-                raise_step(ctl, ERR_G2P, step);
-                return false;
-            }
-            vx = vx + w * g.x;
-            vy = vy + w * g.y;
-            if (liquid) {
-                const float2 lq = llq[row + ox];
-                if (!finitef(lq.x) || lq.x < 0.0f || !finitef(lq.y) || lq.y < 0.0f) {
-                    raise_step_at(ctl, ERR_G2P, step, pid);
-                    return false;
-                }
+            const float w = wx[ox] * wy[oy];
+            const float2 g = lv[row0 + oy * G_TN + ox];
+            const float wgx = w * g.x, wgy = w * g.y;
+            vx = vx + wgx;
+            vy = vy + wgy;
+            a0 = a0 + wgx * dpx[ox];
+            a1 = a1 + wgx * dpy[oy];
+            a2 = a2 + wgy * dpx[ox];
+            a3 = a3 + wgy * dpy[oy];
+        }
+    }
+    if (liquid) {
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy) {
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) {
+                const float w = wx[ox] * wy[oy];
+                const float2 lq = llq[row0 + oy * G_TN + ox];
                 pm = pm + w * lq.x;
                 pvol = pvol + w * lq.y;
             }
-            a0 = a0 + (w * g.x) * dpx;
-            a1 = a1 + (w * g.x) * dpy;
-            a2 = a2 + (w * g.y) * dpx;
-            a3 = a3 + (w * g.y) * dpy;
         }
     }
     if (liquid) {
@@ -1116,7 +1189,7 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     }
     const float rowsum = fmaxf(fabsf(C.x) + fabsf(C.y), fabsf(C.z) + fabsf(C.w));
     const float increment = dt * rowsum;
-    const float required = ceilf(increment / MAX_DEFORMATION_INCREMENT);
+    const float required = ceilf(increment * (1.0f / MAX_DEFORMATION_INCREMENT));  // exact: a power of two
     if (!finitef(required) || required > (float)MAX_DEFORMATION_SUBSTEPS) {
         raise_step_at(ctl, ERR_SUBSTEP_LIMIT, step, pid);
         return false;
@@ -1238,23 +1311,27 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 }
 
 // One 256-thread block per 8x8-bucket tile: stage the 34x34 node patch, run
-// G2P for the tile's particles and write each into its next bucket.
+// G2P for the tile's particles and write each into its next bucket.  The
+// inputs of a thread's next particle are fetched before the current one is
+// processed, so their latency overlaps the current G2P and slot reservation.
 template <bool GENERAL>
 __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
-                                                 SlotBuf out, const Material* __restrict__ mats, Grid grid,
-                                                 uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
+                                                 SlotBuf out, const Material* __restrict__ mats, uint32_t nmat,
+                                                 Grid grid, uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
                                                  uint32_t scene_flags, uint32_t step, int out_buf) {
     __shared__ float2 s_vs[G_TN * G_TN];
     __shared__ float2 s_vw[G_TN * G_TN];
     __shared__ float2 s_vo[G_TN * G_TN];
     __shared__ float2 s_lq[G_TN * G_TN];
     __shared__ uint32_t s_boff[G_TB * G_TB + 1];
+    __shared__ Material s_mat[LDS_MATS];
     if (block_status_set(ctl)) return;
     const Params& P = *Pp;
     const int tid = threadIdx.x;
     const int tbx = (int)(blockIdx.x % tiles_x) * G_TB, tby = (int)(blockIdx.x / tiles_x) * G_TB;
     const int nx0 = tbx * BUCKET_CELLS, ny0 = tby * BUCKET_CELLS;  // first node (= first cell) of the patch
     const int W = (int)P.node_w, H = (int)P.node_h;
+    stage_materials(s_mat, mats, nmat);
     if (tid < 64) {
         const int gbx = tbx + tid % G_TB, gby = tby + tid / G_TB;
         uint32_t c = 0u;
@@ -1271,49 +1348,65 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp
     }
     for (int e = tid; e < G_TN * G_TN; e += G_THREADS) {
         const int nx = nx0 + e % G_TN, ny = ny0 + e / G_TN;
-        float2 a = make_float2(0.f, 0.f), bwv = a, c = a, d = a;
-        if (nx < W && ny < H) {
-            const uint32_t node = (uint32_t)ny * (uint32_t)W + (uint32_t)nx;
-            if (scene_flags & SCENE_SOLID) a = grid.v_solid[node];
-            if (scene_flags & SCENE_WATER) bwv = grid.v_water[node];
-            if (scene_flags & SCENE_OIL) c = grid.v_oil[node];
-            if (scene_flags & (SCENE_WATER | SCENE_OIL)) d = grid.liquid[node];
+        const bool in_grid = nx < W && ny < H;
+        const uint32_t node = (uint32_t)ny * (uint32_t)W + (uint32_t)nx;
+        float2 vs = make_float2(0.f, 0.f), vw = vs, vo = vs, lq = vs;
+        if (in_grid) {
+            if (scene_flags & SCENE_SOLID) vs = grid.v_solid[node];
+            if (scene_flags & SCENE_WATER) vw = grid.v_water[node];
+            if (scene_flags & SCENE_OIL) vo = grid.v_oil[node];
+            if (scene_flags & (SCENE_WATER | SCENE_OIL)) lq = grid.liquid[node];
         }
-        s_vs[e] = a;
-        s_vw[e] = bwv;
-        s_vo[e] = c;
-        s_lq[e] = d;
+        if (scene_flags & SCENE_SOLID) s_vs[e] = vs;
+        if (scene_flags & SCENE_WATER) s_vw[e] = vw;
+        if (scene_flags & SCENE_OIL) s_vo[e] = vo;
+        if (scene_flags & (SCENE_WATER | SCENE_OIL)) s_lq[e] = lq;
     }
     __syncthreads();
     const uint32_t total = s_boff[64];
+
+    // inputs of the thread's current (c_) and next (n_) particle
+    float2 c_p = make_float2(0.f, 0.f), n_p = c_p;
+    float4 c_F = make_float4(0.f, 0.f, 0.f, 0.f), n_F = c_F;
+    uint32_t c_pid = 0u, c_mat = 0u, n_pid = 0u, n_mat = 0u;
+    float c_pv = 1.0f, n_pv = 1.0f;
+    int c_gbx = 0, c_gby = 0, n_gbx = 0, n_gby = 0;
+#define G2P_FETCH(J, PFX)                                                              \
+    do {                                                                               \
+        const uint32_t j_ = (J);                                                       \
+        if (j_ < total) {                                                              \
+            int lo = 0, hi = 63;                                                       \
+            while (lo < hi) {                                                          \
+                const int mid = (lo + hi + 1) >> 1;                                    \
+                if (s_boff[mid] <= j_) lo = mid; else hi = mid - 1;                    \
+            }                                                                          \
+            PFX##gbx = tbx + lo % G_TB;                                                \
+            PFX##gby = tby + lo / G_TB;                                                \
+            const size_t g_ = ((size_t)PFX##gby * bw + (size_t)PFX##gbx) * cap + (j_ - s_boff[lo]); \
+            PFX##p = in.pos[g_];                                                       \
+            PFX##F = in.F[g_];                                                         \
+            PFX##pid = in.pid[g_];                                                     \
+            PFX##mat = in.mat[g_];                                                     \
+            if constexpr (GENERAL) PFX##pv = in.plastic[g_];                           \
+        }                                                                              \
+    } while (0)
+    G2P_FETCH((uint32_t)tid, c_);
     for (uint32_t base = 0; base < total; base += G_THREADS) {
-        const uint32_t j = base + (uint32_t)tid;
-        bool ok = j < total;
+        G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
+        bool ok = base + (uint32_t)tid < total;
         G2POut o;
-        uint32_t pid = 0, mat = 0, dest = 0;
+        uint32_t dest = 0;
         float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
         if (ok) {
-            int lo = 0, hi = 63;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_boff[mid] <= j) lo = mid; else hi = mid - 1;
-            }
-            const int gbx = tbx + lo % G_TB, gby = tby + lo / G_TB;
-            const size_t g = ((size_t)gby * bw + (size_t)gbx) * cap + (j - s_boff[lo]);
-            const float2 p = in.pos[g];
-            const float4 F = in.F[g];
-            pid = in.pid[g];
-            mat = in.mat[g];
-            const float pv = GENERAL ? in.plastic[g] : 1.0f;
-            const Material mt = mats[mat];
+            const Material mt = material_at(s_mat, mats, c_mat);
             Stencil s;
-            if (!make_stencil(P.dx, P.cells_x, P.cells_y, p, s) || ((s.bx + 1) >> 2) != gbx ||
-                ((s.by + 1) >> 2) != gby) {
+            if (!make_stencil(P.dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
+                ((s.by + 1) >> 2) != c_gby) {
                 raise_step(ctl, ERR_STENCIL, step);
                 ok = false;
             } else {
                 const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u ? s_vo : s_vs);
-                ok = g2p_particle<GENERAL>(P, ctl, mt, pid, p, F, pv, s, lv, s_lq, nx0, ny0, step, o);
+                ok = g2p_particle<GENERAL>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o);
             }
             if (ok) {
                 Stencil ns;
@@ -1330,7 +1423,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp
                     // microstep that will start from it
                     const uint32_t pf = particle_stress<true>(mt, o.F, o.plastic, det_comp(o.F), S);
                     if (pf) {
-                        if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], pid);
+                        if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
                         atomicOr(&ctl->deferred[out_buf], pf);
                     }
                 }
@@ -1346,15 +1439,17 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp
                 out.vel[g] = o.vel;
                 out.C[g] = o.C;
                 out.F[g] = o.F;
-                out.pid[g] = pid;
-                out.mat[g] = mat;
+                out.pid[g] = c_pid;
+                out.mat[g] = c_mat;
                 if constexpr (GENERAL) {
                     out.plastic[g] = o.plastic;
                     out.S[g] = S;
                 }
             }
         }
+        c_p = n_p; c_F = n_F; c_pid = n_pid; c_mat = n_mat; c_pv = n_pv; c_gbx = n_gbx; c_gby = n_gby;
     }
+#undef G2P_FETCH
 }
 
 // ------------------------------------------------------------- batch end
@@ -1453,11 +1548,11 @@ hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool
         } else {
             auto p2g = general ? k_p2g_gather<true> : k_p2g_gather<false>;
             MPM_LAUNCH("p2g_grid", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
-                       out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur);
+                       out.cnt, lc.mats, lc.nmat, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur);
         }
         auto g2p = general ? k_g2p<true> : k_g2p<false>;
         MPM_LAUNCH("g2p", g2p, dim3(gtx * gty), dim3(G_THREADS), 0, lc.stream, lc.params, lc.ctl, in, out, lc.mats,
-                   lc.grid, gtx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur ^ 1);
+                   lc.nmat, lc.grid, gtx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur ^ 1);
         cur ^= 1;
     }
     MPM_LAUNCH("commit_last", k_commit_last, dim3(1), dim3(64), 0, lc.stream, lc.ctl, steps, seq);
