@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 256 << 20  # calibration copy of the counter runs
 METRIC = "particle-steps/s @1M particles + %HBM peak; batched & adjoint vs plain step"
 # Algorithmic bytes (float32 SoA): per particle-microstep, read x, v, C, F,
 # volume, material id (56 B) and write x, v, C, F (48 B); per grid node and
@@ -105,6 +106,25 @@ def cpu_baseline(pkg, arr, config, budget_s=12.0):
     return len(arr) * steps / el, steps, el
 
 
+def pmc_sample(pkg, arr, config, mode):
+    """Workload of the FETCH_SIZE / WRITE_SIZE counter runs.  The copy kernel
+    reads and writes COPY_BYTES each, so its counter values per byte scale
+    the counters of the step kernels to HBM bytes."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"[bench] calibration copy of {COPY_BYTES} bytes done")
+    for m in (["gather", "fixed"] if mode == "both" else [mode]):
+        solver = build_solver(pkg, arr, config, m)
+        plan, limit = plan_for(pkg, solver, config)
+        run_steps(pkg, solver, plan, limit, 8)
+        solver.ctx.read_compact()
+        solver.dispose()
+        log(f"[bench] {m}: 8 microsteps sampled")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +133,9 @@ def main():
     ap.add_argument("--cells", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=["both", "gather", "fixed"], default="both")
+    ap.add_argument("--pmc-sample", action="store_true",
+                    help="counter-collection workload: one 256 MiB copy kernel (the HBM byte calibration), then "
+                         "8 untimed microsteps per mode; prints nothing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (each GPU box has one MI355X)",
@@ -122,6 +145,9 @@ def main():
     pkg = __graft_entry__.load_package()
     log(f"[bench] building scene: {args.cells}x{args.cells} cells")
     arr, config = pkg.scenes.water_oil_column(cells=args.cells)
+    if args.pmc_sample:
+        pmc_sample(pkg, arr, config, args.mode)
+        return
     n = len(arr)
     nodes = (args.cells + 3) ** 2
 

@@ -501,6 +501,15 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     return MPM_OK;
 }
 
+// RN(1/b) for the fixed-point kernel's division by particleMass and
+// particleVolume (reciprocal plus one FMA correction, exact for divisors in
+// [2^-20, 2^20]); 0 outside that range selects plain division.
+static float reciprocal_for_division(float b) {
+    if (!(b >= 0x1p-20f && b <= 0x1p20f)) return 0.0f;
+    volatile float one = 1.0f;
+    return one / b;
+}
+
 int mpm_step_batch(mpm_ctx* c, const void* params144, uint32_t step_count, uint32_t measure_speed) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     if (!params144) return fail(c, MPM_EINVAL, "params must not be NULL");
@@ -523,6 +532,9 @@ int mpm_step_batch(mpm_ctx* c, const void* params144, uint32_t step_count, uint3
     if (!isfinite(p.grad_scale) || !(p.grad_scale > 0.0f)) return fail(c, MPM_EINVAL, "params gradient scale is invalid");
     if (p.cfl_on >= 0.5f && (!isfinite(p.wave_bound) || !isfinite(p.vel_bound) || p.vel_bound < 0.0f))
         return fail(c, MPM_EINVAL, "CFL bounds must be finite and non-negative");
+    p.inv_mass_ref = reciprocal_for_division(p.particle_mass);
+    p.inv_volume_ref = reciprocal_for_division(p.particle_volume);
+    p.wall_pad = 0.0f;
     p.drag_target_x = c->interaction[0];
     p.drag_target_y = c->interaction[1];
     p.drag_radius = c->interaction[2];

@@ -56,7 +56,9 @@ struct Params {
     float bond_rest, bond_stiffness, band, reach;              // bonding
     float wave_bound, vel_bound, cfl_on, bond_ref_dt;          // cflBounds
     uint32_t fracture[4];                                      // fracture
-    float wall_friction, wall_pad[3];                          // wall
+    float wall_friction;                                       // wall
+    float inv_mass_ref, inv_volume_ref;  // RN(1/particleMass), RN(1/particleVolume) or 0 (host-set)
+    float wall_pad;
     float drag_target_x, drag_target_y, drag_radius, drag_stiffness; // interaction
     float drag_damping, drag_pad[3];
 };

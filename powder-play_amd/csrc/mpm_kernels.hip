@@ -57,7 +57,7 @@ constexpr int P_NB = P_RB * P_RB;
 constexpr int P_RC = P_RB * BUCKET_CELLS;       // region cells per side
 constexpr int P_CELLS = P_RC * P_RC;
 constexpr int P_THREADS = 1024;
-constexpr int G_CAP = 1536;                     // staged particles per gather chunk
+constexpr int G_CAP = 1408;                     // staged particles per gather chunk
 static_assert(G_CAP <= 2 * P_THREADS, "the gather loads at most two particles a thread");
 static_assert(P_TC * P_TC <= G_CAP, "the ring volumes reuse two staging arrays");
 constexpr int LDS_MATS = 64;                    // material rows cached in LDS
@@ -314,6 +314,17 @@ __device__ __forceinline__ uint32_t bucket_of(const Stencil& s, uint32_t bw) {
     return (uint32_t)((s.by + 1) >> 2) * bw + (uint32_t)((s.bx + 1) >> 2);
 }
 
+// Tile of this block.  Workgroups are dispatched round-robin over the eight
+// XCDs, each with its own L2; handing every XCD a contiguous band of tiles
+// (in tile order) keeps neighbouring tiles, which re-read each other's halo
+// particles and nodes, on the same L2.
+constexpr uint32_t XCDS = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t tiles) {
+    const uint32_t b = blockIdx.x, x = b % XCDS, k = b / XCDS;
+    const uint32_t per = tiles / XCDS, extra = tiles % XCDS;
+    return x * per + min(x, extra) + k;
+}
+
 // ------------------------------------------------------------ batch frame
 __global__ void k_batch_begin(Control* ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -437,6 +448,18 @@ __device__ __forceinline__ float wall_friction(float v, int coord, int extent, f
     float cushion = fminf(WALL_CUSHION_CELLS, (float)extent / 4.0f);
     float infl = 1.0f - fminf(fmaxf(dist / cushion, 0.0f), 1.0f);
     return v * (1.0f - friction * infl);
+}
+
+// a / b for the fixed-point contributions (b = particleMass or
+// particleVolume).  With r = RN(1/b) and b in [2^-20, 2^20] (the host passes
+// r = 0 otherwise) Markstein's correction q + fma(-q, b, a) * r returns the
+// IEEE quotient for |a| in [2^-60, 2^100]; outside that range the quotient
+// rounds to the same fixed-point word or raises the same overflow flag.
+__device__ __forceinline__ float div_ref(float a, float b, float r) {
+    if (r == 0.0f) return a / b;
+    const float q = a * r;
+    const float e = fmaf(-q, b, a);
+    return fmaf(e, r, q);
 }
 
 __device__ __forceinline__ uint32_t fixed_round(float value, int32_t& out) {
@@ -577,7 +600,8 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
         ctl->deferred[out_buf] = 0u;
         ctl->deferred_index[out_buf] = 0xffffffffu;
     }
-    const int tx = (int)(blockIdx.x % tiles_x), ty = (int)(blockIdx.x / tiles_x);
+    const uint32_t tile = xcd_tile(gridDim.x);
+    const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     // this tile owns buckets [7tx, 7tx+7) x [7ty, 7ty+7): clear their next counts
     if (tid < 49) {
         const int obx = 7 * tx + tid % 7, oby = 7 * ty + tid / 7;
@@ -681,13 +705,14 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
     __shared__ float s_fx[G_CAP], s_fy[G_CAP], s_m[G_CAP], s_mvx[G_CAP], s_mvy[G_CAP];
     __shared__ float s_a0[G_CAP], s_a1[G_CAP], s_a2[G_CAP], s_a3[G_CAP], s_vd[G_CAP];
     __shared__ uint32_t s_tag[G_CAP];  // region column | phase << 8; 0xffff = contributes nothing
-    uint32_t* s_pid = s_tag;           // particle indices at provisional places, before staging
+    __shared__ uint32_t s_pid[G_CAP];  // particle indices at provisional places
 
     stage_materials(s_mat, mats, nmat);
     if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     const int tid = threadIdx.x;
     const Params& P = *Pp;
-    const int tx = (int)(blockIdx.x % tiles_x), ty = (int)(blockIdx.x / tiles_x);
+    const uint32_t tile = xcd_tile(gridDim.x);
+    const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     const int lx = tid % 32, ly = tid / 32;
     const int X0 = tx * P_TI, Y0 = ty * P_TI;
     const int W = (int)P.node_w, H = (int)P.node_h;
@@ -730,27 +755,23 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
         for (int e = tid; e <= P_CELLS; e += P_THREADS) s_cell[e] = 0u;
         __syncthreads();
         // (A) positions: region cell of each particle, counted per cell
-        uint32_t rc[PER], tmp[PER], pid[PER];
-        size_t gi[PER];
-        float fx[PER], fy[PER];
+        uint32_t cell[PER], pid[PER], gi[PER];  // cell: region cell << 16 | place in the cell
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint32_t j = c0 + (uint32_t)tid + (uint32_t)(k * P_THREADS);
-            rc[k] = 0xffffffffu;
-            tmp[k] = 0u; pid[k] = 0u; gi[k] = 0; fx[k] = 0.f; fy[k] = 0.f;
+            cell[k] = 0xffffffffu;
+            pid[k] = 0u; gi[k] = 0u;
             if (j >= c1) continue;
             const int rb = region_bucket(s_boff, b0, b1, j);
             const int gbx = rbx0 + rb % P_RB, gby = rby0 + rb / P_RB;
-            gi[k] = ((size_t)gby * bw + (size_t)gbx) * cap + (j - s_boff[rb]);
+            gi[k] = ((uint32_t)gby * bw + (uint32_t)gbx) * cap + (j - s_boff[rb]);
             const float2 p = in.pos[gi[k]];
             pid[k] = in.pid[gi[k]];
             Stencil sc;
             if (make_stencil(P.dx, P.cells_x, P.cells_y, p, sc) && ((sc.bx + 1) >> 2) == gbx &&
                 ((sc.by + 1) >> 2) == gby) {
-                fx[k] = sc.fx;
-                fy[k] = sc.fy;
-                rc[k] = (uint32_t)((sc.by + 1 - cy0) * P_RC + (sc.bx + 1 - cx0));
-                tmp[k] = atomicAdd(&s_cell[rc[k]], 1u);
+                const uint32_t rc = (uint32_t)((sc.by + 1 - cy0) * P_RC + (sc.bx + 1 - cx0));
+                cell[k] = rc << 16 | atomicAdd(&s_cell[rc], 1u);
             } else {
                 flags |= ERR_STENCIL;
             }
@@ -758,28 +779,24 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
         __syncthreads();
         // (B) cell starts
         block_exclusive_scan(s_cell, P_CELLS, s_wave);
-        // (C) ascending particle index inside each cell fixes the summation order
+        // (C) particle indices at their provisional places
 #pragma unroll
         for (int k = 0; k < PER; ++k)
-            if (rc[k] != 0xffffffffu) s_pid[s_cell[rc[k]] + tmp[k]] = pid[k];
+            if (cell[k] != 0xffffffffu) s_pid[s_cell[cell[k] >> 16] + (cell[k] & 0xffffu)] = pid[k];
         __syncthreads();
-        uint32_t li[PER];
+        // (D) rank by particle index inside the cell (the summation order),
+        // then stage the prepared affine (prepareParticle, shaders.ts:808-963)
+        // and the per-particle factors of the contributions
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            li[k] = 0xffffffffu;
-            if (rc[k] == 0xffffffffu) continue;
-            const uint32_t q0 = s_cell[rc[k]], q1 = s_cell[rc[k] + 1];
-            uint32_t r = q0;
-            for (uint32_t q = q0; q < q1; ++q) r += s_pid[q] < pid[k] ? 1u : 0u;
-            li[k] = r;
-        }
-        __syncthreads();
-        // (D) stage: prepared affine (prepareParticle, shaders.ts:808-963) and
-        // the per-particle factors of the contributions
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            if (li[k] == 0xffffffffu) continue;
-            const size_t g = gi[k];
+            if (cell[k] == 0xffffffffu) continue;
+            const uint32_t rc = cell[k] >> 16;
+            const uint32_t q0 = s_cell[rc], q1 = s_cell[rc + 1];
+            uint32_t q = q0;
+            for (uint32_t t = q0; t < q1; ++t) q += s_pid[t] < pid[k] ? 1u : 0u;
+            const uint32_t g = gi[k];
+            Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
+            make_stencil(P.dx, P.cells_x, P.cells_y, in.pos[g], sc);
             const uint32_t mat = in.mat[g];
             const float2 v = in.vel[g];
             const float4 C = in.C[g];
@@ -802,20 +819,19 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
             if (!pf && !finite4(A)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
             // every stencil weight of the particle is used by some node: the
             // per-contribution weight test of the reference, done once
-            if (!pf && !weights_valid(fx[k], fy[k])) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
+            if (!pf && !weights_valid(sc.fx, sc.fy)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
             if (pf) {
                 if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
                 flags |= pf;
             }
-            const uint32_t q = li[k];
-            s_fx[q] = fx[k];
-            s_fy[q] = fy[k];
+            s_fx[q] = sc.fx;
+            s_fy[q] = sc.fy;
             s_m[q] = mt.mass;
             s_mvx[q] = mt.mass * v.x;
             s_mvy[q] = mt.mass * v.y;
             s_a0[q] = A.x; s_a1[q] = A.y; s_a2[q] = A.z; s_a3[q] = A.w;
             s_vd[q] = mt.volume * det;
-            s_tag[q] = pf ? 0xffffu : ((rc[k] % (uint32_t)P_RC) | (mt.phase << 8));
+            s_tag[q] = pf ? 0xffffu : ((rc % (uint32_t)P_RC) | (mt.phase << 8));
         }
         __syncthreads();
         // (E) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region.
@@ -887,30 +903,35 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
 // gather, but every particle of the region adds its rounded int32
 // contributions that land on the tile or its ring into 64-bit LDS words with
 // integer atomics: integer sums do not depend on the order, and an exact sum
-// outside int32 is the accumulator overflow of 65536.
-template <bool GENERAL>
-__global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
+// outside int32 is the accumulator overflow of 65536.  Scenes without solids
+// keep only the eight liquid channels, so two tiles fit in one CU's LDS.
+template <bool GENERAL, bool SOLIDS>
+__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_p2g_fixed(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
                                                        uint32_t* __restrict__ cnt_next,
                                                        const Material* __restrict__ mats, Grid grid,
                                                        uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
                                                        uint32_t keep_fixed, uint32_t scene_flags, uint32_t step,
                                                        int in_buf) {
-    // channels: solid m, qx, qy | water m, qx, qy, V | oil m, qx, qy, V
-    __shared__ unsigned long long s_acc[11][P_TC * P_TC];
+    // words [solid m, qx, qy |] water m, qx, qy, V | oil m, qx, qy, V
+    constexpr int NCH = SOLIDS ? 11 : 8;
+    constexpr int SKIP = SOLIDS ? 0 : 3;  // channel index = word index - SKIP
+    __shared__ unsigned long long s_acc[NCH][P_TC * P_TC];
     __shared__ uint32_t s_boff[P_NB + 1];
 
-    for (int e = threadIdx.x; e < 11 * P_TC * P_TC; e += P_THREADS) (&s_acc[0][0])[e] = 0ull;
+    for (int e = threadIdx.x; e < NCH * P_TC * P_TC; e += P_THREADS) (&s_acc[0][0])[e] = 0ull;
     if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     __syncthreads();
     const int tid = threadIdx.x;
     const Params& P = *Pp;
-    const int tx = (int)(blockIdx.x % tiles_x), ty = (int)(blockIdx.x / tiles_x);
+    const uint32_t tile = xcd_tile(gridDim.x);
+    const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     const int X0 = tx * P_TI, Y0 = ty * P_TI;
     const int W = (int)P.node_w, H = (int)P.node_h;
     const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
 
     uint32_t flags = 0;
     const float mref = P.particle_mass, vref = P.particle_volume, dx = P.dx;
+    const float rm = P.inv_mass_ref, rv = P.inv_volume_ref;
     const uint32_t total = s_boff[P_NB];
     for (uint32_t base = 0; base < total; base += 2u * P_THREADS) {
         // load two particles a thread first, so both loads are in flight
@@ -974,7 +995,11 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restric
                 continue;
             }
             const uint32_t ph = mt.phase;
-            const int c0 = ph == 0u ? 0 : (ph == 1u ? 3 : 7);
+            if (!SOLIDS && ph == 0u) {  // the scene flags list no solid: never taken
+                flags |= ERR_P2G | ERR_INVALID_MATERIAL;
+                continue;
+            }
+            const int c0 = (ph == 0u ? 0 : (ph == 1u ? 3 : 7)) - SKIP;
             const float m = mt.mass;
             const float mvx = m * v[k].x, mvy = m * v[k].y;
             const float vd = mt.volume * det;
@@ -1003,10 +1028,10 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restric
                     }
                     const int node = ly * P_TC + lx;
                     int32_t im, iqx, iqy, iv = 0;
-                    flags |= fixed_round((w * m) / mref, im);
-                    flags |= fixed_round((w * qx) / mref, iqx);
-                    flags |= fixed_round((w * qy) / mref, iqy);
-                    if (ph != 0u) flags |= fixed_round((w * vd) / vref, iv);
+                    flags |= fixed_round(div_ref(w * m, mref, rm), im);
+                    flags |= fixed_round(div_ref(w * qx, mref, rm), iqx);
+                    flags |= fixed_round(div_ref(w * qy, mref, rm), iqy);
+                    if (ph != 0u) flags |= fixed_round(div_ref(w * vd, vref, rv), iv);
                     atomicAdd(&s_acc[c0][node], (unsigned long long)(long long)im);
                     atomicAdd(&s_acc[c0 + 1][node], (unsigned long long)(long long)iqx);
                     atomicAdd(&s_acc[c0 + 2][node], (unsigned long long)(long long)iqy);
@@ -1018,9 +1043,13 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restric
     __syncthreads();
 
     // gatherAtomicGrid decode (shaders.ts:1045-1070) of the wrapped words
-    auto decoded = [&](int k, int node) {
-        const int32_t word = (int32_t)(uint32_t)s_acc[k][node];
-        return ((float)word / FIXED_POINT_SCALE) * (k == 6 || k == 10 ? vref : mref);
+    auto sum_at = [&](int word, int node) -> long long {
+        if (word < SKIP) return 0;
+        return (long long)s_acc[word - SKIP][node];
+    };
+    auto decoded = [&](int word, int node) {
+        const int32_t w32 = (int32_t)(uint32_t)(unsigned long long)sum_at(word, node);
+        return ((float)w32 / FIXED_POINT_SCALE) * (word == 6 || word == 10 ? vref : mref);
     };
     const int lx = tid % 32, ly = tid / 32;
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
@@ -1030,7 +1059,7 @@ __global__ __launch_bounds__(P_THREADS) void k_p2g_fixed(const Params* __restric
         float f[11];
 #pragma unroll
         for (int k = 0; k < 11; ++k) {
-            const long long sum = (long long)s_acc[k][ring];
+            const long long sum = sum_at(k, ring);
             if (sum > 2147483647LL || sum < -2147483648LL) flags |= ERR_FIXED_POINT;
             if (keep_fixed) grid.fixed[(size_t)node * 11 + k] = (int32_t)(uint32_t)(unsigned long long)sum;
             f[k] = decoded(k, ring);
@@ -1328,7 +1357,8 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp
     if (block_status_set(ctl)) return;
     const Params& P = *Pp;
     const int tid = threadIdx.x;
-    const int tbx = (int)(blockIdx.x % tiles_x) * G_TB, tby = (int)(blockIdx.x / tiles_x) * G_TB;
+    const uint32_t tile = xcd_tile(gridDim.x);
+    const int tbx = (int)(tile % tiles_x) * G_TB, tby = (int)(tile / tiles_x) * G_TB;
     const int nx0 = tbx * BUCKET_CELLS, ny0 = tby * BUCKET_CELLS;  // first node (= first cell) of the patch
     const int W = (int)P.node_w, H = (int)P.node_h;
     stage_materials(s_mat, mats, nmat);
@@ -1542,7 +1572,9 @@ hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool
         const SlotBuf& in = lc.buf[cur];
         const SlotBuf& out = lc.buf[cur ^ 1];
         if (lc.fixed_point) {
-            auto p2g = general ? k_p2g_fixed<true> : k_p2g_fixed<false>;
+            const bool solids = (lc.scene_flags & SCENE_SOLID) != 0u;
+            auto p2g = general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
+                               : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
             MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
                        out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.keep_fixed, lc.scene_flags, s, cur);
         } else {

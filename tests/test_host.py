@@ -203,3 +203,30 @@ def test_numerical_error_decode(pkg):  # solver.ts:232-284
     assert err.cfl_details == {"particleSpeed": 3.0, "waveSpeed": 20.0, "velocityLimit": 2.5, "waveSpeedLimit": 20.5}
     assert decode_numerical_error([0] * 8) is None
     assert decode_numerical_error([2, 0xFFFFFFFF, 0, 0, 0, 0, 1, 0]).particle_index is None
+
+
+def test_fixed_point_division_by_reciprocal_matches_ieee(tmp_path):
+    """The fixed-point P2G divides by particleMass / particleVolume (when in
+    [2^-20, 2^20]) with q = a*r, q' = fma(fma(-q, b, a), r, q), r = RN(1/b)
+    (Markstein's correction).  Pin it against the IEEE quotient: equal for
+    numerators in [2^-60, 2^100], and the same fixed-point outcome
+    (rint(q * 2^24), or the overflow flag) for all numerators."""
+    import ctypes
+    import subprocess
+    src = ROOT / "tests" / "fastdiv_check.c"
+    lib = tmp_path / "fastdiv.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC", "-o", str(lib),
+                    str(src), "-lm"], check=True)
+    so = ctypes.CDLL(str(lib))
+    so.fastdiv_check.restype = ctypes.c_uint64
+    so.fastdiv_check.argtypes = [ctypes.c_float, ctypes.c_uint64, ctypes.c_uint64,
+                                 ctypes.POINTER(ctypes.c_uint64)]
+    rng = np.random.default_rng(7)
+    # the kernel takes this path for divisors in [2^-20, 2^20] (else IEEE division)
+    divisors = [0.02, 0.016, 1.0, 0.05, 0.1, 1.25e-4, 3.0, 0.3, 1.0 / 3.0, 6.25e-6, 2.0 ** -20, 2.0 ** 20]
+    divisors += [float(x) for x in np.float32(2.0 ** rng.uniform(-20, 20, 24))]
+    for b in divisors:
+        mismatch = ctypes.c_uint64(0)
+        bad = so.fastdiv_check(b, 12345, 200000, ctypes.byref(mismatch))
+        so.fastdiv_example.restype = ctypes.c_float
+        assert bad == 0 and mismatch.value == 0, (b, bad, mismatch.value, so.fastdiv_example())
