@@ -31,15 +31,31 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 256 << 20  # calibration copy of the counter runs
 METRIC = "particle-steps/s @1M particles + %HBM peak; batched & adjoint vs plain step"
-# Algorithmic bytes (float32 SoA): per particle-microstep, read x, v, C, F,
-# volume, material id (56 B) and write x, v, C, F (48 B); per grid node and
-# liquid phase present, 56 B.
+# Algorithmic bytes per microstep (float32 SoA): per particle, read x, v, C,
+# F, volume and material id (56 B) and write x, v, C, F (48 B); per grid node
+# and liquid phase present, 56 B (P2G writes m, qx, qy, V: 16; the grid
+# update reads them, 16, and writes v, 8; G2P reads v, m, V, 16).  Each
+# kernel is charged its part of that budget: the fused P2G + grid update the
+# particle reads and 40 B per node and phase, G2P the particle writes and the
+# 16 B per node and phase it reads.
 PARTICLE_BYTES = 104
 NODE_PHASE_BYTES = 56
-# P2G + grid-update kernel: particle reads of P2G (56 B) and the node
-# outputs it writes (velocity 8 B per phase + liquid mass/volume 8 B).
-P2G_PARTICLE_BYTES = 56
-G2P_PARTICLE_BYTES = 48 + 32  # writes x, v, C, F; reads x, F, volume and material id
+P2G_PARTICLE_BYTES, P2G_NODE_PHASE_BYTES = 56, 40
+G2P_PARTICLE_BYTES, G2P_NODE_PHASE_BYTES = 48, 16
+TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_water_oil_1m.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of each step kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the calibration copy), or
+    an empty dict when there is none."""
+    import re
+    if not TRAFFIC_PROFILE.exists():
+        return {}
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", line)}
+    return {}
 
 
 def log(msg: str):
@@ -183,13 +199,14 @@ def main():
     kt = kernel_times[main_mode]
     p2g_name = "p2g_grid" if main_mode == "gather" else "p2g_grid_fixed"
     phases = 2  # water and oil
-    p2g_bytes = P2G_PARTICLE_BYTES * n + (8 * phases + 8) * nodes
-    g2p_bytes = G2P_PARTICLE_BYTES * n + (8 * phases + 8) * nodes
+    p2g_bytes = P2G_PARTICLE_BYTES * n + P2G_NODE_PHASE_BYTES * phases * nodes
+    g2p_bytes = G2P_PARTICLE_BYTES * n + G2P_NODE_PHASE_BYTES * phases * nodes
     cand = {p2g_name: (p2g_bytes, kt.get(p2g_name, {}).get("mean_us")),
             "g2p": (g2p_bytes, kt.get("g2p", {}).get("mean_us"))}
     dom = max(cand, key=lambda k: cand[k][1] or 0.0)
     dom_bytes, dom_us = cand[dom]
     achieved = dom_bytes / (dom_us * 1e-6) / 1e9 if dom_us else None
+    traffic = measured_traffic().get(dom) if args.cells == 1000 else None
     step_bytes = PARTICLE_BYTES * n + NODE_PHASE_BYTES * phases * nodes
     step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
 
@@ -222,8 +239,9 @@ def main():
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_us": dom_us},
+                     "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_us": dom_us,
+                     "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)) if traffic else None},
         "kernels_us": kernel_times,
         "cpu_baseline": cpu,
     }

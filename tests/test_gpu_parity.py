@@ -9,18 +9,22 @@
 * overflow regression (transfer-regression.ts:97-170): status 65536 and a
   bitwise rollback;
 * CFL guard failure: status 16384, progress count and rollback;
+* committed golden fixtures (tests/golden): the five-material transfer states,
+  fixed-point words of three particles, one SI Tait tank macrostep;
 * 1M-particle water/oil column: determinism of both paths, conservation and
   oracle parity of a step.
 """
 from __future__ import annotations
 
 import math
+import pathlib
 
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden" / "golden.npz"
 STATE_COLS = ("x", "y", "vx", "vy", "c00", "c01", "c10", "c11", "f00", "f01", "f10", "f11")
 
 
@@ -214,12 +218,14 @@ def test_adaptive_cfl_macrostep_tait_tank(pkg):
         s.dispose()
 
 
-def test_tait_tank_fixed_dt_vs_fp64(pkg, orc):
-    """224 fixed microsteps of the SI Tait tank (the CFL plan of its first
-    macrostep) in float32 on the GPU vs the FP64 oracle."""
-    particles, config, macro_dt, _ = pkg.scenes.tait_tank()
-    steps = 224
-    dt = macro_dt / steps
+def test_tait_tank_macrostep_vs_golden(pkg):
+    """One 1/120 s macrostep (224 fixed microsteps, the CFL plan of the first
+    macrostep) of the SI Tait tank in float32 on the GPU vs the committed FP64
+    fixture (tests/golden/golden.npz, tank_*)."""
+    g = np.load(GOLDEN)
+    dt, steps = float(g["tank_cfg"][0]), int(g["tank_cfg"][1])
+    particles, config, _, _ = pkg.scenes.tait_tank()
+    assert np.array_equal(pkg.layout.validate_input(particles, config).packed, g["tank_in"])
     s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
     try:
         s.step_batch(steps, dt, pkg.scenes.TAIT_TANK_GRAVITY)
@@ -227,16 +233,43 @@ def test_tait_tank_fixed_dt_vs_fp64(pkg, orc):
         recs, _ = s.read_particles(packed=True)
     finally:
         s.dispose()
-    state = pkg.layout.validate_input(particles, config)
-    ocfg = orc.make_config(float(np.float32(dt)), 80, 80, float(np.float32(0.0025)),
-                           float(np.float32(config["particleMass"])), float(np.float32(config["particleVolume"])),
-                           (0.0, float(np.float32(-9.81))), 1.0)
-    expected = orc.run_f64(orc.particles_from_packed(state.packed), ocfg, steps)
-    pos_err = float(np.max(np.abs(as_f64(recs)[:, :2] - oracle_state(orc, expected)[:, :2])))
-    vel_err = float(np.max(np.abs(as_f64(recs)[:, 2:4] - oracle_state(orc, expected)[:, 2:4])))
+    pos_err = float(np.max(np.abs(as_f64(recs)[:, :2] - g["tank_out"][:, :2])))
+    vel_err = float(np.max(np.abs(as_f64(recs)[:, 2:4] - g["tank_out"][:, 2:4])))
     print(f"tait tank {steps} microsteps: max |dx| = {pos_err:.3e} m, max |dv| = {vel_err:.3e} m/s")
     assert pos_err <= 1e-4 * 0.2   # 1e-4 of the 0.2 m box
     assert vel_err <= 1e-3 * 1.0   # 1e-3 of the ~1 m/s speed scale
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+@pytest.mark.parametrize("name", ["five_1", "five_9", "five_45"])
+def test_golden_transfer_fixtures(pkg, mode, name):
+    """The committed 1-, 9- and 45-particle five-material fixtures: 8 steps on
+    the GPU within the reference's 1e-3 (transfer-regression.ts:29)."""
+    g = np.load(GOLDEN)
+    particles = pkg.layout.unpack_particles(g[name + "_in"])
+    _, config = pkg.scenes.five_model_block()
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    recs = run_gpu(pkg, factory, particles, config, 8, 1e-3, {"x": 0.0, "y": 0.0})
+    err = float(np.max(np.abs(as_f64(recs) - g[name + "_out"])))
+    print(f"{name} {mode}: max |GPU - golden| = {err:.3e}")
+    assert err <= 1e-3
+
+
+def test_golden_fixed_point_words(pkg):
+    """Three water particles: the int32 2^24 P2G words equal the committed
+    fixture bit for bit."""
+    g = np.load(GOLDEN)
+    particles = pkg.layout.unpack_particles(g["fixed3_in"])
+    config = {"grid": {"cellsX": 16, "cellsY": 16, "cellSize": 1.0}, "particleMass": 0.02, "particleVolume": 1.0,
+              "material": {"model": "water", "lambda": 8.0, "mu": 0.0, "density": 0.02}}
+    s = pkg.create_experimental_mpm_mls_atomic_gpu_solver(particles, config, device=0, keep_fixed_grid=True)
+    try:
+        s.step(1e-3, {"x": 0.0, "y": 0.0})
+        s.check_numerical_status()
+        words = s.read_fixed_grid()
+    finally:
+        s.dispose()
+    assert np.array_equal(words, g["fixed3_words"])
 
 
 def test_column_1m_deterministic_conservative_and_oracle_close(pkg, orc):

@@ -5,6 +5,7 @@ fixed-point words, brute-force binning).  CPU only."""
 from __future__ import annotations
 
 import math
+import pathlib
 
 import numpy as np
 import pytest
@@ -269,3 +270,37 @@ def test_binning_matches_brute_force(orc, pkg):
         brute = [i for i in range(len(arr))
                  if math.floor(np.float32(pos[i, 0]) - 0.5) == bx and math.floor(np.float32(pos[i, 1]) - 0.5) == by]
         assert list(perm[start[k]:start[k + 1]]) == brute
+
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden" / "golden.npz"
+GOLDEN_COLS = ("x", "y", "vx", "vy", "c00", "c01", "c10", "c11", "f00", "f01", "f10", "f11")
+
+
+@pytest.mark.parametrize("name", ["five_1", "five_9", "five_45"])
+def test_oracle_reproduces_golden_transfer_fixtures(orc, name):
+    """The committed five-material fixtures (tests/golden/make_golden.py):
+    8 FP64 steps reproduce the stored states."""
+    g = np.load(GOLDEN)
+    cfg = orc.make_config(1e-3, 48, 16, 1.0, 0.1, 0.05)
+    out = orc.run_f64(orc.particles_from_packed(g[name + "_in"]), cfg, 8)
+    got = np.stack([out[k] for k in GOLDEN_COLS], axis=1)
+    assert np.max(np.abs(got - g[name + "_out"])) <= 1e-12
+
+
+def test_oracle_reproduces_golden_fixed_point_words(orc):
+    g = np.load(GOLDEN)
+    prm = orc.params_f32(16, 16, 1.0, 0.02, 1.0, 1e-3, 3)
+    words, flags = orc.fixed_p2g_f32(g["fixed3_in"], prm)
+    assert flags == 0 and np.array_equal(words, g["fixed3_words"])
+    # three particles of mass 0.02 = particleMass: the mass words of each sum to
+    # 2^24 up to one rounding per support node
+    assert abs(int(words[:, 3].sum()) - 3 * 2 ** 24) <= 27
+
+
+def test_oracle_reproduces_golden_tank_macrostep(orc):
+    g = np.load(GOLDEN)
+    dt, steps, cell, mass, volume, gy = (float(v) for v in g["tank_cfg"])
+    cfg = orc.make_config(dt, 80, 80, cell, mass, volume, (0.0, gy), 1.0)
+    out = orc.run_f64(orc.particles_from_packed(g["tank_in"]), cfg, int(steps))
+    got = np.stack([out[k] for k in GOLDEN_COLS], axis=1)
+    assert np.max(np.abs(got - g["tank_out"])) <= 1e-12
