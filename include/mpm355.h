@@ -131,6 +131,17 @@ int mpm_read_bins(mpm_ctx* ctx, uint32_t* keys, uint32_t n, uint32_t* bucket_cou
  * the current slots per bucket (grows on demand). */
 int mpm_get_buckets(mpm_ctx* ctx, uint32_t* buckets_x, uint32_t* buckets_y, uint32_t* slots_per_bucket);
 
+/* FP64 mass and momentum of the batch (MpmStepDiagnostics,
+ * reference/solver.ts:96-103, :481-492, :2505-2526).  When enabled, every
+ * mpm_step_batch also sums, in FP64 and in a fixed order, the particle mass
+ * and momentum of the state it starts from and of the state it commits, and
+ * the grid mass and momentum of its last P2G.  out[9] = inputMass, gridMass,
+ * outputMass, inputMomentum x, y, gridMomentum x, y, outputMomentum x, y of
+ * the last batch.  No counterpart call in the reference, whose CPU step
+ * returns these per step. */
+int mpm_set_diagnostics(mpm_ctx* ctx, uint32_t enable);
+int mpm_read_diagnostics(mpm_ctx* ctx, double out[9]);
+
 /* Mean kernel time in microseconds of the named kernel over the last
  * mpm_step_batch calls made while timing was enabled (HIP events on the
  * context's stream). enable != 0 turns timing on and resets counters. */

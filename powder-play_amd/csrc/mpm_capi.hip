@@ -89,6 +89,8 @@ struct mpm_ctx {
     uint32_t last_done_before = 0;   // done_before of the most recently launched batch
     std::string error = "no error";
     KernelTimes* times = nullptr;
+    bool diag_on = false;
+    double* diag_saved = nullptr;  // diagnostics buffer while the reductions are off
 };
 
 namespace {
@@ -532,6 +534,8 @@ int mpm_step_batch(mpm_ctx* c, const void* params144, uint32_t step_count, uint3
     if (!isfinite(p.grad_scale) || !(p.grad_scale > 0.0f)) return fail(c, MPM_EINVAL, "params gradient scale is invalid");
     if (p.cfl_on >= 0.5f && (!isfinite(p.wave_bound) || !isfinite(p.vel_bound) || p.vel_bound < 0.0f))
         return fail(c, MPM_EINVAL, "CFL bounds must be finite and non-negative");
+    p.diag_on = c->diag_on ? 1u : 0u;
+    p.drag_pad[0] = p.drag_pad[1] = 0.0f;
     p.inv_mass_ref = reciprocal_for_division(p.particle_mass);
     p.inv_volume_ref = reciprocal_for_division(p.particle_volume);
     p.wall_pad = 0.0f;
@@ -701,6 +705,44 @@ int mpm_get_buckets(mpm_ctx* c, uint32_t* buckets_x, uint32_t* buckets_y, uint32
     if (buckets_x) *buckets_x = c->lc.bw;
     if (buckets_y) *buckets_y = c->lc.bh;
     if (slots_per_bucket) *slots_per_bucket = c->lc.cap;
+    return MPM_OK;
+}
+
+int mpm_set_diagnostics(mpm_ctx* c, uint32_t enable) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    int s = settle(c);
+    if (s) return s;
+    if (enable && !c->lc.diag) {
+        const uint32_t tiles = ((c->lc.node_w + 27) / 28) * ((c->lc.node_h + 27) / 28);
+        hipError_t e;
+        if ((e = c->mem.alloc((void**)&c->lc.diag_grid, sizeof(double) * 3 * tiles)) != hipSuccess ||
+            (e = c->mem.alloc((void**)&c->lc.diag_part, sizeof(double) * 3 * DIAG_PART_BLOCKS)) != hipSuccess ||
+            (e = c->mem.alloc((void**)&c->lc.diag, sizeof(double) * 9)) != hipSuccess)
+            return hip_fail(c, e, "hipMalloc(diagnostics)");
+        HIP_TRY(c, hipMemset(c->lc.diag_grid, 0, sizeof(double) * 3 * tiles));
+        HIP_TRY(c, hipMemset(c->lc.diag, 0, sizeof(double) * 9));
+    }
+    c->diag_on = enable != 0u;
+    if (!c->diag_on) {
+        // keep the buffers; launches skip the reductions when the pointer is null
+        c->diag_saved = c->lc.diag;
+        c->lc.diag = nullptr;
+    } else if (!c->lc.diag) {
+        c->lc.diag = c->diag_saved;
+    }
+    return MPM_OK;
+}
+
+int mpm_read_diagnostics(mpm_ctx* c, double out[9]) {
+    if (!c || !out) return fail(c, MPM_EINVAL, "context and output must not be NULL");
+    if (!c->lc.diag && !c->diag_saved) return fail(c, MPM_ESTATE, "diagnostics were never enabled");
+    int s = settle(c);
+    if (s) return s;
+    double d[9];
+    HIP_TRY(c, hipMemcpy(d, c->lc.diag ? c->lc.diag : c->diag_saved, sizeof d, hipMemcpyDeviceToHost));
+    // device order: mass (input, grid, output), momentum x (...), momentum y (...)
+    const int order[9] = {0, 1, 2, 3, 6, 4, 7, 5, 8};
+    for (int k = 0; k < 9; ++k) out[k] = d[order[k]];
     return MPM_OK;
 }
 

@@ -60,7 +60,9 @@ struct Params {
     float inv_mass_ref, inv_volume_ref;  // RN(1/particleMass), RN(1/particleVolume) or 0 (host-set)
     float wall_pad;
     float drag_target_x, drag_target_y, drag_radius, drag_stiffness; // interaction
-    float drag_damping, drag_pad[3];
+    float drag_damping;
+    uint32_t diag_on;  // FP64 mass/momentum reductions of the batch (host-set)
+    float drag_pad[2];
 };
 static_assert(sizeof(Params) == 176, "Params must mirror the 144+32 byte blocks");
 
@@ -144,7 +146,13 @@ struct LaunchCtx {
     uint32_t keep_fixed;
     uint32_t scene_flags;
     KernelTimes* timing;   // nullptr when timing is off
+    // FP64 diagnostics (nullptr when off): per-tile grid partials of the last
+    // P2G, per-block particle partials, and the nine reported sums
+    double* diag_grid;
+    double* diag_part;
+    double* diag;
 };
+constexpr uint32_t DIAG_PART_BLOCKS = 1024;  // blocks of the particle-sum kernel
 
 hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure, uint32_t seq);
 // Bucket particles [first, first+count) of the home arrays into buffer b

@@ -509,6 +509,37 @@ __device__ __forceinline__ bool weights_valid(float fx, float fy) {
     return ok;
 }
 
+// Block sum of three doubles in a fixed order (wave shuffles, then the
+// waves in index order); thread 0 writes them to out[0..2].  Every thread of
+// the block calls it.
+__device__ void block_sum3_f64(double a, double b, double c, double* out) {
+    __shared__ double s_part[3][P_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_part[0][wave] = a;
+        s_part[1][wave] = b;
+        s_part[2][wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0, tc = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            ta += s_part[0][w];
+            tb += s_part[1][w];
+            tc += s_part[2][w];
+        }
+        out[0] = ta;
+        out[1] = tb;
+        out[2] = tc;
+    }
+}
+
 // updateGrid (shaders.ts:1485-1746) of one interior node from its summed
 // channels f = {solid m, qx, qy | water m, qx, qy, V | oil m, qx, qy, V}.
 // `nvol(k, ox, oy)` returns the summed water (k = 0) or oil (k = 1) volume of
@@ -694,7 +725,7 @@ template <bool GENERAL>
 __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint32_t* __restrict__ cnt_next,
                   const Material* __restrict__ mats, uint32_t nmat, Grid grid, uint32_t tiles_x, uint32_t bw,
-                  uint32_t bh, uint32_t cap, uint32_t scene_flags, uint32_t step, int in_buf) {
+                  uint32_t bh, uint32_t cap, uint32_t scene_flags, uint32_t step, int in_buf, double* diag_grid) {
     __shared__ uint32_t s_boff[P_NB + 1];    // staged offset of each region bucket
     __shared__ uint32_t s_chunk[P_NB + 1];   // chunk boundaries (region bucket indices)
     __shared__ uint32_t s_nchunks;
@@ -895,6 +926,12 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
                                  return a[ring + oy * P_TC + ox];
                              });
     }
+    if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
+        const bool mine = node_ok && lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI;
+        block_sum3_f64(mine ? (double)acc[0] + (double)acc[3] + (double)acc[7] : 0.0,
+                       mine ? (double)acc[1] + (double)acc[4] + (double)acc[8] : 0.0,
+                       mine ? (double)acc[2] + (double)acc[5] + (double)acc[9] : 0.0, diag_grid + 3 * tile);
+    }
     if (flags) raise_step(ctl, flags, step);
 }
 
@@ -911,7 +948,7 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
                                                        const Material* __restrict__ mats, Grid grid,
                                                        uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
                                                        uint32_t keep_fixed, uint32_t scene_flags, uint32_t step,
-                                                       int in_buf) {
+                                                       int in_buf, double* diag_grid) {
     // words [solid m, qx, qy |] water m, qx, qy, V | oil m, qx, qy, V
     constexpr int NCH = SOLIDS ? 11 : 8;
     constexpr int SKIP = SOLIDS ? 0 : 3;  // channel index = word index - SKIP
@@ -1067,6 +1104,14 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
         flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, f, ERR_GRID, [&](int k, int ox, int oy) {
             return decoded(k == 0 ? 6 : 10, ring + oy * P_TC + ox);
         });
+    }
+    if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
+        const bool mine = lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI && sx < W && sy < H;
+        const int ring = ly * P_TC + lx;
+        block_sum3_f64(mine ? (double)decoded(0, ring) + (double)decoded(3, ring) + (double)decoded(7, ring) : 0.0,
+                       mine ? (double)decoded(1, ring) + (double)decoded(4, ring) + (double)decoded(8, ring) : 0.0,
+                       mine ? (double)decoded(2, ring) + (double)decoded(5, ring) + (double)decoded(9, ring) : 0.0,
+                       diag_grid + 3 * tile);
     }
     if (flags) raise_step(ctl, flags, step);
 }
@@ -1549,6 +1594,47 @@ __global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, SlotBuf buf
     }
 }
 
+// Particle mass and momentum (totalMass / totalMomentum,
+// reference/solver.ts:2505-2526) in FP64: block b sums buckets
+// [b*nb/B, (b+1)*nb/B) in slot order into part[3b..3b+2].
+__global__ __launch_bounds__(256) void k_particle_sums(SlotBuf buf, const Material* __restrict__ mats, uint32_t nb,
+                                                       uint32_t cap, double* part) {
+    const uint32_t b0 = (uint32_t)((uint64_t)blockIdx.x * nb / gridDim.x);
+    const uint32_t b1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * nb / gridDim.x);
+    double m = 0.0, px = 0.0, py = 0.0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t c = min(buf.cnt[b], cap);
+        for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) {
+            const size_t g = (size_t)b * cap + k;
+            const double mass = (double)mats[buf.mat[g]].mass;
+            const float2 v = buf.vel[g];
+            m += mass;
+            px += mass * (double)v.x;
+            py += mass * (double)v.y;
+        }
+    }
+    block_sum3_f64(m, px, py, part + 3 * blockIdx.x);
+}
+
+// Sum n partial triples in a fixed order into out[0], out[off], out[2*off].
+__global__ __launch_bounds__(256) void k_finish_sums(const double* __restrict__ part, uint32_t n, double* out,
+                                                     uint32_t off) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        a += part[3 * i];
+        b += part[3 * i + 1];
+        c += part[3 * i + 2];
+    }
+    __shared__ double s_out[3];
+    block_sum3_f64(a, b, c, s_out);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = s_out[0];
+        out[off] = s_out[1];
+        out[2 * off] = s_out[2];
+    }
+}
+
 inline uint32_t blocks_for(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
 }  // namespace
@@ -1567,6 +1653,14 @@ hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool
     const uint32_t gtx = (lc.bw + G_TB - 1) / G_TB, gty = (lc.bh + G_TB - 1) / G_TB;
     const size_t slots = (size_t)lc.nb * lc.cap;
     MPM_LAUNCH("batch_begin", k_batch_begin, dim3(1), dim3(64), 0, lc.stream, lc.ctl);
+    const bool diag = lc.diag != nullptr;
+    // diag[9] = mass (input, grid, output), momentum x (input, grid, output), momentum y (...)
+    if (diag) {
+        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(DIAG_PART_BLOCKS), dim3(256), 0, lc.stream,
+                   lc.buf[start_buf], lc.mats, lc.nb, lc.cap, lc.diag_part);
+        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(1), dim3(256), 0, lc.stream, lc.diag_part, DIAG_PART_BLOCKS,
+                   lc.diag + 0, 3u);
+    }
     int cur = start_buf;
     for (uint32_t s = 0; s < steps; ++s) {
         const SlotBuf& in = lc.buf[cur];
@@ -1576,11 +1670,13 @@ hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool
             auto p2g = general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
                                : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
             MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
-                       out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.keep_fixed, lc.scene_flags, s, cur);
+                       out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.keep_fixed, lc.scene_flags, s, cur,
+                       lc.diag_grid);
         } else {
             auto p2g = general ? k_p2g_gather<true> : k_p2g_gather<false>;
             MPM_LAUNCH("p2g_grid", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
-                       out.cnt, lc.mats, lc.nmat, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur);
+                       out.cnt, lc.mats, lc.nmat, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur,
+                       lc.diag_grid);
         }
         auto g2p = general ? k_g2p<true> : k_g2p<false>;
         MPM_LAUNCH("g2p", g2p, dim3(gtx * gty), dim3(G_THREADS), 0, lc.stream, lc.params, lc.ctl, in, out, lc.mats,
@@ -1591,6 +1687,14 @@ hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool
     const uint32_t fb = std::min<uint32_t>(blocks_for(slots, 256), 2048u);
     MPM_LAUNCH("fix_rollback", k_fix_rollback, dim3(fb), dim3(256), 0, lc.stream, lc.ctl, lc.buf[0], lc.buf[1], lc.nb,
                lc.cap, (uint32_t)start_buf, steps, general ? 1u : 0u);
+    if (diag) {
+        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(1), dim3(256), 0, lc.stream, lc.diag_grid, ptx * pty,
+                   lc.diag + 1, 3u);
+        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(DIAG_PART_BLOCKS), dim3(256), 0, lc.stream, lc.buf[cur],
+                   lc.mats, lc.nb, lc.cap, lc.diag_part);
+        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(1), dim3(256), 0, lc.stream, lc.diag_part, DIAG_PART_BLOCKS,
+                   lc.diag + 2, 3u);
+    }
     if (measure)
         MPM_LAUNCH("measure_speed", k_measure_speed, dim3(std::min<uint32_t>(blocks_for(slots, 256), 1024u)),
                    dim3(256), 0, lc.stream, lc.ctl, lc.buf[cur], lc.nb, lc.cap);

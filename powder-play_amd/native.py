@@ -35,6 +35,8 @@ SIGNATURES = {
     "mpm_read_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_read_bins": (ctypes.c_int, [_P, _P, _U32, _P, ctypes.c_size_t]),
     "mpm_get_buckets": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mpm_set_diagnostics": (ctypes.c_int, [_P, _U32]),
+    "mpm_read_diagnostics": (ctypes.c_int, [_P, _P]),
     "mpm_set_timing": (ctypes.c_int, [_P, _U32]),
     "mpm_read_timing": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_uint64)]),
@@ -170,6 +172,14 @@ class Context:
         counts = np.zeros(bx * by, np.uint32)
         self._check(self.lib.mpm_read_bins(self.handle, _ptr(keys), n, _ptr(counts), counts.size))
         return keys, counts
+
+    def set_diagnostics(self, enable: bool):
+        self._check(self.lib.mpm_set_diagnostics(self.handle, 1 if enable else 0))
+
+    def read_diagnostics(self) -> np.ndarray:
+        out = np.zeros(9, np.float64)
+        self._check(self.lib.mpm_read_diagnostics(self.handle, _ptr(out)))
+        return out
 
     def set_timing(self, enable: bool):
         self._check(self.lib.mpm_set_timing(self.handle, 1 if enable else 0))

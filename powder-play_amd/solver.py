@@ -473,6 +473,22 @@ class MpmMlsGpuSolver:
             self.ctx.close()
 
     # ------------------------------------------------------- parity helpers
+    def set_diagnostics(self, enable: bool = True):
+        """Turn the FP64 mass/momentum reductions of each batch on or off."""
+        self._assert_active()
+        self.ctx.set_diagnostics(enable)
+
+    def read_diagnostics(self) -> dict:
+        """MpmStepDiagnostics (reference/solver.ts:96-103) of the last batch:
+        input = the state it started from, grid = its last P2G, output = the
+        state it committed; FP64 sums on the GPU."""
+        self._assert_active()
+        d = self.ctx.read_diagnostics()
+        return {"inputMass": float(d[0]), "gridMass": float(d[1]), "outputMass": float(d[2]),
+                "inputMomentum": {"x": float(d[3]), "y": float(d[4])},
+                "gridMomentum": {"x": float(d[5]), "y": float(d[6])},
+                "outputMomentum": {"x": float(d[7]), "y": float(d[8])}}
+
     def read_fixed_grid(self) -> np.ndarray:
         return self.ctx.read_fixed_grid()
 

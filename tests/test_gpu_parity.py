@@ -315,3 +315,53 @@ def test_column_1m_deterministic_conservative_and_oracle_close(pkg, orc):
     assert rel_v <= 1e-4 and rel_x <= 1e-7
     assert max_x <= 1e-4 and max_v <= 2e-3
     assert diag.grid_mass == pytest.approx(diag.input_mass, rel=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+def test_fp64_diagnostics_match_oracle(pkg, orc, mode):
+    """MpmStepDiagnostics (reference/solver.ts:96-103) of one microstep of the
+    five-material block: the FP64 reductions on the GPU against the oracle's
+    FP64 step.  Particle sums of the input agree to rounding; grid and output
+    sums carry the float32 step (1e-6 relative)."""
+    particles, config = pkg.scenes.five_model_block()
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    s = factory(particles, config, device=0)
+    try:
+        s.ctx.set_diagnostics(True)
+        s.step(1e-3, {"x": 0.0, "y": -2.0})
+        s.check_numerical_status()
+        got = s.ctx.read_diagnostics()
+    finally:
+        s.dispose()
+    state = pkg.layout.validate_input(particles, config)
+    f32 = lambda v: float(np.float32(v))  # noqa: E731 - the GPU's float32 configuration values
+    _, d = orc.step_f64(orc.particles_from_packed(state.packed),
+                        orc.make_config(f32(1e-3), 48, 16, 1.0, f32(0.1), f32(0.05), (0.0, -2.0)))
+    want = np.array([d.input_mass, d.grid_mass, d.output_mass, d.input_px, d.input_py, d.grid_px, d.grid_py,
+                     d.output_px, d.output_py])
+    print(f"{mode}: GPU {got}\n        FP64 {want}")
+    # liquid masses are density*volume rounded to float32 on the GPU (as the
+    # reference's shaders do), in FP64 in the oracle: 1e-7 relative
+    assert got[0] == pytest.approx(want[0], rel=1e-7)
+    assert got[2] == pytest.approx(got[0], rel=1e-12)  # mass is carried unchanged
+    assert got[1] == pytest.approx(want[1], rel=1e-6)
+    assert np.allclose(got[3:5], want[3:5], rtol=1e-7, atol=1e-9)
+    assert np.allclose(got[5:], want[5:], rtol=1e-5, atol=1e-6)
+
+
+def test_fp64_conservation_1m_column(pkg):
+    """40 microsteps of the 1M column with diagnostics on: the committed mass
+    equals the input mass exactly (no particle is lost or duplicated by the
+    bucket moves) and the grid mass of the last P2G equals it to f32 rounding."""
+    arr, config = pkg.scenes.water_oil_column(cells=1000)
+    s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
+    try:
+        s.ctx.set_diagnostics(True)
+        s.step_batch(40, 1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY)
+        s.check_numerical_status()
+        d = s.ctx.read_diagnostics()
+    finally:
+        s.dispose()
+    print(f"1M column masses: input {d[0]!r} grid {d[1]!r} output {d[2]!r}")
+    assert d[2] == pytest.approx(d[0], rel=1e-12)
+    assert d[1] == pytest.approx(d[0], rel=1e-6)
