@@ -21,6 +21,8 @@ HERE = pathlib.Path(__file__).resolve().parent
 ROOT = HERE.parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libmpm355.so"
+ADDON = HERE / "mpm355.node"
+NODE_INCLUDE = pathlib.Path(os.environ.get("NODE_INCLUDE", "/usr/include/node"))
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPM355_ARCH", "gfx950")
@@ -64,6 +66,24 @@ def build(verbose: bool = False) -> pathlib.Path:
     return LIB
 
 
+def build_addon(verbose: bool = False) -> pathlib.Path | None:
+    """Compile the Node-API binding (napi/mpm355_napi.c) against node_api.h
+    into mpm355.node beside libmpm355.so (found through $ORIGIN).  Returns
+    None, with a message, when this machine has no Node-API headers."""
+    if not (NODE_INCLUDE / "node_api.h").exists():
+        print(f"[build] {NODE_INCLUDE}/node_api.h not found: Node-API addon not built", file=sys.stderr)
+        return None
+    tmp = ADDON.with_suffix(".node.tmp")
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-Wall", f"-I{NODE_INCLUDE}", f"-I{ROOT / 'include'}", "-o", str(tmp),
+           str(HERE / "napi" / "mpm355_napi.c"), f"-L{HERE}", "-lmpm355", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, ADDON)
+    return ADDON
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
+    build_addon(verbose="-v" in sys.argv)
     print(LIB)

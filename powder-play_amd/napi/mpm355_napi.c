@@ -1,0 +1,408 @@
+/*
+ * Node-API binding of libmpm355 (include/mpm355.h): the native runtime a
+ * TypeScript MpmMlsGpuSolver implementation calls instead of WebGPU
+ * (src/lib/mpm/kernels/solver.ts:106-144).  Every function forwards to one
+ * C-ABI entry point; argument errors become RangeError/TypeError before any
+ * device work (solver.ts:1414-1441) and device failures become Error with a
+ * `code` property.  readCompact returns a Promise resolved from a worker
+ * thread, like the reference's mapAsync readback (solver.ts:2388-2512).
+ */
+#include <node_api.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mpm355.h"
+
+#define CHECK(call)                                                          \
+    do {                                                                     \
+        if ((call) != napi_ok) {                                             \
+            napi_throw_error(env, NULL, "Node-API call failed: " #call);     \
+            return NULL;                                                     \
+        }                                                                    \
+    } while (0)
+
+static napi_value throw_code(napi_env env, int code, mpm_ctx* ctx) {
+    const char* msg = mpm_last_error(ctx);
+    if (code == MPM_EINVAL) {
+        napi_throw_range_error(env, "MPM_EINVAL", msg);
+    } else {
+        const char* name = code == MPM_EDEVICE ? "MPM_EDEVICE" : code == MPM_ESTATE ? "MPM_ESTATE" : "MPM_ECAPACITY";
+        napi_throw_error(env, name, msg);
+    }
+    return NULL;
+}
+
+static void finalize_ctx(napi_env env, void* data, void* hint) {
+    (void)env;
+    (void)hint;
+    mpm_destroy((mpm_ctx*)data);
+}
+
+/* The context of argument 0, or NULL after a thrown TypeError. */
+static mpm_ctx* ctx_arg(napi_env env, napi_value v) {
+    napi_valuetype t;
+    void* p = NULL;
+    if (napi_typeof(env, v, &t) != napi_ok || t != napi_external || napi_get_value_external(env, v, &p) != napi_ok ||
+        !p) {
+        napi_throw_type_error(env, NULL, "expected a solver context");
+        return NULL;
+    }
+    return (mpm_ctx*)p;
+}
+
+static int get_number(napi_env env, napi_value obj, const char* key, double* out, int required) {
+    bool has = false;
+    napi_value v;
+    if (napi_has_named_property(env, obj, key, &has) != napi_ok) return 0;
+    if (!has) {
+        if (required) {
+            napi_throw_type_error(env, NULL, key);
+            return 0;
+        }
+        return 1;
+    }
+    if (napi_get_named_property(env, obj, key, &v) != napi_ok) return 0;
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t == napi_boolean) {
+        bool b;
+        napi_get_value_bool(env, v, &b);
+        *out = b ? 1.0 : 0.0;
+        return 1;
+    }
+    if (t != napi_number || napi_get_value_double(env, v, out) != napi_ok) {
+        napi_throw_type_error(env, NULL, key);
+        return 0;
+    }
+    return 1;
+}
+
+/* A typed array argument as (data, element count), or 0 after a TypeError. */
+static int typed_arg(napi_env env, napi_value v, napi_typedarray_type want, void** data, size_t* count,
+                     int allow_null) {
+    napi_valuetype vt;
+    napi_typeof(env, v, &vt);
+    if (allow_null && (vt == napi_null || vt == napi_undefined)) {
+        *data = NULL;
+        *count = 0;
+        return 1;
+    }
+    bool is = false;
+    napi_is_typedarray(env, v, &is);
+    napi_typedarray_type t;
+    napi_value ab;
+    size_t off;
+    if (!is || napi_get_typedarray_info(env, v, &t, count, data, &ab, &off) != napi_ok || t != want) {
+        napi_throw_type_error(env, NULL, "expected a typed array of the documented element type");
+        return 0;
+    }
+    return 1;
+}
+
+/* create(config, packed: Uint32Array, objectIds: Uint32Array | null, atomic: boolean) */
+static napi_value create(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value argv[4];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 4) {
+        napi_throw_type_error(env, NULL, "create(config, packed, objectIds, atomic)");
+        return NULL;
+    }
+    double cx = 0, cy = 0, cs = 0, pm = 0, pv = 0, wf = 0, capacity = 0, device = -1, slots = 0, keep = 0;
+    if (!get_number(env, argv[0], "cellsX", &cx, 1) || !get_number(env, argv[0], "cellsY", &cy, 1) ||
+        !get_number(env, argv[0], "cellSize", &cs, 1) || !get_number(env, argv[0], "particleMass", &pm, 1) ||
+        !get_number(env, argv[0], "particleVolume", &pv, 1) || !get_number(env, argv[0], "wallFriction", &wf, 0) ||
+        !get_number(env, argv[0], "capacity", &capacity, 0) || !get_number(env, argv[0], "device", &device, 0) ||
+        !get_number(env, argv[0], "bucketSlots", &slots, 0) || !get_number(env, argv[0], "keepFixedGrid", &keep, 0))
+        return NULL;
+    void *packed, *ids;
+    size_t words, nid;
+    if (!typed_arg(env, argv[1], napi_uint32_array, &packed, &words, 0)) return NULL;
+    if (!typed_arg(env, argv[2], napi_uint32_array, &ids, &nid, 1)) return NULL;
+    bool atomic = false;
+    CHECK(napi_get_value_bool(env, argv[3], &atomic));
+    if (words % MPM_PARTICLE_WORDS != 0) {
+        napi_throw_range_error(env, "MPM_EINVAL", "packed particles must be a multiple of 20 words");
+        return NULL;
+    }
+    const uint32_t n = (uint32_t)(words / MPM_PARTICLE_WORDS);
+    if (ids && nid != n) {
+        napi_throw_range_error(env, "MPM_EINVAL", "objectIds must hold one word per particle");
+        return NULL;
+    }
+    mpm_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    cfg.cells_x = (uint32_t)cx;
+    cfg.cells_y = (uint32_t)cy;
+    cfg.cell_size = (float)cs;
+    cfg.particle_mass = (float)pm;
+    cfg.particle_volume = (float)pv;
+    cfg.wall_friction = (float)wf;
+    cfg.capacity = capacity > 0 ? (uint32_t)capacity : n;
+    cfg.p2g_mode = atomic ? MPM_P2G_FIXED_POINT : MPM_P2G_GATHER;
+    cfg.flags = keep != 0 ? MPM_FLAG_KEEP_FIXED_GRID : 0u;
+    cfg.device = (int32_t)device;
+    cfg.bucket_slots = (uint32_t)slots;
+    mpm_ctx* ctx = NULL;
+    const int code = mpm_create(&cfg, (const uint32_t*)packed, (const uint32_t*)ids, n, &ctx);
+    if (code != MPM_OK) return throw_code(env, code, NULL);
+    napi_value ext;
+    CHECK(napi_create_external(env, ctx, finalize_ctx, NULL, &ext));
+    return ext;
+}
+
+/* stepBatch(ctx, params: Uint8Array(144) | ArrayBuffer(144), stepCount, measureSpeed) */
+static napi_value step_batch(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value argv[4];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 4 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* params = NULL;
+    size_t len = 0;
+    bool is_ab = false;
+    napi_is_arraybuffer(env, argv[1], &is_ab);
+    if (is_ab) {
+        CHECK(napi_get_arraybuffer_info(env, argv[1], &params, &len));
+    } else if (!typed_arg(env, argv[1], napi_uint8_array, &params, &len, 0)) {
+        return NULL;
+    }
+    if (len != MPM_PARAMS_BYTES) {
+        napi_throw_range_error(env, "MPM_EINVAL", "step parameters must be 144 bytes");
+        return NULL;
+    }
+    uint32_t steps = 0;
+    bool measure = false;
+    CHECK(napi_get_value_uint32(env, argv[2], &steps));
+    CHECK(napi_get_value_bool(env, argv[3], &measure));
+    const int code = mpm_step_batch(ctx, params, steps, measure ? 1u : 0u);
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+static napi_value compact_object(napi_env env, float speed, uint32_t completed, const uint32_t status[8]) {
+    napi_value out, v, ab, arr;
+    void* data;
+    if (napi_create_object(env, &out) != napi_ok) return NULL;
+    napi_create_double(env, speed, &v);
+    napi_set_named_property(env, out, "maxSpeed", v);
+    napi_create_uint32(env, completed, &v);
+    napi_set_named_property(env, out, "completed", v);
+    napi_create_arraybuffer(env, 32, &data, &ab);
+    memcpy(data, status, 32);
+    napi_create_typedarray(env, napi_uint32_array, 8, ab, 0, &arr);
+    napi_set_named_property(env, out, "status", arr);
+    return out;
+}
+
+/* readCompactSync(ctx) -> { maxSpeed, completed, status: Uint32Array(8) } */
+static napi_value read_compact_sync(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    float speed = 0;
+    uint32_t completed = 0, status[8];
+    const int code = mpm_read_compact(ctx, &speed, &completed, status);
+    if (code != MPM_OK) return throw_code(env, code, ctx);
+    return compact_object(env, speed, completed, status);
+}
+
+typedef struct {
+    napi_async_work work;
+    napi_deferred deferred;
+    napi_ref ctx_ref; /* keeps the context alive while the worker waits */
+    mpm_ctx* ctx;
+    float speed;
+    uint32_t completed, status[8];
+    int code;
+} compact_job;
+
+static void compact_exec(napi_env env, void* p) {
+    (void)env;
+    compact_job* j = (compact_job*)p;
+    j->code = mpm_read_compact(j->ctx, &j->speed, &j->completed, j->status);
+}
+
+static void compact_done(napi_env env, napi_status s, void* p) {
+    compact_job* j = (compact_job*)p;
+    (void)s;
+    if (j->code != MPM_OK) {
+        napi_value msg, err, code;
+        napi_create_string_utf8(env, mpm_last_error(j->ctx), NAPI_AUTO_LENGTH, &msg);
+        napi_create_string_utf8(env, j->code == MPM_EINVAL ? "MPM_EINVAL" : "MPM_EDEVICE", NAPI_AUTO_LENGTH, &code);
+        napi_create_error(env, code, msg, &err);
+        napi_reject_deferred(env, j->deferred, err);
+    } else {
+        napi_resolve_deferred(env, j->deferred, compact_object(env, j->speed, j->completed, j->status));
+    }
+    napi_delete_reference(env, j->ctx_ref);
+    napi_delete_async_work(env, j->work);
+    free(j);
+}
+
+/* readCompact(ctx) -> Promise<{ maxSpeed, completed, status }> */
+static napi_value read_compact(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], promise, name;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    compact_job* j = (compact_job*)calloc(1, sizeof *j);
+    if (!j) {
+        napi_throw_error(env, NULL, "out of memory");
+        return NULL;
+    }
+    j->ctx = ctx;
+    CHECK(napi_create_reference(env, argv[0], 1, &j->ctx_ref));
+    CHECK(napi_create_promise(env, &j->deferred, &promise));
+    CHECK(napi_create_string_utf8(env, "mpm_read_compact", NAPI_AUTO_LENGTH, &name));
+    CHECK(napi_create_async_work(env, NULL, name, compact_exec, compact_done, j, &j->work));
+    CHECK(napi_queue_async_work(env, j->work));
+    return promise;
+}
+
+/* clearStatus(ctx) */
+static napi_value clear_status(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    const int code = mpm_clear_status(ctx);
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+/* counts(ctx) -> { particleCount, particleCapacity, nodeCount } */
+static napi_value counts(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], out, v;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    uint32_t n = 0, cap = 0, nodes = 0;
+    const int code = mpm_get_counts(ctx, &n, &cap, &nodes);
+    if (code != MPM_OK) return throw_code(env, code, ctx);
+    CHECK(napi_create_object(env, &out));
+    napi_create_uint32(env, n, &v);
+    napi_set_named_property(env, out, "particleCount", v);
+    napi_create_uint32(env, cap, &v);
+    napi_set_named_property(env, out, "particleCapacity", v);
+    napi_create_uint32(env, nodes, &v);
+    napi_set_named_property(env, out, "nodeCount", v);
+    return out;
+}
+
+/* readParticles(ctx, indices: Uint32Array | null) -> { records: Uint32Array, objectIds: Uint32Array } */
+static napi_value read_particles(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2], out, rec_ab, ids_ab, rec, ids;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* idx = NULL;
+    size_t nidx = 0;
+    if (argc >= 2 && !typed_arg(env, argv[1], napi_uint32_array, &idx, &nidx, 1)) return NULL;
+    uint32_t n = (uint32_t)nidx;
+    if (!idx) {
+        uint32_t cap = 0, nodes = 0;
+        const int code = mpm_get_counts(ctx, &n, &cap, &nodes);
+        if (code != MPM_OK) return throw_code(env, code, ctx);
+    }
+    void *rd, *id;
+    CHECK(napi_create_arraybuffer(env, (size_t)n * MPM_PARTICLE_WORDS * 4, &rd, &rec_ab));
+    CHECK(napi_create_arraybuffer(env, (size_t)n * 4, &id, &ids_ab));
+    const int code = mpm_read_particles(ctx, (const uint32_t*)idx, n, (uint32_t*)rd, (uint32_t*)id);
+    if (code != MPM_OK) return throw_code(env, code, ctx);
+    CHECK(napi_create_typedarray(env, napi_uint32_array, (size_t)n * MPM_PARTICLE_WORDS, rec_ab, 0, &rec));
+    CHECK(napi_create_typedarray(env, napi_uint32_array, n, ids_ab, 0, &ids));
+    CHECK(napi_create_object(env, &out));
+    napi_set_named_property(env, out, "records", rec);
+    napi_set_named_property(env, out, "objectIds", ids);
+    return out;
+}
+
+/* appendParticles(ctx, packed: Uint32Array, objectIds: Uint32Array | null) */
+static napi_value append_particles(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value argv[3];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 3 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void *packed, *ids;
+    size_t words, nid;
+    if (!typed_arg(env, argv[1], napi_uint32_array, &packed, &words, 0)) return NULL;
+    if (!typed_arg(env, argv[2], napi_uint32_array, &ids, &nid, 1)) return NULL;
+    if (words % MPM_PARTICLE_WORDS != 0 || (ids && nid != words / MPM_PARTICLE_WORDS)) {
+        napi_throw_range_error(env, "MPM_EINVAL", "packed particles and objectIds disagree");
+        return NULL;
+    }
+    const int code = mpm_append_particles(ctx, (const uint32_t*)packed, (const uint32_t*)ids,
+                                          (uint32_t)(words / MPM_PARTICLE_WORDS));
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+/* setInteraction(ctx, Float32Array(8)) */
+static napi_value set_interaction(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* p;
+    size_t n;
+    if (!typed_arg(env, argv[1], napi_float32_array, &p, &n, 0)) return NULL;
+    if (n != 8) {
+        napi_throw_range_error(env, "MPM_EINVAL", "interaction uniform must be 8 floats");
+        return NULL;
+    }
+    const int code = mpm_set_interaction(ctx, (const float*)p);
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+/* setDiagnostics(ctx, enable) / readDiagnostics(ctx) -> Float64Array(9) */
+static napi_value set_diagnostics(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    bool on = false;
+    CHECK(napi_get_value_bool(env, argv[1], &on));
+    const int code = mpm_set_diagnostics(ctx, on ? 1u : 0u);
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+static napi_value read_diagnostics(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], ab, arr;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* data;
+    CHECK(napi_create_arraybuffer(env, 9 * sizeof(double), &data, &ab));
+    const int code = mpm_read_diagnostics(ctx, (double*)data);
+    if (code != MPM_OK) return throw_code(env, code, ctx);
+    CHECK(napi_create_typedarray(env, napi_float64_array, 9, ab, 0, &arr));
+    return arr;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+    napi_property_descriptor p[] = {
+        {"create", 0, create, 0, 0, 0, napi_enumerable, 0},
+        {"stepBatch", 0, step_batch, 0, 0, 0, napi_enumerable, 0},
+        {"readCompact", 0, read_compact, 0, 0, 0, napi_enumerable, 0},
+        {"readCompactSync", 0, read_compact_sync, 0, 0, 0, napi_enumerable, 0},
+        {"clearStatus", 0, clear_status, 0, 0, 0, napi_enumerable, 0},
+        {"counts", 0, counts, 0, 0, 0, napi_enumerable, 0},
+        {"readParticles", 0, read_particles, 0, 0, 0, napi_enumerable, 0},
+        {"appendParticles", 0, append_particles, 0, 0, 0, napi_enumerable, 0},
+        {"setInteraction", 0, set_interaction, 0, 0, 0, napi_enumerable, 0},
+        {"setDiagnostics", 0, set_diagnostics, 0, 0, 0, napi_enumerable, 0},
+        {"readDiagnostics", 0, read_diagnostics, 0, 0, 0, napi_enumerable, 0},
+    };
+    if (napi_define_properties(env, exports, sizeof p / sizeof p[0], p) != napi_ok) return NULL;
+    return exports;
+}
+
+NAPI_MODULE(mpm355, init)

@@ -365,3 +365,56 @@ def test_fp64_conservation_1m_column(pkg):
     print(f"1M column masses: input {d[0]!r} grid {d[1]!r} output {d[2]!r}")
     assert d[2] == pytest.approx(d[0], rel=1e-12)
     assert d[1] == pytest.approx(d[0], rel=1e-6)
+
+
+def test_node_addon_steps_like_the_python_path(pkg, tmp_path):
+    """The Node-API binding (the runtime a TypeScript MpmMlsGpuSolver would
+    call) runs the five-material fixture: 8 microsteps through stepBatch and
+    the Promise of readCompact, bit for bit the state of the Python path."""
+    import json
+    import shutil
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    if shutil.which("node") is None or not (root / "powder-play_amd" / "mpm355.node").exists():
+        pytest.skip("node or the built Node-API addon is missing")
+    particles, config = pkg.scenes.five_model_block()
+    gravity = {"x": 0.0, "y": -1.0}
+    state = pkg.layout.validate_input(particles, config)
+    params = pkg.layout.pack_step_parameters(state, 1e-3, gravity, len(particles), 0, None)
+    (tmp_path / "packed.bin").write_bytes(np.ascontiguousarray(state.packed, np.uint32).tobytes())
+    (tmp_path / "params.bin").write_bytes(params)
+    script = """
+const fs = require('fs');
+const { MpmNativeRuntime } = require('./powder-play_amd/napi/index.js');
+const dir = process.argv[1];
+function u32(file) {
+  const b = fs.readFileSync(file);
+  const u = new Uint32Array(b.length / 4);
+  new Uint8Array(u.buffer).set(b);
+  return u;
+}
+const rt = new MpmNativeRuntime({cellsX: 48, cellsY: 16, cellSize: 1, particleMass: 0.1, particleVolume: 0.05,
+                                 device: 0}, u32(dir + '/packed.bin'), null, false);
+const params = new Uint8Array(fs.readFileSync(dir + '/params.bin'));
+rt.stepBatch(params, 8, true);
+rt.readCompact().then((c) => {
+  const r = rt.readParticles();
+  fs.writeFileSync(dir + '/out.bin', Buffer.from(r.records.buffer));
+  console.log(JSON.stringify({ completed: c.completed, status: Array.from(c.status), maxSpeed: c.maxSpeed }));
+});
+"""
+    print("node addon: 8 microsteps of the five-material fixture", flush=True)
+    r = subprocess.run(["node", "-e", script, str(tmp_path)], cwd=root, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    compact = json.loads(r.stdout)
+    node_recs = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.uint32).reshape(-1, 20)
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+    try:
+        s.step_batch(8, 1e-3, gravity, measure_speed=True)
+        speed, done, status = s.ctx.read_compact()
+        recs, _ = s.read_particles(packed=True)
+    finally:
+        s.dispose()
+    assert compact["completed"] == done == 8 and compact["status"][0] == status[0] == 0
+    assert compact["maxSpeed"] == speed
+    assert np.array_equal(node_recs, recs)

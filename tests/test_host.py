@@ -230,3 +230,53 @@ def test_fixed_point_division_by_reciprocal_matches_ieee(tmp_path):
         bad = so.fastdiv_check(b, 12345, 200000, ctypes.byref(mismatch))
         so.fastdiv_example.restype = ctypes.c_float
         assert bad == 0 and mismatch.value == 0, (b, bad, mismatch.value, so.fastdiv_example())
+
+
+NODE_FUNCTIONS = ("create", "stepBatch", "readCompact", "readCompactSync", "clearStatus", "counts", "readParticles",
+                  "appendParticles", "setInteraction", "setDiagnostics", "readDiagnostics")
+
+
+def node_addon():
+    """The built Node-API binding, building it when missing; skips the test
+    where this machine has no node or no Node-API headers."""
+    import shutil
+    import subprocess  # noqa: F401
+    if shutil.which("node") is None:
+        pytest.skip("node is not installed")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("pp_build", ROOT / "powder-play_amd" / "build.py")
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    if not b.ADDON.exists() and b.build_addon() is None:
+        pytest.skip("no Node-API headers")
+    return b.ADDON
+
+
+def run_node(script: str, timeout: float = 120.0) -> str:
+    import subprocess
+    r = subprocess.run(["node", "-e", script], cwd=ROOT, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_node_addon_exports_and_validates_before_device_work():
+    """The Node-API binding loads, exports the runtime functions and turns an
+    invalid configuration into a RangeError before any device work
+    (solver.ts:1414-1441); a wrong context argument is a TypeError."""
+    node_addon()
+    out = run_node("""
+const { addon } = require('./powder-play_amd/napi/index.js');
+const res = { keys: Object.keys(addon) };
+try { addon.create({cellsX: 0, cellsY: 4, cellSize: 1, particleMass: 1, particleVolume: 1},
+                   new Uint32Array(0), null, false); } catch (e) { res.bad = [e.constructor.name, e.code]; }
+try { addon.create({cellsX: 4, cellsY: 4, cellSize: 1, particleMass: 1, particleVolume: 1},
+                   new Uint32Array(21), null, false); } catch (e) { res.words = [e.constructor.name, e.code]; }
+try { addon.stepBatch({}, new Uint8Array(144), 1, false); } catch (e) { res.ctx = e.constructor.name; }
+console.log(JSON.stringify(res));
+""")
+    import json
+    res = json.loads(out)
+    assert set(NODE_FUNCTIONS) <= set(res["keys"])
+    assert res["bad"] == ["RangeError", "MPM_EINVAL"]
+    assert res["words"] == ["RangeError", "MPM_EINVAL"]
+    assert res["ctx"] == "TypeError"
