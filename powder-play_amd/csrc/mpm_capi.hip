@@ -69,11 +69,14 @@ struct mpm_ctx {
     DeviceArrays mem;       // home arrays, grid, control, params, materials
     DeviceArrays slot_mem;  // bucket slot buffers (reallocated when they grow)
     LaunchCtx lc{};
-    Params* d_params = nullptr;
     Control* d_ctl = nullptr;
     Material* d_mats = nullptr;
     uint32_t mat_capacity = 0;
-    Params* h_param_ring = nullptr;  // pinned
+    SceneDesc* h_desc_ring = nullptr; // pinned ring: one descriptor (with its Params) per launch
+    SceneDesc* d_scene = nullptr;     // device: this context as a one-scene step launch
+    uint32_t* d_p2g_map = nullptr;    // device: tile -> scene (all 0)
+    uint32_t* d_g2p_map = nullptr;
+    uint32_t p2g_tiles = 0, g2p_tiles = 0;
     hipEvent_t ring_events[PARAM_RING] = {};
     bool ring_used[PARAM_RING] = {};
     int ring_next = 0;
@@ -308,6 +311,37 @@ int initial_cap(mpm_ctx* c, const uint32_t* rec, uint32_t n) {
     return (int)std::min(cap, MAX_BUCKET_CAP);
 }
 
+// This context as one scene of a step launch with parameters p; the
+// scene's tiles start at the given bases.
+SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint32_t g2p_base, uint32_t seq) {
+    const LaunchCtx& lc = c->lc;
+    SceneDesc d{};
+    d.P = p;
+    d.ctl = lc.ctl;
+    d.buf[0] = lc.buf[0];
+    d.buf[1] = lc.buf[1];
+    d.mats = lc.mats;
+    d.grid = lc.grid;
+    d.diag_grid = lc.diag ? lc.diag_grid : nullptr;
+    d.diag_part = lc.diag ? lc.diag_part : nullptr;
+    d.diag = lc.diag;
+    d.nmat = lc.nmat;
+    d.bw = lc.bw;
+    d.bh = lc.bh;
+    d.nb = lc.nb;
+    d.cap = lc.cap;
+    d.ptx = (lc.node_w + P_TILE - 1) / P_TILE;
+    d.pty = (lc.node_h + P_TILE - 1) / P_TILE;
+    d.gtx = (lc.bw + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS;
+    d.gty = (lc.bh + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS;
+    d.p2g_base = p2g_base;
+    d.g2p_base = g2p_base;
+    d.scene_flags = lc.scene_flags;
+    d.keep_fixed = lc.keep_fixed;
+    d.seq = seq;
+    return d;
+}
+
 int settle(mpm_ctx* c);
 
 int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t done_before) {
@@ -319,14 +353,28 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     const int slot = c->ring_next;
     c->ring_next = (c->ring_next + 1) % PARAM_RING;
     if (c->ring_used[slot]) HIP_TRY(c, hipEventSynchronize(c->ring_events[slot]));
-    c->h_param_ring[slot] = p;
-    HIP_TRY(c, hipMemcpyAsync(c->d_params, &c->h_param_ring[slot], sizeof(Params), hipMemcpyHostToDevice, c->stream));
+    const uint32_t seq = c->seq++;
+    c->h_desc_ring[slot] = scene_desc(c, p, 0u, 0u, seq);
+    HIP_TRY(c, hipMemcpyAsync(c->d_scene, &c->h_desc_ring[slot], sizeof(SceneDesc), hipMemcpyHostToDevice,
+                              c->stream));
     HIP_TRY(c, hipEventRecord(c->ring_events[slot], c->stream));
     c->ring_used[slot] = true;
-    LaunchCtx lc = c->lc;
-    lc.timing = c->times;
-    const uint32_t seq = c->seq++;
-    HIP_TRY(c, launch_batch(lc, c->cur, steps, measure, seq));
+    const SceneDesc& d = c->h_desc_ring[slot];
+    StepBatch sb{};
+    sb.stream = c->stream;
+    sb.d_scenes = c->d_scene;
+    sb.p2g_scene = c->d_p2g_map;
+    sb.g2p_scene = c->d_g2p_map;
+    sb.nscenes = 1;
+    sb.p2g_tiles = c->p2g_tiles;
+    sb.g2p_tiles = c->g2p_tiles;
+    sb.max_slots = c->lc.nb * c->lc.cap;
+    sb.max_p2g_tiles = c->p2g_tiles;
+    sb.scene_flags = d.scene_flags;
+    sb.fixed_point = c->lc.fixed_point;
+    sb.diag = c->lc.diag != nullptr;
+    sb.timing = c->times;
+    HIP_TRY(c, launch_steps(sb, c->cur, steps, measure));
     c->pending.push_back({p, steps, measure, seq, done_before});
     c->last_done_before = done_before;
     c->cur = (c->cur + (int)(steps & 1u)) & 1;
@@ -460,11 +508,19 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     ALLOC(lc.grid.liquid, nodes * sizeof(float2));
     const bool keep_fixed = c.p2g_mode == MPM_P2G_FIXED_POINT && (c.flags & MPM_FLAG_KEEP_FIXED_GRID);
     if (keep_fixed) ALLOC(lc.grid.fixed, nodes * 11 * sizeof(int32_t));
-    ALLOC(ctx->d_params, sizeof(Params));
     ALLOC(ctx->d_ctl, sizeof(Control));
+    ALLOC(ctx->d_scene, sizeof(SceneDesc));
+    ctx->p2g_tiles = ((lc.node_w + P_TILE - 1) / P_TILE) * ((lc.node_h + P_TILE - 1) / P_TILE);
+    ctx->g2p_tiles = ((lc.bw + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS) * ((lc.bh + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS);
+    ALLOC(ctx->d_p2g_map, sizeof(uint32_t) * ctx->p2g_tiles);
+    ALLOC(ctx->d_g2p_map, sizeof(uint32_t) * ctx->g2p_tiles);
 #undef ALLOC
-    if ((e = hipHostMalloc((void**)&ctx->h_param_ring, sizeof(Params) * PARAM_RING, hipHostMallocDefault)) != hipSuccess)
-        return bail(hip_fail(ctx, e, "hipHostMalloc(params)"));
+    if ((e = hipMemset(ctx->d_p2g_map, 0, sizeof(uint32_t) * ctx->p2g_tiles)) != hipSuccess ||
+        (e = hipMemset(ctx->d_g2p_map, 0, sizeof(uint32_t) * ctx->g2p_tiles)) != hipSuccess)
+        return bail(hip_fail(ctx, e, "hipMemset(tile maps)"));
+    if ((e = hipHostMalloc((void**)&ctx->h_desc_ring, sizeof(SceneDesc) * PARAM_RING, hipHostMallocDefault)) !=
+        hipSuccess)
+        return bail(hip_fail(ctx, e, "hipHostMalloc(descriptors)"));
     if ((e = hipHostMalloc((void**)&ctx->h_ctl, sizeof(Control), hipHostMallocDefault)) != hipSuccess)
         return bail(hip_fail(ctx, e, "hipHostMalloc(control)"));
     for (int k = 0; k < PARAM_RING; ++k)
@@ -483,7 +539,6 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
         return bail(hip_fail(ctx, e, "hipMemcpy(control)"));
 
     lc.stream = ctx->stream;
-    lc.params = ctx->d_params;
     lc.ctl = ctx->d_ctl;
     lc.fixed_point = c.p2g_mode == MPM_P2G_FIXED_POINT ? 1u : 0u;
     lc.keep_fixed = keep_fixed ? 1u : 0u;
@@ -790,7 +845,7 @@ void mpm_destroy(mpm_ctx* c) {
     if (c->d_mats) (void)hipFree(c->d_mats);
     for (int k = 0; k < PARAM_RING; ++k)
         if (c->ring_events[k]) (void)hipEventDestroy(c->ring_events[k]);
-    if (c->h_param_ring) (void)hipHostFree(c->h_param_ring);
+    if (c->h_desc_ring) (void)hipHostFree(c->h_desc_ring);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -122,15 +122,17 @@ struct Grid {
 };
 
 // Scene content flags, host-derived from the particle models.
-enum : uint32_t { SCENE_SOLID = 1u, SCENE_WATER = 2u, SCENE_OIL = 4u, SCENE_TAIT = 8u };
-// Scenes with solids or Tait liquids carry the FP64 stress cache.
-inline bool scene_general(uint32_t flags) { return (flags & (SCENE_SOLID | SCENE_TAIT)) != 0u; }
+enum : uint32_t { SCENE_SOLID = 1u, SCENE_WATER = 2u, SCENE_OIL = 4u, SCENE_TAIT = 8u, SCENE_GENERAL = 16u };
+// Scenes with solids or Tait liquids carry the FP64 stress cache; SCENE_GENERAL
+// asks for it regardless (a scene stepped together with such scenes).
+__host__ __device__ inline bool scene_general(uint32_t flags) {
+    return (flags & (SCENE_SOLID | SCENE_TAIT | SCENE_GENERAL)) != 0u;
+}
 
 struct KernelTimes;  // host-side timing (mpm_timing.h)
 
 struct LaunchCtx {
     hipStream_t stream;
-    const Params* params;  // device
     Control* ctl;          // device
     SlotBuf buf[2];
     Home home;
@@ -154,7 +156,47 @@ struct LaunchCtx {
 };
 constexpr uint32_t DIAG_PART_BLOCKS = 1024;  // blocks of the particle-sum kernel
 
-hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure, uint32_t seq);
+// One scene of a step launch, in device memory.  The step kernels of a
+// launch cover the tiles of every scene; a block finds its scene through a
+// tile -> scene map and reads everything scene-specific from here.
+struct SceneDesc {
+    Params P;           // this scene's step parameters
+    Control* ctl;
+    SlotBuf buf[2];
+    const Material* mats;
+    Grid grid;
+    double* diag_grid;  // per-tile grid sums of the last P2G (diagnostics on), else nullptr
+    double* diag_part;  // particle partial sums
+    double* diag;       // the nine reported sums
+    uint32_t nmat, bw, bh, nb, cap;
+    uint32_t ptx, pty, gtx, gty;  // P2G and G2P tile grids
+    uint32_t p2g_base, g2p_base;  // first tile of this scene in the P2G / G2P launches
+    uint32_t scene_flags, keep_fixed;
+    uint32_t seq;                 // batch sequence number of this scene
+    uint32_t pad;
+};
+
+// Host description of one step launch over `nscenes` scenes that share a
+// stream and start from the same slot buffer.
+struct StepBatch {
+    hipStream_t stream;
+    const SceneDesc* d_scenes;   // device, nscenes entries (each carries its Params)
+    const uint32_t* p2g_scene;   // device: scene of every P2G tile
+    const uint32_t* g2p_scene;   // device: scene of every G2P tile
+    uint32_t nscenes;
+    uint32_t p2g_tiles, g2p_tiles;
+    uint32_t max_slots;          // largest nb * cap of the scenes
+    uint32_t max_p2g_tiles;      // largest per-scene P2G tile count
+    uint32_t scene_flags;        // union of the scenes' flags (kernel variant)
+    uint32_t fixed_point;
+    bool diag;                   // some scene has diagnostics on
+    KernelTimes* timing;
+};
+
+constexpr int P_TILE = 28;      // interior nodes per P2G tile side
+constexpr int G_TILE_BUCKETS = 8;  // buckets per G2P tile side
+
+hipError_t launch_steps(const StepBatch& sb, int start_buf, uint32_t steps, bool measure);
 // Bucket particles [first, first+count) of the home arrays into buffer b
 // (counts must already hold the particles bucketed so far).
 hipError_t launch_bucket(const LaunchCtx& lc, int b, uint32_t first, uint32_t count);

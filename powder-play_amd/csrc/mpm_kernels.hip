@@ -50,7 +50,7 @@ constexpr uint32_t MAX_DEFORMATION_SUBSTEPS = 64u;
 
 // P2G tile: 28x28 interior nodes (7x7 buckets), a one-node ring for the
 // liquid-contact volume gradient (30x30 computed), fed by 9x9 buckets.
-constexpr int P_TI = 28;                        // interior nodes per tile side
+constexpr int P_TI = P_TILE;                    // interior nodes per tile side
 constexpr int P_TC = P_TI + 2;                  // with the one-node ring
 constexpr int P_RB = P_TI / BUCKET_CELLS + 2;   // region buckets per side
 constexpr int P_NB = P_RB * P_RB;
@@ -65,7 +65,7 @@ static_assert(P_TC <= 32 && P_TC * 32 <= P_THREADS, "one node thread per tile no
 static_assert(P_CELLS <= 2 * P_THREADS, "the cell scan takes two cells a thread");
 static_assert(P_NB <= 128, "the bucket scan takes two buckets a lane of one wave");
 // G2P tile: 8x8 buckets (32x32 cells) and their 34x34 node patch.
-constexpr int G_TB = 8;
+constexpr int G_TB = G_TILE_BUCKETS;
 constexpr int G_TC = G_TB * BUCKET_CELLS;
 constexpr int G_TN = G_TC + 2;
 constexpr int G_THREADS = 256;
@@ -326,8 +326,9 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t tiles) {
 }
 
 // ------------------------------------------------------------ batch frame
-__global__ void k_batch_begin(Control* ctl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ void k_batch_begin(const SceneDesc* __restrict__ scenes) {
+    Control* ctl = scenes[blockIdx.x].ctl;  // one block per scene
+    if (threadIdx.x == 0) {
         ctl->completed = 0u;
         ctl->max_speed_bits = 0u;
         // a flag already raised stops the batch before its first microstep
@@ -609,13 +610,22 @@ __device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& gri
     return flags;
 }
 
+// The scene and scene-local tile of this block of a step launch: blocks
+// are spread over the XCDs in bands of consecutive global tiles (xcd_tile),
+// and the tile -> scene map names the scene of each global tile (a
+// one-scene launch skips the lookup).
+#define SCENE_PROLOGUE(MAP, BASE)                                                    \
+    const uint32_t global_tile_ = xcd_tile(gridDim.x);                               \
+    const SceneDesc& S = scenes[nscenes == 1u ? 0u : (MAP)[global_tile_]];           \
+    const uint32_t tile = global_tile_ - S.BASE
+
 // Shared head of both P2G kernels: status gate, deferred stress-cache flags,
 // clearing the next bucket counts this tile owns, and the staged offsets of
 // the 9x9 region buckets (s_boff, one wave, two buckets a lane).  Returns
 // false when a flag is already raised (the whole block returns).
-__device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32_t* cnt_next, uint32_t tiles_x,
-                                         uint32_t bw, uint32_t bh, uint32_t cap, uint32_t step, int in_buf,
-                                         uint32_t* s_boff) {
+__device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32_t* cnt_next, uint32_t tile,
+                                         uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap, uint32_t step,
+                                         int in_buf, uint32_t* s_boff) {
     const int tid = threadIdx.x;
     const int out_buf = in_buf ^ 1;
     // After a failed microstep the output buffer of this one holds the
@@ -631,7 +641,6 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
         ctl->deferred[out_buf] = 0u;
         ctl->deferred_index[out_buf] = 0xffffffffu;
     }
-    const uint32_t tile = xcd_tile(gridDim.x);
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     // this tile owns buckets [7tx, 7tx+7) x [7ty, 7ty+7): clear their next counts
     if (tid < 49) {
@@ -723,9 +732,17 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 // chunks.  The LDS footprint stays under half a CU so two tiles share one.
 template <bool GENERAL>
 __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint32_t* __restrict__ cnt_next,
-                  const Material* __restrict__ mats, uint32_t nmat, Grid grid, uint32_t tiles_x, uint32_t bw,
-                  uint32_t bh, uint32_t cap, uint32_t scene_flags, uint32_t step, int in_buf, double* diag_grid) {
+void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                  uint32_t step, int in_buf) {
+    SCENE_PROLOGUE(tile_scene, p2g_base);
+    const Params* __restrict__ Pp = &S.P;
+    Control* ctl = S.ctl;
+    const SlotBuf in = S.buf[in_buf];
+    uint32_t* cnt_next = S.buf[in_buf ^ 1].cnt;
+    const Material* __restrict__ mats = S.mats;
+    const Grid grid = S.grid;
+    const uint32_t nmat = S.nmat, tiles_x = S.ptx, bw = S.bw, bh = S.bh, cap = S.cap, scene_flags = S.scene_flags;
+    double* diag_grid = S.diag_grid;
     __shared__ uint32_t s_boff[P_NB + 1];    // staged offset of each region bucket
     __shared__ uint32_t s_chunk[P_NB + 1];   // chunk boundaries (region bucket indices)
     __shared__ uint32_t s_nchunks;
@@ -739,10 +756,9 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
     __shared__ uint32_t s_pid[G_CAP];  // particle indices at provisional places
 
     stage_materials(s_mat, mats, nmat);
-    if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    if (!p2g_head(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     const int tid = threadIdx.x;
     const Params& P = *Pp;
-    const uint32_t tile = xcd_tile(gridDim.x);
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     const int lx = tid % 32, ly = tid / 32;
     const int X0 = tx * P_TI, Y0 = ty * P_TI;
@@ -943,12 +959,18 @@ void k_p2g_gather(const Params* __restrict__ Pp, Control* ctl, SlotBuf in, uint3
 // outside int32 is the accumulator overflow of 65536.  Scenes without solids
 // keep only the eight liquid channels, so two tiles fit in one CU's LDS.
 template <bool GENERAL, bool SOLIDS>
-__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_p2g_fixed(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
-                                                       uint32_t* __restrict__ cnt_next,
-                                                       const Material* __restrict__ mats, Grid grid,
-                                                       uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
-                                                       uint32_t keep_fixed, uint32_t scene_flags, uint32_t step,
-                                                       int in_buf, double* diag_grid) {
+__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                 uint32_t step, int in_buf) {
+    SCENE_PROLOGUE(tile_scene, p2g_base);
+    const Params* __restrict__ Pp = &S.P;
+    Control* ctl = S.ctl;
+    const SlotBuf in = S.buf[in_buf];
+    uint32_t* cnt_next = S.buf[in_buf ^ 1].cnt;
+    const Material* __restrict__ mats = S.mats;
+    const Grid grid = S.grid;
+    const uint32_t tiles_x = S.ptx, bw = S.bw, bh = S.bh, cap = S.cap, scene_flags = S.scene_flags;
+    const uint32_t keep_fixed = S.keep_fixed;
+    double* diag_grid = S.diag_grid;
     // words [solid m, qx, qy |] water m, qx, qy, V | oil m, qx, qy, V
     constexpr int NCH = SOLIDS ? 11 : 8;
     constexpr int SKIP = SOLIDS ? 0 : 3;  // channel index = word index - SKIP
@@ -956,11 +978,10 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
     __shared__ uint32_t s_boff[P_NB + 1];
 
     for (int e = threadIdx.x; e < NCH * P_TC * P_TC; e += P_THREADS) (&s_acc[0][0])[e] = 0ull;
-    if (!p2g_head(ctl, in, cnt_next, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    if (!p2g_head(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     __syncthreads();
     const int tid = threadIdx.x;
     const Params& P = *Pp;
-    const uint32_t tile = xcd_tile(gridDim.x);
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     const int X0 = tx * P_TI, Y0 = ty * P_TI;
     const int W = (int)P.node_w, H = (int)P.node_h;
@@ -1389,10 +1410,17 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 // inputs of a thread's next particle are fetched before the current one is
 // processed, so their latency overlaps the current G2P and slot reservation.
 template <bool GENERAL>
-__global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp, Control* ctl, SlotBuf in,
-                                                 SlotBuf out, const Material* __restrict__ mats, uint32_t nmat,
-                                                 Grid grid, uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap,
-                                                 uint32_t scene_flags, uint32_t step, int out_buf) {
+__global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__ scenes,
+                                                 const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                                                 uint32_t step, int out_buf) {
+    SCENE_PROLOGUE(tile_scene, g2p_base);
+    const Params* __restrict__ Pp = &S.P;
+    Control* ctl = S.ctl;
+    const SlotBuf in = S.buf[out_buf ^ 1];
+    const SlotBuf out = S.buf[out_buf];
+    const Material* __restrict__ mats = S.mats;
+    const Grid grid = S.grid;
+    const uint32_t nmat = S.nmat, tiles_x = S.gtx, bw = S.bw, bh = S.bh, cap = S.cap, scene_flags = S.scene_flags;
     __shared__ float2 s_vs[G_TN * G_TN];
     __shared__ float2 s_vw[G_TN * G_TN];
     __shared__ float2 s_vo[G_TN * G_TN];
@@ -1402,7 +1430,6 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp
     if (block_status_set(ctl)) return;
     const Params& P = *Pp;
     const int tid = threadIdx.x;
-    const uint32_t tile = xcd_tile(gridDim.x);
     const int tbx = (int)(tile % tiles_x) * G_TB, tby = (int)(tile / tiles_x) * G_TB;
     const int nx0 = tbx * BUCKET_CELLS, ny0 = tby * BUCKET_CELLS;  // first node (= first cell) of the patch
     const int W = (int)P.node_w, H = (int)P.node_h;
@@ -1528,8 +1555,10 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const Params* __restrict__ Pp
 }
 
 // ------------------------------------------------------------- batch end
-__global__ void k_commit_last(Control* ctl, uint32_t steps, uint32_t seq) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ void k_commit_last(const SceneDesc* __restrict__ scenes, uint32_t steps) {
+    Control* ctl = scenes[blockIdx.x].ctl;  // one block per scene
+    const uint32_t seq = scenes[blockIdx.x].seq;
+    if (threadIdx.x == 0) {
         const uint32_t done = min(ctl->fail_step, steps);
         ctl->completed = done;
         ctl->history[seq % CONTROL_HISTORY] = done;
@@ -1539,9 +1568,14 @@ __global__ void k_commit_last(Control* ctl, uint32_t steps, uint32_t seq) {
 // On a failed microstep the committed state sits in the buffer of the last
 // committed microstep; copy it to the batch's output buffer when their
 // parity differs (commitParticles' whole-step rollback, shaders.ts:3067-3080).
-__global__ __launch_bounds__(256) void k_fix_rollback(Control* ctl, SlotBuf b0, SlotBuf b1, uint32_t nb,
-                                                      uint32_t cap, uint32_t start_buf, uint32_t steps,
-                                                      uint32_t general) {
+__global__ __launch_bounds__(256) void k_fix_rollback(const SceneDesc* __restrict__ scenes, uint32_t start_buf,
+                                                      uint32_t steps) {
+    const SceneDesc& S = scenes[blockIdx.y];  // one row of blocks per scene
+    Control* ctl = S.ctl;
+    const SlotBuf& b0 = S.buf[0];
+    const SlotBuf& b1 = S.buf[1];
+    const uint32_t nb = S.nb, cap = S.cap;
+    const bool general = scene_general(S.scene_flags);
     if (ctl->status[0] == 0u) return;
     const uint32_t committed = (start_buf + ctl->completed) & 1u;
     const uint32_t final_buf = (start_buf + steps) & 1u;
@@ -1574,7 +1608,11 @@ __global__ __launch_bounds__(256) void k_fix_rollback(Control* ctl, SlotBuf b0, 
 // measureParticleSpeedPartials + reduceParticleSpeedPartials
 // (shaders.ts:2463-2533): wave max, then one atomic per block on the
 // non-negative float's bit pattern.
-__global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, SlotBuf buf, uint32_t nb, uint32_t cap) {
+__global__ __launch_bounds__(256) void k_measure_speed(const SceneDesc* __restrict__ scenes, uint32_t which) {
+    const SceneDesc& S = scenes[blockIdx.y];  // one row of blocks per scene
+    Control* ctl = S.ctl;
+    const SlotBuf& buf = S.buf[which];
+    const uint32_t nb = S.nb, cap = S.cap;
     __shared__ float s_w[4];
     float m = 0.0f;
     const size_t total = (size_t)nb * cap;
@@ -1595,10 +1633,14 @@ __global__ __launch_bounds__(256) void k_measure_speed(Control* ctl, SlotBuf buf
 }
 
 // Particle mass and momentum (totalMass / totalMomentum,
-// reference/solver.ts:2505-2526) in FP64: block b sums buckets
-// [b*nb/B, (b+1)*nb/B) in slot order into part[3b..3b+2].
-__global__ __launch_bounds__(256) void k_particle_sums(SlotBuf buf, const Material* __restrict__ mats, uint32_t nb,
-                                                       uint32_t cap, double* part) {
+// reference/solver.ts:2505-2526) in FP64 for the scenes with diagnostics on:
+// block b of a scene sums its buckets [b*nb/B, (b+1)*nb/B) in slot order
+// into diag_part[3b..3b+2].
+__global__ __launch_bounds__(256) void k_particle_sums(const SceneDesc* __restrict__ scenes, uint32_t which) {
+    const SceneDesc& S = scenes[blockIdx.y];  // one row of blocks per scene
+    if (!S.diag) return;
+    const SlotBuf& buf = S.buf[which];
+    const uint32_t nb = S.nb, cap = S.cap;
     const uint32_t b0 = (uint32_t)((uint64_t)blockIdx.x * nb / gridDim.x);
     const uint32_t b1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * nb / gridDim.x);
     double m = 0.0, px = 0.0, py = 0.0;
@@ -1606,19 +1648,26 @@ __global__ __launch_bounds__(256) void k_particle_sums(SlotBuf buf, const Materi
         const uint32_t c = min(buf.cnt[b], cap);
         for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) {
             const size_t g = (size_t)b * cap + k;
-            const double mass = (double)mats[buf.mat[g]].mass;
+            const double mass = (double)S.mats[buf.mat[g]].mass;
             const float2 v = buf.vel[g];
             m += mass;
             px += mass * (double)v.x;
             py += mass * (double)v.y;
         }
     }
-    block_sum3_f64(m, px, py, part + 3 * blockIdx.x);
+    block_sum3_f64(m, px, py, S.diag_part + 3 * blockIdx.x);
 }
 
-// Sum n partial triples in a fixed order into out[0], out[off], out[2*off].
-__global__ __launch_bounds__(256) void k_finish_sums(const double* __restrict__ part, uint32_t n, double* out,
-                                                     uint32_t off) {
+// One block per scene with diagnostics on: the partial triples in a fixed
+// order into diag[slot], diag[slot + 3], diag[slot + 6] (mass, momentum x,
+// momentum y).  grid != 0 sums the per-tile grid partials of the last P2G,
+// otherwise the `parts` particle partials.
+__global__ __launch_bounds__(256) void k_finish_sums(const SceneDesc* __restrict__ scenes, uint32_t grid,
+                                                     uint32_t parts, uint32_t slot) {
+    const SceneDesc& S = scenes[blockIdx.x];
+    if (!S.diag) return;
+    const double* part = grid ? S.diag_grid : S.diag_part;
+    const uint32_t n = grid ? S.ptx * S.pty : parts;
     double a = 0.0, b = 0.0, c = 0.0;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         a += part[3 * i];
@@ -1629,9 +1678,9 @@ __global__ __launch_bounds__(256) void k_finish_sums(const double* __restrict__ 
     block_sum3_f64(a, b, c, s_out);
     __syncthreads();
     if (threadIdx.x == 0) {
-        out[0] = s_out[0];
-        out[off] = s_out[1];
-        out[2 * off] = s_out[2];
+        S.diag[slot] = s_out[0];
+        S.diag[slot + 3] = s_out[1];
+        S.diag[slot + 6] = s_out[2];
     }
 }
 
@@ -1647,57 +1696,55 @@ inline uint32_t blocks_for(size_t n, uint32_t t) { return (uint32_t)((n + t - 1)
         if (_e != hipSuccess) return _e;                                          \
     } while (0)
 
-hipError_t launch_batch(const LaunchCtx& lc, int start_buf, uint32_t steps, bool measure, uint32_t seq) {
-    const bool general = scene_general(lc.scene_flags);
-    const uint32_t ptx = (lc.node_w + P_TI - 1) / P_TI, pty = (lc.node_h + P_TI - 1) / P_TI;
-    const uint32_t gtx = (lc.bw + G_TB - 1) / G_TB, gty = (lc.bh + G_TB - 1) / G_TB;
-    const size_t slots = (size_t)lc.nb * lc.cap;
-    MPM_LAUNCH("batch_begin", k_batch_begin, dim3(1), dim3(64), 0, lc.stream, lc.ctl);
-    const bool diag = lc.diag != nullptr;
-    // diag[9] = mass (input, grid, output), momentum x (input, grid, output), momentum y (...)
-    if (diag) {
-        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(DIAG_PART_BLOCKS), dim3(256), 0, lc.stream,
-                   lc.buf[start_buf], lc.mats, lc.nb, lc.cap, lc.diag_part);
-        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(1), dim3(256), 0, lc.stream, lc.diag_part, DIAG_PART_BLOCKS,
-                   lc.diag + 0, 3u);
+hipError_t launch_steps(const StepBatch& sb, int start_buf, uint32_t steps, bool measure) {
+    const bool general = scene_general(sb.scene_flags);
+    const uint32_t ns = sb.nscenes;
+    struct {
+        hipStream_t stream;
+        KernelTimes* timing;
+    } lc = {sb.stream, sb.timing};
+    // particle partial blocks per scene: one scene uses the whole GPU
+    const uint32_t parts = ns == 1 ? DIAG_PART_BLOCKS : 32u;
+    MPM_LAUNCH("batch_begin", k_batch_begin, dim3(ns), dim3(64), 0, lc.stream, sb.d_scenes);
+    if (sb.diag) {
+        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes,
+                   (uint32_t)start_buf);
+        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u, parts, 0u);
     }
     int cur = start_buf;
     for (uint32_t s = 0; s < steps; ++s) {
-        const SlotBuf& in = lc.buf[cur];
-        const SlotBuf& out = lc.buf[cur ^ 1];
-        if (lc.fixed_point) {
-            const bool solids = (lc.scene_flags & SCENE_SOLID) != 0u;
+        if (sb.fixed_point) {
+            const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
             auto p2g = general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
                                : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
-            MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
-                       out.cnt, lc.mats, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.keep_fixed, lc.scene_flags, s, cur,
-                       lc.diag_grid);
+            MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes,
+                       sb.p2g_scene, sb.nscenes, s, cur);
         } else {
             auto p2g = general ? k_p2g_gather<true> : k_p2g_gather<false>;
-            MPM_LAUNCH("p2g_grid", p2g, dim3(ptx * pty), dim3(P_THREADS), 0, lc.stream, lc.params, lc.ctl, in,
-                       out.cnt, lc.mats, lc.nmat, lc.grid, ptx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur,
-                       lc.diag_grid);
+            MPM_LAUNCH("p2g_grid", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
+                       sb.nscenes, s, cur);
         }
         auto g2p = general ? k_g2p<true> : k_g2p<false>;
-        MPM_LAUNCH("g2p", g2p, dim3(gtx * gty), dim3(G_THREADS), 0, lc.stream, lc.params, lc.ctl, in, out, lc.mats,
-                   lc.nmat, lc.grid, gtx, lc.bw, lc.bh, lc.cap, lc.scene_flags, s, cur ^ 1);
+        MPM_LAUNCH("g2p", g2p, dim3(sb.g2p_tiles), dim3(G_THREADS), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
+                   sb.nscenes, s, cur ^ 1);
         cur ^= 1;
     }
-    MPM_LAUNCH("commit_last", k_commit_last, dim3(1), dim3(64), 0, lc.stream, lc.ctl, steps, seq);
-    const uint32_t fb = std::min<uint32_t>(blocks_for(slots, 256), 2048u);
-    MPM_LAUNCH("fix_rollback", k_fix_rollback, dim3(fb), dim3(256), 0, lc.stream, lc.ctl, lc.buf[0], lc.buf[1], lc.nb,
-               lc.cap, (uint32_t)start_buf, steps, general ? 1u : 0u);
-    if (diag) {
-        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(1), dim3(256), 0, lc.stream, lc.diag_grid, ptx * pty,
-                   lc.diag + 1, 3u);
-        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(DIAG_PART_BLOCKS), dim3(256), 0, lc.stream, lc.buf[cur],
-                   lc.mats, lc.nb, lc.cap, lc.diag_part);
-        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(1), dim3(256), 0, lc.stream, lc.diag_part, DIAG_PART_BLOCKS,
-                   lc.diag + 2, 3u);
+    MPM_LAUNCH("commit_last", k_commit_last, dim3(ns), dim3(64), 0, lc.stream, sb.d_scenes, steps);
+    const uint32_t fb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(sb.max_slots, 256), ns == 1 ? 2048u : 64u));
+    MPM_LAUNCH("fix_rollback", k_fix_rollback, dim3(fb, ns), dim3(256), 0, lc.stream, sb.d_scenes,
+               (uint32_t)start_buf, steps);
+    if (sb.diag) {
+        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 1u, parts, 1u);
+        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes,
+                   (uint32_t)cur);
+        MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u, parts, 2u);
     }
-    if (measure)
-        MPM_LAUNCH("measure_speed", k_measure_speed, dim3(std::min<uint32_t>(blocks_for(slots, 256), 1024u)),
-                   dim3(256), 0, lc.stream, lc.ctl, lc.buf[cur], lc.nb, lc.cap);
+    if (measure) {
+        const uint32_t mb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(sb.max_slots, 256),
+                                                                       ns == 1 ? 1024u : 32u));
+        MPM_LAUNCH("measure_speed", k_measure_speed, dim3(mb, ns), dim3(256), 0, lc.stream, sb.d_scenes,
+                   (uint32_t)cur);
+    }
     return hipSuccess;
 }
 
