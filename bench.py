@@ -122,6 +122,48 @@ def cpu_baseline(pkg, arr, config, budget_s=12.0):
     return len(arr) * steps / el, steps, el
 
 
+def batched_leg(pkg, steps: int):
+    """256 independent Tait tanks (scenes.batched_tanks): `steps` microsteps
+    of all of them in one batch, against the same scenes as separate solvers
+    stepped one after another (each synchronised before the next starts)."""
+    tanks = pkg.scenes.batched_tanks()
+    scenes = [(a, c) for a, c, _ in tanks]
+    gravity = [g for _, _, g in tanks]
+    total = sum(len(a) for a, _, _ in tanks)
+    dt = pkg.scenes.BATCH_DT
+    log(f"[bench] batched: {len(tanks)} tanks, {total} particles, {steps} microsteps")
+    batch = pkg.MpmMlsGpuBatch(scenes, device=0)
+    batch.step_batch(8, dt, gravity)
+    batch.scene(0).ctx.read_compact()
+    t0 = time.perf_counter()
+    batch.step_batch(steps, dt, gravity)
+    batch.scene(0).ctx.read_compact()  # waits for the batch's stream
+    t_batch = time.perf_counter() - t0
+    bad = [i for i in range(len(batch)) if batch.scene(i).ctx.read_compact()[2][0] != 0]
+    batch.dispose()
+    if bad:
+        raise RuntimeError(f"batched scenes {bad[:8]} raised numerical flags")
+    solvers = [pkg.create_mpm_mls_gpu_solver(a, c, device=0) for a, c in scenes]
+    for s, g in zip(solvers, gravity):
+        s.step_batch(8, dt, g)
+        s.ctx.read_compact()
+    t0 = time.perf_counter()
+    for s, g in zip(solvers, gravity):
+        s.step_batch(steps, dt, g)
+        s.ctx.read_compact()
+    t_seq = time.perf_counter() - t0
+    for s in solvers:
+        s.dispose()
+    rate, rate_seq = total * steps / t_batch, total * steps / t_seq
+    log(f"[bench] batched: {1e3 * t_batch / steps:.4f} ms per microstep of all scenes ({rate:.3e} particle-steps/s); "
+        f"sequential {1e3 * t_seq / steps:.4f} ms ({rate_seq:.3e}); x{rate / rate_seq:.1f}")
+    return {"value": rate, "unit": "particle-steps/s", "ms_per_step": 1e3 * t_batch / steps, "scenes": len(tanks),
+            "particles": total, "steps": steps, "sequential_value": rate_seq,
+            "sequential_ms_per_step": 1e3 * t_seq / steps, "speedup_vs_sequential": rate / rate_seq,
+            "workload": f"{len(tanks)} SI-material Tait tanks, {total} particles (log-uniform 500-50,000 per tank), "
+                        "64-512 cell grids, dam-break blocks, random gravity directions, float32 gather P2G"}
+
+
 def pmc_sample(pkg, arr, config, mode):
     """Workload of the FETCH_SIZE / WRITE_SIZE counter runs.  The copy kernel
     reads and writes COPY_BYTES each, so its counter values per byte scale
@@ -149,6 +191,8 @@ def main():
     ap.add_argument("--cells", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=["both", "gather", "fixed"], default="both")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batched-steps", type=int, default=480)
     ap.add_argument("--pmc-sample", action="store_true",
                     help="counter-collection workload: one 256 MiB copy kernel (the HBM byte calibration), then "
                          "8 untimed microsteps per mode; prints nothing")
@@ -210,6 +254,8 @@ def main():
     step_bytes = PARTICLE_BYTES * n + NODE_PHASE_BYTES * phases * nodes
     step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
 
+    batched = None if args.no_batched or args.cells != 1000 else batched_leg(pkg, args.batched_steps)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline: FP64 oracle step, one thread")
@@ -243,6 +289,7 @@ def main():
                      "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_us": dom_us,
                      "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)) if traffic else None},
         "kernels_us": kernel_times,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

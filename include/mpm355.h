@@ -142,13 +142,37 @@ int mpm_get_buckets(mpm_ctx* ctx, uint32_t* buckets_x, uint32_t* buckets_y, uint
 int mpm_set_diagnostics(mpm_ctx* ctx, uint32_t enable);
 int mpm_read_diagnostics(mpm_ctx* ctx, double out[9]);
 
+/* Batched scenes: many independent scenes advanced by the same kernels, one
+ * launch per microstep over the tiles of every scene.  The reference has no
+ * counterpart (one solver per scene, solver.ts:390-406); this serves callers
+ * that step many small scenes, whose single-scene steps would be bound by
+ * launch latency.  Each scene keeps its own status words, rollback and
+ * bucket growth, exactly as a context from mpm_create.
+ *
+ * mpm_batch_create: scenes [0, nscenes) from configs[i], packed20[i] and
+ *   counts[i] (same device and P2G mode for all).
+ * mpm_batch_step: params holds nscenes consecutive 144-byte blocks; every
+ *   scene advances step_count microsteps with its own block.
+ * mpm_batch_scene: the context of one scene, for the per-scene calls
+ *   (mpm_read_compact, mpm_read_particles, mpm_read_status,
+ *   mpm_clear_status, diagnostics...).  It belongs to the batch:
+ *   mpm_destroy ignores it and mpm_batch_destroy releases it. */
+typedef struct mpm_batch mpm_batch;
+int mpm_batch_create(const mpm_config* configs, const uint32_t* const* packed20, const uint32_t* counts,
+                     uint32_t nscenes, mpm_batch** out);
+int mpm_batch_step(mpm_batch* batch, const void* params, uint32_t step_count, uint32_t measure_speed);
+mpm_ctx* mpm_batch_scene(mpm_batch* batch, uint32_t scene);
+uint32_t mpm_batch_count(mpm_batch* batch);
+void mpm_batch_destroy(mpm_batch* batch);
+const char* mpm_batch_last_error(mpm_batch* batch);
+
 /* Mean kernel time in microseconds of the named kernel over the last
  * mpm_step_batch calls made while timing was enabled (HIP events on the
  * context's stream). enable != 0 turns timing on and resets counters. */
 int mpm_set_timing(mpm_ctx* ctx, uint32_t enable);
 int mpm_read_timing(mpm_ctx* ctx, const char* kernel, double* mean_us, uint64_t* launches);
 
-/* dispose (solver.ts:2950-2964). Safe on NULL. */
+/* dispose (solver.ts:2950-2964). Safe on NULL; ignores the scenes of a batch. */
 void mpm_destroy(mpm_ctx* ctx);
 
 /* Message of the last failed call on ctx (or of the last failed mpm_create

@@ -9,6 +9,7 @@ directory name carries a hyphen; see ``load_package`` in ``__graft_entry__``).
 from . import cfl, layout, native, scenes  # noqa: F401
 from .layout import ParticleArrays, RangeError  # noqa: F401
 from .solver import (  # noqa: F401
+    MpmMlsGpuBatch,
     MpmMlsGpuNumericalError,
     MpmMlsGpuSolver,
     create_experimental_mpm_mls_atomic_gpu_solver,
@@ -17,5 +18,5 @@ from .solver import (  # noqa: F401
 
 __all__ = [
     "cfl", "layout", "native", "scenes", "ParticleArrays", "RangeError", "MpmMlsGpuNumericalError",
-    "MpmMlsGpuSolver", "create_mpm_mls_gpu_solver", "create_experimental_mpm_mls_atomic_gpu_solver",
+    "MpmMlsGpuSolver", "MpmMlsGpuBatch", "create_mpm_mls_gpu_solver", "create_experimental_mpm_mls_atomic_gpu_solver",
 ]

@@ -93,7 +93,31 @@ struct mpm_ctx {
     std::string error = "no error";
     KernelTimes* times = nullptr;
     bool diag_on = false;
+    bool shared_stream = false;        // the stream belongs to a batch
+    struct mpm_batch* owner = nullptr; // the batch that owns this scene
     double* diag_saved = nullptr;  // diagnostics buffer while the reductions are off
+};
+
+namespace {
+
+}  // namespace
+
+// Independent scenes stepped together: every step launch covers the tiles
+// of all scenes.  Each scene is a full context (own buffers, status, CFL
+// bookkeeping and bucket growth) on the batch's stream.
+struct mpm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<mpm_ctx*> scenes;
+    std::vector<uint32_t> p2g_base, g2p_base;  // first tile of each scene
+    uint32_t p2g_tiles = 0, g2p_tiles = 0;
+    SceneDesc* d_scenes = nullptr;   // device
+    SceneDesc* h_scenes = nullptr;   // pinned staging
+    uint32_t* d_p2g_map = nullptr;   // device: scene of every tile
+    uint32_t* d_g2p_map = nullptr;
+    hipEvent_t staged = nullptr;     // the staging buffer's last copy
+    bool staged_used = false;
+    std::string error = "no error";
 };
 
 namespace {
@@ -339,6 +363,7 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
     d.scene_flags = lc.scene_flags;
     d.keep_fixed = lc.keep_fixed;
     d.seq = seq;
+    d.start_buf = (uint32_t)c->cur;
     return d;
 }
 
@@ -374,7 +399,7 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     sb.fixed_point = c->lc.fixed_point;
     sb.diag = c->lc.diag != nullptr;
     sb.timing = c->times;
-    HIP_TRY(c, launch_steps(sb, c->cur, steps, measure));
+    HIP_TRY(c, launch_steps(sb, steps, measure));
     c->pending.push_back({p, steps, measure, seq, done_before});
     c->last_done_before = done_before;
     c->cur = (c->cur + (int)(steps & 1u)) & 1;
@@ -424,6 +449,67 @@ int settle(mpm_ctx* c) {
 }
 
 }  // namespace
+
+// RN(1/b) for the fixed-point kernel's division by particleMass and
+// particleVolume (reciprocal plus one FMA correction, exact for divisors in
+// [2^-20, 2^20]); 0 outside that range selects plain division.
+static float reciprocal_for_division(float b) {
+    if (!(b >= 0x1p-20f && b <= 0x1p20f)) return 0.0f;
+    volatile float one = 1.0f;
+    return one / b;
+}
+
+// Decode and check one 144-byte parameter block for context c (the checks
+// stepBatch makes before encoding, solver.ts:1414-1441), and add the parts
+// the host owns: drag uniform, diagnostics switch, exact reciprocals.
+int prepare_params(mpm_ctx* c, const void* params144, Params& p) {
+    if (!params144) return fail(c, MPM_EINVAL, "params must not be NULL");
+    p = Params{};
+    memcpy(&p, params144, MPM_PARAMS_BYTES);
+    const LaunchCtx& lc = c->lc;
+    if (p.cells_x != c->cfg.cells_x || p.cells_y != c->cfg.cells_y || p.node_w != lc.node_w || p.node_h != lc.node_h)
+        return fail(c, MPM_EINVAL, "params grid does not match the solver grid");
+    if (p.node_count != lc.node_count) return fail(c, MPM_EINVAL, "params nodeCount does not match the solver grid");
+    if (p.particle_count != c->n)
+        return fail(c, MPM_EINVAL, "params particleCount %u does not match the %u allocated particle slots",
+                    p.particle_count, c->n);
+    if (p.body_count != 0u) return fail(c, MPM_EINVAL, "bonded triangle bodies are not supported");
+    if (!(p.dt > 0.0f) || !isfinite(p.dt)) return fail(c, MPM_EINVAL, "dt must be greater than zero");
+    if (p.dx != c->cfg.cell_size) return fail(c, MPM_EINVAL, "params cellSize does not match the solver grid");
+    if (p.particle_mass != c->cfg.particle_mass || p.particle_volume != c->cfg.particle_volume)
+        return fail(c, MPM_EINVAL, "params particleMass/particleVolume do not match the solver configuration");
+    if (!isfinite(p.gx) || !isfinite(p.gy)) return fail(c, MPM_EINVAL, "gravity must be finite");
+    if (!isfinite(p.grad_scale) || !(p.grad_scale > 0.0f)) return fail(c, MPM_EINVAL, "params gradient scale is invalid");
+    if (p.cfl_on >= 0.5f && (!isfinite(p.wave_bound) || !isfinite(p.vel_bound) || p.vel_bound < 0.0f))
+        return fail(c, MPM_EINVAL, "CFL bounds must be finite and non-negative");
+    p.diag_on = c->diag_on ? 1u : 0u;
+    p.drag_pad[0] = p.drag_pad[1] = 0.0f;
+    p.inv_mass_ref = reciprocal_for_division(p.particle_mass);
+    p.inv_volume_ref = reciprocal_for_division(p.particle_volume);
+    p.wall_pad = 0.0f;
+    p.drag_target_x = c->interaction[0];
+    p.drag_target_y = c->interaction[1];
+    p.drag_radius = c->interaction[2];
+    p.drag_stiffness = c->interaction[3];
+    p.drag_damping = c->interaction[4];
+    return MPM_OK;
+}
+
+// Release everything a context owns (its stream too unless a batch owns it).
+static void destroy_ctx(mpm_ctx* c) {
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->times;
+    c->slot_mem.release();
+    c->mem.release();
+    if (c->d_mats) (void)hipFree(c->d_mats);
+    for (int k = 0; k < PARAM_RING; ++k)
+        if (c->ring_events[k]) (void)hipEventDestroy(c->ring_events[k]);
+    if (c->h_desc_ring) (void)hipHostFree(c->h_desc_ring);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->stream && !c->shared_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
 
 extern "C" {
 
@@ -558,47 +644,13 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     return MPM_OK;
 }
 
-// RN(1/b) for the fixed-point kernel's division by particleMass and
-// particleVolume (reciprocal plus one FMA correction, exact for divisors in
-// [2^-20, 2^20]); 0 outside that range selects plain division.
-static float reciprocal_for_division(float b) {
-    if (!(b >= 0x1p-20f && b <= 0x1p20f)) return 0.0f;
-    volatile float one = 1.0f;
-    return one / b;
-}
 
 int mpm_step_batch(mpm_ctx* c, const void* params144, uint32_t step_count, uint32_t measure_speed) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
-    if (!params144) return fail(c, MPM_EINVAL, "params must not be NULL");
     if (step_count < 1) return fail(c, MPM_EINVAL, "stepCount must be a positive safe integer.");
     Params p{};
-    memcpy(&p, params144, MPM_PARAMS_BYTES);
-    const LaunchCtx& lc = c->lc;
-    if (p.cells_x != c->cfg.cells_x || p.cells_y != c->cfg.cells_y || p.node_w != lc.node_w || p.node_h != lc.node_h)
-        return fail(c, MPM_EINVAL, "params grid does not match the solver grid");
-    if (p.node_count != lc.node_count) return fail(c, MPM_EINVAL, "params nodeCount does not match the solver grid");
-    if (p.particle_count != c->n)
-        return fail(c, MPM_EINVAL, "params particleCount %u does not match the %u allocated particle slots",
-                    p.particle_count, c->n);
-    if (p.body_count != 0u) return fail(c, MPM_EINVAL, "bonded triangle bodies are not supported");
-    if (!(p.dt > 0.0f) || !isfinite(p.dt)) return fail(c, MPM_EINVAL, "dt must be greater than zero");
-    if (p.dx != c->cfg.cell_size) return fail(c, MPM_EINVAL, "params cellSize does not match the solver grid");
-    if (p.particle_mass != c->cfg.particle_mass || p.particle_volume != c->cfg.particle_volume)
-        return fail(c, MPM_EINVAL, "params particleMass/particleVolume do not match the solver configuration");
-    if (!isfinite(p.gx) || !isfinite(p.gy)) return fail(c, MPM_EINVAL, "gravity must be finite");
-    if (!isfinite(p.grad_scale) || !(p.grad_scale > 0.0f)) return fail(c, MPM_EINVAL, "params gradient scale is invalid");
-    if (p.cfl_on >= 0.5f && (!isfinite(p.wave_bound) || !isfinite(p.vel_bound) || p.vel_bound < 0.0f))
-        return fail(c, MPM_EINVAL, "CFL bounds must be finite and non-negative");
-    p.diag_on = c->diag_on ? 1u : 0u;
-    p.drag_pad[0] = p.drag_pad[1] = 0.0f;
-    p.inv_mass_ref = reciprocal_for_division(p.particle_mass);
-    p.inv_volume_ref = reciprocal_for_division(p.particle_volume);
-    p.wall_pad = 0.0f;
-    p.drag_target_x = c->interaction[0];
-    p.drag_target_y = c->interaction[1];
-    p.drag_radius = c->interaction[2];
-    p.drag_stiffness = c->interaction[3];
-    p.drag_damping = c->interaction[4];
+    const int v = prepare_params(c, params144, p);
+    if (v) return v;
     HIP_TRY(c, hipSetDevice(c->device));
     return launch(c, p, step_count, measure_speed != 0u || p.cfl_on >= 0.5f, 0u);
 }
@@ -836,21 +888,193 @@ int mpm_read_timing(mpm_ctx* c, const char* kernel, double* mean_us, uint64_t* l
 }
 
 void mpm_destroy(mpm_ctx* c) {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    delete c->times;
-    c->slot_mem.release();
-    c->mem.release();
-    if (c->d_mats) (void)hipFree(c->d_mats);
-    for (int k = 0; k < PARAM_RING; ++k)
-        if (c->ring_events[k]) (void)hipEventDestroy(c->ring_events[k]);
-    if (c->h_desc_ring) (void)hipHostFree(c->h_desc_ring);
-    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
+    if (!c || c->owner) return;  // scenes of a batch go with mpm_batch_destroy
+    destroy_ctx(c);
 }
 
+
 const char* mpm_last_error(mpm_ctx* c) { return c ? c->error.c_str() : g_create_error.c_str(); }
+
+
+// ------------------------------------------------------------------ batches
+
+static int batch_fail(mpm_batch* b, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->error = buf; else g_create_error = buf;
+    return code;
+}
+
+void mpm_batch_destroy(mpm_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (mpm_ctx* c : b->scenes) {
+        if (!c) continue;
+        c->owner = nullptr;
+        destroy_ctx(c);
+    }
+    if (b->d_scenes) (void)hipFree(b->d_scenes);
+    if (b->d_p2g_map) (void)hipFree(b->d_p2g_map);
+    if (b->d_g2p_map) (void)hipFree(b->d_g2p_map);
+    if (b->h_scenes) (void)hipHostFree(b->h_scenes);
+    if (b->staged) (void)hipEventDestroy(b->staged);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int mpm_batch_create(const mpm_config* configs, const uint32_t* const* packed20, const uint32_t* counts,
+                     uint32_t nscenes, mpm_batch** out) {
+    if (!out) return batch_fail(nullptr, MPM_EINVAL, "out must not be NULL");
+    *out = nullptr;
+    if (!configs || !packed20 || !counts) return batch_fail(nullptr, MPM_EINVAL, "configs, packed and counts must not be NULL");
+    if (nscenes < 1 || nscenes > 65536) return batch_fail(nullptr, MPM_EINVAL, "scene count must be in [1, 65536]");
+    for (uint32_t i = 1; i < nscenes; ++i) {
+        if (configs[i].device != configs[0].device)
+            return batch_fail(nullptr, MPM_EINVAL, "scene %u: every scene of a batch must use the same device", i);
+        if (configs[i].p2g_mode != configs[0].p2g_mode)
+            return batch_fail(nullptr, MPM_EINVAL, "scene %u: every scene of a batch must use the same P2G mode", i);
+    }
+    mpm_batch* b = new (std::nothrow) mpm_batch();
+    if (!b) return batch_fail(nullptr, MPM_EDEVICE, "out of host memory");
+    auto bail = [&](int code) {
+        const std::string msg = b->error;
+        mpm_batch_destroy(b);
+        g_create_error = msg;
+        return code;
+    };
+    b->scenes.reserve(nscenes);
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        mpm_ctx* c = nullptr;
+        const int r = mpm_create(&configs[i], packed20[i], nullptr, counts[i], &c);
+        if (r != MPM_OK) {
+            b->error = std::string("scene ") + std::to_string(i) + ": " + mpm_last_error(nullptr);
+            return bail(r);
+        }
+        c->owner = b;
+        b->scenes.push_back(c);
+    }
+    hipError_t e;
+    b->device = b->scenes[0]->device;
+    if ((e = hipSetDevice(b->device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming)) != hipSuccess) {
+        batch_fail(b, MPM_EDEVICE, "batch stream: %s", hipGetErrorString(e));
+        return bail(MPM_EDEVICE);
+    }
+    // every scene moves to the batch's stream
+    uint32_t flags = 0;
+    for (mpm_ctx* c : b->scenes) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->stream = b->stream;
+        c->lc.stream = b->stream;
+        c->shared_stream = true;
+        flags |= c->lc.scene_flags;
+    }
+    // one kernel variant serves the batch: when one scene needs the stress
+    // cache, every scene carries it
+    if (scene_general(flags)) {
+        for (mpm_ctx* c : b->scenes) {
+            if (scene_general(c->lc.scene_flags)) continue;
+            c->lc.scene_flags |= SCENE_GENERAL;
+            int r = alloc_slots(c, c->lc.cap);
+            if (!r) r = rebucket_all(c, c->lc.cap);
+            if (r) {
+                b->error = c->error;
+                return bail(r);
+            }
+        }
+    }
+    // tiles of the step launches, scene after scene
+    std::vector<uint32_t> pmap, gmap;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        mpm_ctx* c = b->scenes[i];
+        b->p2g_base.push_back(b->p2g_tiles);
+        b->g2p_base.push_back(b->g2p_tiles);
+        pmap.insert(pmap.end(), c->p2g_tiles, i);
+        gmap.insert(gmap.end(), c->g2p_tiles, i);
+        b->p2g_tiles += c->p2g_tiles;
+        b->g2p_tiles += c->g2p_tiles;
+    }
+    if ((e = hipMalloc((void**)&b->d_scenes, sizeof(SceneDesc) * nscenes)) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_p2g_map, sizeof(uint32_t) * pmap.size())) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_g2p_map, sizeof(uint32_t) * gmap.size())) != hipSuccess ||
+        (e = hipHostMalloc((void**)&b->h_scenes, sizeof(SceneDesc) * nscenes, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMemcpy(b->d_p2g_map, pmap.data(), sizeof(uint32_t) * pmap.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(b->d_g2p_map, gmap.data(), sizeof(uint32_t) * gmap.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+        batch_fail(b, MPM_EDEVICE, "batch buffers: %s", hipGetErrorString(e));
+        return bail(MPM_EDEVICE);
+    }
+    *out = b;
+    return MPM_OK;
+}
+
+mpm_ctx* mpm_batch_scene(mpm_batch* b, uint32_t scene) {
+    if (!b || scene >= b->scenes.size()) return nullptr;
+    return b->scenes[scene];
+}
+
+uint32_t mpm_batch_count(mpm_batch* b) { return b ? (uint32_t)b->scenes.size() : 0u; }
+
+int mpm_batch_step(mpm_batch* b, const void* params, uint32_t step_count, uint32_t measure_speed) {
+    if (!b) return batch_fail(nullptr, MPM_EINVAL, "batch must not be NULL");
+    if (!params) return batch_fail(b, MPM_EINVAL, "params must not be NULL");
+    if (step_count < 1) return batch_fail(b, MPM_EINVAL, "stepCount must be a positive safe integer.");
+    const uint32_t n = (uint32_t)b->scenes.size();
+    std::vector<Params> ps(n);
+    bool measure = measure_speed != 0u;
+    for (uint32_t i = 0; i < n; ++i) {
+        const int r = prepare_params(b->scenes[i], (const uint8_t*)params + (size_t)MPM_PARAMS_BYTES * i, ps[i]);
+        if (r) return batch_fail(b, r, "scene %u: %s", i, b->scenes[i]->error.c_str());
+        measure = measure || ps[i].cfl_on >= 0.5f;
+    }
+    if (hipSetDevice(b->device) != hipSuccess) return batch_fail(b, MPM_EDEVICE, "hipSetDevice failed");
+    for (mpm_ctx* c : b->scenes) {
+        if (c->pending.size() >= MAX_PENDING) {
+            const int r = settle(c);
+            if (r) return batch_fail(b, r, "%s", c->error.c_str());
+        }
+    }
+    if (b->staged_used && hipEventSynchronize(b->staged) != hipSuccess)
+        return batch_fail(b, MPM_EDEVICE, "hipEventSynchronize failed");
+    StepBatch sb{};
+    sb.stream = b->stream;
+    sb.d_scenes = b->d_scenes;
+    sb.p2g_scene = b->d_p2g_map;
+    sb.g2p_scene = b->d_g2p_map;
+    sb.nscenes = n;
+    sb.p2g_tiles = b->p2g_tiles;
+    sb.g2p_tiles = b->g2p_tiles;
+    sb.fixed_point = b->scenes[0]->lc.fixed_point;
+    for (uint32_t i = 0; i < n; ++i) {
+        mpm_ctx* c = b->scenes[i];
+        const uint32_t seq = c->seq++;
+        b->h_scenes[i] = scene_desc(c, ps[i], b->p2g_base[i], b->g2p_base[i], seq);
+        c->pending.push_back({ps[i], step_count, measure, seq, 0u});
+        c->last_done_before = 0u;
+        c->cur = (c->cur + (int)(step_count & 1u)) & 1;
+        sb.max_slots = std::max(sb.max_slots, c->lc.nb * c->lc.cap);
+        sb.max_p2g_tiles = std::max(sb.max_p2g_tiles, c->p2g_tiles);
+        sb.scene_flags |= c->lc.scene_flags;
+        sb.diag = sb.diag || c->lc.diag != nullptr;
+    }
+    hipError_t e;
+    if ((e = hipMemcpyAsync(b->d_scenes, b->h_scenes, sizeof(SceneDesc) * n, hipMemcpyHostToDevice, b->stream)) !=
+            hipSuccess ||
+        (e = hipEventRecord(b->staged, b->stream)) != hipSuccess)
+        return batch_fail(b, MPM_EDEVICE, "staging the batch: %s", hipGetErrorString(e));
+    b->staged_used = true;
+    if ((e = launch_steps(sb, step_count, measure)) != hipSuccess)
+        return batch_fail(b, MPM_EDEVICE, "batch launch: %s", hipGetErrorString(e));
+    return MPM_OK;
+}
+
+const char* mpm_batch_last_error(mpm_batch* b) { return b ? b->error.c_str() : g_create_error.c_str(); }
 
 }  // extern "C"

@@ -173,11 +173,11 @@ struct SceneDesc {
     uint32_t p2g_base, g2p_base;  // first tile of this scene in the P2G / G2P launches
     uint32_t scene_flags, keep_fixed;
     uint32_t seq;                 // batch sequence number of this scene
-    uint32_t pad;
+    uint32_t start_buf;           // slot buffer holding the state the batch starts from
 };
 
 // Host description of one step launch over `nscenes` scenes that share a
-// stream and start from the same slot buffer.
+// stream (each starts from its own slot buffer).
 struct StepBatch {
     hipStream_t stream;
     const SceneDesc* d_scenes;   // device, nscenes entries (each carries its Params)
@@ -196,7 +196,7 @@ struct StepBatch {
 constexpr int P_TILE = 28;      // interior nodes per P2G tile side
 constexpr int G_TILE_BUCKETS = 8;  // buckets per G2P tile side
 
-hipError_t launch_steps(const StepBatch& sb, int start_buf, uint32_t steps, bool measure);
+hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure);
 // Bucket particles [first, first+count) of the home arrays into buffer b
 // (counts must already hold the particles bucketed so far).
 hipError_t launch_bucket(const LaunchCtx& lc, int b, uint32_t first, uint32_t count);

@@ -672,6 +672,29 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
     return true;
 }
 
+// A P2G tile whose 9x9-bucket region holds no particle: its nodes receive
+// nothing, and no particle is within reach of them in this microstep's G2P,
+// so their velocities are not rewritten.  The kept fixed-point words and
+// the diagnostics partial still read zero.
+__device__ void p2g_empty_tile(const Params& P, const Grid& grid, uint32_t keep_fixed, double* diag_grid,
+                               uint32_t tile, int X0, int Y0) {
+    if (keep_fixed) {
+        for (int e = threadIdx.x; e < P_TI * P_TI; e += blockDim.x) {
+            const int sx = X0 + e % P_TI, sy = Y0 + e / P_TI;
+            if (sx < (int)P.node_w && sy < (int)P.node_h) {
+                const size_t node = (size_t)sy * P.node_w + (size_t)sx;
+#pragma unroll
+                for (int k = 0; k < 11; ++k) grid.fixed[node * 11 + k] = 0;
+            }
+        }
+    }
+    if (P.diag_on && threadIdx.x == 0) {
+        diag_grid[3 * tile] = 0.0;
+        diag_grid[3 * tile + 1] = 0.0;
+        diag_grid[3 * tile + 2] = 0.0;
+    }
+}
+
 // Region bucket holding staged particle j of [s_boff[b0], s_boff[b1]).
 __device__ __forceinline__ int region_bucket(const uint32_t* s_boff, int b0, int b1, uint32_t j) {
     int lo = b0, hi = b1 - 1;
@@ -733,8 +756,9 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 template <bool GENERAL>
 __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
-                  uint32_t step, int in_buf) {
+                  uint32_t step) {
     SCENE_PROLOGUE(tile_scene, p2g_base);
+    const int in_buf = (int)((S.start_buf + step) & 1u);
     const Params* __restrict__ Pp = &S.P;
     Control* ctl = S.ctl;
     const SlotBuf in = S.buf[in_buf];
@@ -767,6 +791,10 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
     const bool node_ok = lx < P_TC && ly < P_TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
     __syncthreads();
+    if (s_boff[P_NB] == 0u) {
+        p2g_empty_tile(P, grid, 0u, diag_grid, tile, X0, Y0);
+        return;
+    }
     if (tid == 0) {
         uint32_t n = 0;
         s_chunk[0] = 0u;
@@ -960,8 +988,9 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
 // keep only the eight liquid channels, so two tiles fit in one CU's LDS.
 template <bool GENERAL, bool SOLIDS>
 __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
-                 uint32_t step, int in_buf) {
+                 uint32_t step) {
     SCENE_PROLOGUE(tile_scene, p2g_base);
+    const int in_buf = (int)((S.start_buf + step) & 1u);
     const Params* __restrict__ Pp = &S.P;
     Control* ctl = S.ctl;
     const SlotBuf in = S.buf[in_buf];
@@ -986,6 +1015,10 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
     const int X0 = tx * P_TI, Y0 = ty * P_TI;
     const int W = (int)P.node_w, H = (int)P.node_h;
     const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
+    if (s_boff[P_NB] == 0u) {
+        p2g_empty_tile(P, grid, keep_fixed, diag_grid, tile, X0, Y0);
+        return;
+    }
 
     uint32_t flags = 0;
     const float mref = P.particle_mass, vref = P.particle_volume, dx = P.dx;
@@ -1412,8 +1445,9 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 template <bool GENERAL>
 __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__ scenes,
                                                  const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
-                                                 uint32_t step, int out_buf) {
+                                                 uint32_t step) {
     SCENE_PROLOGUE(tile_scene, g2p_base);
+    const int out_buf = (int)((S.start_buf + step + 1u) & 1u);
     const Params* __restrict__ Pp = &S.P;
     Control* ctl = S.ctl;
     const SlotBuf in = S.buf[out_buf ^ 1];
@@ -1448,6 +1482,8 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
         s_boff[tid] = incl - c;
         if (tid == 63) s_boff[64] = incl;
     }
+    __syncthreads();
+    if (s_boff[64] == 0u) return;  // no particle in this tile
     for (int e = tid; e < G_TN * G_TN; e += G_THREADS) {
         const int nx = nx0 + e % G_TN, ny = ny0 + e / G_TN;
         const bool in_grid = nx < W && ny < H;
@@ -1568,9 +1604,9 @@ __global__ void k_commit_last(const SceneDesc* __restrict__ scenes, uint32_t ste
 // On a failed microstep the committed state sits in the buffer of the last
 // committed microstep; copy it to the batch's output buffer when their
 // parity differs (commitParticles' whole-step rollback, shaders.ts:3067-3080).
-__global__ __launch_bounds__(256) void k_fix_rollback(const SceneDesc* __restrict__ scenes, uint32_t start_buf,
-                                                      uint32_t steps) {
+__global__ __launch_bounds__(256) void k_fix_rollback(const SceneDesc* __restrict__ scenes, uint32_t steps) {
     const SceneDesc& S = scenes[blockIdx.y];  // one row of blocks per scene
+    const uint32_t start_buf = S.start_buf;
     Control* ctl = S.ctl;
     const SlotBuf& b0 = S.buf[0];
     const SlotBuf& b1 = S.buf[1];
@@ -1608,8 +1644,9 @@ __global__ __launch_bounds__(256) void k_fix_rollback(const SceneDesc* __restric
 // measureParticleSpeedPartials + reduceParticleSpeedPartials
 // (shaders.ts:2463-2533): wave max, then one atomic per block on the
 // non-negative float's bit pattern.
-__global__ __launch_bounds__(256) void k_measure_speed(const SceneDesc* __restrict__ scenes, uint32_t which) {
+__global__ __launch_bounds__(256) void k_measure_speed(const SceneDesc* __restrict__ scenes, uint32_t steps) {
     const SceneDesc& S = scenes[blockIdx.y];  // one row of blocks per scene
+    const uint32_t which = (S.start_buf + steps) & 1u;  // the batch's final buffer
     Control* ctl = S.ctl;
     const SlotBuf& buf = S.buf[which];
     const uint32_t nb = S.nb, cap = S.cap;
@@ -1633,12 +1670,14 @@ __global__ __launch_bounds__(256) void k_measure_speed(const SceneDesc* __restri
 }
 
 // Particle mass and momentum (totalMass / totalMomentum,
-// reference/solver.ts:2505-2526) in FP64 for the scenes with diagnostics on:
+// reference/solver.ts:2505-2526) in FP64 for the scenes with diagnostics on,
+// of the state `after` microsteps into the batch (0 or its length):
 // block b of a scene sums its buckets [b*nb/B, (b+1)*nb/B) in slot order
 // into diag_part[3b..3b+2].
-__global__ __launch_bounds__(256) void k_particle_sums(const SceneDesc* __restrict__ scenes, uint32_t which) {
+__global__ __launch_bounds__(256) void k_particle_sums(const SceneDesc* __restrict__ scenes, uint32_t after) {
     const SceneDesc& S = scenes[blockIdx.y];  // one row of blocks per scene
     if (!S.diag) return;
+    const uint32_t which = (S.start_buf + after) & 1u;  // the buffer `after` microsteps into the batch
     const SlotBuf& buf = S.buf[which];
     const uint32_t nb = S.nb, cap = S.cap;
     const uint32_t b0 = (uint32_t)((uint64_t)blockIdx.x * nb / gridDim.x);
@@ -1696,7 +1735,7 @@ inline uint32_t blocks_for(size_t n, uint32_t t) { return (uint32_t)((n + t - 1)
         if (_e != hipSuccess) return _e;                                          \
     } while (0)
 
-hipError_t launch_steps(const StepBatch& sb, int start_buf, uint32_t steps, bool measure) {
+hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure) {
     const bool general = scene_general(sb.scene_flags);
     const uint32_t ns = sb.nscenes;
     struct {
@@ -1707,43 +1746,37 @@ hipError_t launch_steps(const StepBatch& sb, int start_buf, uint32_t steps, bool
     const uint32_t parts = ns == 1 ? DIAG_PART_BLOCKS : 32u;
     MPM_LAUNCH("batch_begin", k_batch_begin, dim3(ns), dim3(64), 0, lc.stream, sb.d_scenes);
     if (sb.diag) {
-        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes,
-                   (uint32_t)start_buf);
+        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u);
         MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u, parts, 0u);
     }
-    int cur = start_buf;
     for (uint32_t s = 0; s < steps; ++s) {
         if (sb.fixed_point) {
             const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
             auto p2g = general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
                                : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
             MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes,
-                       sb.p2g_scene, sb.nscenes, s, cur);
+                       sb.p2g_scene, sb.nscenes, s);
         } else {
             auto p2g = general ? k_p2g_gather<true> : k_p2g_gather<false>;
             MPM_LAUNCH("p2g_grid", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
-                       sb.nscenes, s, cur);
+                       sb.nscenes, s);
         }
         auto g2p = general ? k_g2p<true> : k_g2p<false>;
         MPM_LAUNCH("g2p", g2p, dim3(sb.g2p_tiles), dim3(G_THREADS), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
-                   sb.nscenes, s, cur ^ 1);
-        cur ^= 1;
+                   sb.nscenes, s);
     }
     MPM_LAUNCH("commit_last", k_commit_last, dim3(ns), dim3(64), 0, lc.stream, sb.d_scenes, steps);
     const uint32_t fb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(sb.max_slots, 256), ns == 1 ? 2048u : 64u));
-    MPM_LAUNCH("fix_rollback", k_fix_rollback, dim3(fb, ns), dim3(256), 0, lc.stream, sb.d_scenes,
-               (uint32_t)start_buf, steps);
+    MPM_LAUNCH("fix_rollback", k_fix_rollback, dim3(fb, ns), dim3(256), 0, lc.stream, sb.d_scenes, steps);
     if (sb.diag) {
         MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 1u, parts, 1u);
-        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes,
-                   (uint32_t)cur);
+        MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes, steps);
         MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u, parts, 2u);
     }
     if (measure) {
         const uint32_t mb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(sb.max_slots, 256),
                                                                        ns == 1 ? 1024u : 32u));
-        MPM_LAUNCH("measure_speed", k_measure_speed, dim3(mb, ns), dim3(256), 0, lc.stream, sb.d_scenes,
-                   (uint32_t)cur);
+        MPM_LAUNCH("measure_speed", k_measure_speed, dim3(mb, ns), dim3(256), 0, lc.stream, sb.d_scenes, steps);
     }
     return hipSuccess;
 }

@@ -40,6 +40,12 @@ SIGNATURES = {
     "mpm_set_timing": (ctypes.c_int, [_P, _U32]),
     "mpm_read_timing": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "mpm_batch_create": (ctypes.c_int, [_P, _P, _P, _U32, ctypes.POINTER(_P)]),
+    "mpm_batch_step": (ctypes.c_int, [_P, _P, _U32, _U32]),
+    "mpm_batch_scene": (_P, [_P, _U32]),
+    "mpm_batch_count": (_U32, [_P]),
+    "mpm_batch_destroy": (None, [_P]),
+    "mpm_batch_last_error": (ctypes.c_char_p, [_P]),
     "mpm_destroy": (None, [_P]),
     "mpm_last_error": (ctypes.c_char_p, [_P]),
 }
@@ -87,7 +93,18 @@ def _ptr(a: np.ndarray | None):
 class Context:
     """Owns one ``mpm_ctx``; thin, checked wrappers around the C-ABI."""
 
+    @classmethod
+    def borrowed(cls, handle, cfg: MpmConfig):
+        """A view of a context owned elsewhere (a scene of a batch)."""
+        self = cls.__new__(cls)
+        self.lib = load()
+        self.handle = ctypes.c_void_p(handle)
+        self.cfg = cfg
+        self.owned = False
+        return self
+
     def __init__(self, cfg: MpmConfig, packed: np.ndarray, object_ids: np.ndarray | None):
+        self.owned = True
         self.lib = load()
         packed = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1, 20)
         ids = None if object_ids is None else np.ascontiguousarray(object_ids, dtype=np.uint32)
@@ -192,7 +209,51 @@ class Context:
 
     def close(self):
         if getattr(self, "handle", None):
-            self.lib.mpm_destroy(self.handle)
+            if getattr(self, "owned", True):
+                self.lib.mpm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Owns one ``mpm_batch``: scenes stepped by the same kernel launches."""
+
+    def __init__(self, cfgs: list[MpmConfig], packed: list[np.ndarray]):
+        self.lib = load()
+        n = len(cfgs)
+        arrs = [np.ascontiguousarray(p, dtype=np.uint32).reshape(-1, 20) for p in packed]
+        self._keep = arrs
+        cfg_arr = (MpmConfig * n)(*cfgs)
+        ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in arrs])
+        counts = np.array([a.shape[0] for a in arrs], np.uint32)
+        handle = ctypes.c_void_p()
+        code = self.lib.mpm_batch_create(ctypes.addressof(cfg_arr), ctypes.addressof(ptrs), _ptr(counts), n,
+                                         ctypes.byref(handle))
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_batch_last_error(None).decode())
+        self.handle = handle
+        self.cfgs = cfgs
+        self.scenes = [Context.borrowed(self.lib.mpm_batch_scene(handle, i), cfgs[i]) for i in range(n)]
+
+    def step(self, params: list[bytes], steps: int, measure: bool):
+        blob = b"".join(params)
+        if len(blob) != 144 * len(self.scenes):
+            raise ValueError("one 144-byte parameter block per scene")
+        buf = (ctypes.c_uint8 * len(blob)).from_buffer_copy(blob)
+        code = self.lib.mpm_batch_step(self.handle, ctypes.addressof(buf), steps, 1 if measure else 0)
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_batch_last_error(self.handle).decode())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            for s in self.scenes:
+                s.handle = None
+            self.lib.mpm_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -12,6 +12,9 @@
   velocities U(+-2).
 * ``five_model_block`` — the 45-particle five-material fixture of
   experiments/transfer-regression.ts:34-53.
+* ``batched_tanks`` — independent Tait water tanks of log-uniform size for
+  batched stepping: the SI tank's material and cell size, a dam-break block
+  at one particle per cell in a corner, gravity in a random direction.
 """
 from __future__ import annotations
 
@@ -134,3 +137,42 @@ def five_model_block(models=None):
     config = {"grid": {"cellsX": 48, "cellsY": 16, "cellSize": 1.0}, "particleMass": 0.1,
               "particleVolume": 0.05, "material": FIVE_MODEL_MATERIALS[0], "capacity": 54}
     return particles, config
+
+
+BATCH_SEED = 0x62617463
+BATCH_DT = (1.0 / 120.0) / 224.0  # the SI tank's CFL microstep (first macrostep plan)
+
+
+def batched_tanks(count: int = 256, seed: int = BATCH_SEED, min_particles: int = 500,
+                  max_particles: int = 50_000):
+    """`count` independent SI-material Tait tanks.  Scene i has
+    N_i = round(exp(U(ln min, ln max))) particles at one per cell (cell size
+    2.5 mm, as tait_tank) in a block from cell (2, 2), on a square grid of
+    side round(64 * 8^((ln N_i - ln min) / ln(max / min))) in [64, 512]
+    cells, with gravity 9.81 m/s^2 at an angle U(0, 2 pi).  Draws come from
+    the Gate 4 LCG.  Returns a list of (ParticleArrays, config, gravity)."""
+    u = lcg_uniform(seed, 2 * count)
+    spacing = 0.0025
+    material = tait_material()
+    area = spacing * spacing
+    lo, hi = np.log(min_particles), np.log(max_particles)
+    scenes = []
+    for i in range(count):
+        n = int(round(np.exp(lo + u[2 * i] * (hi - lo))))
+        side = int(round(64 * 8 ** ((np.log(n) - lo) / (hi - lo))))
+        w = int(np.ceil(np.sqrt(n)))
+        k = np.arange(n)
+        x = (2 + k % w + 0.5) * spacing
+        y = (2 + k // w + 0.5) * spacing
+        theta = 2 * np.pi * u[2 * i + 1]
+        arr = ParticleArrays(
+            x=x, y=y, vx=np.zeros(n), vy=np.zeros(n), model=np.full(n, 2, np.uint32),
+            lambda_=np.full(n, material["lambda"]), mu=np.zeros(n), density=np.full(n, material["density"]),
+            volume=np.full(n, area), param=np.full(n, material["equationOfState"]["exponent"]),
+            object_id=np.zeros(n, np.uint32))
+        config = {"grid": {"cellsX": side, "cellsY": side, "cellSize": spacing},
+                  "particleMass": material["density"] * area, "particleVolume": area, "material": material,
+                  "wallFriction": 1.0, "capacity": n}
+        gravity = {"x": 9.81 * float(np.cos(theta)), "y": 9.81 * float(np.sin(theta))}
+        scenes.append((arr, config, gravity))
+    return scenes

@@ -143,6 +143,23 @@ class MpmMlsGpuSolver:
         self._latest_microsteps = 0
         self._drag = None
 
+    @classmethod
+    def _over_context(cls, state, cfl_bounds, ctx, p2g_mode):
+        """A solver over an existing native context (a scene of a batch)."""
+        self = cls.__new__(cls)
+        self.state = state
+        self.cfl_bounds = cfl_bounds
+        self.ctx = ctx
+        self.p2g_mode = p2g_mode
+        self._disposed = False
+        self._cfl_state_usable = True
+        self._speed_measurement_submitted = False
+        self._last_scheduled_velocity_bound = None
+        self._latest_sequence = 0
+        self._latest_microsteps = 0
+        self._drag = None
+        return self
+
     # ----------------------------------------------------------- properties
     @property
     def particle_count(self) -> int:
@@ -497,6 +514,57 @@ class MpmMlsGpuSolver:
 
     def read_bins(self):
         return self.ctx.read_bins()
+
+
+class MpmMlsGpuBatch:
+    """Independent scenes advanced together: one launch per microstep covers
+    the tiles of every scene (mpm_batch_* in include/mpm355.h).  The
+    reference has no counterpart (one solver per scene, solver.ts:390-406).
+    scene(i) is an MpmMlsGpuSolver over scene i: reads, status, diagnostics
+    and single-scene steps work on it as on any solver."""
+
+    def __init__(self, scenes, p2g_mode: int = MPM_P2G_GATHER, device: int = -1):
+        if not scenes:
+            raise RangeError("a batch needs at least one scene")
+        from .native import Batch
+        states, cfgs, bounds = [], [], []
+        for particles, config in scenes:
+            st = layout.validate_input(particles, config)
+            states.append(st)
+            bounds.append(cflmod.derive_cfl_bounds_packed(st.packed, st.particle_mass_f32))
+            cfgs.append(MpmConfig(st.config["grid"]["cellsX"], st.config["grid"]["cellsY"], st.cell_size_f32,
+                                  st.particle_mass_f32, st.particle_volume_f32, st.wall_friction_f32,
+                                  st.particle_capacity, p2g_mode, 0, device, 0))
+        self.native = Batch(cfgs, [st.packed for st in states])
+        self.solvers = [MpmMlsGpuSolver._over_context(st, b, ctx, p2g_mode)
+                        for st, b, ctx in zip(states, bounds, self.native.scenes)]
+
+    def __len__(self) -> int:
+        return len(self.solvers)
+
+    def scene(self, i: int) -> "MpmMlsGpuSolver":
+        return self.solvers[i]
+
+    def step_batch(self, step_count: int, dt, gravity, measure_speed: bool = False):
+        """Advance every scene `step_count` microsteps; dt and gravity are one
+        value for all scenes or one per scene."""
+        if not isinstance(step_count, int) or isinstance(step_count, bool) or step_count < 1:
+            raise RangeError("stepCount must be a positive safe integer.")
+        n = len(self.solvers)
+        dts = list(dt) if isinstance(dt, (list, tuple, np.ndarray)) else [dt] * n
+        gs = list(gravity) if isinstance(gravity, (list, tuple)) else [gravity] * n
+        if len(dts) != n or len(gs) != n:
+            raise RangeError("one dt and one gravity per scene")
+        params = [layout.pack_step_parameters(s.state, d, g, s.allocated_particle_count, 0, None)
+                  for s, d, g in zip(self.solvers, dts, gs)]
+        self.native.step(params, step_count, measure_speed)
+        for s in self.solvers:
+            s._cfl_state_usable = False
+
+    def dispose(self):
+        for s in self.solvers:
+            s._disposed = True
+        self.native.close()
 
 
 def create_mpm_mls_gpu_solver(particles, config: dict, cfl_bounds=None, **kw) -> MpmMlsGpuSolver:

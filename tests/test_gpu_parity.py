@@ -418,3 +418,68 @@ rt.readCompact().then((c) => {
     assert compact["completed"] == done == 8 and compact["status"][0] == status[0] == 0
     assert compact["maxSpeed"] == speed
     assert np.array_equal(node_recs, recs)
+
+
+def _batch_scenes(pkg):
+    """Scenes of different grids, materials and kernel variants: the
+    five-material block (solids), the SI Tait tank, a linear water/oil column
+    (no stress cache of its own) and three tanks of the batched workload."""
+    five, five_cfg = pkg.scenes.five_model_block()
+    tank, tank_cfg, tank_macro, _ = pkg.scenes.tait_tank()
+    col, col_cfg = pkg.scenes.water_oil_column(cells=40)
+    tanks = pkg.scenes.batched_tanks(count=3)
+    scenes = [(five, five_cfg), (tank, tank_cfg), (col, col_cfg)] + [(a, c) for a, c, _ in tanks]
+    dts = [1e-3, tank_macro / 224, 1.0 / 9600.0] + [pkg.scenes.BATCH_DT] * 3
+    gravity = [{"x": 0.0, "y": -2.0}, pkg.scenes.TAIT_TANK_GRAVITY, pkg.scenes.WATER_OIL_GRAVITY] + [g for _, _, g in tanks]
+    return scenes, dts, gravity
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+def test_batch_matches_individual_solvers(pkg, mode):
+    """A batch steps every scene as its own solver would: 24 microsteps of six
+    different scenes in one launch per microstep, bit for bit the state of
+    each scene stepped alone."""
+    scenes, dts, gravity = _batch_scenes(pkg)
+    p2g = pkg.native.MPM_P2G_GATHER if mode == "gather" else pkg.native.MPM_P2G_FIXED_POINT
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    print(f"batch of {len(scenes)} scenes, {mode}", flush=True)
+    batch = pkg.MpmMlsGpuBatch(scenes, p2g_mode=p2g, device=0)
+    try:
+        batch.step_batch(24, dts, gravity)
+        got = []
+        for i in range(len(batch)):
+            batch.scene(i).check_numerical_status()
+            got.append(batch.scene(i).read_particles(packed=True)[0])
+    finally:
+        batch.dispose()
+    for i, ((particles, config), dt, g) in enumerate(zip(scenes, dts, gravity)):
+        s = factory(particles, config, device=0)
+        try:
+            s.step_batch(24, dt, g)
+            s.check_numerical_status()
+            want = s.read_particles(packed=True)[0]
+        finally:
+            s.dispose()
+        assert np.array_equal(got[i], want), f"scene {i}"
+
+
+def test_batched_tanks_conserve_mass(pkg):
+    """The 256-tank batched workload: 48 microsteps in one batch with the FP64
+    diagnostics on; every scene commits all microsteps and carries its mass
+    unchanged, its last grid mass equal to it to float32 rounding."""
+    tanks = pkg.scenes.batched_tanks()
+    print(f"{len(tanks)} tanks, {sum(len(a) for a, _, _ in tanks)} particles", flush=True)
+    batch = pkg.MpmMlsGpuBatch([(a, c) for a, c, _ in tanks], device=0)
+    try:
+        for i in range(len(batch)):
+            batch.scene(i).set_diagnostics(True)
+        batch.step_batch(48, pkg.scenes.BATCH_DT, [g for _, _, g in tanks])
+        for i in range(len(batch)):
+            s = batch.scene(i)
+            speed, done, status = s.ctx.read_compact()
+            assert status[0] == 0 and done == 48, (i, status)
+            d = s.read_diagnostics()
+            assert d["outputMass"] == pytest.approx(d["inputMass"], rel=1e-12), i
+            assert d["gridMass"] == pytest.approx(d["inputMass"], rel=1e-6), i
+    finally:
+        batch.dispose()
