@@ -9,8 +9,10 @@ inputs resident in HBM, the timed region bracketed by stream synchronisation.
 
 Printed JSON line (one): metric/value/unit of the float32 gather path plus
 the fixed-point path's rate, the roofline of the dominant kernel (HIP events
-on the solver's stream; algorithmic bytes per launch), and the CPU baseline
-(the FP64 oracle step, single thread, on a bounded sample of the same scene).
+on the solver's stream; algorithmic bytes per launch), the batched-scenes and
+differentiable-rollout legs, and the CPU baseline (the FP64 oracle step,
+single thread, on a bounded sample of the same scene; and the FP64 oracle
+adjoint of the elastic block).
 """
 from __future__ import annotations
 
@@ -164,6 +166,59 @@ def batched_leg(pkg, steps: int):
                         "64-512 cell grids, dam-break blocks, random gravity directions, float32 gather P2G"}
 
 
+def adjoint_leg(pkg, reps: int):
+    """The differentiable elastic block (scenes.elastic_block: 4,096
+    fixed-corotated particles, 256x256 cells, 200 steps of 1e-3, checkpoints
+    every 20): device time of the forward rollout and of the reverse pass
+    (HIP events on the rollout's stream around the kernel sequences; host
+    copies excluded), median of `reps` forward+backward pairs after one
+    warm-up pair.  The reverse pass includes the recomputation of every
+    segment from its checkpoint."""
+    state, lam, mu, config, target = pkg.scenes.elastic_block()
+    n, T = len(lam), config["steps"]
+    r = pkg.adjoint.ElasticRollout(n, config, segment_steps=20, device=0)
+    fwd, bwd = [], []
+    try:
+        for i in range(reps + 1):
+            final = r.forward(state, lam, mu)
+            d = final[:, :2].astype(np.float64).mean(axis=0) - target
+            bar = np.zeros_like(final)
+            bar[:, :2] = 2.0 * d / n
+            r.backward(bar)
+            t = r.timing()
+            if i:
+                fwd.append(t["forward_ms"])
+                bwd.append(t["backward_ms"])
+        layout = r.layout()
+    finally:
+        r.close()
+    f_ms, b_ms = float(np.median(fwd)), float(np.median(bwd))
+    log(f"[bench] adjoint: forward {f_ms:.3f} ms, reverse {b_ms:.3f} ms for {T} steps of {n} particles "
+        f"(reverse / forward = {b_ms / f_ms:.2f})")
+    return {"workload": f"fixed-corotated elastic block, {n} particles, {config['cellsX']}x{config['cellsY']} cells, "
+                        f"{T} steps of {config['dt']}, checkpoints every {layout['segment_steps']}",
+            "forward_ms": f_ms, "backward_ms": b_ms, "forward_value": n * T / (f_ms * 1e-3),
+            "backward_value": n * T / (b_ms * 1e-3), "unit": "particle-steps/s", "backward_over_forward": b_ms / f_ms,
+            "device_bytes": layout["device_bytes"], "reps": reps}
+
+
+def adjoint_cpu(pkg):
+    """FP64 oracle forward + reverse of the same elastic block, one thread."""
+    from oracle import oracle as orc
+    state, lam, mu, config, target = pkg.scenes.elastic_block()
+    cfg = orc.diff_config(config)
+    t0 = time.perf_counter()
+    final = orc.diff_forward(state, lam, mu, cfg)
+    t1 = time.perf_counter()
+    _, bar = orc.com_loss(final, target)
+    orc.diff_backward(state, lam, mu, cfg, bar)
+    t2 = time.perf_counter()
+    n, T = len(lam), config["steps"]
+    return {"forward_value": n * T / (t1 - t0), "backward_value": n * T / (t2 - t1), "unit": "particle-steps/s",
+            "cores": 1, "kind": "port", "sample": f"one FP64 forward and reverse of the {n}-particle block "
+                                                   f"({t1 - t0:.2f} s + {t2 - t1:.2f} s; the reverse re-runs the forward)"}
+
+
 def pmc_sample(pkg, arr, config, mode):
     """Workload of the FETCH_SIZE / WRITE_SIZE counter runs.  The copy kernel
     reads and writes COPY_BYTES each, so its counter values per byte scale
@@ -193,6 +248,8 @@ def main():
     ap.add_argument("--mode", choices=["both", "gather", "fixed"], default="both")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batched-steps", type=int, default=480)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
+    ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--pmc-sample", action="store_true",
                     help="counter-collection workload: one 256 MiB copy kernel (the HBM byte calibration), then "
                          "8 untimed microsteps per mode; prints nothing")
@@ -255,6 +312,7 @@ def main():
     step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
 
     batched = None if args.no_batched or args.cells != 1000 else batched_leg(pkg, args.batched_steps)
+    adjoint = None if args.no_adjoint else adjoint_leg(pkg, args.adjoint_reps)
 
     cpu = None
     if not args.no_cpu:
@@ -263,6 +321,9 @@ def main():
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": f"{steps} FP64 oracle microsteps of the full {n}-particle scene ({el:.1f} s, after 1 "
                          "warm-up step)"}
+        if adjoint is not None:
+            log("[bench] CPU baseline: FP64 oracle adjoint of the elastic block, one thread")
+            cpu["adjoint"] = adjoint_cpu(pkg)
 
     line = {
         "metric": METRIC,
@@ -290,6 +351,7 @@ def main():
                      "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)) if traffic else None},
         "kernels_us": kernel_times,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -172,6 +172,60 @@ const char* mpm_batch_last_error(mpm_batch* batch);
 int mpm_set_timing(mpm_ctx* ctx, uint32_t enable);
 int mpm_read_timing(mpm_ctx* ctx, const char* kernel, double* mean_us, uint64_t* launches);
 
+/* ---------------- differentiable rollout ----------------
+ * No counterpart in the reference (it has no differentiation code).  A
+ * fixed-corotated elastic solid — the snow stress of constitutiveStressTimesFt
+ * (reference/solver.ts:2171-2203) at hardening factor 1 without the plastic
+ * projection — stepped `steps` times by the solid branch of stepMpm
+ * (reference/solver.ts:396-824) in float32 with an FP64 polar/stress, and the
+ * hand-written reverse of that rollout, for callers that optimise initial
+ * conditions or material parameters.
+ *
+ * State: MPM_DIFF_STATE_FLOATS floats per particle,
+ *   x, y, vx, vy, C00, C01, C10, C11, F00, F01, F10, F11.
+ * Every pointer argument may be host or device memory (copies use
+ * hipMemcpyDefault); the calls return after the results are in place.
+ *
+ * mpm_diff_create: segment_steps is the checkpoint interval (0 = 20); the
+ *   reverse pass recomputes one segment at a time from its checkpoint.
+ * mpm_diff_forward: runs the rollout from state0 with per-particle lambda and
+ *   mu, keeps the checkpoints, writes the final state (state_final may be
+ *   NULL).
+ * mpm_diff_backward: given dL/d(final state), writes dL/d(state0) and
+ *   dL/dlambda, dL/dmu (any output may be NULL); needs a successful forward.
+ *   The derivative through a confined axis (confineAxis) is that of the
+ *   clamped branch: zero.
+ * Numerical anomalies (status bits below, except CONFINED which is
+ * informational) make forward/backward return MPM_ESTATE; the bits stay
+ * readable through mpm_diff_read_status.
+ * mpm_diff_read_timing: device time of the last forward and backward kernel
+ *   sequences (HIP events on the rollout's stream, copies excluded). */
+#define MPM_DIFF_STATE_FLOATS 12
+#define MPM_DIFF_STENCIL 1u     /* a stencil left the grid (that particle was skipped) */
+#define MPM_DIFF_DEFORMATION 2u /* det F <= 0 */
+#define MPM_DIFF_NONFINITE 4u   /* non-finite velocity */
+#define MPM_DIFF_SUBSTEPS 8u    /* more than 64 deformation substeps */
+#define MPM_DIFF_CONFINED 16u   /* a particle was clamped to a wall */
+
+typedef struct mpm_diff mpm_diff;
+typedef struct mpm_diff_config {
+    uint32_t particle_count, cells_x, cells_y, steps, segment_steps;
+    float cell_size, dt, particle_mass, particle_volume;
+    float gravity_x, gravity_y, wall_friction;
+    int32_t device; /* HIP device ordinal; -1 = current */
+} mpm_diff_config;
+
+int mpm_diff_create(const mpm_diff_config* cfg, mpm_diff** out);
+int mpm_diff_forward(mpm_diff* d, const float* state0, const float* lambda, const float* mu, float* state_final);
+int mpm_diff_backward(mpm_diff* d, const float* bar_final, float* bar_state0, float* grad_lambda, float* grad_mu);
+int mpm_diff_read_status(mpm_diff* d, uint32_t* status);
+int mpm_diff_read_timing(mpm_diff* d, double* forward_ms, double* backward_ms);
+int mpm_diff_get_layout(mpm_diff* d, uint32_t* segment_steps, uint32_t* checkpoints, size_t* device_bytes);
+void mpm_diff_destroy(mpm_diff* d);
+/* Message of the last failed call on d (of the last failed mpm_diff_create
+ * on this thread when d is NULL). Never NULL. */
+const char* mpm_diff_last_error(mpm_diff* d);
+
 /* dispose (solver.ts:2950-2964). Safe on NULL; ignores the scenes of a batch. */
 void mpm_destroy(mpm_ctx* ctx);
 

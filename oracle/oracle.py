@@ -7,6 +7,8 @@ its host layer under ``powder-play_amd/``) never imports it.
 The C sources (``mpm_oracle.c``) restate ``stepMpm``
 (src/lib/mpm/reference/solver.ts:396-824) in FP64 and the WGSL fixed-point
 P2G (src/lib/mpm/kernels/shaders.ts:808-1070) as an ordered f32 sequence.
+``mpm_adjoint.c`` holds the FP64 differentiable rollout of the fixed-corotated
+solid and its hand-written reverse pass (the checker of the GPU adjoint).
 """
 from __future__ import annotations
 
@@ -189,3 +191,77 @@ def bin_particles(packed20: np.ndarray, prm: ParamsF32):
     flags = lib().orc_bin(_ptr(np.ascontiguousarray(packed20, np.uint32)), n, ctypes.byref(prm),
                           _ptr(key), _ptr(start), _ptr(perm))
     return key, start, perm, flags
+
+
+# ---------------- differentiable rollout (mpm_adjoint.c) ----------------
+
+class DiffConfig(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("cell_size", ctypes.c_double), ("particle_mass", ctypes.c_double),
+        ("particle_volume", ctypes.c_double), ("gravity_x", ctypes.c_double), ("gravity_y", ctypes.c_double),
+        ("wall_friction", ctypes.c_double), ("cells_x", ctypes.c_int32), ("cells_y", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+    ]
+
+
+def diff_config(cfg: dict, steps: int | None = None) -> DiffConfig:
+    """DiffConfig from an elastic_block-style config dict."""
+    return DiffConfig(cfg["dt"], cfg["cellSize"], cfg["particleMass"], cfg["particleVolume"],
+                      cfg["gravity"][0], cfg["gravity"][1], cfg.get("wallFriction", 0.0),
+                      cfg["cellsX"], cfg["cellsY"], cfg["steps"] if steps is None else steps)
+
+
+def _diff_lib():
+    L = lib()
+    if not getattr(L, "_diff_ready", False):
+        P = ctypes.c_void_p
+        C = ctypes.POINTER(DiffConfig)
+        L.orc_diff_step.argtypes = [C, ctypes.c_int32, P, P, P, P]
+        L.orc_diff_forward.argtypes = [C, ctypes.c_int32, P, P, P, P]
+        L.orc_diff_backward.argtypes = [C, ctypes.c_int32, P, P, P, P, P, P, P, P]
+        L._diff_ready = True
+    return L
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float64)
+
+
+def diff_step(state: np.ndarray, lam, mu, cfg: DiffConfig) -> np.ndarray:
+    s, la, m = _f64(state), _f64(lam), _f64(mu)
+    out = np.zeros_like(s)
+    code = _diff_lib().orc_diff_step(ctypes.byref(cfg), len(la), _ptr(s), _ptr(la), _ptr(m), _ptr(out))
+    if code:
+        raise OracleError(code)
+    return out
+
+
+def diff_forward(state0: np.ndarray, lam, mu, cfg: DiffConfig) -> np.ndarray:
+    s, la, m = _f64(state0), _f64(lam), _f64(mu)
+    out = np.zeros_like(s)
+    code = _diff_lib().orc_diff_forward(ctypes.byref(cfg), len(la), _ptr(s), _ptr(la), _ptr(m), _ptr(out))
+    if code:
+        raise OracleError(code)
+    return out
+
+
+def diff_backward(state0: np.ndarray, lam, mu, cfg: DiffConfig, bar_final: np.ndarray):
+    """Returns (dL/dstate0 (n, 12), dL/dlambda (n,), dL/dmu (n,), confinements)."""
+    s, la, m, b = _f64(state0), _f64(lam), _f64(mu), _f64(bar_final)
+    bar0 = np.zeros_like(s)
+    gl = np.zeros(len(la))
+    gm = np.zeros(len(la))
+    conf = ctypes.c_int64(0)
+    code = _diff_lib().orc_diff_backward(ctypes.byref(cfg), len(la), _ptr(s), _ptr(la), _ptr(m), _ptr(b),
+                                         _ptr(bar0), _ptr(gl), _ptr(gm), ctypes.byref(conf))
+    if code:
+        raise OracleError(code)
+    return bar0, gl, gm, conf.value
+
+
+def com_loss(state: np.ndarray, target) -> tuple[float, np.ndarray]:
+    """L = |mean(x) - target|^2 for equal-mass particles and dL/dstate."""
+    d = state[:, :2].mean(axis=0) - np.asarray(target, np.float64)
+    bar = np.zeros_like(state)
+    bar[:, :2] = 2.0 * d / state.shape[0]
+    return float(d @ d), bar

@@ -3,10 +3,11 @@
 The compute path is ``libmpm355.so`` (hand-written gfx950 kernels behind the
 C-ABI in include/mpm355.h); this package is the host mirror of the reference's
 solver API (src/lib/mpm/kernels/solver.ts:106-144) plus its layout, CFL
-scheduling and scene generators.  Import it as ``powder_play_amd`` (the
+scheduling and scene generators, and the differentiable rollout of a
+fixed-corotated solid (``adjoint``).  Import it as ``powder_play_amd`` (the
 directory name carries a hyphen; see ``load_package`` in ``__graft_entry__``).
 """
-from . import cfl, layout, native, scenes  # noqa: F401
+from . import adjoint, cfl, layout, native, scenes  # noqa: F401
 from .layout import ParticleArrays, RangeError  # noqa: F401
 from .solver import (  # noqa: F401
     MpmMlsGpuBatch,
@@ -17,6 +18,6 @@ from .solver import (  # noqa: F401
 )
 
 __all__ = [
-    "cfl", "layout", "native", "scenes", "ParticleArrays", "RangeError", "MpmMlsGpuNumericalError",
+    "adjoint", "cfl", "layout", "native", "scenes", "ParticleArrays", "RangeError", "MpmMlsGpuNumericalError",
     "MpmMlsGpuSolver", "MpmMlsGpuBatch", "create_mpm_mls_gpu_solver", "create_experimental_mpm_mls_atomic_gpu_solver",
 ]

@@ -46,9 +46,30 @@ SIGNATURES = {
     "mpm_batch_count": (_U32, [_P]),
     "mpm_batch_destroy": (None, [_P]),
     "mpm_batch_last_error": (ctypes.c_char_p, [_P]),
+    "mpm_diff_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "mpm_diff_forward": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "mpm_diff_backward": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "mpm_diff_read_status": (ctypes.c_int, [_P, _P]),
+    "mpm_diff_read_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "mpm_diff_get_layout": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_size_t)]),
+    "mpm_diff_destroy": (None, [_P]),
+    "mpm_diff_last_error": (ctypes.c_char_p, [_P]),
     "mpm_destroy": (None, [_P]),
     "mpm_last_error": (ctypes.c_char_p, [_P]),
 }
+
+MPM_DIFF_STATE_FLOATS = 12
+MPM_DIFF_STENCIL, MPM_DIFF_DEFORMATION, MPM_DIFF_NONFINITE, MPM_DIFF_SUBSTEPS, MPM_DIFF_CONFINED = 1, 2, 4, 8, 16
+
+
+class MpmDiffConfig(ctypes.Structure):
+    _fields_ = [
+        ("particle_count", _U32), ("cells_x", _U32), ("cells_y", _U32), ("steps", _U32),
+        ("segment_steps", _U32), ("cell_size", ctypes.c_float), ("dt", ctypes.c_float),
+        ("particle_mass", ctypes.c_float), ("particle_volume", ctypes.c_float),
+        ("gravity_x", ctypes.c_float), ("gravity_y", ctypes.c_float), ("wall_friction", ctypes.c_float),
+        ("device", ctypes.c_int32),
+    ]
 
 
 class MpmConfig(ctypes.Structure):
@@ -77,6 +98,14 @@ def load() -> ctypes.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        # One HIP runtime per process.  PyTorch wheels bundle their own
+        # libamdhip64 (same soname as ROCm's); loaded first, it is the one this
+        # library binds to.  Loaded after this library, torch would bring a
+        # second runtime that cannot open the GPU this one holds.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
@@ -254,6 +283,57 @@ class Batch:
             for s in self.scenes:
                 s.handle = None
             self.lib.mpm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DiffRollout:
+    """Owns one ``mpm_diff`` (the differentiable rollout).  Array arguments
+    are raw addresses of host or device memory; callers size them."""
+
+    def __init__(self, cfg: MpmDiffConfig):
+        self.lib = load()
+        handle = ctypes.c_void_p()
+        code = self.lib.mpm_diff_create(ctypes.byref(cfg), ctypes.byref(handle))
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_diff_last_error(None).decode())
+        self.handle = handle
+        self.cfg = cfg
+
+    def _check(self, code: int):
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_diff_last_error(self.handle).decode())
+
+    def forward(self, state0: int, lam: int, mu: int, state_final: int | None):
+        self._check(self.lib.mpm_diff_forward(self.handle, state0, lam, mu, state_final))
+
+    def backward(self, bar_final: int, bar_state0: int | None, grad_lambda: int | None, grad_mu: int | None):
+        self._check(self.lib.mpm_diff_backward(self.handle, bar_final, bar_state0, grad_lambda, grad_mu))
+
+    def status(self) -> int:
+        out = np.zeros(1, np.uint32)
+        self._check(self.lib.mpm_diff_read_status(self.handle, _ptr(out)))
+        return int(out[0])
+
+    def timing(self) -> tuple[float, float]:
+        f, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mpm_diff_read_timing(self.handle, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+    def layout(self) -> tuple[int, int, int]:
+        seg, ck = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        nbytes = ctypes.c_size_t()
+        self._check(self.lib.mpm_diff_get_layout(self.handle, _ptr(seg), _ptr(ck), ctypes.byref(nbytes)))
+        return int(seg[0]), int(ck[0]), nbytes.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.mpm_diff_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -15,6 +15,9 @@
 * ``batched_tanks`` — independent Tait water tanks of log-uniform size for
   batched stepping: the SI tank's material and cell size, a dam-break block
   at one particle per cell in a corner, gravity in a random direction.
+* ``elastic_block`` — the differentiable fixed-corotated block: 64x64
+  particles (lambda 200, mu 100, density 1) launched into the floor of a
+  256x256 grid for 200 steps of 1e-3, with a seeded centre-of-mass target.
 """
 from __future__ import annotations
 
@@ -176,3 +179,39 @@ def batched_tanks(count: int = 256, seed: int = BATCH_SEED, min_particles: int =
         gravity = {"x": 9.81 * float(np.cos(theta)), "y": 9.81 * float(np.sin(theta))}
         scenes.append((arr, config, gravity))
     return scenes
+
+
+ELASTIC_SEED = 0x61646A74
+
+
+def elastic_block(seed: int = ELASTIC_SEED, side: int = 64, spacing: float = 0.5, cells: int = 256,
+                  origin=(96.0, 6.0), lam: float = 200.0, mu: float = 100.0, density: float = 1.0,
+                  steps: int = 200, dt: float = 1e-3, gravity=(0.0, -9.81)):
+    """Differentiable elastic block: a side x side lattice of fixed-corotated
+    particles at `spacing` (4 per cell at the defaults) whose lower-left
+    corner is `origin`, in a cells x cells grid of cell size 1, launched
+    toward the floor (wall y = 0).  Every particle gets the block velocity
+    (U(-2, 2), -U(8, 12)) plus its own U(-0.5, 0.5) jitter per axis; the
+    target of the centre-of-mass loss is the initial centre of mass plus
+    (U(-4, 4), U(-2, 2)).  lambda, mu and density are per particle (uniform
+    here); mass = density * spacing^2.  Draws come from the Gate 4 LCG.
+
+    Returns (state0 (n, 12) float64 rows x, y, vx, vy, C00..C11, F00..F11 with
+    C = 0 and F = I, lambda (n,), mu (n,), config dict, target (2,))."""
+    n = side * side
+    u = lcg_uniform(seed, 4 + 2 * n)
+    k = np.arange(n)
+    state = np.zeros((n, 12))
+    state[:, 0] = origin[0] + (k % side + 0.5) * spacing
+    state[:, 1] = origin[1] + (k // side + 0.5) * spacing
+    state[:, 2] = -2.0 + 4.0 * u[0] + (u[4 + 2 * k] - 0.5)
+    state[:, 3] = -(8.0 + 4.0 * u[1]) + (u[5 + 2 * k] - 0.5)
+    state[:, 8] = 1.0
+    state[:, 11] = 1.0
+    area = spacing * spacing
+    config = {"cellsX": cells, "cellsY": cells, "cellSize": 1.0, "dt": dt, "steps": steps,
+              "particleMass": density * area, "particleVolume": area,
+              "gravity": (float(gravity[0]), float(gravity[1])), "wallFriction": 0.0}
+    com = state[:, :2].mean(axis=0)
+    target = com + np.array([-4.0 + 8.0 * u[2], -2.0 + 4.0 * u[3]])
+    return state, np.full(n, lam), np.full(n, mu), config, target
