@@ -207,12 +207,16 @@ int mpm_read_timing(mpm_ctx* ctx, const char* kernel, double* mean_us, uint64_t*
 #define MPM_DIFF_SUBSTEPS 8u    /* more than 64 deformation substeps */
 #define MPM_DIFF_CONFINED 16u   /* a particle was clamped to a wall */
 
+#define MPM_DIFF_NO_GRAPHS 1u     /* config flag: launch kernel by kernel instead of
+                                     replaying captured HIP graphs */
+
 typedef struct mpm_diff mpm_diff;
 typedef struct mpm_diff_config {
     uint32_t particle_count, cells_x, cells_y, steps, segment_steps;
     float cell_size, dt, particle_mass, particle_volume;
     float gravity_x, gravity_y, wall_friction;
     int32_t device; /* HIP device ordinal; -1 = current */
+    uint32_t flags; /* MPM_DIFF_* config flags */
 } mpm_diff_config;
 
 int mpm_diff_create(const mpm_diff_config* cfg, mpm_diff** out);

@@ -39,12 +39,14 @@ def _is_tensor(a) -> bool:
 class ElasticRollout:
     """GPU forward and reverse of a fixed-corotated rollout (one config)."""
 
-    def __init__(self, particle_count: int, config: dict, segment_steps: int = 20, device: int = -1):
+    def __init__(self, particle_count: int, config: dict, segment_steps: int = 20, device: int = -1,
+                 graphs: bool = True):
         g = config.get("gravity", (0.0, 0.0))
         cfg = native.MpmDiffConfig(
             particle_count, config["cellsX"], config["cellsY"], config["steps"], segment_steps,
             config["cellSize"], config["dt"], config["particleMass"], config["particleVolume"],
-            float(g[0]), float(g[1]), config.get("wallFriction", 0.0), device)
+            float(g[0]), float(g[1]), config.get("wallFriction", 0.0), device,
+            0 if graphs else native.MPM_DIFF_NO_GRAPHS)
         self.n = int(particle_count)
         self.steps = int(config["steps"])
         self.native = native.DiffRollout(cfg)

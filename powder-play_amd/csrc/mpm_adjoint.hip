@@ -21,6 +21,8 @@
 //                  the node pass that consumes it
 //   adjacc         float2 per node (dL/dv), float atomics, cleared likewise
 //   qm             float4 per node (dL/dq, dL/dm)
+//   node boxes     int4 (inclusive node range) of each step's P2G; the node
+//                  passes visit only that box
 //   partial        2 float4 per particle: dL/dx and dL/dF carried from the
 //                  G2P reverse to the P2G reverse of the same step
 //   grad lambda/mu double per particle, accumulated over all steps
@@ -29,7 +31,9 @@
 // (G2P of this step fused with P2G of the next) — and so is the reverse:
 // particles (P2G reverse of step t+1 fused with G2P reverse of step t, one
 // thread per particle carries the adjoint between them in registers) and
-// nodes (grid-update reverse).
+// nodes (grid-update reverse).  Both kernel sequences are captured once into
+// HIP graphs and replayed; the status reset, the gradient clears and the
+// host copies stay outside the graphs.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -43,7 +47,7 @@
 
 namespace {
 
-constexpr int PT = 64;   // particles per workgroup (one wave)
+constexpr int PT = 64;   // particles per workgroup: one wave, so the LDS scatter of a block spreads over many CUs
 constexpr int NT = 256;  // nodes per workgroup
 constexpr float CUSHION_CELLS = 8.0f;     // WALL_CUSHION_CELLS, solver.ts:266-283
 constexpr float MAX_INCREMENT = 0.25f;    // MAX_DEFORMATION_INCREMENT
@@ -206,41 +210,131 @@ __device__ inline void gather(const DiffConst& P, const Stencil& s, const float2
     C[3] = c3 * P.gs;
 }
 
-// P2G scatter of one particle state into acc (float atomics).
-__device__ inline uint32_t scatter(const DiffConst& P, float4 xv, float4 C, float4 F, float lam, float mu,
-                                   float4* __restrict__ acc) {
-    Stencil s;
-    if (!make_stencil(P, xv.x, xv.y, s)) return MPM_DIFF_STENCIL;
-    float K[4];
-    const uint32_t err = corot(F, lam, mu, K) ? 0u : (uint32_t)MPM_DIFF_DEFORMATION;
-    const float A0 = C.x * P.pm + K[0] * P.ss, A1 = C.y * P.pm + K[1] * P.ss;
-    const float A2 = C.z * P.pm + K[2] * P.ss, A3 = C.w * P.pm + K[3] * P.ss;
-    const float mvx = P.pm * xv.z, mvy = P.pm * xv.w;
-#pragma unroll
-    for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-        for (int ox = 0; ox < 3; ++ox) {
-            const float w = s.wx[ox] * s.wy[oy];
-            const float dpx = ((float)ox - s.fx) * P.dx, dpy = ((float)oy - s.fy) * P.dx;
-            float* node = reinterpret_cast<float*>(acc + node_key(P, s, ox, oy));
-            unsafeAtomicAdd(node + 0, w * P.pm);
-            unsafeAtomicAdd(node + 1, w * (mvx + A0 * dpx + A1 * dpy));
-            unsafeAtomicAdd(node + 2, w * (mvy + A2 * dpx + A3 * dpy));
-        }
-    return err;
+// ---- block-local pre-reduction of the node scatters ----------------------
+//
+// Device-scope float atomics are the cost of a naive scatter (each one a
+// round trip past the L2).  A workgroup first sums its particles'
+// contributions into an LDS tile spanning the bounding box of their stencils
+// and then adds each touched node once to HBM.  Particles of one workgroup
+// are neighbours in the state order; a workgroup whose box exceeds the tile
+// falls back to direct atomics, which is correct for any order.
+
+constexpr int EMPTY_LO = 0x3fffffff, EMPTY_HI = -0x3fffffff;
+
+struct ScatterTile {
+    int x0, y0, tw, th;  // box of stencil bases; tile is (tw + 2) x (th + 2) nodes
+    bool tiled, empty;
+};
+
+__device__ inline int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
+// Block-wide box of the stencil bases of valid particles (all threads call).
+__device__ inline ScatterTile block_tile(bool ok, int bx, int by, int (*red)[4], int cap_nodes) {
+    int v[4] = {ok ? bx : EMPTY_LO, ok ? by : EMPTY_LO, ok ? bx : EMPTY_HI, ok ? by : EMPTY_HI};
+    v[0] = wave_min(v[0]);
+    v[1] = wave_min(v[1]);
+    v[2] = wave_max(v[2]);
+    v[3] = wave_max(v[3]);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k) red[wave][k] = v[k];
+    __syncthreads();
+    int x0 = red[0][0], y0 = red[0][1], x1 = red[0][2], y1 = red[0][3];
+    for (int w = 1; w < PT / 64; ++w) {
+        x0 = min(x0, red[w][0]);
+        y0 = min(y0, red[w][1]);
+        x1 = max(x1, red[w][2]);
+        y1 = max(y1, red[w][3]);
+    }
+    ScatterTile t;
+    t.empty = x0 > x1;
+    t.x0 = x0;
+    t.y0 = y0;
+    t.tw = t.empty ? 0 : x1 - x0 + 3;
+    t.th = t.empty ? 0 : y1 - y0 + 3;
+    t.tiled = !t.empty && t.tw * t.th <= cap_nodes;
+    return t;
+}
+
+template <int NC>
+__device__ inline void tile_clear(const ScatterTile& t, float* tile) {
+    if (!t.tiled) return;
+    for (int i = threadIdx.x; i < NC * t.tw * t.th; i += PT) tile[i] = 0.0f;
+}
+
+// Adds NC values of one stencil node: into the LDS tile or straight to HBM.
+template <int NC>
+__device__ inline void tile_add(const ScatterTile& t, float* tile, float* global_node, int bx, int by, int ox,
+                                int oy, const float* val) {
+    if (t.tiled) {
+        const int i = (by - t.y0 + oy) * t.tw + (bx - t.x0 + ox), plane = t.tw * t.th;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) atomicAdd(&tile[c * plane + i], val[c]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) unsafeAtomicAdd(global_node + c, val[c]);
+    }
+}
+
+// Adds the tile to HBM, one atomic per touched node and channel.  Node i of
+// the tile is grid node (y0 + i / tw + 1) * W + (x0 + i % tw + 1).
+template <int NC, int STRIDE>
+__device__ inline void tile_flush(const DiffConst& P, const ScatterTile& t, const float* tile, float* global) {
+    if (!t.tiled) return;
+    const int plane = t.tw * t.th;
+    for (int i = threadIdx.x; i < plane; i += PT) {
+        float v[NC];
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            v[c] = tile[c * plane + i];
+            any |= v[c] != 0.0f;
+        }
+        if (!any) continue;
+        const int key = (t.y0 + i / t.tw + 1) * P.W + (t.x0 + i % t.tw + 1);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (v[c] != 0.0f) unsafeAtomicAdd(global + (size_t)key * STRIDE + c, v[c]);
+    }
+}
+
+// Widens the step's node box (inclusive node coordinates) by this block's.
+__device__ inline void box_update(const ScatterTile& t, int4* box) {
+    if (threadIdx.x != 0 || t.empty) return;
+    atomicMin(&box->x, t.x0 + 1);
+    atomicMin(&box->y, t.y0 + 1);
+    atomicMax(&box->z, t.x0 + t.tw);
+    atomicMax(&box->w, t.y0 + t.th);
+}
+
+constexpr int TILE_NODES = 2048;  // LDS tile capacity in nodes (3 channels: 24 KB)
+
 // Forward particle pass: G2P of step t (src -> dst, may alias) and/or P2G of
-// the resulting state.
+// the resulting state into acc, whose node box is widened in `box`.
 template <bool G2P, bool P2G>
 __global__ __launch_bounds__(PT) void k_diff_particles(DiffConst P, const float4* src, float4* dst,
                                                        const float2* __restrict__ nodeV, float4* __restrict__ acc,
-                                                       const float* __restrict__ lam, const float* __restrict__ mu) {
+                                                       const float* __restrict__ lam, const float* __restrict__ mu,
+                                                       int4* __restrict__ box) {
+    __shared__ float tile[3 * TILE_NODES];
+    __shared__ int red[PT / 64][4];
     const uint32_t p = blockIdx.x * PT + threadIdx.x;
-    if (p >= P.n) return;
-    float4 xv = src[3 * p], C = src[3 * p + 1], F = src[3 * p + 2];
+    const bool live = p < P.n;
+    float4 xv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), C = xv, F = xv;
+    if (live) {
+        xv = src[3 * p];
+        C = src[3 * p + 1];
+        F = src[3 * p + 2];
+    }
     uint32_t err = 0;
-    if (G2P) {
+    if (G2P && live) {
         Stencil s;
         if (!make_stencil(P, xv.x, xv.y, s)) {
             err |= MPM_DIFF_STENCIL;
@@ -271,51 +365,108 @@ __global__ __launch_bounds__(PT) void k_diff_particles(DiffConst P, const float4
         dst[3 * p + 1] = C;
         dst[3 * p + 2] = F;
     }
-    if (P2G) err |= scatter(P, xv, C, F, lam[p], mu[p], acc);
+    if (P2G) {
+        Stencil s;
+        const bool ok = live && make_stencil(P, xv.x, xv.y, s);
+        if (live && !ok) err |= MPM_DIFF_STENCIL;
+        const ScatterTile t = block_tile(ok, ok ? s.bx : 0, ok ? s.by : 0, red, TILE_NODES);
+        tile_clear<3>(t, tile);
+        if (t.tiled) __syncthreads();
+        if (ok) {
+            float K[4];
+            if (!corot(F, lam[p], mu[p], K)) err |= MPM_DIFF_DEFORMATION;
+            const float A0 = C.x * P.pm + K[0] * P.ss, A1 = C.y * P.pm + K[1] * P.ss;
+            const float A2 = C.z * P.pm + K[2] * P.ss, A3 = C.w * P.pm + K[3] * P.ss;
+            const float mvx = P.pm * xv.z, mvy = P.pm * xv.w;
+#pragma unroll
+            for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+                for (int ox = 0; ox < 3; ++ox) {
+                    const float w = s.wx[ox] * s.wy[oy];
+                    const float dpx = ((float)ox - s.fx) * P.dx, dpy = ((float)oy - s.fy) * P.dx;
+                    const float val[3] = {w * P.pm, w * (mvx + A0 * dpx + A1 * dpy), w * (mvy + A2 * dpx + A3 * dpy)};
+                    tile_add<3>(t, tile, reinterpret_cast<float*>(acc + node_key(P, s, ox, oy)), s.bx, s.by, ox, oy,
+                                val);
+                }
+        }
+        if (t.tiled) __syncthreads();
+        tile_flush<3, 4>(P, t, tile, reinterpret_cast<float*>(acc));
+        box_update(t, box);
+    }
     if (err) atomicOr(P.status, err);
 }
 
-// Forward node pass: consumes and clears acc, writes the node velocity after
-// the walls (nodeV) and, when recording a segment, (m, vh) for the reverse.
-__global__ __launch_bounds__(NT) void k_diff_nodes(DiffConst P, float4* __restrict__ acc, float2* __restrict__ nodeV,
-                                                   float4* __restrict__ nodeB) {
-    const int k = blockIdx.x * NT + threadIdx.x;
-    if (k >= P.nodes) return;
-    const float4 a = acc[k];
-    acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float xi = (float)(k % P.W - 1), yi = (float)(k / P.W - 1);
-    float vhx = 0.0f, vhy = 0.0f;
-    if (a.x > 0.0f) {
-        vhx = a.y / a.x + P.gdx;
-        vhy = a.z / a.x + P.gdy;
-    }
-    const float vx = (vhx * cushion_factor(vhx, xi, P.ext_x, P.cush_x)) * friction_factor(yi, P.ext_y, P.cush_y, P.friction);
-    const float vy = (vhy * cushion_factor(vhy, yi, P.ext_y, P.cush_y)) * friction_factor(xi, P.ext_x, P.cush_x, P.friction);
-    nodeV[k] = make_float2(vx, vy);
-    if (nodeB) nodeB[k] = make_float4(a.x, vhx, vhy, 0.0f);
-    if (!isfinite(vx) || !isfinite(vy)) atomicOr(P.status, (uint32_t)MPM_DIFF_NONFINITE);
+constexpr int NODE_BLOCKS = 16;  // node passes loop over the step's box
+
+// Grid node of the i-th position of a box (row-major over the box).
+__device__ inline int box_node(const int4& b, int i, const DiffConst& P) {
+    const int bw = b.z - b.x + 1;
+    return (b.y + i / bw) * P.W + (b.x + i % bw);
 }
 
-// Reverse of the grid update of one step: consumes and clears adjacc
-// (dL/dv), writes (dL/dq, dL/dm) per node.
-__global__ __launch_bounds__(NT) void k_diff_nodes_bwd(DiffConst P, float2* __restrict__ adjacc,
-                                                       const float4* __restrict__ nodeB, float4* __restrict__ qm) {
-    const int k = blockIdx.x * NT + threadIdx.x;
-    if (k >= P.nodes) return;
-    const float2 vb = adjacc[k];
-    adjacc[k] = make_float2(0.0f, 0.0f);
-    const float4 b = nodeB[k];
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (b.x > 0.0f) {
-        const float xi = (float)(k % P.W - 1), yi = (float)(k / P.W - 1);
-        const float ax = cushion_factor(b.y, xi, P.ext_x, P.cush_x) * friction_factor(yi, P.ext_y, P.cush_y, P.friction);
-        const float ay = cushion_factor(b.z, yi, P.ext_y, P.cush_y) * friction_factor(xi, P.ext_x, P.cush_x, P.friction);
-        const float hbx = ax * vb.x, hby = ay * vb.y;
-        out.x = hbx / b.x;
-        out.y = hby / b.x;
-        out.z = -(hbx * (b.y - P.gdx) + hby * (b.z - P.gdy)) / b.x;
+// Forward node pass over the box of the step's P2G: consumes and clears acc,
+// writes the node velocity after the walls (nodeV) and, when recording a
+// segment, (m, vh) for the reverse (nodeB) and the box itself (box_save).
+// Resets the other box for the next P2G.
+__global__ __launch_bounds__(NT) void k_diff_nodes(DiffConst P, float4* __restrict__ acc, float2* __restrict__ nodeV,
+                                                   float4* __restrict__ nodeB, const int4* __restrict__ box_in,
+                                                   int4* __restrict__ box_reset, int4* __restrict__ box_save) {
+    const int4 b = *box_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *box_reset = make_int4(EMPTY_LO, EMPTY_LO, EMPTY_HI, EMPTY_HI);
+        if (box_save) *box_save = b;
     }
-    qm[k] = out;
+    if (b.x > b.z) return;
+    const int count = (b.z - b.x + 1) * (b.w - b.y + 1);
+    for (int i = blockIdx.x * NT + threadIdx.x; i < count; i += gridDim.x * NT) {
+        const int k = box_node(b, i, P);
+        const float4 a = acc[k];
+        acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float xi = (float)(k % P.W - 1), yi = (float)(k / P.W - 1);
+        float vhx = 0.0f, vhy = 0.0f;
+        if (a.x > 0.0f) {
+            vhx = a.y / a.x + P.gdx;
+            vhy = a.z / a.x + P.gdy;
+        }
+        const float vx = (vhx * cushion_factor(vhx, xi, P.ext_x, P.cush_x)) *
+                         friction_factor(yi, P.ext_y, P.cush_y, P.friction);
+        const float vy = (vhy * cushion_factor(vhy, yi, P.ext_y, P.cush_y)) *
+                         friction_factor(xi, P.ext_x, P.cush_x, P.friction);
+        nodeV[k] = make_float2(vx, vy);
+        if (nodeB) nodeB[k] = make_float4(a.x, vhx, vhy, 0.0f);
+        if (!isfinite(vx) || !isfinite(vy)) atomicOr(P.status, (uint32_t)MPM_DIFF_NONFINITE);
+    }
+}
+
+// Reverse of the grid update of one step, over that step's box: consumes and
+// clears adjacc (dL/dv), writes (dL/dq, dL/dm) per node.
+__global__ __launch_bounds__(NT) void k_diff_nodes_bwd(DiffConst P, float2* __restrict__ adjacc,
+                                                       const float4* __restrict__ nodeB, float4* __restrict__ qm,
+                                                       const int4* __restrict__ box) {
+    const int4 b = *box;
+    if (b.x > b.z) return;
+    const int count = (b.z - b.x + 1) * (b.w - b.y + 1);
+    for (int i = blockIdx.x * NT + threadIdx.x; i < count; i += gridDim.x * NT) {
+        const int k = box_node(b, i, P);
+        const float2 vb = adjacc[k];
+        adjacc[k] = make_float2(0.0f, 0.0f);
+        const float4 nb = nodeB[k];
+        float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (nb.x > 0.0f) {
+            const float xi = (float)(k % P.W - 1), yi = (float)(k / P.W - 1);
+            const float ax = cushion_factor(nb.y, xi, P.ext_x, P.cush_x) * friction_factor(yi, P.ext_y, P.cush_y, P.friction);
+            const float ay = cushion_factor(nb.z, yi, P.ext_y, P.cush_y) * friction_factor(xi, P.ext_x, P.cush_x, P.friction);
+            const float hbx = ax * vb.x, hby = ay * vb.y;
+            out.x = hbx / nb.x;
+            out.y = hby / nb.x;
+            out.z = -(hbx * (nb.y - P.gdx) + hby * (nb.z - P.gdy)) / nb.x;
+        }
+        qm[k] = out;
+    }
+}
+
+__global__ void k_diff_box_reset(int4* __restrict__ boxes) {
+    if (threadIdx.x < 2) boxes[threadIdx.x] = make_int4(EMPTY_LO, EMPTY_LO, EMPTY_HI, EMPTY_HI);
 }
 
 // Reverse of F' = U^s F (U = I + h C'): Fb = (U^T)^s G and
@@ -354,18 +505,20 @@ __global__ __launch_bounds__(PT) void k_diff_particles_bwd(
     const float2* __restrict__ nodeV, float2* __restrict__ adjacc, float4* __restrict__ partial,
     const float4* __restrict__ bar_in, float4* __restrict__ bar_out, const float* __restrict__ lam,
     const float* __restrict__ mu, double* __restrict__ glam, double* __restrict__ gmu) {
+    __shared__ float tile[2 * TILE_NODES];
+    __shared__ int red[PT / 64][4];
     const uint32_t p = blockIdx.x * PT + threadIdx.x;
-    if (p >= P.n) return;
-    float ax, ay, avx, avy, aC[4], aF[4];
+    const bool live = p < P.n;
+    float ax = 0.0f, ay = 0.0f, avx = 0.0f, avy = 0.0f;
+    float aC[4] = {0.0f, 0.0f, 0.0f, 0.0f}, aF[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t err = 0;
-    if (B3) {
+    if (B3 && live) {
         const float4 xv = s_next[3 * p], C = s_next[3 * p + 1], F = s_next[3 * p + 2];
         const float4 px = partial[2 * p], pF = partial[2 * p + 1];
         ax = px.x;
         ay = px.y;
         double Fb[4] = {pF.x, pF.y, pF.z, pF.w};
         float Ab[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        avx = avy = 0.0f;
         Stencil s;
         float K[4];
         if (!make_stencil(P, xv.x, xv.y, s)) {
@@ -404,7 +557,7 @@ __global__ __launch_bounds__(PT) void k_diff_particles_bwd(
             aC[k] = P.pm * Ab[k];
             aF[k] = (float)Fb[k];
         }
-    } else {
+    } else if (!B3 && live) {
         const float4 b0 = bar_in[3 * p], b1 = bar_in[3 * p + 1], b2 = bar_in[3 * p + 2];
         ax = b0.x;
         ay = b0.y;
@@ -414,12 +567,19 @@ __global__ __launch_bounds__(PT) void k_diff_particles_bwd(
         aF[0] = b2.x; aF[1] = b2.y; aF[2] = b2.z; aF[3] = b2.w;
     }
     if (B1) {
-        const float4 xv = s_cur[3 * p], F = s_cur[3 * p + 2];
+        float4 xv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), F = xv;
+        if (live) {
+            xv = s_cur[3 * p];
+            F = s_cur[3 * p + 2];
+        }
         float xb[2] = {0.0f, 0.0f}, Fb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         Stencil s;
-        if (!make_stencil(P, xv.x, xv.y, s)) {
-            err |= MPM_DIFF_STENCIL;
-        } else {
+        const bool ok = live && make_stencil(P, xv.x, xv.y, s);
+        if (live && !ok) err |= MPM_DIFF_STENCIL;
+        const ScatterTile t = block_tile(ok, ok ? s.bx : 0, ok ? s.by : 0, red, TILE_NODES);
+        tile_clear<2>(t, tile);
+        if (t.tiled) __syncthreads();
+        if (ok) {
             float v[2], Cn[4];
             gather(P, s, nodeV, v, Cn);
             int ns = substeps(Cn, P.dt);
@@ -447,17 +607,20 @@ __global__ __launch_bounds__(PT) void k_diff_particles_bwd(
                     const int key = node_key(P, s, ox, oy);
                     const float2 nv = nodeV[key];
                     const float ux = S0 * dpx + S1 * dpy, uy = S2 * dpx + S3 * dpy;
-                    float* node = reinterpret_cast<float*>(adjacc + key);
-                    unsafeAtomicAdd(node + 0, w * (vgx + ux));
-                    unsafeAtomicAdd(node + 1, w * (vgy + uy));
+                    const float val[2] = {w * (vgx + ux), w * (vgy + uy)};
+                    tile_add<2>(t, tile, reinterpret_cast<float*>(adjacc + key), s.bx, s.by, ox, oy, val);
                     const float wb = nv.x * (vgx + ux) + nv.y * (vgy + uy);
                     xb[0] += wb * s.gx[ox] * s.wy[oy] - w * (S0 * nv.x + S2 * nv.y);
                     xb[1] += wb * s.wx[ox] * s.gy[oy] - w * (S1 * nv.x + S3 * nv.y);
                 }
         }
-        partial[2 * p] = make_float4(xb[0], xb[1], 0.0f, 0.0f);
-        partial[2 * p + 1] = make_float4(Fb[0], Fb[1], Fb[2], Fb[3]);
-    } else {
+        if (t.tiled) __syncthreads();
+        tile_flush<2, 2>(P, t, tile, reinterpret_cast<float*>(adjacc));
+        if (live) {
+            partial[2 * p] = make_float4(xb[0], xb[1], 0.0f, 0.0f);
+            partial[2 * p + 1] = make_float4(Fb[0], Fb[1], Fb[2], Fb[3]);
+        }
+    } else if (live) {
         bar_out[3 * p] = make_float4(ax, ay, avx, avy);
         bar_out[3 * p + 1] = make_float4(aC[0], aC[1], aC[2], aC[3]);
         bar_out[3 * p + 2] = make_float4(aF[0], aF[1], aF[2], aF[3]);
@@ -480,6 +643,8 @@ struct mpm_diff {
     float4* segs = nullptr;     // seg levels (level 0 unused: it is the checkpoint)
     float2* segV = nullptr;     // seg grid levels: node velocity
     float4* segB = nullptr;     // seg grid levels: (m, vh)
+    int4* segBox = nullptr;     // seg grid levels: node box of the step
+    int4* box = nullptr;        // two node boxes: the pending P2G's and the next one's
     float2* fwdV = nullptr;     // forward node velocity
     float4* acc = nullptr;
     float2* adjacc = nullptr;
@@ -493,6 +658,8 @@ struct mpm_diff {
     float* gout = nullptr;      // float copies of the lambda/mu gradients
     uint32_t* status = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool use_graphs = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // captured forward / reverse sequences
     bool have_forward = false;
     uint32_t last_status = 0;
     std::string error;
@@ -523,12 +690,15 @@ int diff_hip(mpm_diff* h, hipError_t e, const char* what) {
     } while (0)
 
 void release(mpm_diff* h) {
-    void* bufs[] = {h->ckpt, h->fin, h->scratch, h->segs, h->segV, h->segB, h->fwdV, h->acc, h->adjacc, h->qm,
+    void* bufs[] = {h->ckpt, h->fin, h->scratch, h->segs, h->segV, h->segB, h->segBox, h->box, h->fwdV, h->acc,
+                    h->adjacc, h->qm,
                     h->partial, h->bar, h->lam, h->mu, h->glam, h->gmu, h->gout, h->status};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t& e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipGraphExec_t& g : h->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -547,19 +717,27 @@ float4* state_at(mpm_diff* h, uint32_t j, uint32_t t) {
     return seg_level(h, j, t - t0);
 }
 
-hipError_t launch_particles(mpm_diff* h, bool g2p, bool p2g, const float4* src, float4* dst, const float2* nodeV) {
+hipError_t launch_particles(mpm_diff* h, bool g2p, bool p2g, const float4* src, float4* dst, const float2* nodeV,
+                            int4* box) {
     const dim3 grid(grid_for(h->P.n, PT)), block(PT);
     if (g2p && p2g)
-        k_diff_particles<true, true><<<grid, block, 0, h->stream>>>(h->P, src, dst, nodeV, h->acc, h->lam, h->mu);
+        k_diff_particles<true, true><<<grid, block, 0, h->stream>>>(h->P, src, dst, nodeV, h->acc, h->lam, h->mu, box);
     else if (g2p)
-        k_diff_particles<true, false><<<grid, block, 0, h->stream>>>(h->P, src, dst, nodeV, h->acc, h->lam, h->mu);
+        k_diff_particles<true, false><<<grid, block, 0, h->stream>>>(h->P, src, dst, nodeV, h->acc, h->lam, h->mu, box);
     else
-        k_diff_particles<false, true><<<grid, block, 0, h->stream>>>(h->P, src, dst, nodeV, h->acc, h->lam, h->mu);
+        k_diff_particles<false, true><<<grid, block, 0, h->stream>>>(h->P, src, dst, nodeV, h->acc, h->lam, h->mu, box);
     return hipGetLastError();
 }
 
-hipError_t launch_nodes(mpm_diff* h, float2* nodeV, float4* nodeB) {
-    k_diff_nodes<<<grid_for((uint32_t)h->P.nodes, NT), NT, 0, h->stream>>>(h->P, h->acc, nodeV, nodeB);
+// Node pass of the P2G whose box is box[q]; resets box[q ^ 1] for the next.
+hipError_t launch_nodes(mpm_diff* h, float2* nodeV, float4* nodeB, int q, int4* box_save) {
+    k_diff_nodes<<<NODE_BLOCKS, NT, 0, h->stream>>>(h->P, h->acc, nodeV, nodeB, h->box + q, h->box + (q ^ 1),
+                                                     box_save);
+    return hipGetLastError();
+}
+
+hipError_t reset_boxes(mpm_diff* h) {
+    k_diff_box_reset<<<1, 64, 0, h->stream>>>(h->box);
     return hipGetLastError();
 }
 
@@ -575,7 +753,7 @@ int mpm_diff_create(const mpm_diff_config* cfg, mpm_diff** out) {
         c.cells_x > 16384 || c.cells_y > 16384 || c.steps < 1 || !(c.cell_size > 0.0f) || !(c.dt > 0.0f) ||
         !(c.particle_mass > 0.0f) || !(c.particle_volume > 0.0f) || !(c.wall_friction >= 0.0f) ||
         !(c.wall_friction <= 1.0f) || !std::isfinite(c.gravity_x) || !std::isfinite(c.gravity_y) ||
-        !std::isfinite(c.cell_size) || !std::isfinite(c.dt))
+        !std::isfinite(c.cell_size) || !std::isfinite(c.dt) || (c.flags & ~(uint32_t)MPM_DIFF_NO_GRAPHS))
         return diff_fail(nullptr, MPM_EINVAL, "invalid rollout configuration");
     mpm_diff* h = new (std::nothrow) mpm_diff();
     if (!h) return diff_fail(nullptr, MPM_EDEVICE, "out of host memory");
@@ -585,6 +763,7 @@ int mpm_diff_create(const mpm_diff_config* cfg, mpm_diff** out) {
         return code;
     };
     h->cfg = c;
+    h->use_graphs = (c.flags & MPM_DIFF_NO_GRAPHS) == 0;
     h->steps = c.steps;
     h->seg = c.segment_steps ? (c.segment_steps < c.steps ? c.segment_steps : c.steps) : (c.steps < 20 ? c.steps : 20);
     h->nseg = (h->steps + h->seg - 1) / h->seg;
@@ -629,6 +808,8 @@ int mpm_diff_create(const mpm_diff_config* cfg, mpm_diff** out) {
     DIFF_ALLOC(h->segs, lvl * h->seg);
     DIFF_ALLOC(h->segV, nodes * h->seg * sizeof(float2));
     DIFF_ALLOC(h->segB, nodes * h->seg * sizeof(float4));
+    DIFF_ALLOC(h->segBox, h->seg * sizeof(int4));
+    DIFF_ALLOC(h->box, 2 * sizeof(int4));
     DIFF_ALLOC(h->fwdV, nodes * sizeof(float2));
     DIFF_ALLOC(h->acc, nodes * sizeof(float4));
     DIFF_ALLOC(h->adjacc, nodes * sizeof(float2));
@@ -680,6 +861,97 @@ int finish(mpm_diff* h, const char* what) {
     return MPM_OK;
 }
 
+// Kernel sequence of the forward rollout: P2G of state 0, then per step the
+// node pass and the particle pass (G2P fused with the next P2G).  Every
+// pointer is fixed for the life of the rollout, so the sequence can be
+// captured once and replayed.
+hipError_t enqueue_forward(mpm_diff* h) {
+    const uint32_t T = h->steps, L = h->seg;
+    int q = 0;
+    hipError_t e = reset_boxes(h);
+    if (e == hipSuccess) e = launch_particles(h, false, true, h->ckpt, nullptr, nullptr, h->box + q);
+    const float4* src = h->ckpt;
+    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
+        e = launch_nodes(h, h->fwdV, nullptr, q, nullptr);
+        q ^= 1;
+        const uint32_t t1 = t + 1;
+        float4* dst = t1 == T ? h->fin : (t1 % L == 0 ? ckpt_level(h, t1 / L) : h->scratch);
+        if (e == hipSuccess) e = launch_particles(h, true, t1 < T, src, dst, h->fwdV, h->box + q);
+        src = dst;
+    }
+    return e;
+}
+
+// Kernel sequence of the reverse pass: for each segment, last first, record
+// its states and grids from the checkpoint, then reverse its steps.
+hipError_t enqueue_backward(mpm_diff* h) {
+    const uint32_t n = h->P.n, T = h->steps, L = h->seg;
+    const size_t nodes = (size_t)h->P.nodes;
+    const dim3 pg(grid_for(n, PT)), pb(PT);
+    hipError_t e = hipSuccess;
+    for (int j = (int)h->nseg - 1; j >= 0 && e == hipSuccess; --j) {
+        const uint32_t t0 = (uint32_t)j * L, len = (T - t0) < L ? (T - t0) : L;
+        int q = 0;
+        e = reset_boxes(h);
+        if (e == hipSuccess) e = launch_particles(h, false, true, ckpt_level(h, j), nullptr, nullptr, h->box + q);
+        if (e == hipSuccess) e = launch_nodes(h, h->segV, h->segB, q, h->segBox);
+        q ^= 1;
+        for (uint32_t k = 1; k < len && e == hipSuccess; ++k) {
+            e = launch_particles(h, true, true, seg_level(h, j, k - 1), seg_level(h, j, k), h->segV + (k - 1) * nodes,
+                                 h->box + q);
+            if (e == hipSuccess) e = launch_nodes(h, h->segV + k * nodes, h->segB + k * nodes, q, h->segBox + k);
+            q ^= 1;
+        }
+        for (int k = (int)len - 1; k >= 0 && e == hipSuccess; --k) {
+            const uint32_t t = t0 + (uint32_t)k;
+            const float4* s_cur = seg_level(h, j, (uint32_t)k);
+            const float2* nv = h->segV + (size_t)k * nodes;
+            if (t + 1 == T)
+                k_diff_particles_bwd<false, true><<<pg, pb, 0, h->stream>>>(
+                    h->P, nullptr, nullptr, s_cur, nv, h->adjacc, h->partial, h->bar, nullptr, h->lam, h->mu,
+                    h->glam, h->gmu);
+            else
+                k_diff_particles_bwd<true, true><<<pg, pb, 0, h->stream>>>(
+                    h->P, state_at(h, j, t + 1), h->qm, s_cur, nv, h->adjacc, h->partial, nullptr, nullptr, h->lam,
+                    h->mu, h->glam, h->gmu);
+            e = hipGetLastError();
+            if (e != hipSuccess) break;
+            k_diff_nodes_bwd<<<NODE_BLOCKS, NT, 0, h->stream>>>(h->P, h->adjacc, h->segB + (size_t)k * nodes, h->qm,
+                                                                h->segBox + k);
+            e = hipGetLastError();
+        }
+    }
+    if (e != hipSuccess) return e;
+    k_diff_particles_bwd<true, false><<<pg, pb, 0, h->stream>>>(h->P, h->ckpt, h->qm, nullptr, nullptr, nullptr,
+                                                               h->partial, nullptr, h->bar, h->lam, h->mu, h->glam,
+                                                               h->gmu);
+    return hipGetLastError();
+}
+
+// Runs sequence 0 (forward) or 1 (reverse): replays its graph, capturing it
+// on first use, or launches the kernels one by one when graphs are off.
+hipError_t run_sequence(mpm_diff* h, int which) {
+    if (!h->use_graphs) return which == 0 ? enqueue_forward(h) : enqueue_backward(h);
+    if (!h->graph[which]) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        const hipError_t le = which == 0 ? enqueue_forward(h) : enqueue_backward(h);
+        e = hipStreamEndCapture(h->stream, &g);
+        if (le != hipSuccess || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return le != hipSuccess ? le : e;
+        }
+        e = hipGraphInstantiate(&h->graph[which], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            h->graph[which] = nullptr;
+            return e;
+        }
+    }
+    return hipGraphLaunch(h->graph[which], h->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -687,7 +959,7 @@ extern "C" {
 int mpm_diff_forward(mpm_diff* h, const float* state0, const float* lambda, const float* mu, float* state_final) {
     if (!h || !state0 || !lambda || !mu) return MPM_EINVAL;
     DIFF_CHECK(hipSetDevice(h->device));
-    const uint32_t n = h->P.n, T = h->steps, L = h->seg;
+    const uint32_t n = h->P.n;
     const size_t lvl = h->level_f4 * sizeof(float4);
     h->have_forward = false;
     DIFF_CHECK(hipMemsetAsync(h->status, 0, sizeof(uint32_t), h->stream));
@@ -695,15 +967,7 @@ int mpm_diff_forward(mpm_diff* h, const float* state0, const float* lambda, cons
     DIFF_CHECK(hipMemcpyAsync(h->lam, lambda, n * sizeof(float), hipMemcpyDefault, h->stream));
     DIFF_CHECK(hipMemcpyAsync(h->mu, mu, n * sizeof(float), hipMemcpyDefault, h->stream));
     DIFF_CHECK(hipEventRecord(h->ev[0], h->stream));
-    DIFF_CHECK(launch_particles(h, false, true, h->ckpt, nullptr, nullptr));
-    const float4* src = h->ckpt;
-    for (uint32_t t = 0; t < T; ++t) {
-        DIFF_CHECK(launch_nodes(h, h->fwdV, nullptr));
-        const uint32_t t1 = t + 1;
-        float4* dst = t1 == T ? h->fin : (t1 % L == 0 ? ckpt_level(h, t1 / L) : h->scratch);
-        DIFF_CHECK(launch_particles(h, true, t1 < T, src, dst, h->fwdV));
-        src = dst;
-    }
+    DIFF_CHECK(run_sequence(h, 0));
     DIFF_CHECK(hipEventRecord(h->ev[1], h->stream));
     if (state_final) DIFF_CHECK(hipMemcpyAsync(state_final, h->fin, lvl, hipMemcpyDefault, h->stream));
     const int rc = finish(h, "forward");
@@ -715,49 +979,14 @@ int mpm_diff_backward(mpm_diff* h, const float* bar_final, float* bar_state0, fl
     if (!h || !bar_final) return MPM_EINVAL;
     if (!h->have_forward) return diff_fail(h, MPM_ESTATE, "backward needs a successful forward first");
     DIFF_CHECK(hipSetDevice(h->device));
-    const uint32_t n = h->P.n, T = h->steps, L = h->seg;
-    const size_t lvl = h->level_f4 * sizeof(float4), nodes = (size_t)h->P.nodes;
-    const dim3 pg(grid_for(n, PT)), pb(PT);
+    const uint32_t n = h->P.n;
+    const size_t lvl = h->level_f4 * sizeof(float4);
     DIFF_CHECK(hipMemsetAsync(h->status, 0, sizeof(uint32_t), h->stream));
     DIFF_CHECK(hipMemcpyAsync(h->bar, bar_final, lvl, hipMemcpyDefault, h->stream));
     DIFF_CHECK(hipMemsetAsync(h->glam, 0, n * sizeof(double), h->stream));
     DIFF_CHECK(hipMemsetAsync(h->gmu, 0, n * sizeof(double), h->stream));
     DIFF_CHECK(hipEventRecord(h->ev[2], h->stream));
-    for (int j = (int)h->nseg - 1; j >= 0; --j) {
-        const uint32_t t0 = (uint32_t)j * L, len = (T - t0) < L ? (T - t0) : L;
-        // Record segment j: states t0 .. t0+len-1 and the grid of each step.
-        DIFF_CHECK(launch_particles(h, false, true, ckpt_level(h, j), nullptr, nullptr));
-        DIFF_CHECK(launch_nodes(h, h->segV, h->segB));
-        for (uint32_t k = 1; k < len; ++k) {
-            DIFF_CHECK(launch_particles(h, true, true, seg_level(h, j, k - 1), seg_level(h, j, k),
-                                        h->segV + (k - 1) * nodes));
-            DIFF_CHECK(launch_nodes(h, h->segV + k * nodes, h->segB + k * nodes));
-        }
-        // Reverse steps t0+len-1 .. t0.
-        for (int k = (int)len - 1; k >= 0; --k) {
-            const uint32_t t = t0 + (uint32_t)k;
-            const float4* s_cur = seg_level(h, j, (uint32_t)k);
-            const float2* nv = h->segV + (size_t)k * nodes;
-            if (t + 1 == T) {
-                k_diff_particles_bwd<false, true><<<pg, pb, 0, h->stream>>>(
-                    h->P, nullptr, nullptr, s_cur, nv, h->adjacc, h->partial, h->bar, nullptr, h->lam, h->mu,
-                    h->glam, h->gmu);
-            } else {
-                const float4* s_next = state_at(h, j, t + 1);
-                k_diff_particles_bwd<true, true><<<pg, pb, 0, h->stream>>>(
-                    h->P, s_next, h->qm, s_cur, nv, h->adjacc, h->partial, nullptr, nullptr, h->lam, h->mu,
-                    h->glam, h->gmu);
-            }
-            DIFF_CHECK(hipGetLastError());
-            k_diff_nodes_bwd<<<grid_for((uint32_t)nodes, NT), NT, 0, h->stream>>>(h->P, h->adjacc,
-                                                                                  h->segB + (size_t)k * nodes, h->qm);
-            DIFF_CHECK(hipGetLastError());
-        }
-    }
-    k_diff_particles_bwd<true, false><<<pg, pb, 0, h->stream>>>(h->P, h->ckpt, h->qm, nullptr, nullptr, nullptr,
-                                                               h->partial, nullptr, h->bar, h->lam, h->mu,
-                                                               h->glam, h->gmu);
-    DIFF_CHECK(hipGetLastError());
+    DIFF_CHECK(run_sequence(h, 1));
     DIFF_CHECK(hipEventRecord(h->ev[3], h->stream));
     k_diff_grad_out<<<grid_for(n, NT), NT, 0, h->stream>>>(n, h->glam, h->gmu, h->gout);
     DIFF_CHECK(hipGetLastError());
@@ -794,6 +1023,7 @@ int mpm_diff_get_layout(mpm_diff* h, uint32_t* segment_steps, uint32_t* checkpoi
     if (device_bytes) {
         const size_t lvl = h->level_f4 * sizeof(float4), nodes = (size_t)h->P.nodes, n = h->P.n;
         *device_bytes = lvl * (h->nseg + 3 + h->seg) + nodes * h->seg * (sizeof(float2) + sizeof(float4)) +
+                        (h->seg + 2) * sizeof(int4) +
                         nodes * (sizeof(float2) * 2 + sizeof(float4) * 2) + n * (2 * sizeof(float4) + 2 * sizeof(float) +
                         2 * sizeof(double) + 2 * sizeof(float)) + sizeof(uint32_t);
     }

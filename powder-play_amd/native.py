@@ -60,6 +60,7 @@ SIGNATURES = {
 
 MPM_DIFF_STATE_FLOATS = 12
 MPM_DIFF_STENCIL, MPM_DIFF_DEFORMATION, MPM_DIFF_NONFINITE, MPM_DIFF_SUBSTEPS, MPM_DIFF_CONFINED = 1, 2, 4, 8, 16
+MPM_DIFF_NO_GRAPHS = 1
 
 
 class MpmDiffConfig(ctypes.Structure):
@@ -68,7 +69,7 @@ class MpmDiffConfig(ctypes.Structure):
         ("segment_steps", _U32), ("cell_size", ctypes.c_float), ("dt", ctypes.c_float),
         ("particle_mass", ctypes.c_float), ("particle_volume", ctypes.c_float),
         ("gravity_x", ctypes.c_float), ("gravity_y", ctypes.c_float), ("wall_friction", ctypes.c_float),
-        ("device", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("flags", _U32),
     ]
 
 

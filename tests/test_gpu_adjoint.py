@@ -36,10 +36,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def run_gpu(pkg, block, bar=None, segment=20, cfg_over=None):
+def run_gpu(pkg, block, bar=None, segment=20, cfg_over=None, graphs=True):
     state, lam, mu, config, target = block
     config = dict(config, **(cfg_over or {}))
-    r = pkg.adjoint.ElasticRollout(len(lam), config, segment_steps=segment, device=0)
+    r = pkg.adjoint.ElasticRollout(len(lam), config, segment_steps=segment, device=0, graphs=graphs)
     try:
         final = r.forward(state, lam, mu)
         if bar is None:
@@ -78,11 +78,11 @@ def test_gradients_match_fp64_adjoint(pkg, block, fp64):
     assert layout["segment_steps"] == 20 and layout["checkpoints"] == 10
 
 
-def test_segment_length_does_not_change_gradients(pkg, block, fp64):
+def test_segment_length_and_graphs_do_not_change_gradients(pkg, block, fp64):
     bar = fp64[1].astype(np.float32)
     base = run_gpu(pkg, block, bar=bar, segment=20)
-    for seg in (7, 200):
-        other = run_gpu(pkg, block, bar=bar, segment=seg)
+    for seg, graphs in ((7, True), (200, True), (20, False)):
+        other = run_gpu(pkg, block, bar=bar, segment=seg, graphs=graphs)
         assert other[6]["segment_steps"] == seg
         for a, b in zip(base[1:4], other[1:4]):
             # float atomics make sums order-dependent: agreement to rounding
