@@ -15,15 +15,19 @@
 * ``batched_tanks`` — independent Tait water tanks of log-uniform size for
   batched stepping: the SI tank's material and cell size, a dam-break block
   at one particle per cell in a corner, gravity in a random direction.
+* ``create_mpm_emitter_particles`` — the deterministic emitter burst of
+  scenes/emitter.ts:49-123, as MpmParticle dicts for append_particles.
 * ``elastic_block`` — the differentiable fixed-corotated block: 64x64
   particles (lambda 200, mu 100, density 1) launched into the floor of a
   256x256 grid for 200 steps of 1e-3, with a seeded centre-of-mass target.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
-from .layout import ParticleArrays
+from .layout import ParticleArrays, RangeError
 
 LCG_A = 1664525
 LCG_C = 1013904223
@@ -215,3 +219,96 @@ def elastic_block(seed: int = ELASTIC_SEED, side: int = 64, spacing: float = 0.5
     com = state[:, :2].mean(axis=0)
     target = com + np.array([-4.0 + 8.0 * u[2], -2.0 + 4.0 * u[3]])
     return state, np.full(n, lam), np.full(n, mu), config, target
+
+
+# ---------------------------------------------------------------- emitter
+MAX_MPM_EMITTER_PARTICLES = 100_000                  # emitter.ts:9
+GATE4_LIQUID_BULK_MODULUS_SCALE = 4                  # gate4.ts:35
+_GOLDEN_ANGLE = math.pi * (3 - math.sqrt(5))
+EMITTER_MATERIALS = {                                # emitter.ts:28-46
+    "water": {"lambda": 40_000 * GATE4_LIQUID_BULK_MODULUS_SCALE, "mu": 0, "model": "water", "density": 0.02},
+    "oil": {"lambda": 30_000 * GATE4_LIQUID_BULK_MODULUS_SCALE, "mu": 0, "model": "oil", "density": 0.016},
+    "neo-hookean": {"lambda": 20, "mu": 10, "model": "neo-hookean"},
+}
+
+
+def _positive_finite(value, name):
+    if not isinstance(value, (int, float)) or isinstance(value, bool) or not math.isfinite(value) or value <= 0:
+        raise RangeError(f"{name} must be a positive finite number")
+    return float(value)
+
+
+def create_mpm_emitter_particles(config: dict) -> list[dict]:
+    """createMpmEmitterParticles (scenes/emitter.ts:49-123): `count`
+    particles of one material around `center`, on a golden-angle spiral
+    (circle) or a jittered lattice (rectangle), all with `velocity`, C = 0,
+    F = I, plastic volume 1 and `particleVolume` (default 25).  Draws come
+    from the seeded LCG state = state * 1664525 + 1013904223 mod 2^32
+    (emitter.ts:164-170); the first draw rotates the spiral, then one draw
+    per particle (circle) or two (rectangle) jitter it by +-6% of the
+    spacing.  Returns MpmParticle-shaped dicts for append_particles."""
+    count = config.get("count")
+    if not isinstance(count, int) or isinstance(count, bool) or count < 1:
+        raise RangeError("count must be a positive safe integer")
+    if count > MAX_MPM_EMITTER_PARTICLES:
+        raise RangeError(f"count must not exceed {MAX_MPM_EMITTER_PARTICLES}")
+    seed = config.get("seed")
+    if not isinstance(seed, int) or isinstance(seed, bool) or seed < 0 or seed > 0xFFFFFFFF:
+        raise RangeError("seed must be an unsigned 32-bit integer")
+    shape = config.get("shape")
+    if shape not in ("rectangle", "circle"):
+        raise RangeError("shape must be rectangle or circle")
+    material = config.get("material")
+    if material not in EMITTER_MATERIALS:
+        raise RangeError("material must be water, oil, or neo-hookean")
+    center, velocity = config.get("center"), config.get("velocity")
+    for vec, name in ((center, "center"), (velocity, "velocity")):
+        if not isinstance(vec, dict):
+            raise TypeError(f"{name} must be a vector")
+        if not all(isinstance(vec.get(k), (int, float)) and math.isfinite(vec.get(k)) for k in ("x", "y")):
+            raise RangeError(f"{name} components must be finite")
+    grid = config.get("grid")
+    if not isinstance(grid, dict):
+        raise TypeError("grid must be an object")
+    cells_x, cells_y = grid.get("cellsX"), grid.get("cellsY")
+    if not all(isinstance(v, int) and not isinstance(v, bool) and v >= 1 for v in (cells_x, cells_y)):
+        raise RangeError("grid cell counts must be positive safe integers")
+    cell_size = _positive_finite(grid.get("cellSize"), "grid.cellSize")
+    spacing = _positive_finite(config.get("spacing"), "spacing")
+    volume = _positive_finite(config.get("particleVolume", 25), "particleVolume")
+    width, height = cells_x * cell_size, cells_y * cell_size
+    cx, cy = float(center["x"]), float(center["y"])
+    if cx < 0 or cx > width or cy < 0 or cy > height:
+        raise RangeError("center must be inside the simulation domain")
+
+    state = seed
+
+    def random() -> float:
+        nonlocal state
+        state = (state * LCG_A + LCG_C) & MASK32
+        return state / 4294967296.0
+
+    columns = math.ceil(math.sqrt(count))
+    rows = math.ceil(count / columns)
+    radius = spacing * math.sqrt(count / math.pi)
+    angular_offset = random() * math.pi * 2
+    mat = EMITTER_MATERIALS[material]
+    out = []
+    for index in range(count):
+        if shape == "circle":
+            radial = math.sqrt((index + 0.5) / count)
+            angle = angular_offset + index * _GOLDEN_ANGLE
+            jitter = (random() - 0.5) * spacing * 0.12
+            dist = max(0.0, radial * radius + jitter)
+            x, y = cx + math.cos(angle) * dist, cy + math.sin(angle) * dist
+        else:
+            column, row = index % columns, index // columns
+            x = cx + (column - (columns - 1) * 0.5) * spacing + (random() - 0.5) * spacing * 0.12
+            y = cy + (row - (rows - 1) * 0.5) * spacing + (random() - 0.5) * spacing * 0.12
+        if x < 0 or x > width or y < 0 or y > height:
+            raise RangeError(f"emitted particle {index} would be outside the simulation domain")
+        out.append({"position": {"x": x, "y": y}, "velocity": {"x": float(velocity["x"]), "y": float(velocity["y"])},
+                    "C": {"m00": 0.0, "m01": 0.0, "m10": 0.0, "m11": 0.0},
+                    "F": {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0},
+                    "plasticVolume": 1.0, "volume": volume, "material": dict(mat)})
+    return out

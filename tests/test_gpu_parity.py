@@ -483,3 +483,39 @@ def test_batched_tanks_conserve_mass(pkg):
             assert d["gridMass"] == pytest.approx(d["inputMass"], rel=1e-6), i
     finally:
         batch.dispose()
+
+
+def test_emitter_burst_appended_then_stepped_matches_oracle(pkg, orc):
+    """appendParticles with an emitter burst (scenes/emitter.ts:49-123) into a
+    running column: every particle is accepted, the appended records are the
+    emitter's, and the next step of the grown state matches the FP64 oracle."""
+    cells = 64
+    arr, config = pkg.scenes.water_oil_column(cells=cells)
+    config = dict(config, capacity=len(arr) + 256)
+    dt, g = 1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY
+    burst = pkg.scenes.create_mpm_emitter_particles({
+        "count": 200, "seed": 7, "center": {"x": 32.0, "y": 20.0}, "shape": "circle", "spacing": 1.0,
+        "velocity": {"x": 0.5, "y": -4.0}, "material": "oil", "particleVolume": 1.0,
+        "grid": {"cellsX": cells, "cellsY": cells, "cellSize": 1.0}})
+    s = pkg.create_mpm_mls_gpu_solver(arr, config, device=0)
+    try:
+        s.step_batch(4, dt, g)
+        res = s.append_particles(burst)
+        assert res["accepted"] == 200 and res["refused"] == 0
+        before = s.read_particles(packed=True)[0]
+        assert before.shape[0] == len(arr) + 200
+        tail = as_f64(before[len(arr):])
+        want_xy = np.array([[p["position"]["x"], p["position"]["y"]] for p in burst])
+        np.testing.assert_allclose(tail[:, :2], want_xy, rtol=0, atol=1e-5)
+        assert np.all(before[len(arr):, 13] == 3)  # oil
+        s.step(dt, g)
+        s.check_numerical_status()
+        got = s.read_particles(packed=True)[0]
+    finally:
+        s.dispose()
+    exp, _ = orc.step_f64(orc.particles_from_packed(before),
+                          orc.make_config(float(np.float32(dt)), cells, cells, 1.0, 0.02, 1.0, (0.0, 90.0)))
+    want, have = oracle_state(orc, exp), as_f64(got)
+    rel_v = float(np.linalg.norm(have[:, 2:4] - want[:, 2:4]) / np.linalg.norm(want[:, 2:4]))
+    assert rel_v <= 1e-4
+    assert float(np.max(np.abs(have[:, :2] - want[:, :2]))) <= 1e-4

@@ -280,3 +280,68 @@ console.log(JSON.stringify(res));
     assert res["bad"] == ["RangeError", "MPM_EINVAL"]
     assert res["words"] == ["RangeError", "MPM_EINVAL"]
     assert res["ctx"] == "TypeError"
+
+
+EMITTER_BASE = {"count": 24, "seed": 0x51EED, "center": {"x": 24, "y": 24}, "shape": "circle", "spacing": 1,
+                "velocity": {"x": 2, "y": -3}, "material": "water",
+                "grid": {"cellsX": 64, "cellsY": 64, "cellSize": 1}}
+
+
+def test_emitter_repeatable_phase_particles(pkg):  # emitter.spec.ts:18-37
+    make = pkg.scenes.create_mpm_emitter_particles
+    first, second = make(EMITTER_BASE), make(EMITTER_BASE)
+    assert first == second and len(first) == 24
+    assert all(p["material"]["model"] == "water" and p["material"]["lambda"] == 40_000 * 4 for p in first)
+    assert all(p["volume"] == 25 for p in first)
+    assert all(p["velocity"] == {"x": 2.0, "y": -3.0} for p in first)
+    assert all(p["position"]["x"] >= 0 and p["position"]["y"] >= 0 for p in first)
+
+
+def test_emitter_oil_solid_rectangle(pkg):  # emitter.spec.ts:39-70
+    make = pkg.scenes.create_mpm_emitter_particles
+    area = 0.25
+    oil = make({**EMITTER_BASE, "material": "oil", "particleVolume": area})
+    solid = make({**EMITTER_BASE, "material": "neo-hookean", "shape": "rectangle", "particleVolume": 100})
+    assert all(p["material"]["model"] == "oil" and p["material"]["lambda"] == 30_000 * 4 for p in oil)
+    assert all(p["volume"] == area for p in oil)
+    assert all(p["material"]["model"] == "neo-hookean" and p["volume"] == 100 for p in solid)
+    assert len({(p["position"]["x"], p["position"]["y"]) for p in solid}) == 24
+
+
+def test_emitter_geometry_closed_forms(pkg):
+    # rectangle: a ceil(sqrt(n)) x rows lattice centred on `center`, each
+    # point within 6% of the spacing of its lattice site; circle: the
+    # golden-angle spiral of radius spacing*sqrt(n/pi), jitter within 6%.
+    make = pkg.scenes.create_mpm_emitter_particles
+    n, s = 30, 2.0
+    rect = make({**EMITTER_BASE, "count": n, "spacing": s, "shape": "rectangle"})
+    cols = math.ceil(math.sqrt(n))
+    rows = math.ceil(n / cols)
+    for i, p in enumerate(rect):
+        gx = 24 + (i % cols - (cols - 1) / 2) * s
+        gy = 24 + (i // cols - (rows - 1) / 2) * s
+        assert abs(p["position"]["x"] - gx) <= 0.06 * s + 1e-12
+        assert abs(p["position"]["y"] - gy) <= 0.06 * s + 1e-12
+    circ = make({**EMITTER_BASE, "count": n, "spacing": s})
+    radius = s * math.sqrt(n / math.pi)
+    for i, p in enumerate(circ):
+        r = math.hypot(p["position"]["x"] - 24, p["position"]["y"] - 24)
+        assert abs(r - math.sqrt((i + 0.5) / n) * radius) <= 0.06 * s + 1e-12
+    # the spiral advances by the golden angle between consecutive particles
+    a0 = math.atan2(circ[5]["position"]["y"] - 24, circ[5]["position"]["x"] - 24)
+    a1 = math.atan2(circ[6]["position"]["y"] - 24, circ[6]["position"]["x"] - 24)
+    golden = math.pi * (3 - math.sqrt(5))
+    assert math.isclose(math.remainder(a1 - a0 - golden, 2 * math.pi), 0.0, abs_tol=1e-9)
+
+
+def test_emitter_rejects_unsafe_inputs(pkg):  # emitter.spec.ts:72-93
+    make = pkg.scenes.create_mpm_emitter_particles
+    RangeError = pkg.layout.RangeError
+    with pytest.raises(RangeError, match="count"):
+        make({**EMITTER_BASE, "count": 0})
+    with pytest.raises(RangeError, match="seed"):
+        make({**EMITTER_BASE, "seed": -1})
+    with pytest.raises(RangeError, match="particleVolume"):
+        make({**EMITTER_BASE, "particleVolume": 0})
+    with pytest.raises(RangeError, match="outside the simulation domain"):
+        make({**EMITTER_BASE, "center": {"x": 0, "y": 0}, "count": 16})
