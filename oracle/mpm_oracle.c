@@ -32,6 +32,15 @@
 #define SNOW_MIN_PLASTIC_VOLUME 0.6
 #define SNOW_MAX_PLASTIC_VOLUME 20.0
 #define SNOW_MAX_HARDENING_EXPONENT 20.0
+/* reference/solver.ts:277-282, materials.ts:44 */
+#define DEFAULT_BOND_STIFFNESS 0.35
+#define BOND_SOLVER_ITERATIONS 2
+#define CONTACT_SKIN_CELLS 0.1
+#define CONTACT_IMPULSE_RELAXATION 0.25
+#define FIELD_BAND_CELLS 2.0
+#define GRID_CONTACT_REACH_CELLS 0.75
+#define FLUID_TANGENTIAL_DRAG_RATE 4.0
+#define ORC_PI 3.14159265358979323846
 
 typedef struct { double m00, m01, m10, m11; } mat2;
 
@@ -283,9 +292,130 @@ static int validate(const orc_particle* in, int32_t n, const orc_config* c) {
     return ORC_OK;
 }
 
+/* ---- bonded triangle bodies (config-2 contact) ---- */
+
+/* Nearest triangle of each grid node within the band (buildTriangleField,
+ * solver.ts:1064-1165).  Arrays are indexed by node key; body = -1 where no
+ * triangle lies within the band. */
+typedef struct {
+    long W, H;
+    int32_t* body;
+    double *dist, *nx, *ny, *b0, *b1, *b2, *ix, *iy;
+    const int32_t* starts;
+    int32_t nbodies;
+} tri_field;
+
+/* triangleDistanceSample (solver.ts:1190-1291) */
+static int tri_sample(double px, double py, const double* t, double* sd, double* nx, double* ny, double* bary) {
+    const double ax = t[0], ay = t[1], bx = t[2], by = t[3], cx = t[4], cy = t[5];
+    const double abx = bx - ax, aby = by - ay, acx = cx - ax, acy = cy - ay;
+    const double area = abx * acy - aby * acx;
+    if (!finite(area) || fabs(area) <= 1e-10) return ORC_ENUMERIC;
+    const double bcx = cx - bx, bcy = cy - by, cax = ax - cx, cay = ay - cy;
+    const double c_ab = abx * (py - ay) - aby * (px - ax);
+    const double c_bc = bcx * (py - by) - bcy * (px - bx);
+    const double c_ca = cax * (py - cy) - cay * (px - cx);
+    const int inside = area > 0 ? (c_ab >= -1e-10 && c_bc >= -1e-10 && c_ca >= -1e-10)
+                                : (c_ab <= 1e-10 && c_bc <= 1e-10 && c_ca <= 1e-10);
+    double best = INFINITY, bnx = 0, bny = 0;
+    for (int e = 0; e < 3; ++e) {
+        const double sx = t[2 * e], sy = t[2 * e + 1], ex = t[(2 * e + 2) % 6], ey = t[(2 * e + 3) % 6];
+        const double dx = ex - sx, dy = ey - sy, len2 = dx * dx + dy * dy;
+        const double tt = fmax(0.0, fmin(1.0, ((px - sx) * dx + (py - sy) * dy) / len2));
+        const double qx = sx + dx * tt, qy = sy + dy * tt;
+        const double ddx = px - qx, ddy = py - qy, d2 = ddx * ddx + ddy * ddy;
+        if (d2 >= best) continue;
+        best = d2;
+        if (e == 0) { bary[0] = 1 - tt; bary[1] = tt; bary[2] = 0; }
+        else if (e == 1) { bary[0] = 0; bary[1] = 1 - tt; bary[2] = tt; }
+        else { bary[0] = tt; bary[1] = 0; bary[2] = 1 - tt; }
+        if (d2 > 1e-20 && !inside) {
+            const double d = sqrt(d2);
+            bnx = ddx * (1 / d);
+            bny = ddy * (1 / d);
+        } else {
+            const double o = area > 0 ? 1 : -1, el = sqrt(len2);
+            bnx = (o * dy) * (1 / el);
+            bny = (-o * dx) * (1 / el);
+        }
+    }
+    const double d = sqrt(best);
+    if (!finite(d) || !finite(bnx) || !finite(bny)) return ORC_ENUMERIC;
+    *sd = inside ? -d : d;
+    *nx = bnx;
+    *ny = bny;
+    return ORC_OK;
+}
+
+static void tri_vertices(const orc_particle* p, int32_t start, double* t) {
+    for (int v = 0; v < 3; ++v) {
+        t[2 * v] = p[start + v].x;
+        t[2 * v + 1] = p[start + v].y;
+    }
+}
+
+static int tri_field_build(const orc_particle* in, const orc_config* c, tri_field* f) {
+    const long nn = f->W * f->H;
+    for (long k = 0; k < nn; ++k) f->body[k] = -1;
+    const double dx = c->cell_size, band = dx * FIELD_BAND_CELLS;
+    for (int32_t b = 0; b < f->nbodies; ++b) {
+        const int32_t start = f->starts[b];
+        double t[6];
+        tri_vertices(in, start, t);
+        const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
+        const double mny = fmin(t[1], fmin(t[3], t[5])), mxy = fmax(t[1], fmax(t[3], t[5]));
+        const long x0 = (long)fmax(-1.0, floor((mnx - band) / dx) - 1), x1 = (long)fmin(c->cells_x + 1, ceil((mxx + band) / dx) + 1);
+        const long y0 = (long)fmax(-1.0, floor((mny - band) / dx) - 1), y1 = (long)fmin(c->cells_y + 1, ceil((mxy + band) / dx) + 1);
+        for (long yi = y0; yi <= y1; ++yi) {
+            const double py = yi * dx;
+            if (py < mny - band || py > mxy + band) continue;
+            for (long xi = x0; xi <= x1; ++xi) {
+                const double px = xi * dx;
+                if (px < mnx - band || px > mxx + band) continue;
+                double sd, nx, ny, bary[3] = {1, 0, 0};
+                int e = tri_sample(px, py, t, &sd, &nx, &ny, bary);
+                if (e) return e;
+                if (sd > band) continue;
+                const long key = (yi + 1) * f->W + (xi + 1);
+                if (f->body[key] >= 0 && sd >= f->dist[key]) continue;
+                f->body[key] = start;
+                f->dist[key] = sd;
+                f->nx[key] = nx;
+                f->ny[key] = ny;
+                f->b0[key] = bary[0];
+                f->b1[key] = bary[1];
+                f->b2[key] = bary[2];
+                f->ix[key] = 0;
+                f->iy[key] = 0;
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* resolveFluidTriangleContact (solver.ts:1462-1505): updates the fluid
+ * velocity and accumulates the impulse on the body. */
+static int fluid_triangle(double fm, double* v, const double* sv, double nx, double ny, double inv_body,
+                          double dt, double* ix, double* iy) {
+    const double inv_eff = 1 / fm + inv_body;
+    if (!finite(inv_eff) || inv_eff <= 0) return ORC_ENUMERIC;
+    const double rx = v[0] - sv[0], ry = v[1] - sv[1];
+    const double rn = rx * nx + ry * ny;
+    const double nimp = rn < 0 ? -rn / inv_eff : 0;
+    const double tx = rx - nx * rn, ty = ry - ny * rn;
+    const double damping = 1 - exp(-FLUID_TANGENTIAL_DRAG_RATE * dt);
+    const double fx = nx * nimp + tx * (-damping / inv_eff), fy = ny * nimp + ty * (-damping / inv_eff);
+    v[0] = v[0] + fx * (1 / fm);
+    v[1] = v[1] + fy * (1 / fm);
+    *ix += -fx;
+    *iy += -fy;
+    if (!finite(v[0]) || !finite(v[1]) || !finite(*ix) || !finite(*iy)) return ORC_ENUMERIC;
+    return ORC_OK;
+}
+
 /* P2G (solver.ts:418-479) + grid update (:494-608). */
 static int p2g_and_update(const orc_particle* in, int32_t n, const orc_config* c, double* G,
-                          orc_diagnostics* diag) {
+                          orc_diagnostics* diag, tri_field* tf) {
     const long W = c->cells_x + 3, H = c->cells_y + 3;
     const double dx = c->cell_size, gscale = 4.0 / (dx * dx);
     memset(G, 0, sizeof(double) * (size_t)(W * H) * N_FIELDS);
@@ -380,6 +510,33 @@ static int p2g_and_update(const orc_particle* in, int32_t n, const orc_config* c
                     }
                 }
             }
+            /* solveFluidTriangleGridContact (solver.ts:1411-1460) */
+            const long key = sy * W + sx;
+            if (tf && tf->body[key] >= 0) {
+                const double radius = sqrt(c->particle_volume / ORC_PI);
+                if (tf->dist[key] <= radius + GRID_CONTACT_REACH_CELLS * dx) {
+                    const int32_t b = tf->body[key];
+                    const double bw[3] = {tf->b0[key], tf->b1[key], tf->b2[key]};
+                    double sv[2] = {0, 0}, inv_body = 0;
+                    for (int v = 0; v < 3; ++v) {
+                        sv[0] += bw[v] * in[b + v].vx;
+                        sv[1] += bw[v] * in[b + v].vy;
+                        inv_body += (bw[v] * bw[v]) / c->particle_mass;
+                    }
+                    if (inv_body > 0) {
+                        if (nd[N_WM] > 0) {
+                            int e = fluid_triangle(nd[N_WM], vel[1], sv, tf->nx[key], tf->ny[key], inv_body, c->dt,
+                                                   &tf->ix[key], &tf->iy[key]);
+                            if (e) return e;
+                        }
+                        if (nd[N_OM] > 0) {
+                            int e = fluid_triangle(nd[N_OM], vel[2], sv, tf->nx[key], tf->ny[key], inv_body, c->dt,
+                                                   &tf->ix[key], &tf->iy[key]);
+                            if (e) return e;
+                        }
+                    }
+                }
+            }
             for (int ph = 0; ph < 3; ++ph) {
                 double vx = wall_cushion(vel[ph][0], (double)xi, (double)c->cells_x);
                 double vy = wall_cushion(vel[ph][1], (double)yi, (double)c->cells_y);
@@ -396,13 +553,62 @@ static int p2g_and_update(const orc_particle* in, int32_t n, const orc_config* c
 int orc_grid_f64(const orc_particle* in, int32_t n, const orc_config* c, double* node_out) {
     int e = validate(in, n, c);
     if (e) return e;
-    return p2g_and_update(in, n, c, node_out, NULL);
+    return p2g_and_update(in, n, c, node_out, NULL, NULL);
 }
 
-int orc_step_f64(const orc_particle* in, int32_t n, const orc_config* c, orc_particle* out,
-                 orc_diagnostics* diag) {
-    int e = validate(in, n, c);
-    if (e) return e;
+/* projectParticleOutsideTriangles (solver.ts:1526-1590) */
+static int project_outside(const orc_config* c, const tri_field* f, double r, double* x, double* y) {
+    stencil_t s;
+    make_stencil(*x, *y, c->cell_size, &s);
+    const double band = c->cell_size * FIELD_BAND_CELLS;
+    double sd = 0, closest = INFINITY, snx = 0, sny = 0;
+    int32_t cb = -1;
+    long keys[9];
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            const double w = s.wx[ox] * s.wy[oy];
+            const long key = (s.by + oy + 1) * f->W + (s.bx + ox + 1);
+            const int has = key >= 0 && key < f->W * f->H && f->body[key] >= 0;
+            keys[oy * 3 + ox] = has ? key : -1;
+            sd += w * (has ? f->dist[key] : band);
+            if (has && f->dist[key] < closest) {
+                closest = f->dist[key];
+                cb = f->body[key];
+                snx = f->nx[key];
+                sny = f->ny[key];
+            }
+        }
+    if (cb < 0 || sd >= r) return ORC_OK;
+    double nx = 0, ny = 0;
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            const long key = keys[oy * 3 + ox];
+            if (key < 0 || f->body[key] != cb) continue;
+            const double w = s.wx[ox] * s.wy[oy];
+            nx += w * f->nx[key];
+            ny += w * f->ny[key];
+        }
+    const double len = hypot(nx, ny);
+    if (len > 1e-12 && finite(len)) {
+        nx = nx / len;
+        ny = ny / len;
+    } else {
+        nx = snx;
+        ny = sny;
+    }
+    const double corr = r - sd;
+    *x = *x + corr * nx;
+    *y = *y + corr * ny;
+    if (!finite(*x) || !finite(*y)) return ORC_ENUMERIC;
+    return ORC_OK;
+}
+
+/* One step of particles already validated: P2G, grid update (with the
+ * triangle field's fluid contact when tf is given), G2P with projection of
+ * liquid particles out of the triangles.  Fills the input and grid sums of dg. */
+static int step_core(const orc_particle* in, int32_t n, const orc_config* c, tri_field* tf, orc_particle* out,
+                     orc_diagnostics* dgp) {
+    int e = ORC_OK;
     const long W = c->cells_x + 3, H = c->cells_y + 3;
     const double dx = c->cell_size, gscale = 4.0 / (dx * dx);
     double* G = (double*)malloc(sizeof(double) * (size_t)(W * H) * N_FIELDS);
@@ -416,7 +622,7 @@ int orc_step_f64(const orc_particle* in, int32_t n, const orc_config* c, orc_par
         dg.input_px += m * in[i].vx;
         dg.input_py += m * in[i].vy;
     }
-    e = p2g_and_update(in, n, c, G, &dg);
+    e = p2g_and_update(in, n, c, G, &dg, tf);
     if (e) { free(G); return e; }
 
     const double Wd = c->cells_x * dx, Hd = c->cells_y * dx;
@@ -483,7 +689,10 @@ int orc_step_f64(const orc_particle* in, int32_t n, const orc_config* c, orc_par
         double bx, bvx, by, bvy, fx, fvx, fy, fvy;
         confine_axis(nx, vx, Wd, &bx, &bvx);
         confine_axis(ny, vy, Hd, &by, &bvy);
-        /* projectParticleOutsideTriangles is the identity without bonded bodies */
+        if (tf && tf->nbodies > 0 && liquid) {
+            e = project_outside(c, tf, sqrt((p->volume * det2(Fn)) / ORC_PI), &bx, &by);
+            if (e) break;
+        }
         confine_axis(bx, bvx, Wd, &fx, &fvx);
         confine_axis(by, bvy, Hd, &fy, &fvy);
         q->x = fx; q->y = fy; q->vx = fvx; q->vy = fvy;
@@ -492,14 +701,28 @@ int orc_step_f64(const orc_particle* in, int32_t n, const orc_config* c, orc_par
         q->plastic_volume = pv;
     }
     free(G);
-    if (e) return e;
+    *dgp = dg;
+    return e;
+}
+
+static void output_sums(const orc_particle* out, int32_t n, const orc_config* c, orc_diagnostics* dg) {
     for (int32_t i = 0; i < n; ++i) {
         if (out[i].model == ORC_MODEL_REMOVED) continue;
         double m = particle_mass(&out[i], c);
-        dg.output_mass += m;
-        dg.output_px += m * out[i].vx;
-        dg.output_py += m * out[i].vy;
+        dg->output_mass += m;
+        dg->output_px += m * out[i].vx;
+        dg->output_py += m * out[i].vy;
     }
+}
+
+int orc_step_f64(const orc_particle* in, int32_t n, const orc_config* c, orc_particle* out,
+                 orc_diagnostics* diag) {
+    int e = validate(in, n, c);
+    if (e) return e;
+    orc_diagnostics dg;
+    e = step_core(in, n, c, NULL, out, &dg);
+    if (e) return e;
+    output_sums(out, n, c, &dg);
     if (diag) *diag = dg;
     return ORC_OK;
 }
@@ -513,6 +736,311 @@ int orc_run_f64(orc_particle* p, int32_t n, const orc_config* c, int32_t steps) 
         memcpy(p, tmp, sizeof(orc_particle) * (size_t)n);
     }
     free(tmp);
+    return ORC_OK;
+}
+
+/* ---- config-2 step: bonded triangle bodies and fracture bonds ---- */
+
+/* collectBondedBodyStarts (solver.ts:1023-1062) and the particle checks of
+ * validateParticles with bonded vertices allowed. */
+static int validate_bodies(const orc_particle* in, int32_t n, const uint32_t* oid, const orc_config* c,
+                           const orc_body_config* bc, int32_t* starts, int32_t* nbodies) {
+    *nbodies = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (in[i].bonded_vertex == 0) continue;
+        if (in[i].bonded_vertex != 1) return ORC_EINVALID;          /* must start at vertex 0 */
+        if (!(bc->bond_rest_length > 0)) return ORC_EINVALID;
+        if (i + 2 >= n) return ORC_EINVALID;
+        for (int v = 0; v < 3; ++v) {
+            const orc_particle* m = &in[i + v];
+            if (m->bonded_vertex != v + 1 || oid[i + v] != oid[i]) return ORC_EINVALID;
+            if (is_liquid(m->model) || m->model == ORC_MODEL_REMOVED) return ORC_EINVALID;
+        }
+        starts[(*nbodies)++] = i;
+        i += 2;
+    }
+    /* the plain checks, with bonded vertices masked */
+    orc_particle* tmp = (orc_particle*)malloc(sizeof(orc_particle) * (size_t)(n > 0 ? n : 1));
+    if (!tmp) return ORC_EINVALID;
+    memcpy(tmp, in, sizeof(orc_particle) * (size_t)n);
+    for (int32_t i = 0; i < n; ++i) tmp[i].bonded_vertex = 0;
+    const int e = validate(tmp, n, c);
+    free(tmp);
+    return e;
+}
+
+/* validateMpmSolidBondGraph (reference/solid-bonds.ts:11-66) */
+static int validate_bonds(const orc_particle* in, int32_t n, const uint32_t* oid, const orc_bond* b, int32_t nb) {
+    for (int32_t k = 0; k < nb; ++k) {
+        const int32_t f = b[k].first, s = b[k].second;
+        if (f < 0 || f >= n || s < 0 || s >= n || f == s) return ORC_EINVALID;
+        if (!(b[k].rest_length > 0) || !finite(b[k].rest_length)) return ORC_EINVALID;
+        if (!(b[k].failure_strain > 0) || !finite(b[k].failure_strain)) return ORC_EINVALID;
+        if (!(b[k].damage >= 0 && b[k].damage <= 1)) return ORC_EINVALID;
+        if (oid[f] != oid[s]) return ORC_EINVALID;
+        if (in[f].bonded_vertex != 0 || in[s].bonded_vertex != 0) return ORC_EINVALID;
+        if (in[f].model != ORC_MODEL_NEO_HOOKEAN || in[s].model != ORC_MODEL_NEO_HOOKEAN) return ORC_EINVALID;
+        for (int32_t j = 0; j < k; ++j)
+            if ((b[j].first == f && b[j].second == s) || (b[j].first == s && b[j].second == f)) return ORC_EINVALID;
+    }
+    return ORC_OK;
+}
+
+/* trianglePairContact (solver.ts:1804-1874): 1 and the contact normal
+ * (pointing from B to A) and penetration, or 0. */
+static int pair_contact(const double* A, const double* B, double skin, double* nx, double* ny, double* pen) {
+    int separated = 0, have = 0;
+    double largest_gap = -INFINITY, smallest_overlap = INFINITY, axx = 0, axy = 0;
+    for (int ti = 0; ti < 2; ++ti) {
+        const double* T = ti == 0 ? A : B;
+        for (int e = 0; e < 3; ++e) {
+            const double ex = T[(2 * e + 2) % 6] - T[2 * e], ey = T[(2 * e + 3) % 6] - T[2 * e + 1];
+            const double el = hypot(ex, ey);
+            if (el <= 1e-10) return 0;
+            const double ux = ey / el, uy = -ex / el;
+            double mina = INFINITY, maxa = -INFINITY, minb = INFINITY, maxb = -INFINITY;
+            for (int v = 0; v < 3; ++v) {
+                const double a = A[2 * v] * ux + A[2 * v + 1] * uy, bb = B[2 * v] * ux + B[2 * v + 1] * uy;
+                mina = fmin(mina, a);
+                maxa = fmax(maxa, a);
+                minb = fmin(minb, bb);
+                maxb = fmax(maxb, bb);
+            }
+            const double gap = fmax(minb - maxa, mina - maxb);
+            if (gap > 0) {
+                separated = 1;
+                if (gap > largest_gap) {
+                    largest_gap = gap;
+                    axx = ux;
+                    axy = uy;
+                    have = 1;
+                }
+            } else if (!separated && -gap < smallest_overlap) {
+                smallest_overlap = -gap;
+                axx = ux;
+                axy = uy;
+                have = 1;
+            }
+        }
+    }
+    if (separated && largest_gap > skin) return 0;
+    if (!have) return 0;
+    const double dx = (A[0] + A[2] + A[4]) / 3 - (B[0] + B[2] + B[4]) / 3;
+    const double dy = (A[1] + A[3] + A[5]) / 3 - (B[1] + B[3] + B[5]) / 3;
+    const double sign = dx * axx + dy * axy >= 0 ? 1 : -1;
+    *nx = sign * axx;
+    *ny = sign * axy;
+    *pen = separated ? skin - largest_gap : skin + smallest_overlap;
+    return *pen > 0;
+}
+
+static void set_confined(orc_particle* q, double x, double vx, double y, double vy, const orc_config* c) {
+    double fx, fvx, fy, fvy;
+    confine_axis(x, vx, c->cells_x * c->cell_size, &fx, &fvx);
+    confine_axis(y, vy, c->cells_y * c->cell_size, &fy, &fvy);
+    q->x = fx;
+    q->vx = fvx;
+    q->y = fy;
+    q->vy = fvy;
+}
+
+/* applyTriangleContactImpulses, enforceBondedTriangleConstraints,
+ * resolveBondedTriangleContacts (solver.ts:1592-1802). */
+static int bodies_after_g2p(const orc_particle* in, orc_particle* out, const uint32_t* oid, const orc_config* c,
+                            const orc_body_config* bc, const tri_field* f) {
+    const int32_t nb = f->nbodies;
+    const double dt = c->dt;
+    /* field impulses, barycentric-weighted, in each body's node order */
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t start = f->starts[b];
+        double imp[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+        double t[6];
+        tri_vertices(in, start, t);
+        const double dx = c->cell_size, band = dx * FIELD_BAND_CELLS;
+        const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
+        const double mny = fmin(t[1], fmin(t[3], t[5])), mxy = fmax(t[1], fmax(t[3], t[5]));
+        const long x0 = (long)fmax(-1.0, floor((mnx - band) / dx) - 1), x1 = (long)fmin(c->cells_x + 1, ceil((mxx + band) / dx) + 1);
+        const long y0 = (long)fmax(-1.0, floor((mny - band) / dx) - 1), y1 = (long)fmin(c->cells_y + 1, ceil((mxy + band) / dx) + 1);
+        for (long yi = y0; yi <= y1; ++yi)
+            for (long xi = x0; xi <= x1; ++xi) {
+                const long key = (yi + 1) * f->W + (xi + 1);
+                if (f->body[key] != start) continue;
+                const double bw[3] = {f->b0[key], f->b1[key], f->b2[key]};
+                for (int v = 0; v < 3; ++v) {
+                    imp[v][0] += bw[v] * f->ix[key];
+                    imp[v][1] += bw[v] * f->iy[key];
+                }
+            }
+        for (int v = 0; v < 3; ++v) {
+            orc_particle* q = &out[start + v];
+            const double m = particle_mass(q, c);
+            q->vx = q->vx + imp[v][0] / m;
+            q->vy = q->vy + imp[v][1] / m;
+            if (!finite(q->vx) || !finite(q->vy)) return ORC_ENUMERIC;
+        }
+    }
+    /* PBD edge constraints */
+    const double ratio = dt / bc->bond_reference_dt;
+    double k = fmin(bc->bond_stiffness * ratio * ratio, 1.0);
+    if (!finite(k)) return ORC_ENUMERIC;
+    k = fmax(0.0, fmin(1.0, k));
+    static const int edges[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t start = f->starts[b];
+        double orig[3][2], pos[3][2];
+        for (int v = 0; v < 3; ++v) {
+            orig[v][0] = pos[v][0] = out[start + v].x;
+            orig[v][1] = pos[v][1] = out[start + v].y;
+        }
+        for (int it = 0; it < BOND_SOLVER_ITERATIONS; ++it)
+            for (int e = 0; e < 3; ++e) {
+                const int a = edges[e][0], s2 = edges[e][1];
+                const double dx = pos[s2][0] - pos[a][0], dy = pos[s2][1] - pos[a][1];
+                const double d = hypot(dx, dy);
+                if (!finite(d) || d <= 1e-10) return ORC_ENUMERIC;
+                const double sc = ((d - bc->bond_rest_length) * k * 0.5) / d;
+                pos[a][0] = pos[a][0] + dx * sc;
+                pos[a][1] = pos[a][1] + dy * sc;
+                pos[s2][0] = pos[s2][0] - dx * sc;
+                pos[s2][1] = pos[s2][1] - dy * sc;
+            }
+        for (int v = 0; v < 3; ++v) {
+            orc_particle* q = &out[start + v];
+            const double vx = q->vx + (pos[v][0] - orig[v][0]) / dt, vy = q->vy + (pos[v][1] - orig[v][1]) / dt;
+            set_confined(q, pos[v][0], vx, pos[v][1], vy, c);
+        }
+    }
+    if (nb < 2) return ORC_OK;
+    /* SAT pair contacts between bodies of different objects */
+    const double skin = c->cell_size * CONTACT_SKIN_CELLS;
+    double* tri = (double*)malloc(sizeof(double) * 6 * (size_t)nb);
+    double* vel = (double*)malloc(sizeof(double) * 2 * (size_t)nb);
+    int32_t* ncont = (int32_t*)calloc((size_t)nb, sizeof(int32_t));
+    int32_t* other = (int32_t*)malloc(sizeof(int32_t) * (size_t)nb * (size_t)nb);
+    double* cn = (double*)malloc(sizeof(double) * 2 * (size_t)nb * (size_t)nb);
+    if (!tri || !vel || !ncont || !other || !cn) {
+        free(tri); free(vel); free(ncont); free(other); free(cn);
+        return ORC_EINVALID;
+    }
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t s0 = f->starts[b];
+        tri_vertices(in, s0, tri + 6 * b);
+        vel[2 * b] = (out[s0].vx + out[s0 + 1].vx + out[s0 + 2].vx) / 3;
+        vel[2 * b + 1] = (out[s0].vy + out[s0 + 1].vy + out[s0 + 2].vy) / 3;
+    }
+    for (int32_t i = 0; i < nb; ++i)
+        for (int32_t j = i + 1; j < nb; ++j) {
+            if (oid[f->starts[i]] == oid[f->starts[j]]) continue;
+            double nx, ny, pen;
+            if (!pair_contact(tri + 6 * i, tri + 6 * j, skin, &nx, &ny, &pen)) continue;
+            const int32_t ki = ncont[i]++, kj = ncont[j]++;
+            other[(size_t)i * nb + ki] = j;
+            cn[2 * ((size_t)i * nb + ki)] = nx;
+            cn[2 * ((size_t)i * nb + ki) + 1] = ny;
+            other[(size_t)j * nb + kj] = i;
+            cn[2 * ((size_t)j * nb + kj)] = -nx;
+            cn[2 * ((size_t)j * nb + kj) + 1] = -ny;
+        }
+    const double body_mass = 3 * c->particle_mass;
+    for (int32_t b = 0; b < nb; ++b) {
+        double ix = 0, iy = 0;
+        for (int32_t q = 0; q < ncont[b]; ++q) {
+            const int32_t o = other[(size_t)b * nb + q];
+            const double nx = cn[2 * ((size_t)b * nb + q)], ny = cn[2 * ((size_t)b * nb + q) + 1];
+            const double w = 1.0 / (double)(ncont[b] > ncont[o] ? ncont[b] : ncont[o]);
+            const double rn = (vel[2 * b] - vel[2 * o]) * nx + (vel[2 * b + 1] - vel[2 * o + 1]) * ny;
+            if (rn < 0) {
+                const double imp = -rn / (2 / body_mass);
+                const double iw = w * CONTACT_IMPULSE_RELAXATION;
+                ix += (iw * imp * nx) / body_mass;
+                iy += (iw * imp * ny) / body_mass;
+            }
+        }
+        for (int v = 0; v < 3; ++v) {
+            orc_particle* q = &out[f->starts[b] + v];
+            set_confined(q, q->x, q->vx + ix, q->y, q->vy + iy, c);
+        }
+    }
+    free(tri); free(vel); free(ncont); free(other); free(cn);
+    return ORC_OK;
+}
+
+/* updateMpmSolidBondDamage (reference/solid-bonds.ts:68-108): bond damage
+ * from the tensile strain of the output positions, then particle damage (the
+ * neo-Hookean `param`) = max over incident bonds. */
+static int fracture_after_step(orc_particle* out, int32_t n, orc_bond* b, int32_t nb) {
+    double* pd = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!pd) return ORC_EINVALID;
+    for (int32_t i = 0; i < n; ++i) pd[i] = -1;
+    for (int32_t k = 0; k < nb; ++k) {
+        const orc_particle *p = &out[b[k].first], *q = &out[b[k].second];
+        const double d = hypot(q->x - p->x, q->y - p->y);
+        if (!finite(d)) { free(pd); return ORC_ENUMERIC; }
+        const double strain = fmax(0.0, (d - b[k].rest_length) / b[k].rest_length);
+        const int broken = b[k].broken || strain >= b[k].failure_strain;
+        const double dmg = broken ? 1.0 : fmax(b[k].damage, fmin(1.0, strain / b[k].failure_strain));
+        b[k].damage = dmg;
+        b[k].broken = broken;
+        pd[b[k].first] = fmax(pd[b[k].first], dmg);
+        pd[b[k].second] = fmax(pd[b[k].second], dmg);
+    }
+    for (int32_t i = 0; i < n; ++i)
+        if (pd[i] >= 0 && pd[i] > out[i].param) out[i].param = pd[i];
+    free(pd);
+    return ORC_OK;
+}
+
+int orc_triangle_sample(double px, double py, const double* tri, double* out6) {
+    double b[3] = {1, 0, 0};
+    const int e = tri_sample(px, py, tri, &out6[0], &out6[1], &out6[2], b);
+    out6[3] = b[0];
+    out6[4] = b[1];
+    out6[5] = b[2];
+    return e;
+}
+
+int orc_triangle_pair_contact(const double* a, const double* b, double skin, double* out3) {
+    return pair_contact(a, b, skin, &out3[0], &out3[1], &out3[2]);
+}
+
+int orc_step_bodies_f64(const orc_particle* in, int32_t n, const uint32_t* object_ids, const orc_config* c,
+                        const orc_body_config* bc, orc_bond* bonds, int32_t nbonds, orc_particle* out,
+                        orc_diagnostics* diag) {
+    int32_t* starts = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n / 3 + 1));
+    if (!starts) return ORC_EINVALID;
+    tri_field f;
+    memset(&f, 0, sizeof f);
+    int32_t nb = 0;
+    int e = validate_bodies(in, n, object_ids, c, bc, starts, &nb);
+    if (!e && nb > 0 && (!(bc->bond_stiffness > 0) || bc->bond_stiffness > 1 || !(bc->bond_reference_dt > 0)))
+        e = ORC_EINVALID;
+    if (!e) e = validate_bonds(in, n, object_ids, bonds, nbonds);
+    double* fbuf = NULL;
+    if (!e && nb > 0) {
+        f.W = c->cells_x + 3;
+        f.H = c->cells_y + 3;
+        const size_t nn = (size_t)(f.W * f.H);
+        fbuf = (double*)malloc(sizeof(double) * nn * 8);
+        f.body = (int32_t*)malloc(sizeof(int32_t) * nn);
+        if (!fbuf || !f.body) e = ORC_EINVALID;
+        else {
+            f.dist = fbuf; f.nx = fbuf + nn; f.ny = fbuf + 2 * nn; f.b0 = fbuf + 3 * nn;
+            f.b1 = fbuf + 4 * nn; f.b2 = fbuf + 5 * nn; f.ix = fbuf + 6 * nn; f.iy = fbuf + 7 * nn;
+            f.starts = starts;
+            f.nbodies = nb;
+            e = tri_field_build(in, c, &f);
+        }
+    }
+    orc_diagnostics dg;
+    if (!e) e = step_core(in, n, c, nb > 0 ? &f : NULL, out, &dg);
+    if (!e && nb > 0) e = bodies_after_g2p(in, out, object_ids, c, bc, &f);
+    if (!e && nbonds > 0) e = fracture_after_step(out, n, bonds, nbonds);
+    free(fbuf);
+    free(f.body);
+    free(starts);
+    if (e) return e;
+    output_sums(out, n, c, &dg);
+    if (diag) *diag = dg;
     return ORC_OK;
 }
 

@@ -27,8 +27,12 @@
  *                   order-independence, and decode == FP64 P2G within the
  *                   2^-24 quantum; see tests/test_oracle.py.
  *   orc_bin:        brute-force O(N * cells) scan on small inputs.
- *   Bonded triangles, triangle contact and fracture bonds (config-2 only) are
- *   not restated: a particle set that uses them is rejected (ORC_EUNSUPPORTED).
+ *   orc_step_bodies_f64: bonded triangle bodies and fracture bonds (config 2),
+ *                   pinned by closed forms of the triangle distance field,
+ *                   the SAT pair contact, the PBD edge solve and the bond
+ *                   damage law, and by momentum bookkeeping; see
+ *                   tests/test_oracle_bodies.py.  orc_step_f64 itself still
+ *                   rejects bonded vertices (ORC_EUNSUPPORTED).
  */
 #ifndef MPM_ORACLE_H
 #define MPM_ORACLE_H
@@ -91,6 +95,40 @@ typedef struct {
  * `out` is only meaningful on ORC_OK. */
 int orc_step_f64(const orc_particle* in, int32_t n, const orc_config* cfg,
                  orc_particle* out, orc_diagnostics* diag);
+
+/* Config-2 extras of the step (solver.ts:80-94): bonded triangle bodies use
+ * bond_rest_length, bond_stiffness (reference default 0.35) and
+ * bond_reference_dt (reference default: dt). */
+typedef struct {
+    double bond_rest_length, bond_stiffness, bond_reference_dt;
+} orc_body_config;
+
+/* A fracture bond (MpmSolidBond, solver.ts:61-67); damage and broken are
+ * updated in place by orc_step_bodies_f64. */
+typedef struct {
+    int32_t first, second;
+    double rest_length, failure_strain, damage;
+    int32_t broken, pad;
+} orc_bond;
+
+/* orc_step_f64 plus bonded triangle bodies and fracture bonds (stepMpm with
+ * buildTriangleField, solveFluidTriangleGridContact,
+ * projectParticleOutsideTriangles, applyTriangleContactImpulses,
+ * enforceBondedTriangleConstraints, resolveBondedTriangleContacts and
+ * updateMpmSolidBondDamage, solver.ts:1023-1874, solid-bonds.ts:68-108).
+ * Particles with bonded_vertex = vertex + 1 form triangles of three
+ * consecutive particles of one object; object_ids has one entry per particle. */
+int orc_step_bodies_f64(const orc_particle* in, int32_t n, const uint32_t* object_ids, const orc_config* cfg,
+                        const orc_body_config* bodies, orc_bond* bonds, int32_t nbonds, orc_particle* out,
+                        orc_diagnostics* diag);
+
+/* The two geometric primitives of the contact code, exposed for pinning:
+ * triangleDistanceSample (solver.ts:1190-1291): out6 = signed distance,
+ * normal x, y, barycentric of the closest point (3); and trianglePairContact
+ * (:1804-1874): returns 1 with out3 = normal x, y (from b towards a),
+ * penetration, or 0 without contact. */
+int orc_triangle_sample(double px, double py, const double* tri6, double* out6);
+int orc_triangle_pair_contact(const double* a6, const double* b6, double skin, double* out3);
 
 /* Run `steps` FP64 steps in place (buffer `p`); returns the first error. */
 int orc_run_f64(orc_particle* p, int32_t n, const orc_config* cfg, int32_t steps);

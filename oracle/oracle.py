@@ -265,3 +265,59 @@ def com_loss(state: np.ndarray, target) -> tuple[float, np.ndarray]:
     bar = np.zeros_like(state)
     bar[:, :2] = 2.0 * d / state.shape[0]
     return float(d @ d), bar
+
+
+# ---------------- config-2 bodies and fracture (orc_step_bodies_f64) ----------------
+
+class BodyConfig(ctypes.Structure):
+    _fields_ = [("bond_rest_length", ctypes.c_double), ("bond_stiffness", ctypes.c_double),
+                ("bond_reference_dt", ctypes.c_double)]
+
+
+BOND_DTYPE = np.dtype([("first", "i4"), ("second", "i4"), ("rest_length", "f8"), ("failure_strain", "f8"),
+                       ("damage", "f8"), ("broken", "i4"), ("pad", "i4")], align=True)
+assert BOND_DTYPE.itemsize == 40
+
+
+def _bodies_lib():
+    L = lib()
+    if not getattr(L, "_bodies_ready", False):
+        P = ctypes.c_void_p
+        L.orc_step_bodies_f64.argtypes = [P, ctypes.c_int32, P, ctypes.POINTER(Config), ctypes.POINTER(BodyConfig),
+                                          P, ctypes.c_int32, P, ctypes.POINTER(Diagnostics)]
+        L.orc_triangle_sample.argtypes = [ctypes.c_double, ctypes.c_double, P, P]
+        L.orc_triangle_pair_contact.argtypes = [P, P, ctypes.c_double, P]
+        L._bodies_ready = True
+    return L
+
+
+def step_bodies_f64(particles: np.ndarray, object_ids, cfg: Config, body_cfg: BodyConfig, bonds=None):
+    """One FP64 step with bonded triangles and fracture bonds.  Returns
+    (particles out, bonds out, diagnostics)."""
+    oid = np.ascontiguousarray(object_ids, np.uint32)
+    b = np.zeros(0, BOND_DTYPE) if bonds is None else np.ascontiguousarray(bonds, BOND_DTYPE).copy()
+    out = np.zeros_like(particles)
+    diag = Diagnostics()
+    code = _bodies_lib().orc_step_bodies_f64(_ptr(particles), len(particles), _ptr(oid), ctypes.byref(cfg),
+                                             ctypes.byref(body_cfg), _ptr(b) if len(b) else None, len(b), _ptr(out),
+                                             ctypes.byref(diag))
+    if code:
+        raise OracleError(code)
+    return out, b, diag
+
+
+def triangle_sample(px: float, py: float, tri) -> np.ndarray:
+    t = np.ascontiguousarray(tri, np.float64).reshape(6)
+    out = np.zeros(6)
+    code = _bodies_lib().orc_triangle_sample(px, py, _ptr(t), _ptr(out))
+    if code:
+        raise OracleError(code)
+    return out
+
+
+def triangle_pair_contact(a, b, skin: float):
+    A = np.ascontiguousarray(a, np.float64).reshape(6)
+    B = np.ascontiguousarray(b, np.float64).reshape(6)
+    out = np.zeros(3)
+    hit = _bodies_lib().orc_triangle_pair_contact(_ptr(A), _ptr(B), skin, _ptr(out))
+    return (out if hit else None)
