@@ -73,7 +73,12 @@ typedef struct mpm_config {
 /* Context creation and upload of the initial packed particles.
  * Replaces createGpuSolver/createRuntime (solver.ts:408-469, :605-1230).
  * packed20: n records of 20 words; object_ids: n words (may be NULL).
- * Bonded triangles (word 19 != 0) are rejected with MPM_EINVAL. */
+ * Bonded triangle bodies are three consecutive records with word 19 = 1, 2, 3
+ * (bondedTriangleVertex + 1) of one object and a solid material
+ * (collectBondedBodyStarts, reference/solver.ts:1023-1062); their contact
+ * field, fluid contact, edge constraints and pair contacts run in every
+ * microstep (at most 1024 bodies; such scenes cannot be batched and cannot
+ * take appended bonded vertices). */
 int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* object_ids,
                uint32_t n, mpm_ctx** out);
 
@@ -104,6 +109,17 @@ int mpm_read_particles(mpm_ctx* ctx, const uint32_t* indices, uint32_t n, uint32
 /* appendParticles device writes (solver.ts:2216-2242): appends n records
  * after the active slots; MPM_ECAPACITY if they do not fit. */
 int mpm_append_particles(mpm_ctx* ctx, const uint32_t* packed20, const uint32_t* object_ids, uint32_t n);
+
+/* Fracture bonds (config.fractureBonds, validated as validateMpmSolidBondGraph,
+ * reference/solid-bonds.ts:11-66) in the reference's 24-byte records
+ * (layout.ts:527-548): u32 first, u32 second, f32 restLength,
+ * f32 failureStrain, f32 damage (1 when broken), u32 broken.  Set once, before
+ * the first step.  Each microstep then updates the bonds' damage from the
+ * tensile strain and raises the endpoints' damage (record word 18), as
+ * updateFractureBonds / applyFractureParticleDamage (shaders.ts:2957-3039).
+ * mpm_read_fracture_bonds returns the committed bond state. */
+int mpm_set_fracture_bonds(mpm_ctx* ctx, const void* bonds24, uint32_t count);
+int mpm_read_fracture_bonds(mpm_ctx* ctx, void* out24, uint32_t count);
 
 /* Drag interaction uniform (solver.ts:1389-1401):
  * p[0..3] = target x, target y, radius (0 = off), stiffness; p[4] = damping. */

@@ -56,7 +56,7 @@ struct Pending {
 
 // Host copy of the per-particle record fields that never change on the GPU.
 struct HostStatic {
-    std::vector<uint32_t> model, object_id;
+    std::vector<uint32_t> model, object_id, vertex;  // vertex: record word 19 (bonded vertex + 1, 0 = none)
     std::vector<float> lambda, mu, density, volume, param;
 };
 
@@ -96,6 +96,12 @@ struct mpm_ctx {
     bool shared_stream = false;        // the stream belongs to a batch
     struct mpm_batch* owner = nullptr; // the batch that owns this scene
     double* diag_saved = nullptr;  // diagnostics buffer while the reductions are off
+    // config-2: bonded triangle bodies (from the records) and fracture bonds
+    std::vector<uint32_t> body_start;  // particle index of each body's vertex 0
+    std::vector<Bond> bonds;           // as set by mpm_set_fracture_bonds
+    std::vector<uint32_t> aux_pid;     // particle of each aux entry
+    DeviceArrays aux_mem;
+    bool stepped = false;              // a step has been launched
 };
 
 namespace {
@@ -153,8 +159,8 @@ int validate_records(mpm_ctx* c, const mpm_config& cfg, const uint32_t* rec, uin
         const uint32_t model = r[13];
         if (model > 4u)
             return fail(c, MPM_EINVAL, "particles[%u].model code %u is not a supported material", i, model);
-        if (r[19] != 0u)
-            return fail(c, MPM_EINVAL, "particles[%u] is a bonded-triangle vertex; bonded bodies are not supported", i);
+        if (r[19] > 3u)
+            return fail(c, MPM_EINVAL, "particles[%u].bondedTriangleVertex must be 0, 1 or 2", i);
         for (int k = 0; k < 19; ++k) {
             if (k == 13) continue;
             if (!isfinite(u2f(r[k]))) return fail(c, MPM_EINVAL, "particles[%u] word %d must be finite", i, k);
@@ -279,6 +285,7 @@ int write_home(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n,
         c->st.volume[j] = u2f(r[17]);
         c->st.param[j] = u2f(r[18]);
         c->st.object_id[j] = ids ? ids[i] : 0u;
+        c->st.vertex[j] = r[19];
         const uint32_t m = r[13];
         c->lc.scene_flags |= m == 2u ? SCENE_WATER : (m == 3u ? SCENE_OIL : SCENE_SOLID);
         if ((m == 2u || m == 3u) && u2f(r[18]) > 0.0f) c->lc.scene_flags |= SCENE_TAIT;
@@ -294,6 +301,118 @@ int write_home(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n,
         HIP_TRY(c, hipMemcpy(h.mat + offset, mat.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     return upload_materials(c);
+}
+
+// collectBondedBodyStarts (reference/solver.ts:1023-1062; layout.ts:452-495):
+// triangles are three consecutive records with vertex codes 1, 2, 3 of one
+// object and a solid material.
+int collect_bodies(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n, std::vector<uint32_t>& starts) {
+    starts.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t* r = rec + 20ull * i;
+        if (r[19] == 0u) continue;
+        if (r[19] != 1u)
+            return fail(c, MPM_EINVAL, "particles[%u].bondedTriangleVertex must start each triangle at zero", i);
+        if (i + 2 >= n) return fail(c, MPM_EINVAL, "bonded triangle at particle %u is incomplete", i);
+        const uint32_t oid = ids ? ids[i] : 0u;
+        for (uint32_t v = 0; v < 3; ++v) {
+            const uint32_t* m = rec + 20ull * (i + v);
+            if (m[19] != v + 1u || (ids ? ids[i + v] : 0u) != oid)
+                return fail(c, MPM_EINVAL,
+                            "bonded triangle at particle %u must contain contiguous vertices 0, 1, and 2 of one object", i);
+            if (m[13] == MODEL_WATER || m[13] == MODEL_OIL)
+                return fail(c, MPM_EINVAL, "bonded triangle vertices must use a solid material");
+        }
+        starts.push_back(i);
+        i += 2;
+    }
+    if (starts.size() > 1024u)
+        return fail(c, MPM_EINVAL, "at most 1024 bonded triangle bodies are supported per scene");
+    return MPM_OK;
+}
+
+// Device tables of the aux particles (triangle vertices, then fracture
+// endpoints in ascending index), the triangle field and the bonds.
+int build_aux(mpm_ctx* c) {
+    c->aux_mem.release();
+    LaunchCtx& lc = c->lc;
+    lc.aux = AuxData{};
+    lc.scene_flags &= ~SCENE_AUX;
+    const uint32_t nbodies = (uint32_t)c->body_start.size(), nbonds = (uint32_t)c->bonds.size();
+    if (nbodies == 0 && nbonds == 0) return MPM_OK;
+    const uint32_t capn = std::max<uint32_t>(c->cfg.capacity, 1u);
+    std::vector<uint32_t> aux_of(capn, NO_AUX), pids;
+    for (uint32_t b = 0; b < nbodies; ++b)
+        for (uint32_t v = 0; v < 3; ++v) {
+            aux_of[c->body_start[b] + v] = (uint32_t)pids.size();
+            pids.push_back(c->body_start[b] + v);
+        }
+    const uint32_t frac0 = (uint32_t)pids.size();
+    std::vector<uint32_t> ends;
+    for (const Bond& bd : c->bonds) {
+        ends.push_back(bd.first);
+        ends.push_back(bd.second);
+    }
+    std::sort(ends.begin(), ends.end());
+    ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+    for (uint32_t e : ends) {
+        aux_of[e] = (uint32_t)pids.size();
+        pids.push_back(e);
+    }
+    const uint32_t naux = (uint32_t)pids.size();
+    std::vector<uint32_t> off(naux + 1, 0u), inc(2 * nbonds);
+    for (const Bond& bd : c->bonds) {
+        off[aux_of[bd.first] + 1]++;
+        off[aux_of[bd.second] + 1]++;
+    }
+    for (uint32_t a = 0; a < naux; ++a) off[a + 1] += off[a];
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint32_t k = 0; k < nbonds; ++k) {
+        inc[cur[aux_of[c->bonds[k].first]]++] = k;
+        inc[cur[aux_of[c->bonds[k].second]]++] = k;
+    }
+    std::vector<float> damage(naux, 0.0f);
+    for (uint32_t a = frac0; a < naux; ++a) damage[a] = c->st.param[pids[a]];
+    std::vector<uint32_t> oid(std::max<uint32_t>(nbodies, 1u), 0u);
+    for (uint32_t b = 0; b < nbodies; ++b) oid[b] = c->st.object_id[c->body_start[b]];
+    AuxData A{};
+    hipError_t e;
+#define AALLOC(ptr, bytes)                                                                             \
+    if ((e = c->aux_mem.alloc((void**)&(ptr), (bytes))) != hipSuccess)                                 \
+        return fail(c, MPM_EDEVICE, "hipMalloc(%zu bytes) for the body tables failed: %s", (size_t)(bytes), \
+                    hipGetErrorString(e));
+    AALLOC(A.aux_of, capn * sizeof(uint32_t));
+    AALLOC(A.aux_pid, naux * sizeof(uint32_t));
+    AALLOC(A.slot[0], naux * sizeof(uint32_t));
+    AALLOC(A.slot[1], naux * sizeof(uint32_t));
+    AALLOC(A.damage[0], naux * sizeof(float));
+    AALLOC(A.damage[1], naux * sizeof(float));
+    AALLOC(A.body_oid, oid.size() * sizeof(uint32_t));
+    AALLOC(A.field, (size_t)lc.node_count * sizeof(FieldNode));
+    AALLOC(A.field_body, (size_t)lc.node_count * sizeof(uint32_t));
+    AALLOC(A.bonds, std::max<uint32_t>(nbonds, 1u) * sizeof(Bond));
+    AALLOC(A.bond_off, (naux + 1) * sizeof(uint32_t));
+    AALLOC(A.bond_inc, std::max<uint32_t>(2 * nbonds, 1u) * sizeof(uint32_t));
+#undef AALLOC
+    HIP_TRY(c, hipMemcpy(A.aux_of, aux_of.data(), capn * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(A.aux_pid, pids.data(), naux * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(A.damage[0], damage.data(), naux * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(A.damage[1], damage.data(), naux * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(A.body_oid, oid.data(), oid.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (nbonds) {
+        HIP_TRY(c, hipMemcpy(A.bonds, c->bonds.data(), nbonds * sizeof(Bond), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(A.bond_inc, inc.data(), 2 * nbonds * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, hipMemcpy(A.bond_off, off.data(), (naux + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(A.field_body, 0, (size_t)lc.node_count * sizeof(uint32_t)));
+    A.nbodies = nbodies;
+    A.naux = naux;
+    A.nbonds = nbonds;
+    A.frac_aux0 = frac0;
+    lc.aux = A;
+    lc.scene_flags |= SCENE_AUX | SCENE_GENERAL;
+    c->aux_pid = pids;
+    return MPM_OK;
 }
 
 // Bucket every live particle of the caller-order arrays into buffer cur,
@@ -364,6 +483,7 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
     d.keep_fixed = lc.keep_fixed;
     d.seq = seq;
     d.start_buf = (uint32_t)c->cur;
+    d.aux = lc.aux;
     return d;
 }
 
@@ -399,6 +519,10 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     sb.fixed_point = c->lc.fixed_point;
     sb.diag = c->lc.diag != nullptr;
     sb.timing = c->times;
+    sb.aux_bodies = c->lc.aux.nbodies;
+    sb.aux_bonds = c->lc.aux.nbonds;
+    sb.node_count = c->lc.node_count;
+    c->stepped = true;
     HIP_TRY(c, launch_steps(sb, steps, measure));
     c->pending.push_back({p, steps, measure, seq, done_before});
     c->last_done_before = done_before;
@@ -473,7 +597,23 @@ int prepare_params(mpm_ctx* c, const void* params144, Params& p) {
     if (p.particle_count != c->n)
         return fail(c, MPM_EINVAL, "params particleCount %u does not match the %u allocated particle slots",
                     p.particle_count, c->n);
-    if (p.body_count != 0u) return fail(c, MPM_EINVAL, "bonded triangle bodies are not supported");
+    const uint32_t nbodies = (uint32_t)c->body_start.size();
+    if (p.body_count != nbodies)
+        return fail(c, MPM_EINVAL, "params bondedBodyCount %u does not match the %u bonded triangle bodies",
+                    p.body_count, nbodies);
+    if (nbodies) {
+        if (!(p.bond_rest > 0.0f) || !isfinite(p.bond_rest))
+            return fail(c, MPM_EINVAL, "bondRestLength is required for bonded triangle particles");
+        if (!(p.bond_stiffness > 0.0f) || !(p.bond_stiffness <= 1.0f))
+            return fail(c, MPM_EINVAL, "bondStiffness must be greater than zero and at most one");
+        if (!(p.bond_ref_dt > 0.0f) || !isfinite(p.bond_ref_dt))
+            return fail(c, MPM_EINVAL, "bondReferenceDt must be greater than zero");
+        if (!(p.reach > 0.0f) || !isfinite(p.reach) || !(p.band > 0.0f))
+            return fail(c, MPM_EINVAL, "params triangle contact band and reach are invalid");
+    }
+    if (p.fracture[1] != (uint32_t)c->bonds.size())
+        return fail(c, MPM_EINVAL, "params fracture bond count %u does not match the %zu bonds", p.fracture[1],
+                    c->bonds.size());
     if (!(p.dt > 0.0f) || !isfinite(p.dt)) return fail(c, MPM_EINVAL, "dt must be greater than zero");
     if (p.dx != c->cfg.cell_size) return fail(c, MPM_EINVAL, "params cellSize does not match the solver grid");
     if (p.particle_mass != c->cfg.particle_mass || p.particle_volume != c->cfg.particle_volume)
@@ -502,6 +642,7 @@ static void destroy_ctx(mpm_ctx* c) {
     delete c->times;
     c->slot_mem.release();
     c->mem.release();
+    c->aux_mem.release();
     if (c->d_mats) (void)hipFree(c->d_mats);
     for (int k = 0; k < PARAM_RING; ++k)
         if (c->ring_events[k]) (void)hipEventDestroy(c->ring_events[k]);
@@ -535,8 +676,10 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     if (capacity < n) return fail(nullptr, MPM_EINVAL, "capacity must be at least the initial particle count");
     if (capacity > (1u << 30)) return fail(nullptr, MPM_EINVAL, "capacity exceeds the supported 2^30 particle slots");
     const uint64_t nodes64 = (uint64_t)(c.cells_x + 3) * (uint64_t)(c.cells_y + 3);
+    std::vector<uint32_t> body_start;
     {
         int v = validate_records(nullptr, c, packed20, n);
+        if (!v) v = collect_bodies(nullptr, packed20, object_ids, n, body_start);
         if (v) return v;
     }
 
@@ -561,7 +704,7 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     lc.nb = lc.bw * lc.bh;
     try {
         HostStatic& s = ctx->st;
-        for (auto* v : {&s.model, &s.object_id}) v->assign(capacity ? capacity : 1, 0u);
+        for (auto* v : {&s.model, &s.object_id, &s.vertex}) v->assign(capacity ? capacity : 1, 0u);
         for (auto* v : {&s.lambda, &s.mu, &s.density, &s.volume, &s.param}) v->assign(capacity ? capacity : 1, 0.0f);
     } catch (...) {
         return bail(fail(ctx, MPM_EDEVICE, "out of host memory"));
@@ -636,6 +779,11 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     lc.n = n;
     const uint32_t cap0 = c.bucket_slots ? std::min<uint32_t>(c.bucket_slots, MAX_BUCKET_CAP)
                                          : (uint32_t)initial_cap(ctx, packed20, n);
+    {
+        ctx->body_start = body_start;
+        int b = build_aux(ctx);
+        if (b) return bail(b);
+    }
     int a = alloc_slots(ctx, cap0);
     if (a) return bail(a);
     int r = rebucket_all(ctx, cap0);
@@ -700,6 +848,17 @@ int mpm_read_particles(mpm_ctx* c, const uint32_t* indices, uint32_t n, uint32_t
         HIP_TRY(c, hipMemcpy(pl.data(), h.plastic, total * sizeof(float), hipMemcpyDeviceToHost));
     }
     const HostStatic& st = c->st;
+    std::vector<float> damage;  // fracture endpoints: the committed damage
+    if (c->lc.aux.naux) {
+        damage.resize(c->lc.aux.naux);
+        HIP_TRY(c, hipMemcpy(damage.data(), c->lc.aux.damage[c->cur], damage.size() * sizeof(float),
+                             hipMemcpyDeviceToHost));
+    }
+    std::vector<float> param_now;
+    if (!damage.empty()) {
+        param_now.assign(st.param.begin(), st.param.begin() + c->n);
+        for (uint32_t a = c->lc.aux.frac_aux0; a < c->lc.aux.naux; ++a) param_now[c->aux_pid[a]] = damage[a];
+    }
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t i = indices ? indices[k] : k;
         uint32_t* r = out20 + 20ull * k;
@@ -707,7 +866,9 @@ int mpm_read_particles(mpm_ctx* c, const uint32_t* indices, uint32_t n, uint32_t
         r[4] = f2u(C[i].x); r[5] = f2u(C[i].y); r[6] = f2u(C[i].z); r[7] = f2u(C[i].w);
         r[8] = f2u(F[i].x); r[9] = f2u(F[i].y); r[10] = f2u(F[i].z); r[11] = f2u(F[i].w);
         r[12] = f2u(pl[i]); r[13] = st.model[i]; r[14] = f2u(st.lambda[i]); r[15] = f2u(st.mu[i]);
-        r[16] = f2u(st.density[i]); r[17] = f2u(st.volume[i]); r[18] = f2u(st.param[i]); r[19] = 0u;
+        r[16] = f2u(st.density[i]); r[17] = f2u(st.volume[i]);
+        r[18] = f2u(param_now.empty() ? st.param[i] : param_now[i]);
+        r[19] = st.vertex[i];
         if (object_ids) object_ids[k] = st.object_id[i];
     }
     return MPM_OK;
@@ -721,6 +882,9 @@ int mpm_append_particles(mpm_ctx* c, const uint32_t* packed20, const uint32_t* o
         return fail(c, MPM_ECAPACITY, "appending %u particles exceeds the capacity of %u", n, c->cfg.capacity);
     int v = validate_records(c, c->cfg, packed20, n);
     if (v) return v;
+    for (uint32_t i = 0; i < n; ++i)
+        if (packed20[20ull * i + 19] != 0u)
+            return fail(c, MPM_EINVAL, "appending bonded triangle vertices is not supported");
     int s = settle(c);
     if (s) return s;
     // committed state -> caller-order arrays, append, bucket everything again
@@ -736,6 +900,63 @@ int mpm_append_particles(mpm_ctx* c, const uint32_t* packed20, const uint32_t* o
         if (a) return a;
     }
     return rebucket_all(c, c->lc.cap);
+}
+
+// validateMpmSolidBondGraph (reference/solid-bonds.ts:11-66) on the packed
+// 24-byte bonds (layout.ts:527-548), then the device tables.
+int mpm_set_fracture_bonds(mpm_ctx* c, const void* bonds24, uint32_t count) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (count > 0 && !bonds24) return fail(c, MPM_EINVAL, "bonds must not be NULL");
+    if (c->stepped) return fail(c, MPM_ESTATE, "fracture bonds must be set before the first step");
+    std::vector<Bond> bonds(count);
+    if (count) memcpy(bonds.data(), bonds24, count * sizeof(Bond));
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (uint32_t k = 0; k < count; ++k) {
+        const Bond& b = bonds[k];
+        if (b.first >= c->n || b.second >= c->n)
+            return fail(c, MPM_EINVAL, "fractureBonds[%u] endpoints must identify existing particles", k);
+        if (b.first == b.second) return fail(c, MPM_EINVAL, "fractureBonds[%u] must connect two distinct particles", k);
+        if (!(b.rest > 0.0f) || !isfinite(b.rest))
+            return fail(c, MPM_EINVAL, "fractureBonds[%u].restLength must be positive and finite", k);
+        if (!(b.failure > 0.0f) || !isfinite(b.failure))
+            return fail(c, MPM_EINVAL, "fractureBonds[%u].failureStrain must be positive and finite", k);
+        if (!(b.damage >= 0.0f && b.damage <= 1.0f))
+            return fail(c, MPM_EINVAL, "fractureBonds[%u].damage must be between zero and one", k);
+        if (b.broken > 1u) return fail(c, MPM_EINVAL, "fractureBonds[%u].broken must be 0 or 1", k);
+        if (c->st.object_id[b.first] != c->st.object_id[b.second])
+            return fail(c, MPM_EINVAL, "fractureBonds[%u] must connect particles of one object", k);
+        if (c->st.vertex[b.first] != 0u || c->st.vertex[b.second] != 0u)
+            return fail(c, MPM_EINVAL, "fractureBonds[%u] cannot reuse bonded triangle vertices", k);
+        if (c->st.model[b.first] != MODEL_NEO_HOOKEAN || c->st.model[b.second] != MODEL_NEO_HOOKEAN)
+            return fail(c, MPM_EINVAL, "fractureBonds[%u] endpoints must be neo-hookean solids", k);
+        pairs.emplace_back(std::min(b.first, b.second), std::max(b.first, b.second));
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> sorted = pairs;
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(c, MPM_EINVAL, "fractureBonds duplicates an existing particle pair");
+    int st = settle(c);
+    if (st) return st;
+    c->bonds = bonds;
+    int a = build_aux(c);
+    if (a) return a;
+    if (!scene_general(c->lc.scene_flags) || !c->lc.buf[0].S) {
+        a = alloc_slots(c, c->lc.cap);
+        if (a) return a;
+    }
+    return rebucket_all(c, c->lc.cap);
+}
+
+int mpm_read_fracture_bonds(mpm_ctx* c, void* out24, uint32_t count) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (count != (uint32_t)c->bonds.size())
+        return fail(c, MPM_EINVAL, "the context holds %zu fracture bonds, not %u", c->bonds.size(), count);
+    if (count == 0) return MPM_OK;
+    if (!out24) return fail(c, MPM_EINVAL, "output must not be NULL");
+    int s = settle(c);
+    if (s) return s;
+    HIP_TRY(c, hipMemcpy(out24, c->lc.aux.bonds, count * sizeof(Bond), hipMemcpyDeviceToHost));
+    return MPM_OK;
 }
 
 int mpm_set_interaction(mpm_ctx* c, const float p[8]) {
@@ -956,6 +1177,10 @@ int mpm_batch_create(const mpm_config* configs, const uint32_t* const* packed20,
         }
         c->owner = b;
         b->scenes.push_back(c);
+        if (c->lc.aux.naux) {
+            b->error = std::string("scene ") + std::to_string(i) + ": bonded triangle bodies cannot be batched";
+            return bail(MPM_EINVAL);
+        }
     }
     hipError_t e;
     b->device = b->scenes[0]->device;

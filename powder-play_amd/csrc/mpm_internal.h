@@ -121,8 +121,44 @@ struct Grid {
     int32_t* fixed;     // node_count*11 or nullptr
 };
 
-// Scene content flags, host-derived from the particle models.
-enum : uint32_t { SCENE_SOLID = 1u, SCENE_WATER = 2u, SCENE_OIL = 4u, SCENE_TAIT = 8u, SCENE_GENERAL = 16u };
+// Scene content flags, host-derived from the particle models.  SCENE_AUX:
+// the scene has bonded triangle bodies or fracture bonds, whose particles are
+// reached by index through the aux table below.
+enum : uint32_t { SCENE_SOLID = 1u, SCENE_WATER = 2u, SCENE_OIL = 4u, SCENE_TAIT = 8u, SCENE_GENERAL = 16u,
+                  SCENE_AUX = 32u };
+
+// Config-2 bodies and bonds.  Triangle field of one node (buildTriangleField,
+// shaders.ts:1324-1436): signed distance, outward normal and barycentric
+// weights of the nearest point of the nearest triangle within the band, and
+// the impulse the fluid contact puts on that body at this node.
+struct FieldNode {
+    float dist, nx, ny, b0, b1, b2, ix, iy;
+};
+// A fracture bond in the reference's 24-byte layout (layout.ts:527-548).
+struct Bond {
+    uint32_t first, second;
+    float rest, failure, damage;
+    uint32_t broken;
+};
+static_assert(sizeof(Bond) == 24, "fracture bond record is 24 bytes");
+// Particles the contact and fracture kernels address by index ("aux"): the
+// vertices of body b are aux 3b, 3b+1, 3b+2; then every fracture-bond
+// endpoint in ascending particle index.  Both slot buffers keep where each
+// aux particle sits; the bucket and G2P kernels write it.
+struct AuxData {
+    uint32_t* aux_of;      // per particle index: aux index or NO_AUX
+    uint32_t* aux_pid;     // per aux: particle index
+    uint32_t* slot[2];     // per aux and slot buffer: slot
+    float* damage[2];      // per aux and slot buffer: neo-Hookean damage (fracture endpoints)
+    uint32_t* body_oid;    // per body: object id
+    FieldNode* field;      // per node
+    uint32_t* field_body;  // per node: body index + 1, 0 = no triangle within the band
+    Bond* bonds;
+    uint32_t* bond_off;    // per aux + 1: CSR offsets into bond_inc
+    uint32_t* bond_inc;    // incident bond indices
+    uint32_t nbodies, naux, nbonds, frac_aux0;  // frac_aux0 = 3 * nbodies
+};
+constexpr uint32_t NO_AUX = 0xffffffffu;
 // Scenes with solids or Tait liquids carry the FP64 stress cache; SCENE_GENERAL
 // asks for it regardless (a scene stepped together with such scenes).
 __host__ __device__ inline bool scene_general(uint32_t flags) {
@@ -147,6 +183,7 @@ struct LaunchCtx {
     uint32_t fixed_point;  // 0 = float gather, 1 = int32 fixed point
     uint32_t keep_fixed;
     uint32_t scene_flags;
+    AuxData aux;           // bodies / fracture (zeros without SCENE_AUX)
     KernelTimes* timing;   // nullptr when timing is off
     // FP64 diagnostics (nullptr when off): per-tile grid partials of the last
     // P2G, per-block particle partials, and the nine reported sums
@@ -172,6 +209,7 @@ struct SceneDesc {
     uint32_t ptx, pty, gtx, gty;  // P2G and G2P tile grids
     uint32_t p2g_base, g2p_base;  // first tile of this scene in the P2G / G2P launches
     uint32_t scene_flags, keep_fixed;
+    AuxData aux;
     uint32_t seq;                 // batch sequence number of this scene
     uint32_t start_buf;           // slot buffer holding the state the batch starts from
 };
@@ -189,6 +227,8 @@ struct StepBatch {
     uint32_t max_p2g_tiles;      // largest per-scene P2G tile count
     uint32_t scene_flags;        // union of the scenes' flags (kernel variant)
     uint32_t fixed_point;
+    uint32_t aux_bodies, aux_bonds;  // SCENE_AUX (one scene): body and bond counts
+    uint32_t node_count;             // SCENE_AUX: the scene's grid nodes
     bool diag;                   // some scene has diagnostics on
     KernelTimes* timing;
 };

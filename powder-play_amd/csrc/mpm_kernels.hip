@@ -47,6 +47,13 @@ constexpr double MAX_NEO_HOOKEAN_STRETCH = 20.0;
 constexpr float MAX_DEFORMATION_INCREMENT = 0.25f;  // a power of two (dividing by it is exact scaling)
 static_assert(MAX_DEFORMATION_INCREMENT == 0.25f, "the substep count scales by the exact reciprocal");
 constexpr uint32_t MAX_DEFORMATION_SUBSTEPS = 64u;
+// bonded triangle bodies (shaders.ts:175-177, materials.ts:44, reference/solver.ts:277-282)
+constexpr float TRIANGLE_FIELD_BAND_CELLS = 2.0f;
+constexpr float TRIANGLE_CONTACT_SKIN_CELLS = 0.1f;
+constexpr float TRIANGLE_CONTACT_IMPULSE_RELAXATION = 0.25f;
+constexpr float TRIANGLE_FLUID_TANGENTIAL_DRAG_RATE = 4.0f;
+constexpr int BONDED_TRIANGLE_SOLVER_ITERATIONS = 2;
+constexpr float PI_F = 3.14159265358979f;
 
 // P2G tile: 28x28 interior nodes (7x7 buckets), a one-node ring for the
 // liquid-contact volume gradient (30x30 computed), fed by 9x9 buckets.
@@ -346,7 +353,8 @@ __global__ void k_clear_counts(uint32_t* cnt, uint32_t nb) {
 template <bool GENERAL>
 __global__ __launch_bounds__(256) void k_bucket(Control* ctl, Home home, const Material* __restrict__ mats,
                                                 SlotBuf out, uint32_t first, uint32_t count, float dx,
-                                                uint32_t cells_x, uint32_t cells_y, uint32_t bw, uint32_t cap) {
+                                                uint32_t cells_x, uint32_t cells_y, uint32_t bw, uint32_t cap,
+                                                AuxData aux, int buf) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = first + t;
     bool active = t < count;
@@ -377,11 +385,19 @@ __global__ __launch_bounds__(256) void k_bucket(Control* ctl, Home home, const M
     out.F[g] = F;
     out.pid[g] = i;
     out.mat[g] = m;
+    Material mt = mats[m];
+    if (aux.aux_of) {  // bodies / fracture: where the particle sits, and its current damage
+        const uint32_t a = aux.aux_of[i];
+        if (a != NO_AUX) {
+            aux.slot[buf][a] = (uint32_t)g;
+            if (a >= aux.frac_aux0) mt.param = aux.damage[buf][a];
+        }
+    }
     if constexpr (GENERAL) {
         const float pv = home.plastic[i];
         out.plastic[g] = pv;
         float4 S;
-        const uint32_t pf = particle_stress<true>(mats[m], F, pv, det_comp(F), S);
+        const uint32_t pf = particle_stress<true>(mt, F, pv, det_comp(F), S);
         out.S[g] = S;
         if (pf) {
             if (pf & ERR_P2G) raise_step_at(ctl, pf, 0u, i); else raise_step(ctl, pf, 0u);
@@ -545,10 +561,15 @@ __device__ void block_sum3_f64(double a, double b, double c, double* out) {
 // channels f = {solid m, qx, qy | water m, qx, qy, V | oil m, qx, qy, V}.
 // `nvol(k, ox, oy)` returns the summed water (k = 0) or oil (k = 1) volume of
 // the neighbour at offset (ox, oy).  Returns the status bits to raise.
-template <typename NVol>
+// No fluid-triangle contact (scenes without bonded bodies).
+struct NoTriangles {
+    __device__ uint32_t operator()(uint32_t, float, float, float2&, float2&) const { return 0u; }
+};
+
+template <typename NVol, typename Tri = NoTriangles>
 __device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& grid, uint32_t scene_flags,
                                                 uint32_t node, int nx, int ny, const float (&f)[11],
-                                                uint32_t nonfinite_bits, NVol nvol) {
+                                                uint32_t nonfinite_bits, NVol nvol, Tri tri = Tri()) {
     const float f_sm = f[0], f_sqx = f[1], f_sqy = f[2], f_wm = f[3], f_wqx = f[4], f_wqy = f[5], f_wv = f[6];
     const float f_om = f[7], f_oqx = f[8], f_oqy = f[9], f_ov = f[10];
     uint32_t flags = 0;
@@ -594,6 +615,8 @@ __device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& gri
             }
         }
     }
+    // solveFluidTriangleGridContact (shaders.ts:1596-1665) in scenes with bodies
+    flags |= tri(node, f_wm, f_om, vw, vo);
     if (!finite2(vs) || !finite2(vw) || !finite2(vo)) return flags | ERR_GRID | ERR_TRIANGLE_CONTACT;
     const int cxn = (int)P.cells_x, cyn = (int)P.cells_y;
     vs.x = wall_cushion(vs.x, nx, cxn); vs.y = wall_cushion(vs.y, ny, cyn);
@@ -607,6 +630,58 @@ __device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& gri
     if (scene_flags & SCENE_WATER) grid.v_water[node] = vw;
     if (scene_flags & SCENE_OIL) grid.v_oil[node] = vo;
     if (scene_flags & (SCENE_WATER | SCENE_OIL)) grid.liquid[node] = make_float2(f_wm + f_om, f_wv + f_ov);
+    return flags;
+}
+
+// resolveFluidTriangleContact (shaders.ts:1624-1665): the fluid phase's
+// velocity after the normal and tangential-drag impulse, and the reaction
+// impulse on the body accumulated in (ix, iy).
+__device__ __forceinline__ uint32_t fluid_triangle_contact(float fm, float2& v, float svx, float svy, float nx,
+                                                           float ny, float inv_body, float dt, float& ix,
+                                                           float& iy) {
+    const float inv_eff = 1.0f / fm + inv_body;
+    if (!finitef(inv_eff) || inv_eff <= 0.0f) return ERR_TRIANGLE_CONTACT;
+    const float rx = v.x - svx, ry = v.y - svy;
+    const float rn = rx * nx + ry * ny;
+    const float nimp = rn < 0.0f ? -rn / inv_eff : 0.0f;
+    const float tx = rx - nx * rn, ty = ry - ny * rn;
+    const float damping = 1.0f - expf(-TRIANGLE_FLUID_TANGENTIAL_DRAG_RATE * dt);
+    const float fx = nx * nimp + tx * (-damping / inv_eff), fy = ny * nimp + ty * (-damping / inv_eff);
+    v.x = v.x + fx * (1.0f / fm);
+    v.y = v.y + fy * (1.0f / fm);
+    ix -= fx;
+    iy -= fy;
+    return (finite2(v) && finitef(ix) && finitef(iy)) ? 0u : ERR_TRIANGLE_CONTACT;
+}
+
+// Fluid contact of one node with the triangle of its field, whose vertex
+// velocities come from the step's input buffer; the node's field keeps the
+// impulse on the body.
+__device__ __forceinline__ uint32_t fluid_triangle_node(const Params& P, const AuxData& A, const SlotBuf& in,
+                                                        int in_buf, uint32_t node, float wm, float om, float2& vw,
+                                                        float2& vo) {
+    if (A.nbodies == 0u || (wm <= 0.0f && om <= 0.0f)) return 0u;
+    const uint32_t bp1 = A.field_body[node];
+    if (bp1 == 0u) return 0u;
+    const FieldNode fn = A.field[node];
+    if (fn.dist > P.reach) return 0u;
+    const uint32_t a0 = 3u * (bp1 - 1u);
+    const float bw[3] = {fn.b0, fn.b1, fn.b2};
+    float svx = 0.0f, svy = 0.0f, inv_body = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const float2 pv = in.vel[A.slot[in_buf][a0 + v]];
+        svx += bw[v] * pv.x;
+        svy += bw[v] * pv.y;
+        inv_body += (bw[v] * bw[v]) / P.particle_mass;
+    }
+    if (!(inv_body > 0.0f)) return 0u;
+    float ix = 0.0f, iy = 0.0f;
+    uint32_t flags = 0u;
+    if (wm > 0.0f) flags |= fluid_triangle_contact(wm, vw, svx, svy, fn.nx, fn.ny, inv_body, P.dt, ix, iy);
+    if (om > 0.0f) flags |= fluid_triangle_contact(om, vo, svx, svy, fn.nx, fn.ny, inv_body, P.dt, ix, iy);
+    A.field[node].ix = ix;
+    A.field[node].iy = iy;
     return flags;
 }
 
@@ -753,7 +828,7 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 // than G_CAP particles is processed in chunks of whole bucket rows (single
 // buckets only when one bucket row exceeds G_CAP); the sums carry across
 // chunks.  The LDS footprint stays under half a CU so two tiles share one.
-template <bool GENERAL>
+template <bool GENERAL, bool AUX = false>
 __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                   uint32_t step) {
@@ -964,11 +1039,20 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     __syncthreads();
     if (node_ok && lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI) {
         const uint32_t node = (uint32_t)sy * (uint32_t)W + (uint32_t)sx;
-        flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, acc, ERR_P2G | ERR_NONFINITE_ACCUMULATION,
-                             [&](int k, int ox, int oy) {
-                                 const float* a = k == 0 ? s_vw : s_vo;
-                                 return a[ring + oy * P_TC + ox];
-                             });
+        auto nvol = [&](int k, int ox, int oy) {
+            const float* a = k == 0 ? s_vw : s_vo;
+            return a[ring + oy * P_TC + ox];
+        };
+        if constexpr (AUX) {
+            const AuxData& A = S.aux;
+            flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, acc, ERR_P2G | ERR_NONFINITE_ACCUMULATION,
+                                 nvol, [&](uint32_t nd, float wm, float om, float2& vw, float2& vo) {
+                                     return fluid_triangle_node(P, A, in, in_buf, nd, wm, om, vw, vo);
+                                 });
+        } else {
+            flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, acc, ERR_P2G | ERR_NONFINITE_ACCUMULATION,
+                                 nvol);
+        }
     }
     if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
         const bool mine = node_ok && lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI;
@@ -986,7 +1070,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
 // integer atomics: integer sums do not depend on the order, and an exact sum
 // outside int32 is the accumulator overflow of 65536.  Scenes without solids
 // keep only the eight liquid channels, so two tiles fit in one CU's LDS.
-template <bool GENERAL, bool SOLIDS>
+template <bool GENERAL, bool SOLIDS, bool AUX = false>
 __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                  uint32_t step) {
     SCENE_PROLOGUE(tile_scene, p2g_base);
@@ -1155,9 +1239,16 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
             if (keep_fixed) grid.fixed[(size_t)node * 11 + k] = (int32_t)(uint32_t)(unsigned long long)sum;
             f[k] = decoded(k, ring);
         }
-        flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, f, ERR_GRID, [&](int k, int ox, int oy) {
-            return decoded(k == 0 ? 6 : 10, ring + oy * P_TC + ox);
-        });
+        auto nvol = [&](int k, int ox, int oy) { return decoded(k == 0 ? 6 : 10, ring + oy * P_TC + ox); };
+        if constexpr (AUX) {
+            const AuxData& A = S.aux;
+            flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, f, ERR_GRID, nvol,
+                                 [&](uint32_t nd, float wm, float om, float2& vw, float2& vo) {
+                                     return fluid_triangle_node(P, A, in, in_buf, nd, wm, om, vw, vo);
+                                 });
+        } else {
+            flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, f, ERR_GRID, nvol);
+        }
     }
     if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
         const bool mine = lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI && sx < W && sy < H;
@@ -1241,10 +1332,67 @@ __device__ __forceinline__ float2 confine_axis(float x, float v, float maximum) 
 // raised a flag (the microstep then rolls back).
 struct G2POut { float2 pos, vel; float4 C, F; float plastic; };
 
-template <bool GENERAL>
+// projectParticleOutsideTriangles (shaders.ts:1801-1854): a liquid particle
+// of radius r whose interpolated signed distance to the nearest triangle is
+// below r moves out along the interpolated normal of that triangle.  In FP64:
+// the stencil fraction of a float32 coordinate near 2^12 cells keeps only
+// ~14 bits, which would move the interpolated distance by 1e-4 of a cell.
+__device__ float2 project_out_of_triangles(const Params& P, const AuxData& A, float2 p, float r) {
+    const double dx = P.dx, gx = (double)p.x / dx, gy = (double)p.y / dx;
+    const double bxf = floor(gx - 0.5), byf = floor(gy - 0.5);
+    if (!(bxf >= -1.0 && byf >= -1.0 && bxf + 2.0 <= (double)P.cells_x + 1.0 && byf + 2.0 <= (double)P.cells_y + 1.0))
+        return p;
+    const int bx = (int)bxf, by = (int)byf;
+    const double fx = gx - bxf, fy = gy - byf;
+    const double wx[3] = {0.5 * (1.5 - fx) * (1.5 - fx), 0.75 - (fx - 1.0) * (fx - 1.0), 0.5 * (fx - 0.5) * (fx - 0.5)};
+    const double wy[3] = {0.5 * (1.5 - fy) * (1.5 - fy), 0.75 - (fy - 1.0) * (fy - 1.0), 0.5 * (fy - 0.5) * (fy - 0.5)};
+    const double band = dx * TRIANGLE_FIELD_BAND_CELLS;
+    double sd = 0.0, closest = INFINITY, snx = 0.0, sny = 0.0;
+    uint32_t cb = 0u;
+    const uint32_t W = P.node_w;
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            const uint32_t node = (uint32_t)(by + oy + 1) * W + (uint32_t)(bx + ox + 1);
+            const uint32_t b = A.field_body[node];
+            double d = band;
+            if (b != 0u) {
+                const FieldNode fn = A.field[node];
+                d = fn.dist;
+                if (d < closest) {
+                    closest = d;
+                    cb = b;
+                    snx = fn.nx;
+                    sny = fn.ny;
+                }
+            }
+            sd += wx[ox] * wy[oy] * d;
+        }
+    if (cb == 0u || sd >= (double)r) return p;
+    double nx = 0.0, ny = 0.0;
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            const uint32_t node = (uint32_t)(by + oy + 1) * W + (uint32_t)(bx + ox + 1);
+            if (A.field_body[node] != cb) continue;
+            const FieldNode fn = A.field[node];
+            nx += wx[ox] * wy[oy] * fn.nx;
+            ny += wx[ox] * wy[oy] * fn.ny;
+        }
+    const double len = sqrt(nx * nx + ny * ny);
+    if (len > 1e-12 && isfinite(len)) {
+        nx = nx / len;
+        ny = ny / len;
+    } else {
+        nx = snx;
+        ny = sny;
+    }
+    const double corr = (double)r - sd;
+    return make_float2((float)((double)p.x + corr * nx), (float)((double)p.y + corr * ny));
+}
+
+template <bool GENERAL, bool AUX = false>
 __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
                              float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
-                             uint32_t step, G2POut& o) {
+                             uint32_t step, G2POut& o, const AuxData* A = nullptr, bool cfl_here = false) {
     const uint32_t model = mt.model;
     const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
@@ -1419,14 +1567,26 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     }
     // projectLiquidFromTriangles is the identity without bonded bodies; the
     // second confineAxis pass is then idempotent (shaders.ts:2376-2408).
-    const float2 bxv = confine_axis(np.x, vx, P.dom_w);
-    const float2 byv = confine_axis(np.y, vy, P.dom_h);
+    float2 bxv = confine_axis(np.x, vx, P.dom_w);
+    float2 byv = confine_axis(np.y, vy, P.dom_h);
+    if constexpr (AUX) {
+        if (liquid && A->nbodies != 0u) {
+            const float r = (float)sqrt(((double)mt.volume * ndet) / 3.14159265358979323846);
+            const float2 q = project_out_of_triangles(P, *A, make_float2(bxv.x, byv.x), r);
+            if (!finite2(q)) {
+                raise_step(ctl, ERR_TRIANGLE_CONTACT, step);
+                return false;
+            }
+            bxv = confine_axis(q.x, bxv.y, P.dom_w);
+            byv = confine_axis(q.y, byv.y, P.dom_h);
+        }
+    }
     o.pos = make_float2(bxv.x, byv.x);
     o.vel = make_float2(bxv.y, byv.y);
     o.C = C;
     o.F = Fn;
     o.plastic = pvn;
-    if (P.cfl_on >= 0.5f && P.body_count == 0u) {
+    if (P.cfl_on >= 0.5f && (P.body_count == 0u || cfl_here)) {
         const float speed = sqrtf(o.vel.x * o.vel.x + o.vel.y * o.vel.y);
         const float ws = wave_speed_bound<GENERAL>(P, model, Fn, mt.lambda, mt.mu, mt.density, mt.volume, mt.param);
         if (!finite2(o.vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
@@ -1442,7 +1602,7 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 // G2P for the tile's particles and write each into its next bucket.  The
 // inputs of a thread's next particle are fetched before the current one is
 // processed, so their latency overlaps the current G2P and slot reservation.
-template <bool GENERAL>
+template <bool GENERAL, bool AUX = false>
 __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__ scenes,
                                                  const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                                                  uint32_t step) {
@@ -1528,6 +1688,8 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
             if constexpr (GENERAL) PFX##pv = in.plastic[g_];                           \
         }                                                                              \
     } while (0)
+    const AuxData* const Ax = &S.aux;
+    uint32_t* const S_aux_slot_out = AUX ? S.aux.slot[out_buf] : nullptr;
     G2P_FETCH((uint32_t)tid, c_);
     for (uint32_t base = 0; base < total; base += G_THREADS) {
         G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
@@ -1535,8 +1697,17 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
         G2POut o;
         uint32_t dest = 0;
         float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t aux = NO_AUX;
         if (ok) {
-            const Material mt = material_at(s_mat, mats, c_mat);
+            Material mt = material_at(s_mat, mats, c_mat);
+            bool cfl_here = false;
+            if constexpr (AUX) {
+                aux = Ax->aux_of[c_pid];
+                // fracture endpoints carry their own damage (the neo-Hookean parameter)
+                if (aux != NO_AUX && aux >= Ax->frac_aux0) mt.param = Ax->damage[out_buf ^ 1][aux];
+                // triangle vertices are checked after the body contacts (validateCflParticles)
+                cfl_here = aux == NO_AUX || aux >= Ax->frac_aux0;
+            }
             Stencil s;
             if (!make_stencil(P.dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
                 ((s.by + 1) >> 2) != c_gby) {
@@ -1544,7 +1715,8 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
                 ok = false;
             } else {
                 const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u ? s_vo : s_vs);
-                ok = g2p_particle<GENERAL>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o);
+                ok = g2p_particle<GENERAL, AUX>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
+                                                Ax, cfl_here);
             }
             if (ok) {
                 Stencil ns;
@@ -1583,11 +1755,507 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
                     out.plastic[g] = o.plastic;
                     out.S[g] = S;
                 }
+                if constexpr (AUX) {
+                    if (aux != NO_AUX) S_aux_slot_out[aux] = (uint32_t)g;
+                }
             }
         }
         c_p = n_p; c_F = n_F; c_pid = n_pid; c_mat = n_mat; c_pv = n_pv; c_gbx = n_gbx; c_gby = n_gby;
     }
 #undef G2P_FETCH
+}
+
+// ------------------------------------------ config-2 bodies and fracture
+// Scenes with bonded triangle bodies or fracture bonds run up to three more
+// kernels per microstep around P2G and G2P: the triangle field before P2G,
+// then, on the G2P output, the body contacts and the fracture update.  They
+// cover one scene (such scenes are not batched).
+
+// triangleDistanceSample (reference/solver.ts:1190-1291) in FP64: signed
+// distance, outward normal and barycentric weights of the closest point.
+__device__ bool tri_sample64(double px, double py, const double* t, double& sd, double& nx, double& ny,
+                             double* bary) {
+    const double ax = t[0], ay = t[1], bx = t[2], by = t[3], cx = t[4], cy = t[5];
+    const double abx = bx - ax, aby = by - ay, acx = cx - ax, acy = cy - ay;
+    const double area = abx * acy - aby * acx;
+    if (!isfinite(area) || fabs(area) <= 1e-10) return false;
+    const double bcx = cx - bx, bcy = cy - by, cax = ax - cx, cay = ay - cy;
+    const double c_ab = abx * (py - ay) - aby * (px - ax);
+    const double c_bc = bcx * (py - by) - bcy * (px - bx);
+    const double c_ca = cax * (py - cy) - cay * (px - cx);
+    const bool inside = area > 0 ? (c_ab >= -1e-10 && c_bc >= -1e-10 && c_ca >= -1e-10)
+                                 : (c_ab <= 1e-10 && c_bc <= 1e-10 && c_ca <= 1e-10);
+    double best = INFINITY, bnx = 0.0, bny = 0.0;
+    bary[0] = 1.0;
+    bary[1] = 0.0;
+    bary[2] = 0.0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double sx = t[2 * e], sy = t[2 * e + 1], ex = t[(2 * e + 2) % 6], ey = t[(2 * e + 3) % 6];
+        const double dx = ex - sx, dy = ey - sy, len2 = dx * dx + dy * dy;
+        const double tt = fmax(0.0, fmin(1.0, ((px - sx) * dx + (py - sy) * dy) / len2));
+        const double qx = sx + dx * tt, qy = sy + dy * tt;
+        const double ddx = px - qx, ddy = py - qy, d2 = ddx * ddx + ddy * ddy;
+        if (d2 >= best) continue;
+        best = d2;
+        if (e == 0) { bary[0] = 1 - tt; bary[1] = tt; bary[2] = 0; }
+        else if (e == 1) { bary[0] = 0; bary[1] = 1 - tt; bary[2] = tt; }
+        else { bary[0] = tt; bary[1] = 0; bary[2] = 1 - tt; }
+        if (d2 > 1e-20 && !inside) {
+            const double d = sqrt(d2);
+            bnx = ddx * (1.0 / d);
+            bny = ddy * (1.0 / d);
+        } else {
+            const double o = area > 0 ? 1.0 : -1.0, el = sqrt(len2);
+            bnx = (o * dy) * (1.0 / el);
+            bny = (-o * dx) * (1.0 / el);
+        }
+    }
+    const double d = sqrt(best);
+    if (!isfinite(d) || !isfinite(bnx) || !isfinite(bny)) return false;
+    sd = inside ? -d : d;
+    nx = bnx;
+    ny = bny;
+    return true;
+}
+
+__device__ __forceinline__ void load_triangle(const SlotBuf& b, const uint32_t* slot, uint32_t body, double* t) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const float2 p = b.pos[slot[3u * body + (uint32_t)v]];
+        t[2 * v] = p.x;
+        t[2 * v + 1] = p.y;
+    }
+}
+
+constexpr int FIELD_THREADS = 256;
+constexpr int FIELD_CHUNK = 256;  // triangles staged in LDS at a time
+
+// buildTriangleField (reference/solver.ts:1064-1165; shaders.ts:1324-1436):
+// for every node, the nearest triangle of the step's input state whose
+// signed distance is within the band (ties keep the first body), and a
+// cleared contact impulse.
+__global__ __launch_bounds__(FIELD_THREADS) void k_tri_field(const SceneDesc* __restrict__ scenes, uint32_t step) {
+    const SceneDesc& S = scenes[0];
+    Control* ctl = S.ctl;
+    if (block_status_set(ctl)) return;
+    const Params& P = S.P;
+    const AuxData& A = S.aux;
+    const int in_buf = (int)((S.start_buf + step) & 1u);
+    const SlotBuf in = S.buf[in_buf];
+    __shared__ double s_tri[FIELD_CHUNK][6];
+    __shared__ double s_box[FIELD_CHUNK][4];
+    const uint32_t node = blockIdx.x * FIELD_THREADS + threadIdx.x;
+    const bool live = node < P.node_count;
+    const double dx = P.dx, band = (double)P.dx * TRIANGLE_FIELD_BAND_CELLS;
+    const double px = (double)((int)(node % P.node_w) - 1) * dx, py = (double)((int)(node / P.node_w) - 1) * dx;
+    double best = INFINITY, bnx = 0.0, bny = 0.0, bb[3] = {0.0, 0.0, 0.0};
+    uint32_t best_body = 0u, err = 0u, bad_body = NO_AUX;
+    for (uint32_t c0 = 0; c0 < A.nbodies; c0 += FIELD_CHUNK) {
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < FIELD_CHUNK && c0 + b < A.nbodies; b += FIELD_THREADS) {
+            double t[6];
+            load_triangle(in, A.slot[in_buf], c0 + b, t);
+            for (int k = 0; k < 6; ++k) s_tri[b][k] = t[k];
+            s_box[b][0] = fmin(t[0], fmin(t[2], t[4])) - band;
+            s_box[b][1] = fmax(t[0], fmax(t[2], t[4])) + band;
+            s_box[b][2] = fmin(t[1], fmin(t[3], t[5])) - band;
+            s_box[b][3] = fmax(t[1], fmax(t[3], t[5])) + band;
+        }
+        __syncthreads();
+        if (!live) continue;
+        const uint32_t nchunk = min((uint32_t)FIELD_CHUNK, A.nbodies - c0);
+        for (uint32_t b = 0; b < nchunk; ++b) {
+            if (px < s_box[b][0] || px > s_box[b][1] || py < s_box[b][2] || py > s_box[b][3]) continue;
+            double sd, nx, ny, bary[3];
+            if (!tri_sample64(px, py, s_tri[b], sd, nx, ny, bary)) {
+                err |= ERR_TRIANGLE_CONTACT;
+                bad_body = min(bad_body, c0 + b);
+                continue;
+            }
+            if (sd <= band && sd < best) {
+                best = sd;
+                best_body = c0 + b + 1u;
+                bnx = nx;
+                bny = ny;
+                bb[0] = bary[0];
+                bb[1] = bary[1];
+                bb[2] = bary[2];
+            }
+        }
+    }
+    if (live) {
+        A.field_body[node] = best_body;
+        FieldNode fn;
+        fn.dist = best_body ? (float)best : (float)band;
+        fn.nx = (float)bnx;
+        fn.ny = (float)bny;
+        fn.b0 = (float)bb[0];
+        fn.b1 = (float)bb[1];
+        fn.b2 = (float)bb[2];
+        fn.ix = 0.0f;
+        fn.iy = 0.0f;
+        A.field[node] = fn;
+    }
+    if (err) raise_step_at(ctl, err, step, A.aux_pid[3u * bad_body]);
+}
+
+// trianglePairContact (reference/solver.ts:1804-1874) in FP64.
+__device__ bool pair_contact64(const double* Aa, const double* Bb, double skin, double& nx, double& ny) {
+    bool separated = false, have = false;
+    double largest_gap = -INFINITY, smallest_overlap = INFINITY, axx = 0.0, axy = 0.0;
+    for (int ti = 0; ti < 2; ++ti) {
+        const double* T = ti == 0 ? Aa : Bb;
+        for (int e = 0; e < 3; ++e) {
+            const double ex = T[(2 * e + 2) % 6] - T[2 * e], ey = T[(2 * e + 3) % 6] - T[2 * e + 1];
+            const double el = sqrt(ex * ex + ey * ey);
+            if (el <= 1e-10) return false;
+            const double ux = ey / el, uy = -ex / el;
+            double mina = INFINITY, maxa = -INFINITY, minb = INFINITY, maxb = -INFINITY;
+            for (int v = 0; v < 3; ++v) {
+                const double a = Aa[2 * v] * ux + Aa[2 * v + 1] * uy, bv = Bb[2 * v] * ux + Bb[2 * v + 1] * uy;
+                mina = fmin(mina, a);
+                maxa = fmax(maxa, a);
+                minb = fmin(minb, bv);
+                maxb = fmax(maxb, bv);
+            }
+            const double gap = fmax(minb - maxa, mina - maxb);
+            if (gap > 0) {
+                separated = true;
+                if (gap > largest_gap) {
+                    largest_gap = gap;
+                    axx = ux;
+                    axy = uy;
+                    have = true;
+                }
+            } else if (!separated && -gap < smallest_overlap) {
+                smallest_overlap = -gap;
+                axx = ux;
+                axy = uy;
+                have = true;
+            }
+        }
+    }
+    if (separated && largest_gap > skin) return false;
+    if (!have) return false;
+    const double dxc = (Aa[0] + Aa[2] + Aa[4]) / 3 - (Bb[0] + Bb[2] + Bb[4]) / 3;
+    const double dyc = (Aa[1] + Aa[3] + Aa[5]) / 3 - (Bb[1] + Bb[3] + Bb[5]) / 3;
+    const double sign = dxc * axx + dyc * axy >= 0 ? 1.0 : -1.0;
+    nx = sign * axx;
+    ny = sign * axy;
+    const double pen = separated ? skin - largest_gap : skin + smallest_overlap;
+    return pen > 0;
+}
+
+__device__ __forceinline__ void copy_slot(const SlotBuf& b, size_t from, size_t to) {
+    b.pos[to] = b.pos[from];
+    b.vel[to] = b.vel[from];
+    b.C[to] = b.C[from];
+    b.F[to] = b.F[from];
+    b.pid[to] = b.pid[from];
+    b.mat[to] = b.mat[from];
+    b.S[to] = b.S[from];
+    b.plastic[to] = b.plastic[from];
+}
+
+constexpr int BODY_THREADS = 256;
+constexpr uint32_t MAX_BODIES = 1024;  // bodies per scene (checked on the host)
+
+// After G2P: applyTriangleContactImpulses, enforceBondedTriangleConstraints,
+// resolveBondedTriangleContacts (reference/solver.ts:1592-1802;
+// shaders.ts:2535-2955), the CFL check of the triangle vertices deferred from
+// G2P (validateCflParticles, shaders.ts:2423-2454), and moving a vertex whose
+// new position left its bucket.  One block.
+__global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __restrict__ scenes, uint32_t step) {
+    const SceneDesc& S = scenes[0];
+    Control* ctl = S.ctl;
+    if (block_status_set(ctl)) return;
+    const Params& P = S.P;
+    const AuxData& A = S.aux;
+    const int in_buf = (int)((S.start_buf + step) & 1u), out_buf = in_buf ^ 1;
+    const SlotBuf in = S.buf[in_buf];
+    const SlotBuf out = S.buf[out_buf];
+    const uint32_t* in_slot = A.slot[in_buf];
+    uint32_t* out_slot = A.slot[out_buf];
+    const uint32_t nb = A.nbodies, tid = threadIdx.x;
+    __shared__ float s_red[BODY_THREADS][6];
+    __shared__ double s_tri[MAX_BODIES][6];
+    __shared__ float s_vel[MAX_BODIES][2];
+    __shared__ uint32_t s_cnt[MAX_BODIES];
+    uint32_t err = 0u, err_index = NO_AUX;
+    const double dx = P.dx, band = (double)P.dx * TRIANGLE_FIELD_BAND_CELLS;
+    const float inv_mass = 1.0f / P.particle_mass;
+
+    // field impulses on each body, barycentric-weighted, summed in a fixed order
+    for (uint32_t b = 0; b < nb; ++b) {
+        double t[6];
+        load_triangle(in, in_slot, b, t);
+        const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
+        const double mny = fmin(t[1], fmin(t[3], t[5])), mxy = fmax(t[1], fmax(t[3], t[5]));
+        const int x0 = (int)fmax(-1.0, floor((mnx - band) / dx) - 1.0);
+        const int x1 = (int)fmin((double)P.cells_x + 1.0, ceil((mxx + band) / dx) + 1.0);
+        const int y0 = (int)fmax(-1.0, floor((mny - band) / dx) - 1.0);
+        const int y1 = (int)fmin((double)P.cells_y + 1.0, ceil((mxy + band) / dx) + 1.0);
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (x1 >= x0 && y1 >= y0) {
+            const uint32_t w = (uint32_t)(x1 - x0 + 1), count = w * (uint32_t)(y1 - y0 + 1);
+            for (uint32_t i = tid; i < count; i += BODY_THREADS) {
+                const uint32_t node = (uint32_t)(y0 + (int)(i / w) + 1) * P.node_w + (uint32_t)(x0 + (int)(i % w) + 1);
+                if (A.field_body[node] != b + 1u) continue;
+                const FieldNode fn = A.field[node];
+                acc[0] += fn.b0 * fn.ix;
+                acc[1] += fn.b0 * fn.iy;
+                acc[2] += fn.b1 * fn.ix;
+                acc[3] += fn.b1 * fn.iy;
+                acc[4] += fn.b2 * fn.ix;
+                acc[5] += fn.b2 * fn.iy;
+            }
+        }
+        for (int k = 0; k < 6; ++k) s_red[tid][k] = acc[k];
+        __syncthreads();
+        for (uint32_t stride = BODY_THREADS / 2; stride > 0; stride >>= 1) {
+            if (tid < stride)
+                for (int k = 0; k < 6; ++k) s_red[tid][k] += s_red[tid + stride][k];
+            __syncthreads();
+        }
+        if (tid == 0) {
+            for (int v = 0; v < 3; ++v) {
+                const uint32_t sl = out_slot[3u * b + (uint32_t)v];
+                float2 vel = out.vel[sl];
+                vel.x = vel.x + s_red[0][2 * v] * inv_mass;
+                vel.y = vel.y + s_red[0][2 * v + 1] * inv_mass;
+                if (!finite2(vel)) {
+                    err |= ERR_TRIANGLE_CONTACT;
+                    err_index = min(err_index, A.aux_pid[3u * b]);
+                }
+                out.vel[sl] = vel;
+            }
+        }
+        __syncthreads();
+    }
+
+    // two Gauss-Seidel passes over the edges of each triangle, in FP64 offsets
+    // from the first vertex (exact differences of the float32 positions), so
+    // the velocity change displacement / dt keeps no rounding of the large
+    // world coordinates; then confinement
+    const double ratio = (double)P.dt / (double)P.bond_ref_dt;
+    double k = fmin((double)P.bond_stiffness * ratio * ratio, 1.0);
+    k = isfinite(k) ? fmax(0.0, fmin(1.0, k)) : 0.0;
+    const double rest = P.bond_rest;
+    for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
+        uint32_t sl[3];
+        float2 orig[3];
+        double ox[3], oy[3], px[3], py[3];
+        for (int v = 0; v < 3; ++v) {
+            sl[v] = out_slot[3u * b + (uint32_t)v];
+            orig[v] = out.pos[sl[v]];
+        }
+        for (int v = 0; v < 3; ++v) {
+            ox[v] = px[v] = (double)orig[v].x - (double)orig[0].x;
+            oy[v] = py[v] = (double)orig[v].y - (double)orig[0].y;
+        }
+        bool ok = true;
+        for (int it = 0; it < BONDED_TRIANGLE_SOLVER_ITERATIONS && ok; ++it)
+            for (int e = 0; e < 3; ++e) {
+                const int a = e, c = (e + 1) % 3;
+                const double ddx = px[c] - px[a], ddy = py[c] - py[a];
+                const double d = sqrt(ddx * ddx + ddy * ddy);
+                if (!isfinite(d) || d <= 1e-10) {
+                    ok = false;
+                    break;
+                }
+                const double sc = ((d - rest) * k * 0.5) / d;
+                px[a] += ddx * sc;
+                py[a] += ddy * sc;
+                px[c] -= ddx * sc;
+                py[c] -= ddy * sc;
+            }
+        if (!ok) {
+            err |= ERR_TRIANGLE_BOND;
+            err_index = min(err_index, A.aux_pid[3u * b]);
+            continue;
+        }
+        for (int v = 0; v < 3; ++v) {
+            const double dxv = px[v] - ox[v], dyv = py[v] - oy[v];
+            const float2 vel = out.vel[sl[v]];
+            const float2 bx = confine_axis((float)((double)orig[v].x + dxv), (float)((double)vel.x + dxv / P.dt), P.dom_w);
+            const float2 by = confine_axis((float)((double)orig[v].y + dyv), (float)((double)vel.y + dyv / P.dt), P.dom_h);
+            out.pos[sl[v]] = make_float2(bx.x, by.x);
+            out.vel[sl[v]] = make_float2(bx.y, by.y);
+        }
+    }
+    __syncthreads();
+
+    // pair contacts between bodies of different objects: triangles of the
+    // input state, mean velocities after the edge solve
+    if (nb >= 2u) {
+        for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
+            double t[6];
+            load_triangle(in, in_slot, b, t);
+            for (int q = 0; q < 6; ++q) s_tri[b][q] = t[q];
+            float vx = 0.0f, vy = 0.0f;
+            for (int v = 0; v < 3; ++v) {
+                const float2 vel = out.vel[out_slot[3u * b + (uint32_t)v]];
+                vx += vel.x;
+                vy += vel.y;
+            }
+            s_vel[b][0] = vx / 3.0f;
+            s_vel[b][1] = vy / 3.0f;
+        }
+        __syncthreads();
+        const double skin = (double)P.dx * TRIANGLE_CONTACT_SKIN_CELLS;
+        for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
+            uint32_t c = 0u;
+            for (uint32_t j = 0; j < nb; ++j) {
+                if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
+                double nx, ny;
+                if (b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny) : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny))
+                    ++c;
+            }
+            s_cnt[b] = c;
+        }
+        __syncthreads();
+        const float body_mass = 3.0f * P.particle_mass;
+        for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
+            float ix = 0.0f, iy = 0.0f;
+            if (s_cnt[b] != 0u) {
+                for (uint32_t j = 0; j < nb; ++j) {
+                    if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
+                    double nx, ny;
+                    const bool hit = b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
+                                           : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny);
+                    if (!hit) continue;
+                    if (b > j) {
+                        nx = -nx;
+                        ny = -ny;
+                    }
+                    const float w = 1.0f / (float)max(s_cnt[b], s_cnt[j]);
+                    const float rn = (s_vel[b][0] - s_vel[j][0]) * (float)nx + (s_vel[b][1] - s_vel[j][1]) * (float)ny;
+                    if (rn < 0.0f) {
+                        const float imp = -rn / (2.0f / body_mass);
+                        const float iw = w * TRIANGLE_CONTACT_IMPULSE_RELAXATION;
+                        ix += (iw * imp * (float)nx) / body_mass;
+                        iy += (iw * imp * (float)ny) / body_mass;
+                    }
+                }
+            }
+            for (int v = 0; v < 3; ++v) {
+                const uint32_t sl = out_slot[3u * b + (uint32_t)v];
+                const float2 p = out.pos[sl], vel = out.vel[sl];
+                const float2 bx = confine_axis(p.x, vel.x + ix, P.dom_w);
+                const float2 by = confine_axis(p.y, vel.y + iy, P.dom_h);
+                out.pos[sl] = make_float2(bx.x, by.x);
+                out.vel[sl] = make_float2(bx.y, by.y);
+            }
+        }
+        __syncthreads();
+    }
+
+    // CFL check of the vertices (their velocities changed after G2P)
+    if (P.cfl_on >= 0.5f) {
+        for (uint32_t a = tid; a < 3u * nb; a += BODY_THREADS) {
+            const uint32_t sl = out_slot[a];
+            const float2 vel = out.vel[sl];
+            const Material mt = S.mats[out.mat[sl]];
+            const float speed = sqrtf(vel.x * vel.x + vel.y * vel.y);
+            const float ws = wave_speed_bound<true>(P, mt.model, out.F[sl], mt.lambda, mt.mu, mt.density, mt.volume,
+                                                    mt.param);
+            if (!finite2(vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
+                record_failure(ctl, 3u, A.aux_pid[a], speed, ws, P.vel_bound, P.wave_bound);
+                raise_step(ctl, ERR_CFL_BOUND, step);
+            }
+        }
+    }
+    if (err) raise_step_at(ctl, err, step, err_index);
+    __syncthreads();
+
+    // a vertex moved by the contacts keeps the bucket invariant: re-slot it
+    if (tid == 0) {
+        const uint32_t cap = S.cap;
+        for (uint32_t a = 0; a < 3u * nb; ++a) {
+            const uint32_t src = out_slot[a];
+            Stencil st;
+            if (!make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st)) {
+                raise_step(ctl, ERR_STENCIL, step);
+                break;
+            }
+            const uint32_t nbk = bucket_of(st, S.bw), obk = src / cap;
+            if (nbk == obk) continue;
+            const uint32_t kk = out.cnt[nbk];
+            if (kk >= cap) {
+                raise_step(ctl, ERR_INTERNAL_REBUCKET, step);
+                break;
+            }
+            const uint32_t dst = nbk * cap + kk;
+            out.cnt[nbk] = kk + 1u;
+            copy_slot(out, src, dst);
+            const uint32_t last = out.cnt[obk] - 1u;
+            out.cnt[obk] = last;
+            const uint32_t L = obk * cap + last;
+            if (L != src) {
+                copy_slot(out, L, src);
+                const uint32_t moved = A.aux_of[out.pid[src]];
+                if (moved != NO_AUX) out_slot[moved] = src;
+            }
+            out_slot[a] = dst;
+        }
+    }
+}
+
+// updateFractureBonds + applyFractureParticleDamage (shaders.ts:2957-3039;
+// reference/solid-bonds.ts:68-108): bond damage from the tensile strain of
+// the output positions, then each endpoint's damage = max(its damage, its
+// bonds'), and the endpoint's stress cache with that damage.  One block.
+__global__ __launch_bounds__(BODY_THREADS) void k_fracture(const SceneDesc* __restrict__ scenes, uint32_t step) {
+    const SceneDesc& S = scenes[0];
+    Control* ctl = S.ctl;
+    if (block_status_set(ctl)) return;
+    const AuxData& A = S.aux;
+    const int in_buf = (int)((S.start_buf + step) & 1u), out_buf = in_buf ^ 1;
+    const SlotBuf out = S.buf[out_buf];
+    const uint32_t* out_slot = A.slot[out_buf];
+    const uint32_t tid = threadIdx.x;
+    uint32_t err = 0u, err_index = NO_AUX;
+    for (uint32_t k = tid; k < A.nbonds; k += BODY_THREADS) {
+        Bond bd = A.bonds[k];
+        if (bd.broken != 0u) {
+            bd.damage = 1.0f;
+        } else {
+            const float2 p1 = out.pos[out_slot[A.aux_of[bd.first]]], p2 = out.pos[out_slot[A.aux_of[bd.second]]];
+            const float sx = p2.x - p1.x, sy = p2.y - p1.y;
+            const float d = sqrtf(sx * sx + sy * sy);
+            if (!finitef(d)) {
+                err |= ERR_FRACTURE_BOND;
+                err_index = min(err_index, bd.first);
+                continue;
+            }
+            const float strain = fmaxf(0.0f, (d - bd.rest) / bd.rest);
+            bd.damage = fmaxf(bd.damage, fminf(1.0f, strain / bd.failure));
+            if (strain >= bd.failure) {
+                bd.damage = 1.0f;
+                bd.broken = 1u;
+            }
+        }
+        A.bonds[k] = bd;
+    }
+    __syncthreads();
+    for (uint32_t a = A.frac_aux0 + tid; a < A.naux; a += BODY_THREADS) {
+        float dmg = fminf(fmaxf(A.damage[in_buf][a], 0.0f), 1.0f);
+        for (uint32_t q = A.bond_off[a]; q < A.bond_off[a + 1]; ++q) dmg = fmaxf(dmg, A.bonds[A.bond_inc[q]].damage);
+        A.damage[out_buf][a] = dmg;
+        const uint32_t sl = out_slot[a];
+        Material mt = S.mats[out.mat[sl]];
+        mt.param = dmg;
+        const float4 F = out.F[sl];
+        float4 Sc;
+        const uint32_t pf = particle_stress<true>(mt, F, out.plastic[sl], det_comp(F), Sc);
+        out.S[sl] = Sc;
+        if (pf) {
+            if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], A.aux_pid[a]);
+            atomicOr(&ctl->deferred[out_buf], pf);
+        }
+    }
+    if (err) raise_step_at(ctl, err, step, err_index);
 }
 
 // ------------------------------------------------------------- batch end
@@ -1621,6 +2289,13 @@ __global__ __launch_bounds__(256) void k_fix_rollback(const SceneDesc* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->deferred[final_buf] = ctl->deferred[committed];
         ctl->deferred_index[final_buf] = ctl->deferred_index[committed];
+    }
+    if (blockIdx.x == 0) {  // the aux particles keep their slots and damage
+        const AuxData& A = S.aux;
+        for (uint32_t a = threadIdx.x; a < A.naux; a += blockDim.x) {
+            A.slot[final_buf][a] = A.slot[committed][a];
+            A.damage[final_buf][a] = A.damage[committed][a];
+        }
     }
     const size_t total = (size_t)nb * cap;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
@@ -1749,21 +2424,30 @@ hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure) {
         MPM_LAUNCH("particle_sums", k_particle_sums, dim3(parts, ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u);
         MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u, parts, 0u);
     }
+    const bool aux = (sb.scene_flags & SCENE_AUX) != 0u;
     for (uint32_t s = 0; s < steps; ++s) {
+        if (aux && sb.aux_bodies)
+            MPM_LAUNCH("tri_field", k_tri_field, dim3(blocks_for(sb.node_count, FIELD_THREADS)), dim3(FIELD_THREADS),
+                       0, lc.stream, sb.d_scenes, s);
         if (sb.fixed_point) {
             const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
-            auto p2g = general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
-                               : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
+            auto p2g = aux ? k_p2g_fixed<true, true, true>
+                           : general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
+                                     : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
             MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes,
                        sb.p2g_scene, sb.nscenes, s);
         } else {
-            auto p2g = general ? k_p2g_gather<true> : k_p2g_gather<false>;
+            auto p2g = aux ? k_p2g_gather<true, true> : general ? k_p2g_gather<true> : k_p2g_gather<false>;
             MPM_LAUNCH("p2g_grid", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
                        sb.nscenes, s);
         }
-        auto g2p = general ? k_g2p<true> : k_g2p<false>;
+        auto g2p = aux ? k_g2p<true, true> : general ? k_g2p<true> : k_g2p<false>;
         MPM_LAUNCH("g2p", g2p, dim3(sb.g2p_tiles), dim3(G_THREADS), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
                    sb.nscenes, s);
+        if (aux && sb.aux_bodies)
+            MPM_LAUNCH("bodies", k_bodies, dim3(1), dim3(BODY_THREADS), 0, lc.stream, sb.d_scenes, s);
+        if (aux && sb.aux_bonds)
+            MPM_LAUNCH("fracture", k_fracture, dim3(1), dim3(BODY_THREADS), 0, lc.stream, sb.d_scenes, s);
     }
     MPM_LAUNCH("commit_last", k_commit_last, dim3(ns), dim3(64), 0, lc.stream, sb.d_scenes, steps);
     const uint32_t fb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(sb.max_slots, 256), ns == 1 ? 2048u : 64u));
@@ -1792,7 +2476,7 @@ hipError_t launch_bucket(const LaunchCtx& lc, int b, uint32_t first, uint32_t co
     const float dx = lc.cell_size;
     auto kern = scene_general(lc.scene_flags) ? k_bucket<true> : k_bucket<false>;
     MPM_LAUNCH("bucket", kern, dim3(blocks_for(count, 256)), dim3(256), 0, lc.stream, lc.ctl, lc.home, lc.mats,
-               lc.buf[b], first, count, dx, lc.cells_x, lc.cells_y, lc.bw, lc.cap);
+               lc.buf[b], first, count, dx, lc.cells_x, lc.cells_y, lc.bw, lc.cap, lc.aux, b);
     return hipSuccess;
 }
 

@@ -127,6 +127,7 @@ class ParticleArrays:
     F: np.ndarray | None = None
     plastic_volume: np.ndarray | None = None
     object_id: np.ndarray | None = None
+    bonded_vertex: np.ndarray | None = None  # bondedTriangleVertex (0, 1, 2) or -1
 
     def __len__(self) -> int:
         return int(np.asarray(self.x).shape[0])
@@ -155,6 +156,9 @@ class ValidatedInput:
     gradient_scale: float
     gradient_scale_f32: float
     object_ranges: list = field(default_factory=list)
+    bonded_body_count: int = 0
+    fracture_bonds: np.ndarray | None = None     # (bonds, 6) uint32: the 24-byte records
+    fracture_incidence_count: int = 0
 
 
 def _validate_config(config: dict):
@@ -213,13 +217,16 @@ def particle_arrays_from_dicts(particles, default_material: dict, particle_volum
     F = np.zeros((n, 4))
     model = np.zeros(n, np.uint32)
     oid = np.zeros(n, np.uint32)
+    vertex = np.full(n, -1, np.int64)
     for i, p in enumerate(particles):
         if not isinstance(p, dict):
             raise TypeError(f"particles[{i}] must be an object")
         name = f"particles[{i}]"
         mat = default_material if p.get("material") is None else validate_material(p["material"], f"{name}.material")
-        if p.get("bondedTriangleVertex") is not None:
-            raise RangeError(f"{name}.bondedTriangleVertex: bonded triangle bodies are not supported by this solver")
+        bv = p.get("bondedTriangleVertex")
+        if bv is not None and (isinstance(bv, bool) or bv not in (0, 1, 2)):
+            raise RangeError(f"{name}.bondedTriangleVertex must be 0, 1, or 2")
+        vertex[i] = -1 if bv is None else int(bv)
         pos, vel = p.get("position"), p.get("velocity")
         for vec, nm in ((pos, "position"), (vel, "velocity")):
             if not isinstance(vec, dict):
@@ -257,7 +264,8 @@ def particle_arrays_from_dicts(particles, default_material: dict, particle_volum
         cols["param"][i] = material_param(mat, 0.0 if damage is None else damage)
         oid[i] = int(oid_v)
     return ParticleArrays(cols["x"], cols["y"], cols["vx"], cols["vy"], model, cols["lambda_"], cols["mu"],
-                          cols["density"], cols["volume"], cols["param"], C, F, cols["pv"], oid)
+                          cols["density"], cols["volume"], cols["param"], C, F, cols["pv"], oid,
+                          vertex if np.any(vertex >= 0) else None)
 
 
 def _f32_checked(values: np.ndarray, name: str, positive=False) -> np.ndarray:
@@ -312,6 +320,9 @@ def pack_arrays(arr: ParticleArrays, domain_w: float, domain_h: float, domain_w3
         raise RangeError("material.density must remain positive in the supported float32 range")
     fv[:, 17] = _f32_checked(arr.volume, "volume", positive=True)
     fv[:, 18] = _f32_checked(arr.param, "material parameter")
+    if arr.bonded_vertex is not None:
+        bv = np.asarray(arr.bonded_vertex, np.int64)
+        out[:, 19] = np.where(bv >= 0, bv + 1, 0).astype(np.uint32)
     packed_mass = np.where(liquid, fv[:, 16] * fv[:, 17], np.float32(particle_mass_f32))
     if not np.all(np.isfinite(packed_mass)) or np.any(packed_mass <= 0):
         raise RangeError("particle mass is outside the supported float32 range")
@@ -377,23 +388,111 @@ def validate_input(particles, config: dict) -> ValidatedInput:
                 raise RangeError(f"particles with objectId {o} must be contiguous for GPU cloning")
             seen.add(o)
             ranges.append({"objectId": o, "start": int(s_), "count": int(e_ - s_)})
+    # bonded triangle bodies per object range (layout.ts:456-494)
+    vertex = packed[:, 19].astype(np.int64) - 1
+    bodies = 0
+    for r in ranges:
+        v = vertex[r["start"]:r["start"] + r["count"]]
+        if not np.any(v >= 0):
+            continue
+        if bond_rest is None:
+            raise RangeError(f"object {r['objectId']} uses bonded triangles but bondRestLength is missing")
+        if r["count"] % 3 != 0:
+            raise RangeError(f"bonded object {r['objectId']} must contain complete three-particle triangles")
+        if not np.array_equal(v, np.arange(r["count"]) % 3):
+            raise RangeError(f"bonded object {r['objectId']} must store contiguous vertices 0, 1, and 2")
+        m = packed[r["start"]:r["start"] + r["count"], 13]
+        if np.any((m == 2) | (m == 3)):
+            raise RangeError("bonded triangle vertices must use a solid material")
+        bodies += r["count"] // 3
+    if bodies > 0 and bond_rest_f32 <= 0:
+        raise RangeError("bondRestLength must remain positive in float32")
+    fracture = config.get("fractureBonds")
+    bonds_packed, incidence = None, 0
+    if fracture is not None:
+        bonds_packed = pack_fracture_bonds(fracture, packed, oid)
+        incidence = 2 * bonds_packed.shape[0]
     cfg = {"grid": {"cellsX": cells_x, "cellsY": cells_y, "cellSize": cell_size}, "particleMass": particle_mass,
            "particleVolume": particle_volume, "bondStiffness": bond_stiffness, "material": material,
            "wallFriction": wall_friction, "capacity": capacity}
+    if fracture is not None:
+        cfg["fractureBonds"] = fracture
     if bond_rest is not None:
         cfg["bondRestLength"] = bond_rest
     if bond_ref is not None:
         cfg["bondReferenceDt"] = bond_ref
     return ValidatedInput(cfg, n, capacity, packed, oid, domain_w, domain_h, dw32, dh32, cell_size_f32, mass_f32,
                           volume_f32, bond_rest_f32, bond_stiffness_f32, bond_ref_f32, lambda_f32, mu_f32, wall_f32,
-                          gradient_scale, grad_f32, ranges)
+                          gradient_scale, grad_f32, ranges, bodies, bonds_packed, incidence)
+
+
+def pack_fracture_bonds(bonds, packed: np.ndarray, oid: np.ndarray) -> np.ndarray:
+    """validateMpmSolidBondGraph (reference/solid-bonds.ts:11-66) and the
+    24-byte bond records (layout.ts:497-548): u32 first, u32 second,
+    f32 restLength, f32 failureStrain, f32 damage (1 when broken), u32 broken."""
+    if not isinstance(bonds, (list, tuple)):
+        raise TypeError("fractureBonds must be an array")
+    n = packed.shape[0]
+    out = np.zeros((len(bonds), 6), np.uint32)
+    fv = out.view(np.float32)
+    seen = set()
+    for k, b in enumerate(bonds):
+        if not isinstance(b, dict):
+            raise TypeError(f"fractureBonds[{k}] must be an object")
+        ends = []
+        for key in ("firstParticle", "secondParticle"):
+            v = b.get(key)
+            if isinstance(v, bool) or not isinstance(v, int) or v < 0 or v >= n:
+                raise RangeError(f"fractureBonds[{k}].{key} must identify an existing particle")
+            ends.append(v)
+        a, c = ends
+        if a == c:
+            raise RangeError(f"fractureBonds[{k}] must connect two distinct particles")
+        rest = b.get("restLength")
+        if not isinstance(rest, (int, float)) or not math.isfinite(rest) or rest <= 0:
+            raise RangeError(f"fractureBonds[{k}].restLength must be positive and finite")
+        strain = b.get("failureStrain")
+        if not isinstance(strain, (int, float)) or not math.isfinite(strain) or strain <= 0:
+            raise RangeError(f"fractureBonds[{k}].failureStrain must be positive and finite")
+        damage = b.get("damage", 0.0)
+        if not isinstance(damage, (int, float)) or not math.isfinite(damage) or damage < 0 or damage > 1:
+            raise RangeError(f"fractureBonds[{k}].damage must be between zero and one")
+        broken = b.get("broken")
+        if broken is not None and not isinstance(broken, bool):
+            raise TypeError(f"fractureBonds[{k}].broken must be a boolean")
+        if oid[a] != oid[c]:
+            raise RangeError(f"fractureBonds[{k}] must connect particles of one object")
+        if packed[a, 19] != 0 or packed[c, 19] != 0:
+            raise RangeError(f"fractureBonds[{k}] cannot reuse bonded triangle vertices")
+        if packed[a, 13] != 0 or packed[c, 13] != 0:
+            raise RangeError(f"fractureBonds[{k}] endpoints must be neo-hookean solids")
+        pair = (min(a, c), max(a, c))
+        if pair in seen:
+            raise RangeError(f"fractureBonds[{k}] duplicates an existing particle pair")
+        seen.add(pair)
+        out[k, 0], out[k, 1] = a, c
+        fv[k, 2] = finite_float32(rest, f"fractureBonds[{k}].restLength", True)
+        fv[k, 3] = finite_float32(strain, f"fractureBonds[{k}].failureStrain", True)
+        fv[k, 4] = finite_float32(1.0 if broken else damage, f"fractureBonds[{k}].damage")
+        out[k, 5] = 1 if broken else 0
+    return out
+
+
+def unpack_fracture_bonds(records: np.ndarray) -> list[dict]:
+    """24-byte bond records back to MpmSolidBond dicts."""
+    rec = np.asarray(records, np.uint32).reshape(-1, 6)
+    fv = rec.view(np.float32)
+    return [{"firstParticle": int(r[0]), "secondParticle": int(r[1]), "restLength": float(f[2]),
+             "failureStrain": float(f[3]), "damage": float(f[4]), "broken": bool(r[5])} for r, f in zip(rec, fv)]
 
 
 def pack_step_parameters(state: ValidatedInput, dt: float, gravity: dict, particle_count: int | None = None,
-                         bonded_body_count: int = 0, cfl_guard: dict | None = None,
+                         bonded_body_count: int | None = None, cfl_guard: dict | None = None,
                          bond_reference_dt: float | None = None) -> bytes:
     """packMpmMlsGpuStepParameters (layout.ts:896-997): 144 bytes."""
     particle_count = state.particle_count if particle_count is None else particle_count
+    if bonded_body_count is None:
+        bonded_body_count = state.bonded_body_count
     if not isinstance(particle_count, int) or particle_count < 0 or particle_count > state.particle_capacity:
         raise RangeError("particleCount must be an integer within the allocated GPU capacity")
     if particle_count * 9 > INT32_MAX:
@@ -429,7 +528,9 @@ def pack_step_parameters(state: ValidatedInput, dt: float, gravity: dict, partic
         math.sqrt(state.particle_volume_f32 / math.pi) + float(cs) * 0.75,
         wave, vel, 0.0 if cfl_guard is None else 1.0, ref32,
     ], np.float32).tobytes()
-    fracture = struct.pack("<4I", 0, 0, 0, 0)
+    nbonds = 0 if state.fracture_bonds is None else state.fracture_bonds.shape[0]
+    fracture = struct.pack("<4I", state.particle_count if nbonds else 0, nbonds, state.fracture_incidence_count,
+                           1 if 0 < bonded_body_count <= 512 else 0)
     wall = np.array([state.wall_friction_f32, 0, 0, 0], np.float32).tobytes()
     out = ints + floats + fracture + wall
     assert len(out) == SIMULATION_PARAMS_BYTES
@@ -464,5 +565,7 @@ def unpack_particles(records: np.ndarray, object_ids: np.ndarray | None = None) 
             mat["frictionCoefficient"] = fv[i, 18]
         elif name == "neo-hookean":
             p["damage"] = fv[i, 18]
+        if rec[i, 19]:
+            p["bondedTriangleVertex"] = int(rec[i, 19]) - 1
         out.append(p)
     return out

@@ -30,6 +30,8 @@ SIGNATURES = {
     "mpm_read_particles": (ctypes.c_int, [_P, _P, _U32, _P, _P]),
     "mpm_append_particles": (ctypes.c_int, [_P, _P, _P, _U32]),
     "mpm_set_interaction": (ctypes.c_int, [_P, _P]),
+    "mpm_set_fracture_bonds": (ctypes.c_int, [_P, _P, _U32]),
+    "mpm_read_fracture_bonds": (ctypes.c_int, [_P, _P, _U32]),
     "mpm_get_counts": (ctypes.c_int, [_P, _P, _P, _P]),
     "mpm_read_fixed_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_read_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
@@ -189,6 +191,15 @@ class Context:
         packed = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1, 20)
         ids = None if object_ids is None else np.ascontiguousarray(object_ids, dtype=np.uint32)
         self._check(self.lib.mpm_append_particles(self.handle, _ptr(packed), _ptr(ids), packed.shape[0]))
+
+    def set_fracture_bonds(self, records: np.ndarray):
+        rec = np.ascontiguousarray(records, dtype=np.uint32).reshape(-1, 6)
+        self._check(self.lib.mpm_set_fracture_bonds(self.handle, _ptr(rec), rec.shape[0]))
+
+    def read_fracture_bonds(self, count: int) -> np.ndarray:
+        out = np.zeros((count, 6), np.uint32)
+        self._check(self.lib.mpm_read_fracture_bonds(self.handle, _ptr(out), count))
+        return out
 
     def set_interaction(self, values):
         arr = np.zeros(8, np.float32)

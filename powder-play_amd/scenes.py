@@ -15,6 +15,8 @@
 * ``batched_tanks`` — independent Tait water tanks of log-uniform size for
   batched stepping: the SI tank's material and cell size, a dam-break block
   at one particle per cell in a corner, gravity in a random direction.
+* ``gate4_scene`` — the multi-material Gate 4 scene: water, oil, 50 bonded
+  triangle pebbles and a fracture solid (scenes/gate4.ts:184-357).
 * ``create_mpm_emitter_particles`` — the deterministic emitter burst of
   scenes/emitter.ts:49-123, as MpmParticle dicts for append_particles.
 * ``elastic_block`` — the differentiable fixed-corotated block: 64x64
@@ -312,3 +314,83 @@ def create_mpm_emitter_particles(config: dict) -> list[dict]:
                     "F": {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0},
                     "plasticVolume": 1.0, "volume": volume, "material": dict(mat)})
     return out
+
+
+# ---------------------------------------------------------------- Gate 4
+GATE4_FIXED_STEP_SECONDS = 1.0 / 240.0               # gate4.ts:11
+GATE4_DEFAULT_GRAVITY = {"x": 0.0, "y": 90.0}        # gate4.ts:12
+GATE4_PEBBLE_EDGE_LENGTH = 220 * 10 * 0.005 * 12     # gate4.ts:63-67 (132)
+GATE4_PEBBLE_MATERIAL = {"lambda": 20.0, "mu": 10.0, "model": "neo-hookean"}  # gate4.ts:72-76
+
+
+def gate4_scene(seed: int = GATE4_SCENE_SEED, mpm_grid: bool = True, liquid_area: float | None = None):
+    """createMpmGate4Scene + createMpmGate4StepConfig (scenes/gate4.ts:184-357):
+    2,000 water and 1,500 oil particles on 18-unit lattices, 50 three-particle
+    neo-Hookean pebbles (bonded triangles of edge 132, each its own object)
+    and a 6x4 fracture solid with 53 bonds (failure strain 0.12), velocities
+    from the scene LCG (+-64 liquids, +-240 pebble vertices, one +-80 vector
+    for the fracture solid).  mpm_grid=True is the MPM page's configuration
+    (kernels/README.md:116-127): a 220 x 600 grid of cell size 10, liquid
+    reference area 1,600 and the whole scene shifted 4,500 units in +y;
+    False is the 220 x 150 code grid with liquid area 400.
+    Returns (particles as MpmParticle dicts, solver config, gravity)."""
+    area = liquid_area if liquid_area is not None else (1600.0 if mpm_grid else 400.0)
+    shift = 4500.0 if mpm_grid else 0.0
+    cells_y = 600 if mpm_grid else 150
+    state = seed & MASK32
+
+    def random() -> float:
+        nonlocal state
+        state = (state * LCG_A + LCG_C) & MASK32
+        return state / 4294967296.0
+
+    def velocity(vmax):
+        return {"x": (random() * 2 - 1) * vmax, "y": (random() * 2 - 1) * vmax}
+
+    zero = {"m00": 0.0, "m01": 0.0, "m10": 0.0, "m11": 0.0}
+    ident = {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0}
+    particles = []
+    for oid, start_y, rows, mat in ((1, 640.0, 40, GATE4_WATER), (2, 100.0, 30, GATE4_OIL)):  # gate4.ts:95-118
+        for row in range(rows):
+            for col in range(50):
+                particles.append({"position": {"x": 100.0 + col * 18.0, "y": start_y + row * 18.0 + shift},
+                                  "velocity": velocity(64.0), "C": dict(zero), "F": dict(ident), "objectId": oid,
+                                  "plasticVolume": 1.0, "volume": area, "material": dict(mat)})
+    columns, count = 8, 50
+    rows = math.ceil(count / columns)
+    edge = GATE4_PEBBLE_EDGE_LENGTH
+    for k in range(count):
+        row = k // columns
+        in_row = min(columns, count - row * columns)
+        col = k - row * columns + (columns - in_row) * 0.5
+        cx = 700.0 + (col - (columns - 1) / 2) * 150.0
+        cy = 569.0 + (row - (rows - 1) / 2) * 150.0 + shift
+        verts = [(cx - edge * 0.5, cy - edge / (2 * math.sqrt(3))), (cx + edge * 0.5, cy - edge / (2 * math.sqrt(3))),
+                 (cx, cy + edge / math.sqrt(3))]
+        for v, (x, y) in enumerate(verts):
+            particles.append({"position": {"x": x, "y": y}, "velocity": velocity(240.0), "C": dict(zero),
+                              "F": dict(ident), "objectId": 3 + k, "bondedTriangleVertex": v, "plasticVolume": 1.0,
+                              "volume": 100.0, "material": dict(GATE4_PEBBLE_MATERIAL)})
+    fracture_start = len(particles)
+    fvel = velocity(80.0)
+    for row in range(4):
+        for col in range(6):
+            particles.append({"position": {"x": 1520.0 + (col - 2.5) * 20.0, "y": 1380.0 + (row - 1.5) * 20.0 + shift},
+                              "velocity": dict(fvel), "C": dict(zero), "F": dict(ident), "objectId": 3 + count,
+                              "plasticVolume": 1.0, "volume": 100.0, "material": dict(GATE4_PEBBLE_MATERIAL)})
+    bonds = []
+    for row in range(4):
+        for col in range(6):
+            a = fracture_start + row * 6 + col
+            if col + 1 < 6:
+                bonds.append({"firstParticle": a, "secondParticle": a + 1, "restLength": 20.0, "failureStrain": 0.12})
+            if row + 1 < 4:
+                bonds.append({"firstParticle": a, "secondParticle": a + 6, "restLength": 20.0, "failureStrain": 0.12})
+            if row + 1 < 4 and col + 1 < 6:
+                bonds.append({"firstParticle": a, "secondParticle": a + 7, "restLength": 20.0 * math.sqrt(2),
+                              "failureStrain": 0.12})
+    assert len(particles) == 3674 and len(bonds) == 53
+    config = {"grid": {"cellsX": 220, "cellsY": cells_y, "cellSize": 10.0}, "particleMass": 4.0,
+              "particleVolume": 100.0, "bondRestLength": edge, "bondReferenceDt": GATE4_FIXED_STEP_SECONDS,
+              "material": dict(GATE4_PEBBLE_MATERIAL), "fractureBonds": bonds, "capacity": len(particles)}
+    return particles, config, dict(GATE4_DEFAULT_GRAVITY)

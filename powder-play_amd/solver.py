@@ -134,6 +134,8 @@ class MpmMlsGpuSolver:
                         self.state.wall_friction_f32, self.state.particle_capacity, p2g_mode,
                         MPM_FLAG_KEEP_FIXED_GRID if keep_fixed_grid else 0, device, bucket_slots)
         self.ctx = Context(cfg, self.state.packed, self.state.object_ids)
+        if self.state.fracture_bonds is not None:
+            self.ctx.set_fracture_bonds(self.state.fracture_bonds)
         self.p2g_mode = p2g_mode
         self._disposed = False
         self._cfl_state_usable = True
@@ -206,7 +208,7 @@ class MpmMlsGpuSolver:
             if not math.isfinite(cfl_velocity_bound) or cfl_velocity_bound < 0:
                 raise RangeError("velocityBound must be finite and non-negative.")
             guard = {"waveSpeedBound": self.cfl_bounds.waveSpeedBound, "velocityBound": cfl_velocity_bound}
-        params = layout.pack_step_parameters(self.state, dt, gravity, self.allocated_particle_count, 0, guard,
+        params = layout.pack_step_parameters(self.state, dt, gravity, self.allocated_particle_count, None, guard,
                                              bond_reference_dt)
         try:
             self.ctx.step_batch(params, step_count, measure_speed or guard is not None)
@@ -483,6 +485,13 @@ class MpmMlsGpuSolver:
     def end_drag(self):
         self._drag = None
         self.ctx.set_interaction([0.0] * 8)
+
+    def read_fracture_bonds(self) -> list[dict]:
+        """The committed state of the fracture bonds (damage, broken), as the
+        reference's step result reports them (reference/solver.ts:815-819)."""
+        self._assert_active()
+        n = 0 if self.state.fracture_bonds is None else self.state.fracture_bonds.shape[0]
+        return layout.unpack_fracture_bonds(self.ctx.read_fracture_bonds(n))
 
     def dispose(self):
         if not self._disposed:
