@@ -182,6 +182,45 @@ uint32_t mpm_batch_count(mpm_batch* batch);
 void mpm_batch_destroy(mpm_batch* batch);
 const char* mpm_batch_last_error(mpm_batch* batch);
 
+/* ---------------- kernel tuning ----------------
+ * No counterpart in the reference (its WGSL kernels have fixed workgroup
+ * sizes).  The launch shape of the step kernels and the bucket capacity,
+ * for the autotuner (powder-play_amd/autotune.py), which times candidates on
+ * a caller's scene and caches the fastest per (GPU architecture, scene
+ * class).  A step's results do not depend on the tuning: fixed-point P2G
+ * sums are integers, the float gather sums each node in a tile-independent
+ * order (the one exception, a region bucket row holding more particles than
+ * one gather chunk, needs ~8 particles per cell), and G2P is per particle.
+ *
+ * mpm_tuning fields:
+ *   p2g_tile         P2G tile side in grid nodes: 28 (default), 20 or 12.
+ *   g2p_tile         G2P tile side in 4x4-cell buckets: 8 (default) or 4.
+ *   g2p_threads      threads of a G2P block: 128, 256 (default) or 512 for
+ *                    8-bucket tiles; 128 or 256 for 4-bucket tiles.  Particles
+ *                    per wavefront = the tile's particles / (g2p_threads/64).
+ *   bucket_capacity  slots per bucket: 0 keeps the current capacity, else a
+ *                    multiple of 16 in [32, 1024]; it still grows on demand.
+ * mpm_set_tuning: a shape that is not compiled, or a bad capacity, returns
+ *   MPM_EINVAL with no state change; the scene of a batch takes the batch's
+ *   shape (mpm_batch_set_tuning) and returns MPM_EINVAL for another shape.
+ * mpm_get_tuning: the shape the next step runs and the current capacity.
+ *   Scenes with bonded triangle bodies or fracture bonds run the default
+ *   shape; a P2G tile whose gather chunk (448 particles for 12, 960 for 20)
+ *   is below the bucket capacity runs the 28-node tile.
+ * mpm_device_arch: the architecture name (e.g. "gfx950:sramecc+:xnack-")
+ *   and compute-unit count of a device, the autotuner's cache key. */
+typedef struct {
+    uint32_t p2g_tile;
+    uint32_t g2p_tile;
+    uint32_t g2p_threads;
+    uint32_t bucket_capacity;
+} mpm_tuning;
+int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* tuning);
+int mpm_get_tuning(mpm_ctx* ctx, mpm_tuning* out);
+int mpm_batch_set_tuning(mpm_batch* batch, const mpm_tuning* tuning);
+int mpm_batch_get_tuning(mpm_batch* batch, mpm_tuning* out);
+int mpm_device_arch(int32_t device, char* arch, uint32_t arch_bytes, uint32_t* compute_units);
+
 /* Mean kernel time in microseconds of the named kernel over the last
  * mpm_step_batch calls made while timing was enabled (HIP events on the
  * context's stream). enable != 0 turns timing on and resets counters. */

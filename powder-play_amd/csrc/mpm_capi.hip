@@ -74,9 +74,9 @@ struct mpm_ctx {
     uint32_t mat_capacity = 0;
     SceneDesc* h_desc_ring = nullptr; // pinned ring: one descriptor (with its Params) per launch
     SceneDesc* d_scene = nullptr;     // device: this context as a one-scene step launch
-    uint32_t* d_p2g_map = nullptr;    // device: tile -> scene (all 0)
+    uint32_t* d_p2g_map = nullptr;    // device: tile -> scene (all 0; a one-scene launch does not read it)
     uint32_t* d_g2p_map = nullptr;
-    uint32_t p2g_tiles = 0, g2p_tiles = 0;
+    Shape shape = DEFAULT_SHAPE;      // requested launch shape (mpm_set_tuning)
     hipEvent_t ring_events[PARAM_RING] = {};
     bool ring_used[PARAM_RING] = {};
     int ring_next = 0;
@@ -117,6 +117,9 @@ struct mpm_batch {
     std::vector<mpm_ctx*> scenes;
     std::vector<uint32_t> p2g_base, g2p_base;  // first tile of each scene
     uint32_t p2g_tiles = 0, g2p_tiles = 0;
+    Shape shape = DEFAULT_SHAPE;      // requested launch shape (mpm_batch_set_tuning)
+    Shape map_shape{};                // shape the tile maps were built for (zeros: none yet)
+    uint32_t map_capacity = 0;        // entries the device tile maps hold
     SceneDesc* d_scenes = nullptr;   // device
     SceneDesc* h_scenes = nullptr;   // pinned staging
     uint32_t* d_p2g_map = nullptr;   // device: scene of every tile
@@ -456,7 +459,8 @@ int initial_cap(mpm_ctx* c, const uint32_t* rec, uint32_t n) {
 
 // This context as one scene of a step launch with parameters p; the
 // scene's tiles start at the given bases.
-SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint32_t g2p_base, uint32_t seq) {
+SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint32_t g2p_base, uint32_t seq,
+                     const Shape& shape) {
     const LaunchCtx& lc = c->lc;
     SceneDesc d{};
     d.P = p;
@@ -473,10 +477,10 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
     d.bh = lc.bh;
     d.nb = lc.nb;
     d.cap = lc.cap;
-    d.ptx = (lc.node_w + P_TILE - 1) / P_TILE;
-    d.pty = (lc.node_h + P_TILE - 1) / P_TILE;
-    d.gtx = (lc.bw + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS;
-    d.gty = (lc.bh + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS;
+    d.ptx = (lc.node_w + shape.p2g_tile - 1) / shape.p2g_tile;
+    d.pty = (lc.node_h + shape.p2g_tile - 1) / shape.p2g_tile;
+    d.gtx = (lc.bw + shape.g2p_tile - 1) / shape.g2p_tile;
+    d.gty = (lc.bh + shape.g2p_tile - 1) / shape.g2p_tile;
     d.p2g_base = p2g_base;
     d.g2p_base = g2p_base;
     d.scene_flags = lc.scene_flags;
@@ -489,6 +493,15 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
 
 int settle(mpm_ctx* c);
 
+// Shape a step of this context runs with: scenes with bodies or bonds run the
+// default kernels only, and a P2G tile whose gather chunk is smaller than the
+// bucket capacity falls back to the default tile (a bucket must fit a chunk).
+Shape effective_shape(const mpm_ctx* c, Shape s) {
+    if (c->lc.scene_flags & SCENE_AUX) return DEFAULT_SHAPE;
+    if (c->lc.cap > p2g_chunk_for(s.p2g_tile)) s.p2g_tile = DEFAULT_SHAPE.p2g_tile;
+    return s;
+}
+
 int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t done_before) {
     if (c->pending.size() >= MAX_PENDING) {
         // keep the replay list within the control block's history
@@ -499,7 +512,8 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     c->ring_next = (c->ring_next + 1) % PARAM_RING;
     if (c->ring_used[slot]) HIP_TRY(c, hipEventSynchronize(c->ring_events[slot]));
     const uint32_t seq = c->seq++;
-    c->h_desc_ring[slot] = scene_desc(c, p, 0u, 0u, seq);
+    const Shape shape = effective_shape(c, c->shape);
+    c->h_desc_ring[slot] = scene_desc(c, p, 0u, 0u, seq, shape);
     HIP_TRY(c, hipMemcpyAsync(c->d_scene, &c->h_desc_ring[slot], sizeof(SceneDesc), hipMemcpyHostToDevice,
                               c->stream));
     HIP_TRY(c, hipEventRecord(c->ring_events[slot], c->stream));
@@ -511,10 +525,11 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     sb.p2g_scene = c->d_p2g_map;
     sb.g2p_scene = c->d_g2p_map;
     sb.nscenes = 1;
-    sb.p2g_tiles = c->p2g_tiles;
-    sb.g2p_tiles = c->g2p_tiles;
+    sb.p2g_tiles = p2g_tiles_for(c->lc.node_w, c->lc.node_h, shape.p2g_tile);
+    sb.g2p_tiles = g2p_tiles_for(c->lc.bw, c->lc.bh, shape.g2p_tile);
     sb.max_slots = c->lc.nb * c->lc.cap;
-    sb.max_p2g_tiles = c->p2g_tiles;
+    sb.max_p2g_tiles = sb.p2g_tiles;
+    sb.shape = shape;
     sb.scene_flags = d.scene_flags;
     sb.fixed_point = c->lc.fixed_point;
     sb.diag = c->lc.diag != nullptr;
@@ -739,13 +754,14 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     if (keep_fixed) ALLOC(lc.grid.fixed, nodes * 11 * sizeof(int32_t));
     ALLOC(ctx->d_ctl, sizeof(Control));
     ALLOC(ctx->d_scene, sizeof(SceneDesc));
-    ctx->p2g_tiles = ((lc.node_w + P_TILE - 1) / P_TILE) * ((lc.node_h + P_TILE - 1) / P_TILE);
-    ctx->g2p_tiles = ((lc.bw + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS) * ((lc.bh + G_TILE_BUCKETS - 1) / G_TILE_BUCKETS);
-    ALLOC(ctx->d_p2g_map, sizeof(uint32_t) * ctx->p2g_tiles);
-    ALLOC(ctx->d_g2p_map, sizeof(uint32_t) * ctx->g2p_tiles);
+    // tile maps sized for the most tiles any shape makes
+    const uint32_t max_p2g = p2g_tiles_for(lc.node_w, lc.node_h, MIN_P2G_TILE);
+    const uint32_t max_g2p = g2p_tiles_for(lc.bw, lc.bh, 4u);
+    ALLOC(ctx->d_p2g_map, sizeof(uint32_t) * max_p2g);
+    ALLOC(ctx->d_g2p_map, sizeof(uint32_t) * max_g2p);
 #undef ALLOC
-    if ((e = hipMemset(ctx->d_p2g_map, 0, sizeof(uint32_t) * ctx->p2g_tiles)) != hipSuccess ||
-        (e = hipMemset(ctx->d_g2p_map, 0, sizeof(uint32_t) * ctx->g2p_tiles)) != hipSuccess)
+    if ((e = hipMemset(ctx->d_p2g_map, 0, sizeof(uint32_t) * max_p2g)) != hipSuccess ||
+        (e = hipMemset(ctx->d_g2p_map, 0, sizeof(uint32_t) * max_g2p)) != hipSuccess)
         return bail(hip_fail(ctx, e, "hipMemset(tile maps)"));
     if ((e = hipHostMalloc((void**)&ctx->h_desc_ring, sizeof(SceneDesc) * PARAM_RING, hipHostMallocDefault)) !=
         hipSuccess)
@@ -1036,12 +1052,98 @@ int mpm_get_buckets(mpm_ctx* c, uint32_t* buckets_x, uint32_t* buckets_y, uint32
     return MPM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+Shape batch_shape(const mpm_batch* b);
+
+bool tuning_valid(const mpm_tuning* t, std::string& why) {
+    const Shape s = {t->p2g_tile, t->g2p_tile, t->g2p_threads};
+    if (!shape_supported(s)) {
+        why = "unsupported launch shape: p2g_tile " + std::to_string(t->p2g_tile) + ", g2p_tile " +
+              std::to_string(t->g2p_tile) + ", g2p_threads " + std::to_string(t->g2p_threads);
+        return false;
+    }
+    const uint32_t cap = t->bucket_capacity;
+    if (cap != 0u && (cap % 16u != 0u || cap < MIN_BUCKET_CAP || cap > MAX_BUCKET_CAP)) {
+        why = "bucket_capacity must be 0 or a multiple of 16 in [" + std::to_string(MIN_BUCKET_CAP) + ", " +
+              std::to_string(MAX_BUCKET_CAP) + "]";
+        return false;
+    }
+    return true;
+}
+
+// Re-bucket the committed state with `cap` slots per bucket (doubling while a
+// bucket overflows).
+int set_bucket_capacity(mpm_ctx* c, uint32_t cap) {
+    if (cap == 0u || cap == c->lc.cap) return MPM_OK;
+    int s = settle(c);
+    if (s) return s;
+    HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int a = alloc_slots(c, cap);
+    if (a) return a;
+    return rebucket_all(c, cap);
+}
+
+void tuning_of(const Shape& s, uint32_t cap, mpm_tuning* out) {
+    out->p2g_tile = s.p2g_tile;
+    out->g2p_tile = s.g2p_tile;
+    out->g2p_threads = s.g2p_threads;
+    out->bucket_capacity = cap;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_set_tuning(mpm_ctx* c, const mpm_tuning* t) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!t) return fail(c, MPM_EINVAL, "tuning must not be NULL");
+    std::string why;
+    if (!tuning_valid(t, why)) return fail(c, MPM_EINVAL, "%s", why.c_str());
+    const Shape s = {t->p2g_tile, t->g2p_tile, t->g2p_threads};
+    if (c->owner) {
+        const Shape& bs = c->owner->shape;
+        if (bs.p2g_tile != s.p2g_tile || bs.g2p_tile != s.g2p_tile || bs.g2p_threads != s.g2p_threads)
+            return fail(c, MPM_EINVAL, "the scenes of a batch share its launch shape: use mpm_batch_set_tuning");
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MPM_EDEVICE, "hipSetDevice failed");
+    const int r = set_bucket_capacity(c, t->bucket_capacity);
+    if (r) return r;
+    c->shape = s;
+    return MPM_OK;
+}
+
+int mpm_get_tuning(mpm_ctx* c, mpm_tuning* out) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!out) return fail(c, MPM_EINVAL, "out must not be NULL");
+    tuning_of(c->owner ? batch_shape(c->owner) : effective_shape(c, c->shape), c->lc.cap, out);
+    return MPM_OK;
+}
+
+int mpm_device_arch(int32_t device, char* arch, uint32_t arch_bytes, uint32_t* compute_units) {
+    hipDeviceProp_t prop;
+    const hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+        return MPM_EDEVICE;
+    }
+    if (arch && arch_bytes) {
+        strncpy(arch, prop.gcnArchName, arch_bytes - 1);
+        arch[arch_bytes - 1] = '\0';
+    }
+    if (compute_units) *compute_units = (uint32_t)prop.multiProcessorCount;
+    return MPM_OK;
+}
+
 int mpm_set_diagnostics(mpm_ctx* c, uint32_t enable) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     int s = settle(c);
     if (s) return s;
     if (enable && !c->lc.diag) {
-        const uint32_t tiles = ((c->lc.node_w + 27) / 28) * ((c->lc.node_h + 27) / 28);
+        const uint32_t tiles = p2g_tiles_for(c->lc.node_w, c->lc.node_h, MIN_P2G_TILE);  // any shape
         hipError_t e;
         if ((e = c->mem.alloc((void**)&c->lc.diag_grid, sizeof(double) * 3 * tiles)) != hipSuccess ||
             (e = c->mem.alloc((void**)&c->lc.diag_part, sizeof(double) * 3 * DIAG_PART_BLOCKS)) != hipSuccess ||
@@ -1147,6 +1249,65 @@ void mpm_batch_destroy(mpm_batch* b) {
     delete b;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The shape every scene of the batch can run (one kernel variant serves the
+// launch).
+Shape batch_shape(const mpm_batch* b) {
+    Shape s = b->shape;
+    for (const mpm_ctx* c : b->scenes) s = effective_shape(c, s);
+    return s;
+}
+
+// Tiles of the step launches for a shape, scene after scene, and the device
+// tile -> scene maps; rebuilt only when the shape changes.  The caller has
+// made sure no launch of the batch still reads the old maps.
+int batch_maps(mpm_batch* b, const Shape& shape) {
+    if (b->map_shape.p2g_tile == shape.p2g_tile && b->map_shape.g2p_tile == shape.g2p_tile &&
+        b->map_shape.g2p_threads == shape.g2p_threads)
+        return MPM_OK;
+    std::vector<uint32_t> pmap, gmap;
+    b->p2g_base.clear();
+    b->g2p_base.clear();
+    b->p2g_tiles = b->g2p_tiles = 0;
+    for (uint32_t i = 0; i < (uint32_t)b->scenes.size(); ++i) {
+        const mpm_ctx* c = b->scenes[i];
+        const uint32_t pt = p2g_tiles_for(c->lc.node_w, c->lc.node_h, shape.p2g_tile);
+        const uint32_t gt = g2p_tiles_for(c->lc.bw, c->lc.bh, shape.g2p_tile);
+        b->p2g_base.push_back(b->p2g_tiles);
+        b->g2p_base.push_back(b->g2p_tiles);
+        pmap.insert(pmap.end(), pt, i);
+        gmap.insert(gmap.end(), gt, i);
+        b->p2g_tiles += pt;
+        b->g2p_tiles += gt;
+    }
+    hipError_t e;
+    const uint32_t need = (uint32_t)std::max(pmap.size(), gmap.size());
+    if (need > b->map_capacity) {
+        if (b->d_p2g_map) (void)hipFree(b->d_p2g_map);
+        if (b->d_g2p_map) (void)hipFree(b->d_g2p_map);
+        b->d_p2g_map = b->d_g2p_map = nullptr;
+        b->map_capacity = 0;
+        if ((e = hipMalloc((void**)&b->d_p2g_map, sizeof(uint32_t) * need)) != hipSuccess ||
+            (e = hipMalloc((void**)&b->d_g2p_map, sizeof(uint32_t) * need)) != hipSuccess)
+            return batch_fail(b, MPM_EDEVICE, "batch tile maps: %s", hipGetErrorString(e));
+        b->map_capacity = need;
+    }
+    if ((e = hipMemcpy(b->d_p2g_map, pmap.data(), sizeof(uint32_t) * pmap.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(b->d_g2p_map, gmap.data(), sizeof(uint32_t) * gmap.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+        return batch_fail(b, MPM_EDEVICE, "batch tile maps: %s", hipGetErrorString(e));
+    b->map_shape = shape;
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int mpm_batch_create(const mpm_config* configs, const uint32_t* const* packed20, const uint32_t* counts,
                      uint32_t nscenes, mpm_batch** out) {
     if (!out) return batch_fail(nullptr, MPM_EINVAL, "out must not be NULL");
@@ -1214,28 +1375,13 @@ int mpm_batch_create(const mpm_config* configs, const uint32_t* const* packed20,
             }
         }
     }
-    // tiles of the step launches, scene after scene
-    std::vector<uint32_t> pmap, gmap;
-    for (uint32_t i = 0; i < nscenes; ++i) {
-        mpm_ctx* c = b->scenes[i];
-        b->p2g_base.push_back(b->p2g_tiles);
-        b->g2p_base.push_back(b->g2p_tiles);
-        pmap.insert(pmap.end(), c->p2g_tiles, i);
-        gmap.insert(gmap.end(), c->g2p_tiles, i);
-        b->p2g_tiles += c->p2g_tiles;
-        b->g2p_tiles += c->g2p_tiles;
-    }
     if ((e = hipMalloc((void**)&b->d_scenes, sizeof(SceneDesc) * nscenes)) != hipSuccess ||
-        (e = hipMalloc((void**)&b->d_p2g_map, sizeof(uint32_t) * pmap.size())) != hipSuccess ||
-        (e = hipMalloc((void**)&b->d_g2p_map, sizeof(uint32_t) * gmap.size())) != hipSuccess ||
-        (e = hipHostMalloc((void**)&b->h_scenes, sizeof(SceneDesc) * nscenes, hipHostMallocDefault)) != hipSuccess ||
-        (e = hipMemcpy(b->d_p2g_map, pmap.data(), sizeof(uint32_t) * pmap.size(), hipMemcpyHostToDevice)) !=
-            hipSuccess ||
-        (e = hipMemcpy(b->d_g2p_map, gmap.data(), sizeof(uint32_t) * gmap.size(), hipMemcpyHostToDevice)) !=
-            hipSuccess) {
+        (e = hipHostMalloc((void**)&b->h_scenes, sizeof(SceneDesc) * nscenes, hipHostMallocDefault)) != hipSuccess) {
         batch_fail(b, MPM_EDEVICE, "batch buffers: %s", hipGetErrorString(e));
         return bail(MPM_EDEVICE);
     }
+    const int m = batch_maps(b, batch_shape(b));
+    if (m) return bail(m);
     *out = b;
     return MPM_OK;
 }
@@ -1268,6 +1414,15 @@ int mpm_batch_step(mpm_batch* b, const void* params, uint32_t step_count, uint32
     }
     if (b->staged_used && hipEventSynchronize(b->staged) != hipSuccess)
         return batch_fail(b, MPM_EDEVICE, "hipEventSynchronize failed");
+    const Shape shape = batch_shape(b);
+    if (!(shape.p2g_tile == b->map_shape.p2g_tile && shape.g2p_tile == b->map_shape.g2p_tile &&
+          shape.g2p_threads == b->map_shape.g2p_threads)) {
+        // launches already queued read the old maps: let them finish first
+        if (hipStreamSynchronize(b->stream) != hipSuccess)
+            return batch_fail(b, MPM_EDEVICE, "hipStreamSynchronize failed");
+        const int m = batch_maps(b, shape);
+        if (m) return m;
+    }
     StepBatch sb{};
     sb.stream = b->stream;
     sb.d_scenes = b->d_scenes;
@@ -1277,15 +1432,16 @@ int mpm_batch_step(mpm_batch* b, const void* params, uint32_t step_count, uint32
     sb.p2g_tiles = b->p2g_tiles;
     sb.g2p_tiles = b->g2p_tiles;
     sb.fixed_point = b->scenes[0]->lc.fixed_point;
+    sb.shape = shape;
     for (uint32_t i = 0; i < n; ++i) {
         mpm_ctx* c = b->scenes[i];
         const uint32_t seq = c->seq++;
-        b->h_scenes[i] = scene_desc(c, ps[i], b->p2g_base[i], b->g2p_base[i], seq);
+        b->h_scenes[i] = scene_desc(c, ps[i], b->p2g_base[i], b->g2p_base[i], seq, shape);
         c->pending.push_back({ps[i], step_count, measure, seq, 0u});
         c->last_done_before = 0u;
         c->cur = (c->cur + (int)(step_count & 1u)) & 1;
         sb.max_slots = std::max(sb.max_slots, c->lc.nb * c->lc.cap);
-        sb.max_p2g_tiles = std::max(sb.max_p2g_tiles, c->p2g_tiles);
+        sb.max_p2g_tiles = std::max(sb.max_p2g_tiles, p2g_tiles_for(c->lc.node_w, c->lc.node_h, shape.p2g_tile));
         sb.scene_flags |= c->lc.scene_flags;
         sb.diag = sb.diag || c->lc.diag != nullptr;
     }
@@ -1297,6 +1453,29 @@ int mpm_batch_step(mpm_batch* b, const void* params, uint32_t step_count, uint32
     b->staged_used = true;
     if ((e = launch_steps(sb, step_count, measure)) != hipSuccess)
         return batch_fail(b, MPM_EDEVICE, "batch launch: %s", hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int mpm_batch_set_tuning(mpm_batch* b, const mpm_tuning* t) {
+    if (!b) return batch_fail(nullptr, MPM_EINVAL, "batch must not be NULL");
+    if (!t) return batch_fail(b, MPM_EINVAL, "tuning must not be NULL");
+    std::string why;
+    if (!tuning_valid(t, why)) return batch_fail(b, MPM_EINVAL, "%s", why.c_str());
+    if (hipSetDevice(b->device) != hipSuccess) return batch_fail(b, MPM_EDEVICE, "hipSetDevice failed");
+    for (uint32_t i = 0; i < (uint32_t)b->scenes.size(); ++i) {
+        const int r = set_bucket_capacity(b->scenes[i], t->bucket_capacity);
+        if (r) return batch_fail(b, r, "scene %u: %s", i, b->scenes[i]->error.c_str());
+    }
+    b->shape = {t->p2g_tile, t->g2p_tile, t->g2p_threads};
+    return MPM_OK;
+}
+
+int mpm_batch_get_tuning(mpm_batch* b, mpm_tuning* out) {
+    if (!b) return batch_fail(nullptr, MPM_EINVAL, "batch must not be NULL");
+    if (!out) return batch_fail(b, MPM_EINVAL, "out must not be NULL");
+    uint32_t cap = 0;
+    for (const mpm_ctx* c : b->scenes) cap = std::max(cap, c->lc.cap);
+    tuning_of(batch_shape(b), cap, out);
     return MPM_OK;
 }
 

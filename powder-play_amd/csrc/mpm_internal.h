@@ -193,6 +193,31 @@ struct LaunchCtx {
 };
 constexpr uint32_t DIAG_PART_BLOCKS = 1024;  // blocks of the particle-sum kernel
 
+// Launch shape of the step kernels: the P2G tile side in grid nodes, the
+// G2P tile side in buckets and the threads of a G2P block (particles per
+// wavefront = particles of a G2P tile / its waves).  Every supported shape is
+// compiled; a step's results do not depend on the shape.
+struct Shape {
+    uint32_t p2g_tile, g2p_tile, g2p_threads;
+};
+constexpr Shape DEFAULT_SHAPE = {28u, 8u, 256u};
+constexpr uint32_t MIN_P2G_TILE = 12u;  // the most P2G tiles a grid can have
+// Particles the float-gather P2G stages per chunk, for each tile side: the
+// particles of one bucket must fit one chunk.
+constexpr uint32_t p2g_chunk_for(uint32_t tile) { return tile == 28u ? 1408u : (tile == 20u ? 960u : 448u); }
+inline bool shape_supported(const Shape& s) {
+    const bool p = s.p2g_tile == 28u || s.p2g_tile == 20u || s.p2g_tile == 12u;
+    const bool g = (s.g2p_tile == 8u && (s.g2p_threads == 128u || s.g2p_threads == 256u || s.g2p_threads == 512u)) ||
+                   (s.g2p_tile == 4u && (s.g2p_threads == 128u || s.g2p_threads == 256u));
+    return p && g;
+}
+inline uint32_t p2g_tiles_for(uint32_t node_w, uint32_t node_h, uint32_t tile) {
+    return ((node_w + tile - 1) / tile) * ((node_h + tile - 1) / tile);
+}
+inline uint32_t g2p_tiles_for(uint32_t bw, uint32_t bh, uint32_t tile) {
+    return ((bw + tile - 1) / tile) * ((bh + tile - 1) / tile);
+}
+
 // One scene of a step launch, in device memory.  The step kernels of a
 // launch cover the tiles of every scene; a block finds its scene through a
 // tile -> scene map and reads everything scene-specific from here.
@@ -229,12 +254,10 @@ struct StepBatch {
     uint32_t fixed_point;
     uint32_t aux_bodies, aux_bonds;  // SCENE_AUX (one scene): body and bond counts
     uint32_t node_count;             // SCENE_AUX: the scene's grid nodes
+    Shape shape;                     // launch shape (DEFAULT_SHAPE for SCENE_AUX)
     bool diag;                   // some scene has diagnostics on
     KernelTimes* timing;
 };
-
-constexpr int P_TILE = 28;      // interior nodes per P2G tile side
-constexpr int G_TILE_BUCKETS = 8;  // buckets per G2P tile side
 
 hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure);
 // Bucket particles [first, first+count) of the home arrays into buffer b

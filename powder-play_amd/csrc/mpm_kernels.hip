@@ -53,29 +53,49 @@ constexpr float TRIANGLE_CONTACT_SKIN_CELLS = 0.1f;
 constexpr float TRIANGLE_CONTACT_IMPULSE_RELAXATION = 0.25f;
 constexpr float TRIANGLE_FLUID_TANGENTIAL_DRAG_RATE = 4.0f;
 constexpr int BONDED_TRIANGLE_SOLVER_ITERATIONS = 2;
-constexpr float PI_F = 3.14159265358979f;
 
-// P2G tile: 28x28 interior nodes (7x7 buckets), a one-node ring for the
-// liquid-contact volume gradient (30x30 computed), fed by 9x9 buckets.
-constexpr int P_TI = P_TILE;                    // interior nodes per tile side
-constexpr int P_TC = P_TI + 2;                  // with the one-node ring
-constexpr int P_RB = P_TI / BUCKET_CELLS + 2;   // region buckets per side
-constexpr int P_NB = P_RB * P_RB;
-constexpr int P_RC = P_RB * BUCKET_CELLS;       // region cells per side
-constexpr int P_CELLS = P_RC * P_RC;
-constexpr int P_THREADS = 1024;
-constexpr int G_CAP = 1408;                     // staged particles per gather chunk
-static_assert(G_CAP <= 2 * P_THREADS, "the gather loads at most two particles a thread");
-static_assert(P_TC * P_TC <= G_CAP, "the ring volumes reuse two staging arrays");
+// P2G tile of TI x TI interior nodes (TI/4 buckets a side) and a one-node
+// ring for the liquid-contact volume gradient ((TI+2)^2 nodes computed), fed
+// by the (TI/4 + 2)^2 buckets around it.  Three sides are compiled:
+//   28: 1024 threads, two tiles a CU (LDS), 32 node threads to a row
+//   20:  512 threads, three tiles a CU
+//   12:  256 threads, six tiles a CU
+// Smaller tiles re-read more region buckets (the ring of buckets around a
+// tile) but keep more blocks in flight per CU.
+template <int TI_>
+struct P2GTile {
+    static constexpr int TI = TI_;                     // interior nodes per tile side
+    static constexpr int TC = TI + 2;                  // with the one-node ring
+    static constexpr int OWN = TI / BUCKET_CELLS;      // buckets a tile owns per side
+    static constexpr int RB = OWN + 2;                 // region buckets per side
+    static constexpr int NB = RB * RB;
+    static constexpr int RC = RB * BUCKET_CELLS;       // region cells per side
+    static constexpr int CELLS = RC * RC;
+    static constexpr int THREADS = TI == 28 ? 1024 : (TI == 20 ? 512 : 256);
+    static constexpr int CAP = (int)p2g_chunk_for(TI);  // staged particles per gather chunk
+    static constexpr int ROW = (TC <= 32 && 32 * TC <= THREADS) ? 32 : TC;  // node threads per tile row
+    // waves a SIMD keeps of the gather: its LDS admits 2 x 16, 3 x 8 or 6 x 4
+    // waves a CU, so the smaller tiles may use 80 registers instead of 64
+    static constexpr int GATHER_WAVES_PER_EU = TI == 28 ? 8 : 6;
+    static_assert(TI == 28 || TI == 20 || TI == 12, "compiled tile sides");
+    static_assert(TI % BUCKET_CELLS == 0, "tiles hold whole buckets");
+    static_assert(CAP <= 2 * THREADS, "the gather loads at most two particles a thread");
+    static_assert(TC * TC <= CAP, "the ring volumes reuse two staging arrays");
+    static_assert(ROW * TC <= THREADS, "one node thread per tile node");
+    static_assert(CELLS <= 2 * THREADS, "the cell scan takes two cells a thread");
+    static_assert(NB <= 128, "the bucket scan takes two buckets a lane of one wave");
+    static_assert(OWN * OWN <= THREADS, "one thread clears each owned bucket count");
+};
 constexpr int LDS_MATS = 64;                    // material rows cached in LDS
-static_assert(P_TC <= 32 && P_TC * 32 <= P_THREADS, "one node thread per tile node, 32 to a row");
-static_assert(P_CELLS <= 2 * P_THREADS, "the cell scan takes two cells a thread");
-static_assert(P_NB <= 128, "the bucket scan takes two buckets a lane of one wave");
-// G2P tile: 8x8 buckets (32x32 cells) and their 34x34 node patch.
-constexpr int G_TB = G_TILE_BUCKETS;
-constexpr int G_TC = G_TB * BUCKET_CELLS;
-constexpr int G_TN = G_TC + 2;
-constexpr int G_THREADS = 256;
+// G2P tile: TB x TB buckets (4TB x 4TB cells) and their (4TB+2)^2 node patch.
+template <int TB_>
+struct G2PTile {
+    static constexpr int TB = TB_;
+    static constexpr int TC = TB * BUCKET_CELLS;
+    static constexpr int TN = TC + 2;
+    static constexpr int NBK = TB * TB;  // buckets of the tile
+    static_assert(NBK <= 64, "one wave scans the bucket counts");
+};
 
 __device__ __forceinline__ bool finitef(float v) { return (__float_as_uint(v) & 0x7f800000u) != 0x7f800000u; }
 __device__ __forceinline__ bool finite2(float2 v) { return finitef(v.x) && finitef(v.y); }
@@ -530,7 +550,7 @@ __device__ __forceinline__ bool weights_valid(float fx, float fy) {
 // waves in index order); thread 0 writes them to out[0..2].  Every thread of
 // the block calls it.
 __device__ void block_sum3_f64(double a, double b, double c, double* out) {
-    __shared__ double s_part[3][P_THREADS / 64];
+    __shared__ double s_part[3][16];  // blocks of up to 1024 threads
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off, 64);
@@ -698,6 +718,7 @@ __device__ __forceinline__ uint32_t fluid_triangle_node(const Params& P, const A
 // clearing the next bucket counts this tile owns, and the staged offsets of
 // the 9x9 region buckets (s_boff, one wave, two buckets a lane).  Returns
 // false when a flag is already raised (the whole block returns).
+template <class T>
 __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32_t* cnt_next, uint32_t tile,
                                          uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap, uint32_t step,
                                          int in_buf, uint32_t* s_boff) {
@@ -717,20 +738,20 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
         ctl->deferred_index[out_buf] = 0xffffffffu;
     }
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
-    // this tile owns buckets [7tx, 7tx+7) x [7ty, 7ty+7): clear their next counts
-    if (tid < 49) {
-        const int obx = 7 * tx + tid % 7, oby = 7 * ty + tid / 7;
+    // this tile owns buckets [OWN tx, OWN tx + OWN) x [OWN ty, OWN ty + OWN): clear their next counts
+    if (tid < T::OWN * T::OWN) {
+        const int obx = T::OWN * tx + tid % T::OWN, oby = T::OWN * ty + tid / T::OWN;
         if (obx < (int)bw && oby < (int)bh) cnt_next[(uint32_t)oby * bw + (uint32_t)obx] = 0u;
     }
-    const int rbx0 = tx * P_TI / BUCKET_CELLS - 1, rby0 = ty * P_TI / BUCKET_CELLS - 1;
+    const int rbx0 = tx * T::OWN - 1, rby0 = ty * T::OWN - 1;
     if (tid < 64) {
         uint32_t c[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int b = tid + 64 * k;
-            const int gbx = rbx0 + b % P_RB, gby = rby0 + b / P_RB;
+            const int gbx = rbx0 + b % T::RB, gby = rby0 + b / T::RB;
             c[k] = 0u;
-            if (b < P_NB && gbx >= 0 && gby >= 0 && gbx < (int)bw && gby < (int)bh)
+            if (b < T::NB && gbx >= 0 && gby >= 0 && gbx < (int)bw && gby < (int)bh)
                 c[k] = min(in.cnt[(uint32_t)gby * bw + (uint32_t)gbx], cap);
         }
         uint32_t i0 = c[0], i1 = c[1];
@@ -740,9 +761,9 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
             if (tid >= off) { i0 += t0; i1 += t1; }
         }
         const uint32_t tot0 = __shfl(i0, 63, 64), tot1 = __shfl(i1, 63, 64);
-        s_boff[tid] = i0 - c[0];
-        if (tid + 64 < P_NB) s_boff[tid + 64] = tot0 + i1 - c[1];
-        if (tid == 0) s_boff[P_NB] = tot0 + tot1;
+        if (tid < T::NB) s_boff[tid] = i0 - c[0];
+        if (tid + 64 < T::NB) s_boff[tid + 64] = tot0 + i1 - c[1];
+        if (tid == 0) s_boff[T::NB] = tot0 + tot1;
     }
     return true;
 }
@@ -751,11 +772,12 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
 // nothing, and no particle is within reach of them in this microstep's G2P,
 // so their velocities are not rewritten.  The kept fixed-point words and
 // the diagnostics partial still read zero.
+template <int TI>
 __device__ void p2g_empty_tile(const Params& P, const Grid& grid, uint32_t keep_fixed, double* diag_grid,
                                uint32_t tile, int X0, int Y0) {
     if (keep_fixed) {
-        for (int e = threadIdx.x; e < P_TI * P_TI; e += blockDim.x) {
-            const int sx = X0 + e % P_TI, sy = Y0 + e / P_TI;
+        for (int e = threadIdx.x; e < TI * TI; e += blockDim.x) {
+            const int sx = X0 + e % TI, sy = Y0 + e / TI;
             if (sx < (int)P.node_w && sy < (int)P.node_h) {
                 const size_t node = (size_t)sy * P.node_w + (size_t)sx;
 #pragma unroll
@@ -780,8 +802,9 @@ __device__ __forceinline__ int region_bucket(const uint32_t* s_boff, int b0, int
     return lo;
 }
 
-// Exclusive scan of s[0, n) in place (n <= 2 * P_THREADS); s[n] receives the
+// Exclusive scan of s[0, n) in place (n <= 2 * THREADS); s[n] receives the
 // total.  Every thread of the block must call it; it ends with a barrier.
+template <int THREADS>
 __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i0 = 2 * tid;
@@ -796,7 +819,7 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     if (wave == 0) {
-        constexpr int WAVES = P_THREADS / 64;
+        constexpr int WAVES = THREADS / 64;
         const uint32_t w = lane < WAVES ? s_wave[lane] : 0u;
         uint32_t wi = w;
 #pragma unroll
@@ -811,27 +834,33 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
     const uint32_t base = s_wave[wave] + incl - pair;
     if (i0 < n) s[i0] = base;
     if (i0 + 1 < n) s[i0 + 1] = base + a;
-    if (tid == 0) s[n] = s_wave[P_THREADS / 64];
+    if (tid == 0) s[n] = s_wave[THREADS / 64];
     __syncthreads();
 }
 
 // Float P2G: LDS-staged gather (p2g, shaders.ts:1072-1211) fused with
-// updateGrid (:1485-1799).  One 1024-thread block per 28x28-node tile: it
-// gathers the tile and a one-node ring of complete sums, then updates the
-// interior (the ring supplies the volume gradient).
+// updateGrid (:1485-1799).  One block per TI x TI-node tile: it gathers the
+// tile and a one-node ring of complete sums, then updates the interior (the
+// ring supplies the volume gradient).
 //
-// The tile's particles are those of the 9x9 buckets around it (a 36x36-cell
-// region).  They are staged in LDS in region-row-major cell order, ascending
-// particle index inside a cell, so each node sums its three stencil rows as
-// three contiguous ranges in a fixed order (rows, then cells, then particle
-// index): the float sums are reproducible run to run.  A region holding more
-// than G_CAP particles is processed in chunks of whole bucket rows (single
-// buckets only when one bucket row exceeds G_CAP); the sums carry across
-// chunks.  The LDS footprint stays under half a CU so two tiles share one.
-template <bool GENERAL, bool AUX = false>
-__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+// The tile's particles are those of the region buckets around it (for TI =
+// 28, 9x9 buckets: a 36x36-cell region).  They are staged in LDS in
+// region-row-major cell order, ascending particle index inside a cell, so
+// each node sums its three stencil rows as three contiguous ranges in a fixed
+// order (rows, then cells, then particle index).  That order does not depend
+// on the tile: the float sums are the same run to run and for every tile
+// side.  A region holding more than CAP particles is processed in chunks of
+// whole bucket rows, which keeps the order; only a single bucket row above
+// CAP is split into single buckets, and then a node whose stencil row spans
+// two buckets of different chunks adds them bucket by bucket.  The sums carry
+// across chunks.
+template <int TI, bool GENERAL, bool AUX = false>
+__global__ __launch_bounds__(P2GTile<TI>::THREADS)
+__attribute__((amdgpu_waves_per_eu(P2GTile<TI>::GATHER_WAVES_PER_EU, 8)))
 void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                   uint32_t step) {
+    using T = P2GTile<TI>;
+    constexpr int G_CAP = T::CAP;
     SCENE_PROLOGUE(tile_scene, p2g_base);
     const int in_buf = (int)((S.start_buf + step) & 1u);
     const Params* __restrict__ Pp = &S.P;
@@ -842,12 +871,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     const Grid grid = S.grid;
     const uint32_t nmat = S.nmat, tiles_x = S.ptx, bw = S.bw, bh = S.bh, cap = S.cap, scene_flags = S.scene_flags;
     double* diag_grid = S.diag_grid;
-    __shared__ uint32_t s_boff[P_NB + 1];    // staged offset of each region bucket
-    __shared__ uint32_t s_chunk[P_NB + 1];   // chunk boundaries (region bucket indices)
+    __shared__ uint32_t s_boff[T::NB + 1];    // staged offset of each region bucket
+    __shared__ uint32_t s_chunk[T::NB + 1];   // chunk boundaries (region bucket indices)
     __shared__ uint32_t s_nchunks;
-    __shared__ uint32_t s_wave[P_THREADS / 64 + 1];
+    __shared__ uint32_t s_wave[T::THREADS / 64 + 1];
     __shared__ Material s_mat[LDS_MATS];
-    __shared__ uint32_t s_cell[P_CELLS + 1]; // per region cell: count, then start
+    __shared__ uint32_t s_cell[T::CELLS + 1]; // per region cell: count, then start
     // staged particle records in gather order
     __shared__ float s_fx[G_CAP], s_fy[G_CAP], s_m[G_CAP], s_mvx[G_CAP], s_mvy[G_CAP];
     __shared__ float s_a0[G_CAP], s_a1[G_CAP], s_a2[G_CAP], s_a3[G_CAP], s_vd[G_CAP];
@@ -855,27 +884,27 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     __shared__ uint32_t s_pid[G_CAP];  // particle indices at provisional places
 
     stage_materials(s_mat, mats, nmat);
-    if (!p2g_head(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    if (!p2g_head<T>(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     const int tid = threadIdx.x;
     const Params& P = *Pp;
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
-    const int lx = tid % 32, ly = tid / 32;
-    const int X0 = tx * P_TI, Y0 = ty * P_TI;
+    const int lx = tid % T::ROW, ly = tid / T::ROW;
+    const int X0 = tx * TI, Y0 = ty * TI;
     const int W = (int)P.node_w, H = (int)P.node_h;
     const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
-    const bool node_ok = lx < P_TC && ly < P_TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
+    const bool node_ok = lx < T::TC && ly < T::TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
     __syncthreads();
-    if (s_boff[P_NB] == 0u) {
-        p2g_empty_tile(P, grid, 0u, diag_grid, tile, X0, Y0);
+    if (s_boff[T::NB] == 0u) {
+        p2g_empty_tile<TI>(P, grid, 0u, diag_grid, tile, X0, Y0);
         return;
     }
     if (tid == 0) {
         uint32_t n = 0;
         s_chunk[0] = 0u;
-        if (s_boff[P_NB] > (uint32_t)G_CAP) {
-            for (int r = 0; r < P_RB; ++r) {
-                const int lo = r * P_RB, hi = lo + P_RB;
+        if (s_boff[T::NB] > (uint32_t)G_CAP) {
+            for (int r = 0; r < T::RB; ++r) {
+                const int lo = r * T::RB, hi = lo + T::RB;
                 if (s_boff[hi] - s_boff[s_chunk[n]] <= (uint32_t)G_CAP) continue;
                 if ((int)s_chunk[n] < lo) s_chunk[++n] = (uint32_t)lo;
                 if (s_boff[hi] - s_boff[lo] > (uint32_t)G_CAP) {
@@ -885,7 +914,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 }
             }
         }
-        s_chunk[++n] = P_NB;
+        s_chunk[++n] = T::NB;
         s_nchunks = n;
     }
     __syncthreads();
@@ -897,30 +926,30 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     const float dx = P.dx;
     const uint32_t nchunks = s_nchunks;
     const int cx0 = rbx0 * BUCKET_CELLS, cy0 = rby0 * BUCKET_CELLS;  // first region cell (stencil-base coords)
-    constexpr int PER = G_CAP / P_THREADS + (G_CAP % P_THREADS ? 1 : 0);  // staged particles a thread
+    constexpr int PER = G_CAP / T::THREADS + (G_CAP % T::THREADS ? 1 : 0);  // staged particles a thread
 
     for (uint32_t ch = 0; ch < nchunks; ++ch) {
         const int b0 = (int)s_chunk[ch], b1 = (int)s_chunk[ch + 1];
         const uint32_t c0 = s_boff[b0], c1 = s_boff[b1];
-        for (int e = tid; e <= P_CELLS; e += P_THREADS) s_cell[e] = 0u;
+        for (int e = tid; e <= T::CELLS; e += T::THREADS) s_cell[e] = 0u;
         __syncthreads();
         // (A) positions: region cell of each particle, counted per cell
         uint32_t cell[PER], pid[PER], gi[PER];  // cell: region cell << 16 | place in the cell
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const uint32_t j = c0 + (uint32_t)tid + (uint32_t)(k * P_THREADS);
+            const uint32_t j = c0 + (uint32_t)tid + (uint32_t)(k * T::THREADS);
             cell[k] = 0xffffffffu;
             pid[k] = 0u; gi[k] = 0u;
             if (j >= c1) continue;
             const int rb = region_bucket(s_boff, b0, b1, j);
-            const int gbx = rbx0 + rb % P_RB, gby = rby0 + rb / P_RB;
+            const int gbx = rbx0 + rb % T::RB, gby = rby0 + rb / T::RB;
             gi[k] = ((uint32_t)gby * bw + (uint32_t)gbx) * cap + (j - s_boff[rb]);
             const float2 p = in.pos[gi[k]];
             pid[k] = in.pid[gi[k]];
             Stencil sc;
             if (make_stencil(P.dx, P.cells_x, P.cells_y, p, sc) && ((sc.bx + 1) >> 2) == gbx &&
                 ((sc.by + 1) >> 2) == gby) {
-                const uint32_t rc = (uint32_t)((sc.by + 1 - cy0) * P_RC + (sc.bx + 1 - cx0));
+                const uint32_t rc = (uint32_t)((sc.by + 1 - cy0) * T::RC + (sc.bx + 1 - cx0));
                 cell[k] = rc << 16 | atomicAdd(&s_cell[rc], 1u);
             } else {
                 flags |= ERR_STENCIL;
@@ -928,7 +957,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
         }
         __syncthreads();
         // (B) cell starts
-        block_exclusive_scan(s_cell, P_CELLS, s_wave);
+        block_exclusive_scan<T::THREADS>(s_cell, T::CELLS, s_wave);
         // (C) particle indices at their provisional places
 #pragma unroll
         for (int k = 0; k < PER; ++k)
@@ -981,7 +1010,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             s_mvy[q] = mt.mass * v.y;
             s_a0[q] = A.x; s_a1[q] = A.y; s_a2[q] = A.z; s_a3[q] = A.w;
             s_vd[q] = mt.volume * det;
-            s_tag[q] = pf ? 0xffffu : ((rc % (uint32_t)P_RC) | (mt.phase << 8));
+            s_tag[q] = pf ? 0xffffu : ((rc % (uint32_t)T::RC) | (mt.phase << 8));
         }
         __syncthreads();
         // (E) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region.
@@ -991,7 +1020,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             uint32_t q0[3], n[3];
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
-                const int row = (ly + 1 + dy) * P_RC + lx + 1;
+                const int row = (ly + 1 + dy) * T::RC + lx + 1;
                 q0[dy] = s_cell[row];
                 n[dy] = s_cell[row + 3] - q0[dy];
             }
@@ -1031,17 +1060,17 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     // ring volumes for the gradient (the staging arrays are free now)
     float* s_vw = s_fx;
     float* s_vo = s_fy;
-    const int ring = ly * P_TC + lx;
-    if (lx < P_TC && ly < P_TC) {
+    const int ring = ly * T::TC + lx;
+    if (lx < T::TC && ly < T::TC) {
         s_vw[ring] = node_ok ? acc[6] : 0.0f;
         s_vo[ring] = node_ok ? acc[10] : 0.0f;
     }
     __syncthreads();
-    if (node_ok && lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI) {
+    if (node_ok && lx >= 1 && lx <= TI && ly >= 1 && ly <= TI) {
         const uint32_t node = (uint32_t)sy * (uint32_t)W + (uint32_t)sx;
         auto nvol = [&](int k, int ox, int oy) {
             const float* a = k == 0 ? s_vw : s_vo;
-            return a[ring + oy * P_TC + ox];
+            return a[ring + oy * T::TC + ox];
         };
         if constexpr (AUX) {
             const AuxData& A = S.aux;
@@ -1055,7 +1084,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
         }
     }
     if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
-        const bool mine = node_ok && lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI;
+        const bool mine = node_ok && lx >= 1 && lx <= TI && ly >= 1 && ly <= TI;
         block_sum3_f64(mine ? (double)acc[0] + (double)acc[3] + (double)acc[7] : 0.0,
                        mine ? (double)acc[1] + (double)acc[4] + (double)acc[8] : 0.0,
                        mine ? (double)acc[2] + (double)acc[5] + (double)acc[9] : 0.0, diag_grid + 3 * tile);
@@ -1070,9 +1099,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
 // integer atomics: integer sums do not depend on the order, and an exact sum
 // outside int32 is the accumulator overflow of 65536.  Scenes without solids
 // keep only the eight liquid channels, so two tiles fit in one CU's LDS.
-template <bool GENERAL, bool SOLIDS, bool AUX = false>
-__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+template <int TI, bool GENERAL, bool SOLIDS, bool AUX = false>
+__global__ __launch_bounds__(P2GTile<TI>::THREADS)
+__attribute__((amdgpu_waves_per_eu((TI == 20 && SOLIDS) ? 6 : 8, 8)))  // LDS: 3 x 8 waves a CU for 20 with solids
+void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                  uint32_t step) {
+    using T = P2GTile<TI>;
     SCENE_PROLOGUE(tile_scene, p2g_base);
     const int in_buf = (int)((S.start_buf + step) & 1u);
     const Params* __restrict__ Pp = &S.P;
@@ -1087,28 +1119,28 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
     // words [solid m, qx, qy |] water m, qx, qy, V | oil m, qx, qy, V
     constexpr int NCH = SOLIDS ? 11 : 8;
     constexpr int SKIP = SOLIDS ? 0 : 3;  // channel index = word index - SKIP
-    __shared__ unsigned long long s_acc[NCH][P_TC * P_TC];
-    __shared__ uint32_t s_boff[P_NB + 1];
+    __shared__ unsigned long long s_acc[NCH][T::TC * T::TC];
+    __shared__ uint32_t s_boff[T::NB + 1];
 
-    for (int e = threadIdx.x; e < NCH * P_TC * P_TC; e += P_THREADS) (&s_acc[0][0])[e] = 0ull;
-    if (!p2g_head(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    for (int e = threadIdx.x; e < NCH * T::TC * T::TC; e += T::THREADS) (&s_acc[0][0])[e] = 0ull;
+    if (!p2g_head<T>(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     __syncthreads();
     const int tid = threadIdx.x;
     const Params& P = *Pp;
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
-    const int X0 = tx * P_TI, Y0 = ty * P_TI;
+    const int X0 = tx * TI, Y0 = ty * TI;
     const int W = (int)P.node_w, H = (int)P.node_h;
     const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
-    if (s_boff[P_NB] == 0u) {
-        p2g_empty_tile(P, grid, keep_fixed, diag_grid, tile, X0, Y0);
+    if (s_boff[T::NB] == 0u) {
+        p2g_empty_tile<TI>(P, grid, keep_fixed, diag_grid, tile, X0, Y0);
         return;
     }
 
     uint32_t flags = 0;
     const float mref = P.particle_mass, vref = P.particle_volume, dx = P.dx;
     const float rm = P.inv_mass_ref, rv = P.inv_volume_ref;
-    const uint32_t total = s_boff[P_NB];
-    for (uint32_t base = 0; base < total; base += 2u * P_THREADS) {
+    const uint32_t total = s_boff[T::NB];
+    for (uint32_t base = 0; base < total; base += 2u * T::THREADS) {
         // load two particles a thread first, so both loads are in flight
         uint32_t pid[2], mat[2];
         int gbx[2], gby[2];
@@ -1118,16 +1150,16 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
         bool have[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const uint32_t j = base + (uint32_t)tid + (uint32_t)(k * P_THREADS);
+            const uint32_t j = base + (uint32_t)tid + (uint32_t)(k * T::THREADS);
             have[k] = j < total;
             pid[k] = 0u; mat[k] = 0u; gbx[k] = 0; gby[k] = 0;
             p[k] = make_float2(0.f, 0.f); v[k] = p[k];
             C[k] = make_float4(0.f, 0.f, 0.f, 0.f); F[k] = C[k]; S[k] = C[k];
             pv[k] = 1.0f;
             if (!have[k]) continue;
-            const int rb = region_bucket(s_boff, 0, P_NB, j);
-            gbx[k] = rbx0 + rb % P_RB;
-            gby[k] = rby0 + rb / P_RB;
+            const int rb = region_bucket(s_boff, 0, T::NB, j);
+            gbx[k] = rbx0 + rb % T::RB;
+            gby[k] = rby0 + rb / T::RB;
             const size_t g = ((size_t)gby[k] * bw + (size_t)gbx[k]) * cap + (j - s_boff[rb]);
             p[k] = in.pos[g];
             pid[k] = in.pid[g];
@@ -1150,9 +1182,9 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
                 continue;
             }
             // tile-local node of stencil offset (0, 0); the tile's nodes are
-            // storage nodes [X0-1, X0+P_TI] x [Y0-1, Y0+P_TI]
+            // storage nodes [X0-1, X0+TI] x [Y0-1, Y0+TI]
             const int nx0 = sc.bx + 2 - X0, ny0 = sc.by + 2 - Y0;
-            if (nx0 + 2 < 0 || ny0 + 2 < 0 || nx0 >= P_TC || ny0 >= P_TC) continue;
+            if (nx0 + 2 < 0 || ny0 + 2 < 0 || nx0 >= T::TC || ny0 >= T::TC) continue;
             const Material mt = mats[mat[k]];
             const float det = det_comp(F[k]);
             uint32_t pf = 0;
@@ -1187,12 +1219,12 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
 #pragma unroll
             for (int oy = 0; oy < 3; ++oy) {
                 const int ly = ny0 + oy;
-                if (ly < 0 || ly >= P_TC) continue;
+                if (ly < 0 || ly >= T::TC) continue;
                 const float dpy = ((float)oy - sc.fy) * dx;
 #pragma unroll
                 for (int ox = 0; ox < 3; ++ox) {
                     const int lx = nx0 + ox;
-                    if (lx < 0 || lx >= P_TC) continue;
+                    if (lx < 0 || lx >= T::TC) continue;
                     const float dpx = ((float)ox - sc.fx) * dx;
                     const float w = wx[ox] * wy[oy];
                     const float qx = (mvx + A.x * dpx) + A.y * dpy;
@@ -1201,7 +1233,7 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
                         flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
                         continue;
                     }
-                    const int node = ly * P_TC + lx;
+                    const int node = ly * T::TC + lx;
                     int32_t im, iqx, iqy, iv = 0;
                     flags |= fixed_round(div_ref(w * m, mref, rm), im);
                     flags |= fixed_round(div_ref(w * qx, mref, rm), iqx);
@@ -1226,10 +1258,10 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
         const int32_t w32 = (int32_t)(uint32_t)(unsigned long long)sum_at(word, node);
         return ((float)w32 / FIXED_POINT_SCALE) * (word == 6 || word == 10 ? vref : mref);
     };
-    const int lx = tid % 32, ly = tid / 32;
+    const int lx = tid % T::ROW, ly = tid / T::ROW;
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
-    if (lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI && sx < W && sy < H) {
-        const int ring = ly * P_TC + lx;
+    if (lx >= 1 && lx <= TI && ly >= 1 && ly <= TI && sx < W && sy < H) {
+        const int ring = ly * T::TC + lx;
         const uint32_t node = (uint32_t)sy * (uint32_t)W + (uint32_t)sx;
         float f[11];
 #pragma unroll
@@ -1239,7 +1271,7 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
             if (keep_fixed) grid.fixed[(size_t)node * 11 + k] = (int32_t)(uint32_t)(unsigned long long)sum;
             f[k] = decoded(k, ring);
         }
-        auto nvol = [&](int k, int ox, int oy) { return decoded(k == 0 ? 6 : 10, ring + oy * P_TC + ox); };
+        auto nvol = [&](int k, int ox, int oy) { return decoded(k == 0 ? 6 : 10, ring + oy * T::TC + ox); };
         if constexpr (AUX) {
             const AuxData& A = S.aux;
             flags |= update_node(P, grid, scene_flags, node, sx - 1, sy - 1, f, ERR_GRID, nvol,
@@ -1251,8 +1283,8 @@ __global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)
         }
     }
     if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
-        const bool mine = lx >= 1 && lx <= P_TI && ly >= 1 && ly <= P_TI && sx < W && sy < H;
-        const int ring = ly * P_TC + lx;
+        const bool mine = lx >= 1 && lx <= TI && ly >= 1 && ly <= TI && sx < W && sy < H;
+        const int ring = ly * T::TC + lx;
         block_sum3_f64(mine ? (double)decoded(0, ring) + (double)decoded(3, ring) + (double)decoded(7, ring) : 0.0,
                        mine ? (double)decoded(1, ring) + (double)decoded(4, ring) + (double)decoded(8, ring) : 0.0,
                        mine ? (double)decoded(2, ring) + (double)decoded(5, ring) + (double)decoded(9, ring) : 0.0,
@@ -1389,7 +1421,7 @@ __device__ float2 project_out_of_triangles(const Params& P, const AuxData& A, fl
     return make_float2((float)((double)p.x + corr * nx), (float)((double)p.y + corr * ny));
 }
 
-template <bool GENERAL, bool AUX = false>
+template <int TN, bool GENERAL, bool AUX = false>
 __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
                              float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
                              uint32_t step, G2POut& o, const AuxData* A = nullptr, bool cfl_here = false) {
@@ -1411,13 +1443,13 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
         dpx[o] = ((float)o - s.fx) * dx;
         dpy[o] = ((float)o - s.fy) * dx;
     }
-    const int row0 = (s.by + 1 - ny0) * G_TN + (s.bx + 1 - nx0);
+    const int row0 = (s.by + 1 - ny0) * TN + (s.bx + 1 - nx0);
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy) {
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
             const float w = wx[ox] * wy[oy];
-            const float2 g = lv[row0 + oy * G_TN + ox];
+            const float2 g = lv[row0 + oy * TN + ox];
             const float wgx = w * g.x, wgy = w * g.y;
             vx = vx + wgx;
             vy = vy + wgy;
@@ -1433,7 +1465,7 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 #pragma unroll
             for (int ox = 0; ox < 3; ++ox) {
                 const float w = wx[ox] * wy[oy];
-                const float2 lq = llq[row0 + oy * G_TN + ox];
+                const float2 lq = llq[row0 + oy * TN + ox];
                 pm = pm + w * lq.x;
                 pvol = pvol + w * lq.y;
             }
@@ -1598,14 +1630,19 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     return true;
 }
 
-// One 256-thread block per 8x8-bucket tile: stage the 34x34 node patch, run
-// G2P for the tile's particles and write each into its next bucket.  The
-// inputs of a thread's next particle are fetched before the current one is
-// processed, so their latency overlaps the current G2P and slot reservation.
-template <bool GENERAL, bool AUX = false>
+// One block of G_THREADS threads per G_TB x G_TB-bucket tile: stage the
+// tile's node patch, run G2P for its particles and write each into its next
+// bucket.  The inputs of a thread's next particle are fetched before the
+// current one is processed, so their latency overlaps the current G2P and
+// slot reservation.  Compiled shapes: 8x8 buckets with 128, 256 or 512
+// threads, 4x4 buckets with 128 or 256.
+template <int G_TB, int G_THREADS, bool GENERAL, bool AUX = false>
 __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__ scenes,
                                                  const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                                                  uint32_t step) {
+    using T = G2PTile<G_TB>;
+    constexpr int G_TN = T::TN, NBK = T::NBK;
+    static_assert(G_THREADS >= 64, "the first wave scans the bucket counts");
     SCENE_PROLOGUE(tile_scene, g2p_base);
     const int out_buf = (int)((S.start_buf + step + 1u) & 1u);
     const Params* __restrict__ Pp = &S.P;
@@ -1619,7 +1656,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
     __shared__ float2 s_vw[G_TN * G_TN];
     __shared__ float2 s_vo[G_TN * G_TN];
     __shared__ float2 s_lq[G_TN * G_TN];
-    __shared__ uint32_t s_boff[G_TB * G_TB + 1];
+    __shared__ uint32_t s_boff[NBK + 1];
     __shared__ Material s_mat[LDS_MATS];
     if (block_status_set(ctl)) return;
     const Params& P = *Pp;
@@ -1631,19 +1668,19 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
     if (tid < 64) {
         const int gbx = tbx + tid % G_TB, gby = tby + tid / G_TB;
         uint32_t c = 0u;
-        if (gbx < (int)bw && gby < (int)bh) c = min(in.cnt[(uint32_t)gby * bw + (uint32_t)gbx], cap);
-        // exclusive scan of the 64 counts in one wave
+        if (tid < NBK && gbx < (int)bw && gby < (int)bh) c = min(in.cnt[(uint32_t)gby * bw + (uint32_t)gbx], cap);
+        // exclusive scan of the tile's counts in one wave
         uint32_t incl = c;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t t = __shfl_up(incl, off, 64);
             if (tid >= off) incl += t;
         }
-        s_boff[tid] = incl - c;
-        if (tid == 63) s_boff[64] = incl;
+        if (tid < NBK) s_boff[tid] = incl - c;
+        if (tid == NBK - 1) s_boff[NBK] = incl;
     }
     __syncthreads();
-    if (s_boff[64] == 0u) return;  // no particle in this tile
+    if (s_boff[NBK] == 0u) return;  // no particle in this tile
     for (int e = tid; e < G_TN * G_TN; e += G_THREADS) {
         const int nx = nx0 + e % G_TN, ny = ny0 + e / G_TN;
         const bool in_grid = nx < W && ny < H;
@@ -1661,7 +1698,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
         if (scene_flags & (SCENE_WATER | SCENE_OIL)) s_lq[e] = lq;
     }
     __syncthreads();
-    const uint32_t total = s_boff[64];
+    const uint32_t total = s_boff[NBK];
 
     // inputs of the thread's current (c_) and next (n_) particle
     float2 c_p = make_float2(0.f, 0.f), n_p = c_p;
@@ -1673,7 +1710,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
     do {                                                                               \
         const uint32_t j_ = (J);                                                       \
         if (j_ < total) {                                                              \
-            int lo = 0, hi = 63;                                                       \
+            int lo = 0, hi = NBK - 1;                                                  \
             while (lo < hi) {                                                          \
                 const int mid = (lo + hi + 1) >> 1;                                    \
                 if (s_boff[mid] <= j_) lo = mid; else hi = mid - 1;                    \
@@ -1715,7 +1752,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
                 ok = false;
             } else {
                 const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u ? s_vo : s_vs);
-                ok = g2p_particle<GENERAL, AUX>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
+                ok = g2p_particle<G_TN, GENERAL, AUX>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
                                                 Ax, cfl_here);
             }
             if (ok) {
@@ -2400,6 +2437,53 @@ __global__ __launch_bounds__(256) void k_finish_sums(const SceneDesc* __restrict
 
 inline uint32_t blocks_for(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+// Kernel and block size of the step kernels for a launch shape.  Scenes
+// with bodies or bonds (AUX) run the default shape only.
+using StepKernel = void (*)(const SceneDesc*, const uint32_t*, uint32_t, uint32_t);
+struct StepChoice {
+    StepKernel fn;
+    uint32_t threads;
+};
+
+template <int TI>
+StepChoice p2g_choice(bool fixed_point, bool general, bool solids) {
+    StepKernel fn;
+    if (fixed_point)
+        fn = general ? (solids ? k_p2g_fixed<TI, true, true> : k_p2g_fixed<TI, true, false>)
+                     : (solids ? k_p2g_fixed<TI, false, true> : k_p2g_fixed<TI, false, false>);
+    else
+        fn = general ? k_p2g_gather<TI, true> : k_p2g_gather<TI, false>;
+    return {fn, (uint32_t)P2GTile<TI>::THREADS};
+}
+
+StepChoice pick_p2g(const StepBatch& sb, bool general, bool aux) {
+    const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
+    if (aux)
+        return {sb.fixed_point ? k_p2g_fixed<28, true, true, true> : k_p2g_gather<28, true, true>,
+                (uint32_t)P2GTile<28>::THREADS};
+    switch (sb.shape.p2g_tile) {
+        case 20u: return p2g_choice<20>(sb.fixed_point != 0u, general, solids);
+        case 12u: return p2g_choice<12>(sb.fixed_point != 0u, general, solids);
+        default: return p2g_choice<28>(sb.fixed_point != 0u, general, solids);
+    }
+}
+
+template <int TB, int THREADS>
+StepChoice g2p_choice(bool general) {
+    return {general ? k_g2p<TB, THREADS, true> : k_g2p<TB, THREADS, false>, (uint32_t)THREADS};
+}
+
+StepChoice pick_g2p(const StepBatch& sb, bool general, bool aux) {
+    if (aux) return {k_g2p<8, 256, true, true>, 256u};
+    if (sb.shape.g2p_tile == 4u) return sb.shape.g2p_threads == 128u ? g2p_choice<4, 128>(general)
+                                                                     : g2p_choice<4, 256>(general);
+    switch (sb.shape.g2p_threads) {
+        case 128u: return g2p_choice<8, 128>(general);
+        case 512u: return g2p_choice<8, 512>(general);
+        default: return g2p_choice<8, 256>(general);
+    }
+}
+
 }  // namespace
 
 #define MPM_LAUNCH(name, ...)                                                     \
@@ -2425,24 +2509,17 @@ hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure) {
         MPM_LAUNCH("finish_sums", k_finish_sums, dim3(ns), dim3(256), 0, lc.stream, sb.d_scenes, 0u, parts, 0u);
     }
     const bool aux = (sb.scene_flags & SCENE_AUX) != 0u;
+    if (!aux && !shape_supported(sb.shape)) return hipErrorInvalidValue;
+    const StepChoice p2g = pick_p2g(sb, general, aux);
+    const StepChoice g2p = pick_g2p(sb, general, aux);
+    const char* p2g_name = sb.fixed_point ? "p2g_grid_fixed" : "p2g_grid";
     for (uint32_t s = 0; s < steps; ++s) {
         if (aux && sb.aux_bodies)
             MPM_LAUNCH("tri_field", k_tri_field, dim3(blocks_for(sb.node_count, FIELD_THREADS)), dim3(FIELD_THREADS),
                        0, lc.stream, sb.d_scenes, s);
-        if (sb.fixed_point) {
-            const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
-            auto p2g = aux ? k_p2g_fixed<true, true, true>
-                           : general ? (solids ? k_p2g_fixed<true, true> : k_p2g_fixed<true, false>)
-                                     : (solids ? k_p2g_fixed<false, true> : k_p2g_fixed<false, false>);
-            MPM_LAUNCH("p2g_grid_fixed", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes,
-                       sb.p2g_scene, sb.nscenes, s);
-        } else {
-            auto p2g = aux ? k_p2g_gather<true, true> : general ? k_p2g_gather<true> : k_p2g_gather<false>;
-            MPM_LAUNCH("p2g_grid", p2g, dim3(sb.p2g_tiles), dim3(P_THREADS), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
-                       sb.nscenes, s);
-        }
-        auto g2p = aux ? k_g2p<true, true> : general ? k_g2p<true> : k_g2p<false>;
-        MPM_LAUNCH("g2p", g2p, dim3(sb.g2p_tiles), dim3(G_THREADS), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
+        MPM_LAUNCH(p2g_name, p2g.fn, dim3(sb.p2g_tiles), dim3(p2g.threads), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
+                   sb.nscenes, s);
+        MPM_LAUNCH("g2p", g2p.fn, dim3(sb.g2p_tiles), dim3(g2p.threads), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
                    sb.nscenes, s);
         if (aux && sb.aux_bodies)
             MPM_LAUNCH("bodies", k_bodies, dim3(1), dim3(BODY_THREADS), 0, lc.stream, sb.d_scenes, s);

@@ -42,6 +42,11 @@ SIGNATURES = {
     "mpm_set_timing": (ctypes.c_int, [_P, _U32]),
     "mpm_read_timing": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "mpm_set_tuning": (ctypes.c_int, [_P, _P]),
+    "mpm_get_tuning": (ctypes.c_int, [_P, _P]),
+    "mpm_batch_set_tuning": (ctypes.c_int, [_P, _P]),
+    "mpm_batch_get_tuning": (ctypes.c_int, [_P, _P]),
+    "mpm_device_arch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_char_p, _U32, _P]),
     "mpm_batch_create": (ctypes.c_int, [_P, _P, _P, _U32, ctypes.POINTER(_P)]),
     "mpm_batch_step": (ctypes.c_int, [_P, _P, _U32, _U32]),
     "mpm_batch_scene": (_P, [_P, _U32]),
@@ -248,6 +253,14 @@ class Context:
         self._check(self.lib.mpm_read_timing(self.handle, kernel.encode(), ctypes.byref(mean), ctypes.byref(launches)))
         return mean.value, launches.value
 
+    def set_tuning(self, tuning: "MpmTuning"):
+        self._check(self.lib.mpm_set_tuning(self.handle, ctypes.byref(tuning)))
+
+    def get_tuning(self) -> "MpmTuning":
+        out = MpmTuning()
+        self._check(self.lib.mpm_get_tuning(self.handle, ctypes.byref(out)))
+        return out
+
     def close(self):
         if getattr(self, "handle", None):
             if getattr(self, "owned", True):
@@ -259,6 +272,29 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class MpmTuning(ctypes.Structure):
+    """``mpm_tuning``: launch shape of the step kernels and bucket capacity."""
+    _fields_ = [("p2g_tile", _U32), ("g2p_tile", _U32), ("g2p_threads", _U32), ("bucket_capacity", _U32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "MpmTuning":
+        return cls(*(int(d.get(k, 0)) for k, _ in cls._fields_))
+
+
+def device_arch(device: int = 0) -> tuple[str, int]:
+    """(architecture name, compute units) of a HIP device."""
+    lib = load()
+    buf = ctypes.create_string_buffer(128)
+    cus = ctypes.c_uint32()
+    code = lib.mpm_device_arch(device, buf, 128, ctypes.byref(cus))
+    if code != MPM_OK:
+        raise NativeError(code, lib.mpm_last_error(None).decode())
+    return buf.value.decode(), int(cus.value)
 
 
 class Batch:
@@ -289,6 +325,18 @@ class Batch:
         code = self.lib.mpm_batch_step(self.handle, ctypes.addressof(buf), steps, 1 if measure else 0)
         if code != MPM_OK:
             raise NativeError(code, self.lib.mpm_batch_last_error(self.handle).decode())
+
+    def set_tuning(self, tuning: "MpmTuning"):
+        code = self.lib.mpm_batch_set_tuning(self.handle, ctypes.byref(tuning))
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_batch_last_error(self.handle).decode())
+
+    def get_tuning(self) -> "MpmTuning":
+        out = MpmTuning()
+        code = self.lib.mpm_batch_get_tuning(self.handle, ctypes.byref(out))
+        if code != MPM_OK:
+            raise NativeError(code, self.lib.mpm_batch_last_error(self.handle).decode())
+        return out
 
     def close(self):
         if getattr(self, "handle", None):

@@ -122,7 +122,11 @@ class MpmMlsGpuSolver:
 
     def __init__(self, particles, config: dict, cfl_bounds: cflmod.CflBounds | None = None,
                  p2g_mode: int = MPM_P2G_GATHER, keep_fixed_grid: bool = False, device: int = -1,
-                 bucket_slots: int = 0):
+                 bucket_slots: int = 0, tuning: dict | str | None = None):
+        """tuning: None runs the default launch shape; a dict sets one
+        (autotune.DEFAULT_TUNING's keys); "auto" takes the cached tuning of
+        this scene's class, tuning and caching it first when there is none
+        (autotune.tuning_for)."""
         self.state = layout.validate_input(particles, config)
         derived = cfl_bounds
         if derived is None:
@@ -137,6 +141,13 @@ class MpmMlsGpuSolver:
         if self.state.fracture_bonds is not None:
             self.ctx.set_fracture_bonds(self.state.fracture_bonds)
         self.p2g_mode = p2g_mode
+        if tuning is not None:
+            from . import autotune
+            if isinstance(tuning, str):
+                if tuning != "auto":
+                    raise RangeError('tuning must be None, a dict or "auto"')
+                tuning = autotune.tuning_for(particles, config, p2g_mode, device=max(device, 0))
+            autotune.apply(self, tuning)
         self._disposed = False
         self._cfl_state_usable = True
         self._speed_measurement_submitted = False
@@ -532,7 +543,7 @@ class MpmMlsGpuBatch:
     scene(i) is an MpmMlsGpuSolver over scene i: reads, status, diagnostics
     and single-scene steps work on it as on any solver."""
 
-    def __init__(self, scenes, p2g_mode: int = MPM_P2G_GATHER, device: int = -1):
+    def __init__(self, scenes, p2g_mode: int = MPM_P2G_GATHER, device: int = -1, tuning: dict | None = None):
         if not scenes:
             raise RangeError("a batch needs at least one scene")
         from .native import Batch
@@ -545,6 +556,9 @@ class MpmMlsGpuBatch:
                                   st.particle_mass_f32, st.particle_volume_f32, st.wall_friction_f32,
                                   st.particle_capacity, p2g_mode, 0, device, 0))
         self.native = Batch(cfgs, [st.packed for st in states])
+        if tuning is not None:
+            from .native import MpmTuning
+            self.native.set_tuning(MpmTuning.from_dict(tuning))
         self.solvers = [MpmMlsGpuSolver._over_context(st, b, ctx, p2g_mode)
                         for st, b, ctx in zip(states, bounds, self.native.scenes)]
 
