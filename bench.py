@@ -64,9 +64,9 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_solver(pkg, arr, config, mode, timing=False):
+def build_solver(pkg, arr, config, mode, timing=False, tuning=None):
     factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
-    s = factory(arr, config, device=0)
+    s = factory(arr, config, device=0, tuning=tuning)
     if timing:
         s.ctx.set_timing(True)
     return s
@@ -219,7 +219,29 @@ def adjoint_cpu(pkg):
                                                    f"({t1 - t0:.2f} s + {t2 - t1:.2f} s; the reverse re-runs the forward)"}
 
 
-def pmc_sample(pkg, arr, config, mode):
+def choose_tuning(pkg, arr, config, mode, how):
+    """The launch shape a mode is measured with: "default", "auto" (the
+    autotuner on a scratch copy of the scene, before and outside every timed
+    region) or explicit "p2g_tile,g2p_tile,g2p_threads,bucket_capacity".
+    Returns (tuning dict or None, report)."""
+    if how == "default":
+        return None, {"source": "default", "tuning": dict(pkg.autotune.DEFAULT_TUNING)}
+    if how != "auto":
+        keys = ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity")
+        t = dict(zip(keys, (int(v) for v in how.split(","))))
+        return t, {"source": "explicit", "tuning": t}
+    code = pkg.native.MPM_P2G_GATHER if mode == "gather" else pkg.native.MPM_P2G_FIXED_POINT
+    t0 = time.perf_counter()
+    r = pkg.autotune.autotune(arr, config, code, gravity=pkg.scenes.WATER_OIL_GRAVITY, steps=20, warmup=4)
+    el = time.perf_counter() - t0
+    cands = {part: {k: round(v["p2g_us" if part == "p2g" else "g2p_us" if part == "g2p" else "step_us"], 2)
+                    for k, v in r["candidates"][part].items()} for part in r["candidates"]}
+    log(f"[bench] {mode}: autotuned {r['tuning']} in {el:.1f} s")
+    return r["tuning"], {"source": "autotune", "tuning": r["tuning"], "seconds": round(el, 2),
+                         "class": r["class"], "device": r["device"], "candidates_us": cands}
+
+
+def pmc_sample(pkg, arr, config, mode, tuning_how="default"):
     """Workload of the FETCH_SIZE / WRITE_SIZE counter runs.  The copy kernel
     reads and writes COPY_BYTES each, so its counter values per byte scale
     the counters of the step kernels to HBM bytes."""
@@ -230,7 +252,8 @@ def pmc_sample(pkg, arr, config, mode):
     torch.cuda.synchronize()
     log(f"[bench] calibration copy of {COPY_BYTES} bytes done")
     for m in (["gather", "fixed"] if mode == "both" else [mode]):
-        solver = build_solver(pkg, arr, config, m)
+        tuning, _ = choose_tuning(pkg, arr, config, m, tuning_how)
+        solver = build_solver(pkg, arr, config, m, tuning=tuning)
         plan, limit = plan_for(pkg, solver, config)
         run_steps(pkg, solver, plan, limit, 8)
         solver.ctx.read_compact()
@@ -250,6 +273,9 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=480)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=5)
+    ap.add_argument("--tuning", default="auto",
+                    help='launch shape: "auto" (autotune before timing), "default", or '
+                         '"p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
     ap.add_argument("--pmc-sample", action="store_true",
                     help="counter-collection workload: one 256 MiB copy kernel (the HBM byte calibration), then "
                          "8 untimed microsteps per mode; prints nothing")
@@ -263,16 +289,19 @@ def main():
     log(f"[bench] building scene: {args.cells}x{args.cells} cells")
     arr, config = pkg.scenes.water_oil_column(cells=args.cells)
     if args.pmc_sample:
-        pmc_sample(pkg, arr, config, args.mode)
+        pmc_sample(pkg, arr, config, args.mode, args.tuning)
         return
     n = len(arr)
     nodes = (args.cells + 3) ** 2
 
     results = {}
     kernel_times = {}
+    tunings = {}
     microdt = None
     for mode in (["gather", "fixed"] if args.mode == "both" else [args.mode]):
-        solver = build_solver(pkg, arr, config, mode)
+        tuning, tunings[mode] = choose_tuning(pkg, arr, config, mode, args.tuning)
+        solver = build_solver(pkg, arr, config, mode, tuning=tuning)
+        tunings[mode]["effective"] = solver.ctx.get_tuning().as_dict()
         plan, limit = plan_for(pkg, solver, config)
         microdt = plan.microDt
         log(f"[bench] {mode}: plan {plan.stepCount} microsteps of {plan.microDt:.4e} s; timing {args.steps} steps")
@@ -282,7 +311,7 @@ def main():
         log(f"[bench] {mode}: {1e3 * el / args.steps:.4f} ms/step, {n * args.steps / el:.4e} particle-steps/s")
         solver.dispose()
         # per-kernel times with HIP events on the solver's stream (separate run)
-        solver = build_solver(pkg, arr, config, mode, timing=True)
+        solver = build_solver(pkg, arr, config, mode, timing=True, tuning=tuning)
         run_steps(pkg, solver, plan, limit, min(args.steps, 2 * plan.stepCount))
         solver.ctx.read_compact()
         kt = {}
@@ -350,6 +379,7 @@ def main():
                      "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_us": dom_us,
                      "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)) if traffic else None},
         "kernels_us": kernel_times,
+        "tuning": tunings,
         "batched": batched,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

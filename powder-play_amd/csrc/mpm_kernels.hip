@@ -727,8 +727,9 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
     // After a failed microstep the output buffer of this one holds the
     // committed state: leave its counts and deferred flags untouched.
     if (block_status_set(ctl)) return false;
-    if (blockIdx.x == 0 && tid == 0) {
-        // stress-cache failures of the state this microstep starts from
+    if (tile == 0u && tid == 0) {
+        // stress-cache failures of the state this microstep starts from (the
+        // scene's first tile; in a batch every scene has one)
         const uint32_t d = ctl->deferred[in_buf];
         if (d) {
             if (d & ERR_P2G) atomicMin(&ctl->status[1], ctl->deferred_index[in_buf]);
@@ -877,9 +878,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     __shared__ uint32_t s_wave[T::THREADS / 64 + 1];
     __shared__ Material s_mat[LDS_MATS];
     __shared__ uint32_t s_cell[T::CELLS + 1]; // per region cell: count, then start
-    // staged particle records in gather order
-    __shared__ float s_fx[G_CAP], s_fy[G_CAP], s_m[G_CAP], s_mvx[G_CAP], s_mvy[G_CAP];
-    __shared__ float s_a0[G_CAP], s_a1[G_CAP], s_a2[G_CAP], s_a3[G_CAP], s_vd[G_CAP];
+    // staged particle records in gather order, packed so that a contribution
+    // reads them in three wide LDS loads (and consecutive lanes read
+    // consecutive 16-byte records, which keeps the banks apart)
+    __shared__ float4 s_r0[G_CAP];     // stencil fraction fx, fy; momentum m vx, m vy
+    __shared__ float4 s_r1[G_CAP];     // affine A (row-major)
+    __shared__ float2 s_r2[G_CAP];     // mass, volume x det F
     __shared__ uint32_t s_tag[G_CAP];  // region column | phase << 8; 0xffff = contributes nothing
     __shared__ uint32_t s_pid[G_CAP];  // particle indices at provisional places
 
@@ -1003,13 +1007,9 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
                 flags |= pf;
             }
-            s_fx[q] = sc.fx;
-            s_fy[q] = sc.fy;
-            s_m[q] = mt.mass;
-            s_mvx[q] = mt.mass * v.x;
-            s_mvy[q] = mt.mass * v.y;
-            s_a0[q] = A.x; s_a1[q] = A.y; s_a2[q] = A.z; s_a3[q] = A.w;
-            s_vd[q] = mt.volume * det;
+            s_r0[q] = make_float4(sc.fx, sc.fy, mt.mass * v.x, mt.mass * v.y);
+            s_r1[q] = A;
+            s_r2[q] = make_float2(mt.mass, mt.volume * det);
             s_tag[q] = pf ? 0xffffu : ((rc % (uint32_t)T::RC) | (mt.phase << 8));
         }
         __syncthreads();
@@ -1034,21 +1034,23 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 const uint32_t ph = tag >> 8;
                 const int ox = lx + 3 - (int)(tag & 0xffu);
                 const int oy = 2 - dy;
-                const float fx = s_fx[q], fy = s_fy[q];
+                const float4 r0 = s_r0[q], r1 = s_r1[q];
+                const float2 r2 = s_r2[q];
+                const float fx = r0.x, fy = r0.y;
                 const float w = qweight(fx, ox) * qweight(fy, oy);
                 const float dpx = ((float)ox - fx) * dx;
                 const float dpy = ((float)oy - fy) * dx;
-                const float qx = (s_mvx[q] + s_a0[q] * dpx) + s_a1[q] * dpy;
-                const float qy = (s_mvy[q] + s_a2[q] * dpx) + s_a3[q] * dpy;
+                const float qx = (r0.z + r1.x * dpx) + r1.y * dpy;
+                const float qy = (r0.w + r1.z * dpx) + r1.w * dpy;
                 if (!finitef(qx) || !finitef(qy)) {
                     flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
                     continue;
                 }
-                const float wm = w * s_m[q], wqx = w * qx, wqy = w * qy;
+                const float wm = w * r2.x, wqx = w * qx, wqy = w * qy;
                 if (ph == 0u) {
                     acc[0] += wm; acc[1] += wqx; acc[2] += wqy;
                 } else {
-                    const float wv = w * s_vd[q];
+                    const float wv = w * r2.y;
                     if (ph == 1u) { acc[3] += wm; acc[4] += wqx; acc[5] += wqy; acc[6] += wv; }
                     else { acc[7] += wm; acc[8] += wqx; acc[9] += wqy; acc[10] += wv; }
                 }
@@ -1058,8 +1060,8 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     }
 
     // ring volumes for the gradient (the staging arrays are free now)
-    float* s_vw = s_fx;
-    float* s_vo = s_fy;
+    float* s_vw = reinterpret_cast<float*>(s_r0);
+    float* s_vo = s_vw + G_CAP;
     const int ring = ly * T::TC + lx;
     if (lx < T::TC && ly < T::TC) {
         s_vw[ring] = node_ok ? acc[6] : 0.0f;
