@@ -47,16 +47,20 @@ G2P_PARTICLE_BYTES, G2P_NODE_PHASE_BYTES = 48, 16
 TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_water_oil_1m.md"
 
 
-def measured_traffic():
+def measured_traffic(tuning: dict | None):
     """HBM bytes per launch of each step kernel from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE, scaled by the calibration copy), or
-    an empty dict when there is none."""
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the calibration copy), when
+    it was measured with the launch shape this run uses; else an empty dict."""
     import re
-    if not TRAFFIC_PROFILE.exists():
+    if not TRAFFIC_PROFILE.exists() or not tuning:
         return {}
+    shape = ",".join(str(tuning[k]) for k in ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity"))
     for line in TRAFFIC_PROFILE.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
-            return {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", line)}
+            m = re.search(r"shape=([\d,]+)", line)
+            if not m or m.group(1) != shape:
+                return {}
+            return {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", line) if k != "shape"}
     return {}
 
 
@@ -336,7 +340,7 @@ def main():
     dom = max(cand, key=lambda k: cand[k][1] or 0.0)
     dom_bytes, dom_us = cand[dom]
     achieved = dom_bytes / (dom_us * 1e-6) / 1e9 if dom_us else None
-    traffic = measured_traffic().get(dom) if args.cells == 1000 else None
+    traffic = measured_traffic(tunings[main_mode].get("effective")).get(dom) if args.cells == 1000 else None
     step_bytes = PARTICLE_BYTES * n + NODE_PHASE_BYTES * phases * nodes
     step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
 

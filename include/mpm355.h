@@ -193,7 +193,7 @@ const char* mpm_batch_last_error(mpm_batch* batch);
  * one gather chunk, needs ~8 particles per cell), and G2P is per particle.
  *
  * mpm_tuning fields:
- *   p2g_tile         P2G tile side in grid nodes: 28 (default), 20 or 12.
+ *   p2g_tile         P2G tile side in grid nodes: 28, 20 (default) or 12.
  *   g2p_tile         G2P tile side in 4x4-cell buckets: 8 (default) or 4.
  *   g2p_threads      threads of a G2P block: 128, 256 (default) or 512 for
  *                    8-bucket tiles; 128 or 256 for 4-bucket tiles.  Particles
@@ -204,8 +204,8 @@ const char* mpm_batch_last_error(mpm_batch* batch);
  *   MPM_EINVAL with no state change; the scene of a batch takes the batch's
  *   shape (mpm_batch_set_tuning) and returns MPM_EINVAL for another shape.
  * mpm_get_tuning: the shape the next step runs and the current capacity.
- *   Scenes with bonded triangle bodies or fracture bonds run the default
- *   shape; a P2G tile whose gather chunk (448 particles for 12, 960 for 20)
+ *   Scenes with bonded triangle bodies or fracture bonds run the shape
+ *   {28, 8, 256}; a P2G tile whose gather chunk (448 particles for 12, 960 for 20)
  *   is below the bucket capacity runs the 28-node tile.
  * mpm_device_arch: the architecture name (e.g. "gfx950:sramecc+:xnack-")
  *   and compute-unit count of a device, the autotuner's cache key. */

@@ -43,7 +43,8 @@ from . import layout
 from .native import MPM_P2G_FIXED_POINT, MPM_P2G_GATHER, MpmTuning, device_arch
 
 CACHE_VERSION = 1
-DEFAULT_TUNING = {"p2g_tile": 28, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
+DEFAULT_TUNING = {"p2g_tile": 20, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
+BODIES_TUNING = {"p2g_tile": 28, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
 P2G_TILES = (28, 20, 12)
 G2P_SHAPES = ((8, 256), (8, 512), (8, 128), (4, 256), (4, 128))
 # particles one float-gather chunk stages for each P2G tile side: a bucket
@@ -227,14 +228,14 @@ def autotune(particles, config: dict, p2g_mode: int = MPM_P2G_GATHER, *, dt: flo
                 table[key] = r
 
         if state.bonded_body_count > 0 or state.fracture_bonds is not None:
-            best = dict(DEFAULT_TUNING)  # bodies and bonds run the default shape only
+            best = dict(BODIES_TUNING)  # bodies and bonds run this shape only
         else:
             for _ in range(2):
                 for t in p2g_tiles:
                     if P2G_CHUNK[t] < derived_cap:
                         continue
                     timed(log["p2g"], str(t), _tuning(t, 8, 256), "p2g_us")
-            p2g_tile = int(_best(log["p2g"], "p2g_us")) if log["p2g"] else 28
+            p2g_tile = int(_best(log["p2g"], "p2g_us")) if log["p2g"] else 28  # only 28 holds every bucket
             for _ in range(2):
                 for tb, th in g2p_shapes:
                     timed(log["g2p"], f"{tb}x{th}", _tuning(p2g_tile, tb, th), "g2p_us")

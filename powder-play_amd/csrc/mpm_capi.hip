@@ -493,12 +493,12 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
 
 int settle(mpm_ctx* c);
 
-// Shape a step of this context runs with: scenes with bodies or bonds run the
-// default kernels only, and a P2G tile whose gather chunk is smaller than the
-// bucket capacity falls back to the default tile (a bucket must fit a chunk).
+// Shape a step of this context runs with: scenes with bodies or bonds run
+// AUX_SHAPE only, and a P2G tile whose gather chunk is smaller than the bucket
+// capacity falls back to the 28-node tile (a bucket must fit a chunk).
 Shape effective_shape(const mpm_ctx* c, Shape s) {
-    if (c->lc.scene_flags & SCENE_AUX) return DEFAULT_SHAPE;
-    if (c->lc.cap > p2g_chunk_for(s.p2g_tile)) s.p2g_tile = DEFAULT_SHAPE.p2g_tile;
+    if (c->lc.scene_flags & SCENE_AUX) return AUX_SHAPE;
+    if (c->lc.cap > p2g_chunk_for(s.p2g_tile)) s.p2g_tile = 28u;
     return s;
 }
 

@@ -200,7 +200,10 @@ constexpr uint32_t DIAG_PART_BLOCKS = 1024;  // blocks of the particle-sum kerne
 struct Shape {
     uint32_t p2g_tile, g2p_tile, g2p_threads;
 };
-constexpr Shape DEFAULT_SHAPE = {28u, 8u, 256u};
+constexpr Shape DEFAULT_SHAPE = {20u, 8u, 256u};
+// Scenes with bonded bodies or fracture bonds run one shape: the largest P2G
+// tile, whose gather chunk holds any bucket.
+constexpr Shape AUX_SHAPE = {28u, 8u, 256u};
 constexpr uint32_t MIN_P2G_TILE = 12u;  // the most P2G tiles a grid can have
 // Particles the float-gather P2G stages per chunk, for each tile side: the
 // particles of one bucket must fit one chunk.
@@ -254,7 +257,7 @@ struct StepBatch {
     uint32_t fixed_point;
     uint32_t aux_bodies, aux_bonds;  // SCENE_AUX (one scene): body and bond counts
     uint32_t node_count;             // SCENE_AUX: the scene's grid nodes
-    Shape shape;                     // launch shape (DEFAULT_SHAPE for SCENE_AUX)
+    Shape shape;                     // launch shape (AUX_SHAPE for SCENE_AUX)
     bool diag;                   // some scene has diagnostics on
     KernelTimes* timing;
 };

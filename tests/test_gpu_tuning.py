@@ -84,7 +84,7 @@ def run(pkg, mode, particles, config, tuning=None, steps=12, dt=None, capacity_c
 def test_every_shape_is_bit_identical(pkg, mode, scene):
     particles, config = scenes(pkg)[scene]
     ref, dt, eff = run(pkg, mode, particles, config)
-    assert eff["p2g_tile"] == 28 and eff["g2p_tile"] == 8 and eff["g2p_threads"] == 256
+    assert eff["p2g_tile"] == 20 and eff["g2p_tile"] == 8 and eff["g2p_threads"] == 256
     for tile in pkg.autotune.P2G_TILES:
         for tb, th in pkg.autotune.G2P_SHAPES:
             want = {"p2g_tile": tile, "g2p_tile": tb, "g2p_threads": th, "bucket_capacity": 0}
@@ -136,7 +136,7 @@ def test_small_tile_falls_back_when_a_bucket_exceeds_its_chunk(pkg):
         s.dispose()
 
 
-def test_scene_with_bodies_runs_the_default_shape(pkg):
+def test_scene_with_bodies_runs_its_one_shape(pkg):
     particles, config, gravity = pkg.scenes.gate4_scene()
     s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
     try:
@@ -169,9 +169,9 @@ def test_batch_shape_is_bit_identical(pkg, mode):
             b.dispose()
 
     ref, eff = go(None)
-    assert eff["p2g_tile"] == 28
+    assert eff["p2g_tile"] == 20
     for t in ({"p2g_tile": 12, "g2p_tile": 4, "g2p_threads": 128, "bucket_capacity": 0},
-              {"p2g_tile": 20, "g2p_tile": 8, "g2p_threads": 512, "bucket_capacity": 0}):
+              {"p2g_tile": 28, "g2p_tile": 8, "g2p_threads": 512, "bucket_capacity": 0}):
         got, eff = go(t)
         assert (eff["p2g_tile"], eff["g2p_tile"], eff["g2p_threads"]) == (t["p2g_tile"], t["g2p_tile"], t["g2p_threads"])
         for a, r in zip(got, ref):
