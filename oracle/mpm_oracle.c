@@ -414,44 +414,143 @@ static int fluid_triangle(double fm, double* v, const double* sv, double nx, dou
 }
 
 /* P2G (solver.ts:418-479) + grid update (:494-608). */
+/* P2G contributions of one particle (solver.ts:418-479) into G. */
+static int p2g_particle(const orc_particle* p, const orc_config* c, double* G) {
+    const long W = c->cells_x + 3;
+    const double dx = c->cell_size, gscale = 4.0 / (dx * dx);
+    stencil_t s;
+    make_stencil(p->x, p->y, dx, &s);
+    if (s.bx < -1 || s.by < -1 || s.bx + 2 > c->cells_x + 1 || s.by + 2 > c->cells_y + 1) return ORC_EINVALID;
+    mat2 F = {p->f00, p->f01, p->f10, p->f11};
+    mat2 PFt;
+    int e = stress_times_ft(p, F, &PFt);
+    if (e) return e;
+    double m = particle_mass(p, c);
+    double ss = -c->dt * p->volume * gscale;
+    double liqV = is_liquid(p->model) ? p->volume * det2(F) : 0;
+    mat2 A = {p->c00 * m + PFt.m00 * ss, p->c01 * m + PFt.m01 * ss, p->c10 * m + PFt.m10 * ss,
+              p->c11 * m + PFt.m11 * ss};
+    if (!mat_finite(A)) return ORC_ENUMERIC;
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            double w = s.wx[ox] * s.wy[oy];
+            if (w == 0) continue;
+            long key = (s.by + oy + 1) * W + (s.bx + ox + 1);
+            double* nd = G + key * N_FIELDS;
+            double dpx = (ox - s.fx) * dx, dpy = (oy - s.fy) * dx;
+            double qx = m * p->vx + A.m00 * dpx + A.m01 * dpy;
+            double qy = m * p->vy + A.m10 * dpx + A.m11 * dpy;
+            if (p->model == ORC_MODEL_WATER) {
+                nd[N_WM] += w * m; nd[N_WQX] += w * qx; nd[N_WQY] += w * qy; nd[N_WV] += w * liqV;
+            } else if (p->model == ORC_MODEL_OIL) {
+                nd[N_OM] += w * m; nd[N_OQX] += w * qx; nd[N_OQY] += w * qy; nd[N_OV] += w * liqV;
+            } else {
+                nd[N_M] += w * m; nd[N_QX] += w * qx; nd[N_QY] += w * qy;
+            }
+        }
+    return ORC_OK;
+}
+
+/* Grid update of node (sx, sy) (solver.ts:1293-1524): velocities from the
+ * summed momenta, liquid and fluid-triangle contact, walls.  Reads the
+ * volumes of the neighbours, writes only this node's velocities. */
+static int grid_node(const orc_particle* in, const orc_config* c, double* G, long sx, long sy,
+                     tri_field* tf) {
+    const long W = c->cells_x + 3, H = c->cells_y + 3;
+    const double dx = c->cell_size;
+    const double gmag = hypot(c->gravity_x, c->gravity_y);
+    const double vscale = 1.0 / (2 * dx * dx);
+    double* nd = G + (sy * W + sx) * N_FIELDS;
+    long xi = sx - 1, yi = sy - 1;
+    /* updateGridVelocity (solver.ts:1507-1524) */
+    double vel[3][2];
+    const int mi[3] = {N_M, N_WM, N_OM};
+    for (int ph = 0; ph < 3; ++ph) {
+        double mass = nd[mi[ph]];
+        if (mass <= 0) { vel[ph][0] = vel[ph][1] = 0; continue; }
+        vel[ph][0] = nd[mi[ph] + 1] / mass + c->dt * c->gravity_x;
+        vel[ph][1] = nd[mi[ph] + 2] / mass + c->dt * c->gravity_y;
+    }
+    /* solveLiquidGridContact + volumeGradient (solver.ts:1293-1409) */
+    if (nd[N_WM] > 0 && nd[N_OM] > 0 && nd[N_WV] > 0 && nd[N_OV] > 0 && xi >= 0 && xi <= c->cells_x &&
+        yi >= 0 && yi <= c->cells_y) {
+#define VOL(X, Y, F) (((X) < 0 || (X) >= W || (Y) < 0 || (Y) >= H) ? 0.0 : G[((Y) * W + (X)) * N_FIELDS + (F)])
+        double wgx = (VOL(sx + 1, sy, N_WV) - VOL(sx - 1, sy, N_WV)) * vscale;
+        double wgy = (VOL(sx, sy + 1, N_WV) - VOL(sx, sy - 1, N_WV)) * vscale;
+        double ogx = (VOL(sx + 1, sy, N_OV) - VOL(sx - 1, sy, N_OV)) * vscale;
+        double ogy = (VOL(sx, sy + 1, N_OV) - VOL(sx, sy - 1, N_OV)) * vscale;
+#undef VOL
+        double gx = ogx - wgx, gy = ogy - wgy;
+        double gl = hypot(gx, gy);
+        if (finite(gl) && gl > 1e-12) {
+            double nx = gx / gl, ny = gy / gl;
+            double wd = nd[N_WM] / nd[N_WV], od = nd[N_OM] / nd[N_OV];
+            double dscale = fmax(wd, od);
+            double contrast = (od - wd) / dscale;
+            double align = gmag > 0 ? (nx * c->gravity_x + ny * c->gravity_y) / gmag : 0;
+            if (!finite(wd) || !finite(od) || !finite(contrast)) return ORC_ENUMERIC;
+            if (contrast * align < -LIQUID_STABLE_ORDER_EPSILON) { nx = -nx; ny = -ny; }
+            double rel = (vel[2][0] - vel[1][0]) * nx + (vel[2][1] - vel[1][1]) * ny;
+            double overlap = fmin(1.0, (2 * fmin(nd[N_WV], nd[N_OV])) / (nd[N_WV] + nd[N_OV]));
+            double minsep = fmax(LIQUID_CONTACT_SEPARATION_FRACTION * overlap * dx /
+                                     LIQUID_CONTACT_RELAXATION_SECONDS,
+                                 fmax(-rel, 0.0) * LIQUID_CONTACT_APPROACH_SPEED_MULTIPLIER);
+            if (rel < minsep) {
+                double imp = (minsep - rel) / (1 / nd[N_WM] + 1 / nd[N_OM]);
+                vel[1][0] -= (imp / nd[N_WM]) * nx;
+                vel[1][1] -= (imp / nd[N_WM]) * ny;
+                vel[2][0] += (imp / nd[N_OM]) * nx;
+                vel[2][1] += (imp / nd[N_OM]) * ny;
+            }
+        }
+    }
+    /* solveFluidTriangleGridContact (solver.ts:1411-1460) */
+    const long key = sy * W + sx;
+    if (tf && tf->body[key] >= 0) {
+        const double radius = sqrt(c->particle_volume / ORC_PI);
+        if (tf->dist[key] <= radius + GRID_CONTACT_REACH_CELLS * dx) {
+            const int32_t b = tf->body[key];
+            const double bw[3] = {tf->b0[key], tf->b1[key], tf->b2[key]};
+            double sv[2] = {0, 0}, inv_body = 0;
+            for (int v = 0; v < 3; ++v) {
+                sv[0] += bw[v] * in[b + v].vx;
+                sv[1] += bw[v] * in[b + v].vy;
+                inv_body += (bw[v] * bw[v]) / c->particle_mass;
+            }
+            if (inv_body > 0) {
+                if (nd[N_WM] > 0) {
+                    int e = fluid_triangle(nd[N_WM], vel[1], sv, tf->nx[key], tf->ny[key], inv_body, c->dt,
+                                           &tf->ix[key], &tf->iy[key]);
+                    if (e) return e;
+                }
+                if (nd[N_OM] > 0) {
+                    int e = fluid_triangle(nd[N_OM], vel[2], sv, tf->nx[key], tf->ny[key], inv_body, c->dt,
+                                           &tf->ix[key], &tf->iy[key]);
+                    if (e) return e;
+                }
+            }
+        }
+    }
+    for (int ph = 0; ph < 3; ++ph) {
+        double vx = wall_cushion(vel[ph][0], (double)xi, (double)c->cells_x);
+        double vy = wall_cushion(vel[ph][1], (double)yi, (double)c->cells_y);
+        vx = wall_friction(vx, (double)yi, (double)c->cells_y, c->wall_friction);
+        vy = wall_friction(vy, (double)xi, (double)c->cells_x, c->wall_friction);
+        if (!finite(vx) || !finite(vy)) return ORC_ENUMERIC;
+        nd[mi[ph] + 3] = vx;
+        nd[mi[ph] + 4] = vy;
+    }
+    return ORC_OK;
+}
+
 static int p2g_and_update(const orc_particle* in, int32_t n, const orc_config* c, double* G,
                           orc_diagnostics* diag, tri_field* tf) {
     const long W = c->cells_x + 3, H = c->cells_y + 3;
-    const double dx = c->cell_size, gscale = 4.0 / (dx * dx);
     memset(G, 0, sizeof(double) * (size_t)(W * H) * N_FIELDS);
     for (int32_t i = 0; i < n; ++i) {
-        const orc_particle* p = &in[i];
-        if (p->model == ORC_MODEL_REMOVED) continue;
-        stencil_t s;
-        make_stencil(p->x, p->y, dx, &s);
-        if (s.bx < -1 || s.by < -1 || s.bx + 2 > c->cells_x + 1 || s.by + 2 > c->cells_y + 1) return ORC_EINVALID;
-        mat2 F = {p->f00, p->f01, p->f10, p->f11};
-        mat2 PFt;
-        int e = stress_times_ft(p, F, &PFt);
+        if (in[i].model == ORC_MODEL_REMOVED) continue;
+        int e = p2g_particle(&in[i], c, G);
         if (e) return e;
-        double m = particle_mass(p, c);
-        double ss = -c->dt * p->volume * gscale;
-        double liqV = is_liquid(p->model) ? p->volume * det2(F) : 0;
-        mat2 A = {p->c00 * m + PFt.m00 * ss, p->c01 * m + PFt.m01 * ss, p->c10 * m + PFt.m10 * ss,
-                  p->c11 * m + PFt.m11 * ss};
-        if (!mat_finite(A)) return ORC_ENUMERIC;
-        for (int oy = 0; oy < 3; ++oy)
-            for (int ox = 0; ox < 3; ++ox) {
-                double w = s.wx[ox] * s.wy[oy];
-                if (w == 0) continue;
-                long key = (s.by + oy + 1) * W + (s.bx + ox + 1);
-                double* nd = G + key * N_FIELDS;
-                double dpx = (ox - s.fx) * dx, dpy = (oy - s.fy) * dx;
-                double qx = m * p->vx + A.m00 * dpx + A.m01 * dpy;
-                double qy = m * p->vy + A.m10 * dpx + A.m11 * dpy;
-                if (p->model == ORC_MODEL_WATER) {
-                    nd[N_WM] += w * m; nd[N_WQX] += w * qx; nd[N_WQY] += w * qy; nd[N_WV] += w * liqV;
-                } else if (p->model == ORC_MODEL_OIL) {
-                    nd[N_OM] += w * m; nd[N_OQX] += w * qx; nd[N_OQY] += w * qy; nd[N_OV] += w * liqV;
-                } else {
-                    nd[N_M] += w * m; nd[N_QX] += w * qx; nd[N_QY] += w * qy;
-                }
-            }
     }
     double gm = 0, gpx = 0, gpy = 0;
     for (long k = 0; k < W * H; ++k) {
@@ -462,90 +561,10 @@ static int p2g_and_update(const orc_particle* in, int32_t n, const orc_config* c
     }
     if (diag) { diag->grid_mass = gm; diag->grid_px = gpx; diag->grid_py = gpy; }
 
-    const double gmag = hypot(c->gravity_x, c->gravity_y);
-    const double vscale = 1.0 / (2 * dx * dx);
     for (long sy = 0; sy < H; ++sy)
         for (long sx = 0; sx < W; ++sx) {
-            double* nd = G + (sy * W + sx) * N_FIELDS;
-            long xi = sx - 1, yi = sy - 1;
-            /* updateGridVelocity (solver.ts:1507-1524) */
-            double vel[3][2];
-            const int mi[3] = {N_M, N_WM, N_OM};
-            for (int ph = 0; ph < 3; ++ph) {
-                double mass = nd[mi[ph]];
-                if (mass <= 0) { vel[ph][0] = vel[ph][1] = 0; continue; }
-                vel[ph][0] = nd[mi[ph] + 1] / mass + c->dt * c->gravity_x;
-                vel[ph][1] = nd[mi[ph] + 2] / mass + c->dt * c->gravity_y;
-            }
-            /* solveLiquidGridContact + volumeGradient (solver.ts:1293-1409) */
-            if (nd[N_WM] > 0 && nd[N_OM] > 0 && nd[N_WV] > 0 && nd[N_OV] > 0 && xi >= 0 && xi <= c->cells_x &&
-                yi >= 0 && yi <= c->cells_y) {
-#define VOL(X, Y, F) (((X) < 0 || (X) >= W || (Y) < 0 || (Y) >= H) ? 0.0 : G[((Y) * W + (X)) * N_FIELDS + (F)])
-                double wgx = (VOL(sx + 1, sy, N_WV) - VOL(sx - 1, sy, N_WV)) * vscale;
-                double wgy = (VOL(sx, sy + 1, N_WV) - VOL(sx, sy - 1, N_WV)) * vscale;
-                double ogx = (VOL(sx + 1, sy, N_OV) - VOL(sx - 1, sy, N_OV)) * vscale;
-                double ogy = (VOL(sx, sy + 1, N_OV) - VOL(sx, sy - 1, N_OV)) * vscale;
-#undef VOL
-                double gx = ogx - wgx, gy = ogy - wgy;
-                double gl = hypot(gx, gy);
-                if (finite(gl) && gl > 1e-12) {
-                    double nx = gx / gl, ny = gy / gl;
-                    double wd = nd[N_WM] / nd[N_WV], od = nd[N_OM] / nd[N_OV];
-                    double dscale = fmax(wd, od);
-                    double contrast = (od - wd) / dscale;
-                    double align = gmag > 0 ? (nx * c->gravity_x + ny * c->gravity_y) / gmag : 0;
-                    if (!finite(wd) || !finite(od) || !finite(contrast)) return ORC_ENUMERIC;
-                    if (contrast * align < -LIQUID_STABLE_ORDER_EPSILON) { nx = -nx; ny = -ny; }
-                    double rel = (vel[2][0] - vel[1][0]) * nx + (vel[2][1] - vel[1][1]) * ny;
-                    double overlap = fmin(1.0, (2 * fmin(nd[N_WV], nd[N_OV])) / (nd[N_WV] + nd[N_OV]));
-                    double minsep = fmax(LIQUID_CONTACT_SEPARATION_FRACTION * overlap * dx /
-                                             LIQUID_CONTACT_RELAXATION_SECONDS,
-                                         fmax(-rel, 0.0) * LIQUID_CONTACT_APPROACH_SPEED_MULTIPLIER);
-                    if (rel < minsep) {
-                        double imp = (minsep - rel) / (1 / nd[N_WM] + 1 / nd[N_OM]);
-                        vel[1][0] -= (imp / nd[N_WM]) * nx;
-                        vel[1][1] -= (imp / nd[N_WM]) * ny;
-                        vel[2][0] += (imp / nd[N_OM]) * nx;
-                        vel[2][1] += (imp / nd[N_OM]) * ny;
-                    }
-                }
-            }
-            /* solveFluidTriangleGridContact (solver.ts:1411-1460) */
-            const long key = sy * W + sx;
-            if (tf && tf->body[key] >= 0) {
-                const double radius = sqrt(c->particle_volume / ORC_PI);
-                if (tf->dist[key] <= radius + GRID_CONTACT_REACH_CELLS * dx) {
-                    const int32_t b = tf->body[key];
-                    const double bw[3] = {tf->b0[key], tf->b1[key], tf->b2[key]};
-                    double sv[2] = {0, 0}, inv_body = 0;
-                    for (int v = 0; v < 3; ++v) {
-                        sv[0] += bw[v] * in[b + v].vx;
-                        sv[1] += bw[v] * in[b + v].vy;
-                        inv_body += (bw[v] * bw[v]) / c->particle_mass;
-                    }
-                    if (inv_body > 0) {
-                        if (nd[N_WM] > 0) {
-                            int e = fluid_triangle(nd[N_WM], vel[1], sv, tf->nx[key], tf->ny[key], inv_body, c->dt,
-                                                   &tf->ix[key], &tf->iy[key]);
-                            if (e) return e;
-                        }
-                        if (nd[N_OM] > 0) {
-                            int e = fluid_triangle(nd[N_OM], vel[2], sv, tf->nx[key], tf->ny[key], inv_body, c->dt,
-                                                   &tf->ix[key], &tf->iy[key]);
-                            if (e) return e;
-                        }
-                    }
-                }
-            }
-            for (int ph = 0; ph < 3; ++ph) {
-                double vx = wall_cushion(vel[ph][0], (double)xi, (double)c->cells_x);
-                double vy = wall_cushion(vel[ph][1], (double)yi, (double)c->cells_y);
-                vx = wall_friction(vx, (double)yi, (double)c->cells_y, c->wall_friction);
-                vy = wall_friction(vy, (double)xi, (double)c->cells_x, c->wall_friction);
-                if (!finite(vx) || !finite(vy)) return ORC_ENUMERIC;
-                nd[mi[ph] + 3] = vx;
-                nd[mi[ph] + 4] = vy;
-            }
+            int e = grid_node(in, c, G, sx, sy, tf);
+            if (e) return e;
         }
     return ORC_OK;
 }
@@ -606,11 +625,91 @@ static int project_outside(const orc_config* c, const tri_field* f, double r, do
 /* One step of particles already validated: P2G, grid update (with the
  * triangle field's fluid contact when tf is given), G2P with projection of
  * liquid particles out of the triangles.  Fills the input and grid sums of dg. */
+/* G2P of one particle (solver.ts:610-773) from the updated grid G: q
+ * receives the particle after the step. */
+static int g2p_particle(const orc_particle* p, orc_particle* q, const orc_config* c, const double* G,
+                        const tri_field* tf) {
+    const long W = c->cells_x + 3;
+    const double dx = c->cell_size, gscale = 4.0 / (dx * dx);
+    const double Wd = c->cells_x * dx, Hd = c->cells_y * dx;
+    int e = ORC_OK;
+    *q = *p;
+    if (p->model == ORC_MODEL_REMOVED) return ORC_OK;
+    /* G2P (solver.ts:610-773) */
+    stencil_t s;
+    make_stencil(p->x, p->y, dx, &s);
+    const int water = p->model == ORC_MODEL_WATER, oil = p->model == ORC_MODEL_OIL, liquid = water || oil;
+    double vx = 0, vy = 0, c00 = 0, c01 = 0, c10 = 0, c11 = 0, lm = 0, lv = 0;
+    for (int oy = 0; oy < 3; ++oy)
+        for (int ox = 0; ox < 3; ++ox) {
+            double w = s.wx[ox] * s.wy[oy];
+            if (w == 0) continue;
+            long key = (s.by + oy + 1) * W + (s.bx + ox + 1);
+            const double* nd = G + key * N_FIELDS;
+            double dpx = (ox - s.fx) * dx, dpy = (oy - s.fy) * dx;
+            double nvx = water ? nd[N_WVX] : oil ? nd[N_OVX] : nd[N_VX];
+            double nvy = water ? nd[N_WVY] : oil ? nd[N_OVY] : nd[N_VY];
+            vx += w * nvx;
+            vy += w * nvy;
+            if (liquid) {
+                lm += w * (nd[N_WM] + nd[N_OM]);
+                lv += w * (nd[N_WV] + nd[N_OV]);
+            }
+            c00 += w * nvx * dpx; c01 += w * nvx * dpy;
+            c10 += w * nvy * dpx; c11 += w * nvy * dpy;
+        }
+    mat2 F = {p->f00, p->f01, p->f10, p->f11};
+    if (liquid) {
+        if (lm <= 0 || lv <= 0) return ORC_ENUMERIC;
+        double m = particle_mass(p, c);
+        double pd = m / (p->volume * det2(F));
+        double ld = lm / lv;
+        if (!finite(pd) || !finite(ld) || pd <= 0 || ld <= 0) return ORC_ENUMERIC;
+        double rel = (pd - ld) / pd;
+        vx += c->dt * c->gravity_x * rel;
+        vy += c->dt * c->gravity_y * rel;
+    }
+    mat2 C = {c00 * gscale, c01 * gscale, c10 * gscale, c11 * gscale};
+    if (!finite(vx) || !finite(vy) || !mat_finite(C)) return ORC_ENUMERIC;
+    mat2 trial;
+    e = update_deformation(F, C, c->dt, &trial);
+    if (e) return e;
+    double J = det2(trial);
+    if (!finite(J) || J <= 0) return ORC_EDETERMINANT;
+    mat2 Fn = trial;
+    double pv = p->plastic_volume;
+    if (p->model == ORC_MODEL_SNOW) e = project_snow(trial, pv, &Fn, &pv);
+    else if (p->model == ORC_MODEL_POWDER) e = project_powder(trial, p, &Fn);
+    else if (liquid) e = project_liquid(trial, &Fn);
+    else {
+        double tr = trial.m00 * trial.m00 + trial.m01 * trial.m01 + trial.m10 * trial.m10 + trial.m11 * trial.m11;
+        if (!finite(tr)) return ORC_ENUMERIC;
+        if (J < MIN_NEO_HOOKEAN_STRETCH * sqrt(tr) || tr > MAX_NEO_HOOKEAN_STRETCH * MAX_NEO_HOOKEAN_STRETCH)
+            e = project_nh(trial, &Fn);
+    }
+    if (e) return e;
+    double nx = p->x + c->dt * vx, ny = p->y + c->dt * vy;
+    if (!finite(nx) || !finite(ny)) return ORC_ENUMERIC;
+    double bx, bvx, by, bvy, fx, fvx, fy, fvy;
+    confine_axis(nx, vx, Wd, &bx, &bvx);
+    confine_axis(ny, vy, Hd, &by, &bvy);
+    if (tf && tf->nbodies > 0 && liquid) {
+        e = project_outside(c, tf, sqrt((p->volume * det2(Fn)) / ORC_PI), &bx, &by);
+        if (e) return e;
+    }
+    confine_axis(bx, bvx, Wd, &fx, &fvx);
+    confine_axis(by, bvy, Hd, &fy, &fvy);
+    q->x = fx; q->y = fy; q->vx = fvx; q->vy = fvy;
+    q->c00 = C.m00; q->c01 = C.m01; q->c10 = C.m10; q->c11 = C.m11;
+    q->f00 = Fn.m00; q->f01 = Fn.m01; q->f10 = Fn.m10; q->f11 = Fn.m11;
+    q->plastic_volume = pv;
+    return ORC_OK;
+}
+
 static int step_core(const orc_particle* in, int32_t n, const orc_config* c, tri_field* tf, orc_particle* out,
                      orc_diagnostics* dgp) {
     int e = ORC_OK;
     const long W = c->cells_x + 3, H = c->cells_y + 3;
-    const double dx = c->cell_size, gscale = 4.0 / (dx * dx);
     double* G = (double*)malloc(sizeof(double) * (size_t)(W * H) * N_FIELDS);
     if (!G) return ORC_EINVALID;
     orc_diagnostics dg;
@@ -625,81 +724,7 @@ static int step_core(const orc_particle* in, int32_t n, const orc_config* c, tri
     e = p2g_and_update(in, n, c, G, &dg, tf);
     if (e) { free(G); return e; }
 
-    const double Wd = c->cells_x * dx, Hd = c->cells_y * dx;
-    for (int32_t i = 0; i < n && !e; ++i) {
-        const orc_particle* p = &in[i];
-        orc_particle* q = &out[i];
-        *q = *p;
-        if (p->model == ORC_MODEL_REMOVED) continue;
-        /* G2P (solver.ts:610-773) */
-        stencil_t s;
-        make_stencil(p->x, p->y, dx, &s);
-        const int water = p->model == ORC_MODEL_WATER, oil = p->model == ORC_MODEL_OIL, liquid = water || oil;
-        double vx = 0, vy = 0, c00 = 0, c01 = 0, c10 = 0, c11 = 0, lm = 0, lv = 0;
-        for (int oy = 0; oy < 3; ++oy)
-            for (int ox = 0; ox < 3; ++ox) {
-                double w = s.wx[ox] * s.wy[oy];
-                if (w == 0) continue;
-                long key = (s.by + oy + 1) * W + (s.bx + ox + 1);
-                const double* nd = G + key * N_FIELDS;
-                double dpx = (ox - s.fx) * dx, dpy = (oy - s.fy) * dx;
-                double nvx = water ? nd[N_WVX] : oil ? nd[N_OVX] : nd[N_VX];
-                double nvy = water ? nd[N_WVY] : oil ? nd[N_OVY] : nd[N_VY];
-                vx += w * nvx;
-                vy += w * nvy;
-                if (liquid) {
-                    lm += w * (nd[N_WM] + nd[N_OM]);
-                    lv += w * (nd[N_WV] + nd[N_OV]);
-                }
-                c00 += w * nvx * dpx; c01 += w * nvx * dpy;
-                c10 += w * nvy * dpx; c11 += w * nvy * dpy;
-            }
-        mat2 F = {p->f00, p->f01, p->f10, p->f11};
-        if (liquid) {
-            if (lm <= 0 || lv <= 0) { e = ORC_ENUMERIC; break; }
-            double m = particle_mass(p, c);
-            double pd = m / (p->volume * det2(F));
-            double ld = lm / lv;
-            if (!finite(pd) || !finite(ld) || pd <= 0 || ld <= 0) { e = ORC_ENUMERIC; break; }
-            double rel = (pd - ld) / pd;
-            vx += c->dt * c->gravity_x * rel;
-            vy += c->dt * c->gravity_y * rel;
-        }
-        mat2 C = {c00 * gscale, c01 * gscale, c10 * gscale, c11 * gscale};
-        if (!finite(vx) || !finite(vy) || !mat_finite(C)) { e = ORC_ENUMERIC; break; }
-        mat2 trial;
-        e = update_deformation(F, C, c->dt, &trial);
-        if (e) break;
-        double J = det2(trial);
-        if (!finite(J) || J <= 0) { e = ORC_EDETERMINANT; break; }
-        mat2 Fn = trial;
-        double pv = p->plastic_volume;
-        if (p->model == ORC_MODEL_SNOW) e = project_snow(trial, pv, &Fn, &pv);
-        else if (p->model == ORC_MODEL_POWDER) e = project_powder(trial, p, &Fn);
-        else if (liquid) e = project_liquid(trial, &Fn);
-        else {
-            double tr = trial.m00 * trial.m00 + trial.m01 * trial.m01 + trial.m10 * trial.m10 + trial.m11 * trial.m11;
-            if (!finite(tr)) { e = ORC_ENUMERIC; break; }
-            if (J < MIN_NEO_HOOKEAN_STRETCH * sqrt(tr) || tr > MAX_NEO_HOOKEAN_STRETCH * MAX_NEO_HOOKEAN_STRETCH)
-                e = project_nh(trial, &Fn);
-        }
-        if (e) break;
-        double nx = p->x + c->dt * vx, ny = p->y + c->dt * vy;
-        if (!finite(nx) || !finite(ny)) { e = ORC_ENUMERIC; break; }
-        double bx, bvx, by, bvy, fx, fvx, fy, fvy;
-        confine_axis(nx, vx, Wd, &bx, &bvx);
-        confine_axis(ny, vy, Hd, &by, &bvy);
-        if (tf && tf->nbodies > 0 && liquid) {
-            e = project_outside(c, tf, sqrt((p->volume * det2(Fn)) / ORC_PI), &bx, &by);
-            if (e) break;
-        }
-        confine_axis(bx, bvx, Wd, &fx, &fvx);
-        confine_axis(by, bvy, Hd, &fy, &fvy);
-        q->x = fx; q->y = fy; q->vx = fvx; q->vy = fvy;
-        q->c00 = C.m00; q->c01 = C.m01; q->c10 = C.m10; q->c11 = C.m11;
-        q->f00 = Fn.m00; q->f01 = Fn.m01; q->f10 = Fn.m10; q->f11 = Fn.m11;
-        q->plastic_volume = pv;
-    }
+    for (int32_t i = 0; i < n && !e; ++i) e = g2p_particle(&in[i], &out[i], c, G, tf);
     free(G);
     *dgp = dg;
     return e;
@@ -737,6 +762,115 @@ int orc_run_f64(orc_particle* p, int32_t n, const orc_config* c, int32_t steps) 
     }
     free(tmp);
     return ORC_OK;
+}
+
+/* All-core timing variant of orc_run_f64 (the CPU baseline of the
+ * benchmark; the checker is always orc_step_f64).  Per step: validation in
+ * chunks, then P2G scattered by bands of stencil-base rows with `threads`
+ * OpenMP threads.  A particle whose base row is r writes node rows r..r+2, so
+ * bands of at least three rows taken in two passes, even bands then odd
+ * ones, never write a node at the same time.  The grid update is a parallel
+ * loop over node rows (a node reads its neighbours' volumes and writes only
+ * its own velocities) and G2P a parallel loop over particles into the other
+ * of two buffers.  Node sums follow the band order rather than the particle
+ * order, so results equal orc_run_f64's to rounding. */
+int orc_run_f64_parallel(orc_particle* p, int32_t n, const orc_config* c, int32_t steps, int32_t threads) {
+    if (threads < 1) threads = 1;
+    const long W = c->cells_x + 3, H = c->cells_y + 3;
+    const long rows = (long)c->cells_y + 2;  /* stencil-base rows by + 1 in [0, cells_y + 1] */
+    long band = rows / (4L * threads);
+    if (band < 3) band = 3;
+    const long nbands = (rows + band - 1) / band;
+    double* G = (double*)malloc(sizeof(double) * (size_t)(W * H) * N_FIELDS);
+    orc_particle* tmp = (orc_particle*)malloc(sizeof(orc_particle) * (size_t)(n > 0 ? n : 1));
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t* band_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    long* start = (long*)malloc(sizeof(long) * (size_t)(nbands + 1));
+    int err = ORC_OK;
+    if (!G || !tmp || !order || !band_of || !start) err = ORC_EINVALID;
+    orc_particle* cur = p;  /* the state the step starts from; the two buffers swap each step */
+    orc_particle* nxt = tmp;
+    for (int32_t st = 0; st < steps && !err; ++st) {
+        /* the first failing chunk's error, as the serial scan would report */
+        const int32_t chunk = (n + threads - 1) / threads;
+        int first = threads, code = validate(cur, 0, c);
+        if (!code) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int t = 0; t < threads; ++t) {
+                const int32_t lo = t * chunk, hi = lo + chunk < n ? lo + chunk : n;
+                const int e = lo < hi ? validate(cur + lo, hi - lo, c) : ORC_OK;
+                if (e) {
+#pragma omp critical
+                    {
+                        if (t < first) { first = t; code = e; }
+                    }
+                }
+            }
+        }
+        err = code;
+        if (err) break;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int32_t i = 0; i < n; ++i) {
+            if (cur[i].model == ORC_MODEL_REMOVED) { band_of[i] = -1; continue; }
+            stencil_t s;
+            make_stencil(cur[i].x, cur[i].y, c->cell_size, &s);
+            long r = s.by + 1;
+            if (r < 0) r = 0;
+            if (r >= rows) r = rows - 1;
+            band_of[i] = (int32_t)(r / band);
+        }
+        for (long b = 0; b <= nbands; ++b) start[b] = 0;
+        for (int32_t i = 0; i < n; ++i)
+            if (band_of[i] >= 0) ++start[band_of[i] + 1];
+        for (long b = 0; b < nbands; ++b) start[b + 1] += start[b];
+        for (int32_t i = 0; i < n; ++i)
+            if (band_of[i] >= 0) order[start[band_of[i]]++] = i;
+        for (long b = nbands; b > 0; --b) start[b] = start[b - 1];
+        start[0] = 0;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (long k = 0; k < W * H * N_FIELDS; ++k) G[k] = 0.0;
+        for (int parity = 0; parity < 2; ++parity) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+            for (long b = parity; b < nbands; b += 2)
+                for (long k = start[b]; k < start[b + 1]; ++k) {
+                    const int e = p2g_particle(&cur[order[k]], c, G);
+                    if (e) {
+#pragma omp atomic write
+                        err = e;
+                    }
+                }
+        }
+        if (err) break;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (long sy = 0; sy < H; ++sy)
+            for (long sx = 0; sx < W; ++sx) {
+                const int e = grid_node(cur, c, G, sx, sy, NULL);
+                if (e) {
+#pragma omp atomic write
+                    err = e;
+                }
+            }
+        if (err) break;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int32_t i = 0; i < n; ++i) {
+            const int e = g2p_particle(&cur[i], &nxt[i], c, G, NULL);
+            if (e) {
+#pragma omp atomic write
+                err = e;
+            }
+        }
+        if (err) break;
+        orc_particle* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    if (!err && cur != p) memcpy(p, cur, sizeof(orc_particle) * (size_t)n);
+    free(G);
+    free(tmp);
+    free(order);
+    free(band_of);
+    free(start);
+    return err;
 }
 
 /* ---- config-2 step: bonded triangle bodies and fracture bonds ---- */

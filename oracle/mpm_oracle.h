@@ -133,6 +133,12 @@ int orc_triangle_pair_contact(const double* a6, const double* b6, double skin, d
 /* Run `steps` FP64 steps in place (buffer `p`); returns the first error. */
 int orc_run_f64(orc_particle* p, int32_t n, const orc_config* cfg, int32_t steps);
 
+/* orc_run_f64 on `threads` OpenMP threads: the all-core CPU baseline of the
+ * benchmark, never a checker.  P2G sums follow bands of grid rows instead of
+ * the particle order, so results equal orc_run_f64's to rounding
+ * (tests/test_oracle.py compares them). */
+int orc_run_f64_parallel(orc_particle* p, int32_t n, const orc_config* cfg, int32_t steps, int32_t threads);
+
 /* FP64 grid after P2G + grid update of one step (for component checks).
  * node_out has 17 doubles per node (nodeCount = (cx+3)*(cy+3)):
  * [mass, qx, qy, vx, vy, wm, wqx, wqy, wvx, wvy, wV, om, oqx, oqy, ovx, ovy, oV]. */

@@ -143,6 +143,19 @@ def run_f64(particles: np.ndarray, cfg: Config, steps: int) -> np.ndarray:
     return p
 
 
+def run_f64_parallel(particles: np.ndarray, cfg: Config, steps: int, threads: int) -> np.ndarray:
+    """run_f64 on `threads` OpenMP threads (the all-core CPU baseline; equal
+    to run_f64 to rounding)."""
+    L = lib()
+    L.orc_run_f64_parallel.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Config), ctypes.c_int32,
+                                       ctypes.c_int32]
+    p = particles.copy()
+    code = L.orc_run_f64_parallel(_ptr(p), len(p), ctypes.byref(cfg), steps, threads)
+    if code:
+        raise OracleError(code)
+    return p
+
+
 GRID_FIELDS = ["m", "qx", "qy", "vx", "vy", "wm", "wqx", "wqy", "wvx", "wvy", "wV",
                "om", "oqx", "oqy", "ovx", "ovy", "oV"]
 

@@ -304,3 +304,30 @@ def test_oracle_reproduces_golden_tank_macrostep(orc):
     out = orc.run_f64(orc.particles_from_packed(g["tank_in"]), cfg, int(steps))
     got = np.stack([out[k] for k in GOLDEN_COLS], axis=1)
     assert np.max(np.abs(got - g["tank_out"])) <= 1e-12
+
+
+@pytest.mark.parametrize("threads", [1, 3, 4])
+def test_all_core_baseline_equals_the_serial_oracle(orc, pkg, threads):
+    """orc_run_f64_parallel (the benchmark's all-core CPU baseline) steps the
+    same scene as orc_run_f64, P2G sums reordered by row bands: equal to
+    rounding, and an invalid particle fails it the same way."""
+    rng = np.random.default_rng(11)
+    arr, config = pkg.scenes.water_oil_column(cells=40)
+    parts = orc.particles_from_packed(pkg.layout.validate_input(arr, config).packed)
+    # a few solids among the liquids, to run every constitutive branch
+    for k, model in enumerate((0, 1, 4)):
+        idx = rng.choice(len(parts), 30, replace=False) + 0
+        parts["model"][idx] = model
+        parts["lambda_"][idx], parts["mu"][idx], parts["param"][idx] = 40.0, 20.0, 0.3 if model == 4 else 0.0
+        parts["volume"][idx] = 1.0
+    c = orc.make_config(1e-3, 40, 40, 1.0, 0.02, 1.0, (0.0, -9.81))
+    serial = orc.run_f64(parts, c, 4)
+    par = orc.run_f64_parallel(parts, c, 4, threads)
+    for name in ("x", "y", "vx", "vy", "c00", "c01", "c10", "c11", "f00", "f01", "f10", "f11", "plastic_volume"):
+        a, b = serial[name], par[name]
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-11 * max(1.0, float(np.abs(a).max()))), name
+    assert np.array_equal(serial["model"], par["model"])
+    bad = parts.copy()
+    bad["x"][5] = -1.0  # outside the domain
+    with pytest.raises(orc.OracleError):
+        orc.run_f64_parallel(bad, c, 1, threads)
