@@ -110,22 +110,122 @@ def timed(pkg, solver, plan, limit, steps, warmup):
     return dt, speed
 
 
-def cpu_baseline(pkg, arr, config, budget_s=12.0):
+def baseline_threads() -> int:
+    """Host threads of the all-core CPU baseline: OMP_NUM_THREADS when set
+    (the GPU host's CPU share), else the CPU count, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(pkg, arr, config, budget_s=12.0, threads=1):
+    """The FP64 oracle step on the same scene: one thread (orc_run_f64) or
+    `threads` OpenMP threads (orc_run_f64_parallel).  Returns
+    (particle-steps/s, steps, seconds)."""
     from oracle import oracle as orc
     state = pkg.layout.validate_input(arr, config)
     parts = orc.particles_from_packed(state.packed)
     cfg = orc.make_config(1.0 / 9600.0, config["grid"]["cellsX"], config["grid"]["cellsY"], 1.0, 0.02, 1.0,
                           (0.0, 90.0))
-    orc.run_f64(parts, cfg, 1)  # warm-up step
+
+    def step(p):
+        return orc.run_f64(p, cfg, 1) if threads == 1 else orc.run_f64_parallel(p, cfg, 1, threads)
+
+    parts = step(parts)  # warm-up step
     steps = 0
     t0 = time.perf_counter()
     while True:
-        parts = orc.run_f64(parts, cfg, 1)
+        parts = step(parts)
         steps += 1
         el = time.perf_counter() - t0
         if (steps >= 3 and el > budget_s * 0.5) or el > budget_s:
             break
     return len(arr) * steps / el, steps, el
+
+
+def cfl_macrosteps(pkg, solver, macro_dt, gravity, count):
+    """`count` macrosteps through the adaptive CFL loop (stepCflBatchAdaptive,
+    solver.ts:1724-2021), each planned from the bound the previous one
+    recommends, as the reference's page loop does.  Returns the microsteps
+    run."""
+    cell = solver.state.config["grid"]["cellSize"]
+    g = float(np.hypot(gravity["x"], gravity["y"]))
+    bound = max(pkg.cfl.calculate_velocity_bound(solver.cfl_bounds.velocityBound, macro_dt, g),
+                solver._last_scheduled_velocity_bound or 0.0)
+    micro = 0
+    for _ in range(count):
+        plan = pkg.cfl.calculate_microsteps(macro_dt, cell, solver.cfl_bounds.waveSpeedBound, bound, 1024)
+        m = solver.step_cfl_batch_adaptive(1, macro_dt, plan, bound, 1024, gravity)
+        micro += m.attemptedMicroStepCount
+        # the next bound covers what the GPU measured and never drops below
+        # the envelope the solver last scheduled (a retry or continuation
+        # inside the adaptive loop may have raised it)
+        bound = max(bound, pkg.cfl.calculate_velocity_bound(m.recommendedVelocityBound, macro_dt, g),
+                    solver._last_scheduled_velocity_bound or 0.0)
+    solver.check_numerical_status()
+    return micro
+
+
+def small_scenes_leg(pkg, macrosteps: int = 12):
+    """The reference's own scenes, which are launch-bound at this size: the SI
+    Tait tank (1,600 particles, 1/120 s macrosteps, both P2G modes) and the
+    Gate 4 scene (3,674 particles with bonded pebbles and a fracture solid,
+    1/240 s macrosteps).  Each is stepped through the adaptive CFL loop with
+    its host work and status readback per macrostep, after one warm-up
+    macrostep; rate = particles x microsteps / wall seconds."""
+    out = {}
+    particles, config, macro_dt, _ = pkg.scenes.tait_tank()
+    n = len(particles)
+    for mode in ("gather", "fixed"):
+        factory = (pkg.create_mpm_mls_gpu_solver if mode == "gather"
+                   else pkg.create_experimental_mpm_mls_atomic_gpu_solver)
+        s = factory(particles, config, device=0)
+        try:
+            cfl_macrosteps(pkg, s, macro_dt, pkg.scenes.TAIT_TANK_GRAVITY, 1)
+            t0 = time.perf_counter()
+            micro = cfl_macrosteps(pkg, s, macro_dt, pkg.scenes.TAIT_TANK_GRAVITY, macrosteps)
+            el = time.perf_counter() - t0
+        finally:
+            s.dispose()
+        out[f"tait_tank_{mode}"] = {"value": n * micro / el, "unit": "particle-steps/s", "particles": n,
+                                    "macrosteps": macrosteps, "microsteps": micro,
+                                    "ms_per_microstep": 1e3 * el / micro, "ms_per_macrostep": 1e3 * el / macrosteps}
+        log(f"[bench] Tait tank ({mode}): {micro} microsteps in {1e3 * el:.1f} ms, {n * micro / el:.3e} particle-steps/s")
+    # Gate 4 with the reference's own batch protocol (docs/PERFORMANCE-MLS-MPM.md:
+    # 622-636): the 220 x 150 grid of cell size 10, liquid area 100, fixed
+    # dt = 1/240 and gravity (0, 90); 64 warm-up steps, then 12 batches of 8
+    # steps, each synchronised; median and p95 per step
+    particles, config, gravity = pkg.scenes.gate4_scene(mpm_grid=False, liquid_area=100.0)
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+    try:
+        n = s.particle_count
+        for _ in range(8):
+            s.step_batch(8, 1.0 / 240.0, gravity)
+        s.ctx.read_compact()
+        per_step = []
+        for _ in range(12):
+            t0 = time.perf_counter()
+            s.step_batch(8, 1.0 / 240.0, gravity)
+            s.ctx.read_compact()
+            per_step.append((time.perf_counter() - t0) / 8)
+        s.check_numerical_status()
+    finally:
+        s.dispose()
+    med = float(np.median(per_step))
+    out["gate4"] = {"value": n / med, "unit": "particle-steps/s", "particles": n,
+                    "ms_per_step_median": 1e3 * med, "ms_per_step_p95": 1e3 * float(np.quantile(per_step, 0.95)),
+                    "protocol": "220x150 grid, cell 10, liquid area 100, dt 1/240, gravity (0, 90); 64 warm-up "
+                                "steps, 12 batches of 8 steps, wall clock per batch including its synchronisation"}
+    log(f"[bench] Gate 4: {1e3 * med:.4f} ms per step (median of 12 batches of 8), {n / med:.3e} particle-steps/s")
+    out["reference_published"] = {
+        "tait_tank": "23.89e6 particle-steps/s, 1,600 particles, 423-microstep batch, Apple Metal 3 "
+                     "(docs/PERFORMANCE-MLS-MPM.md:837-851); ~4.8e7 with fixed-point P2G "
+                     "(docs/REPORT-FLUID-SOLVER-PROTOTYPES.md:207-229)",
+        "gate4": "0.216618 ms per step (median, GPU timestamps) on Apple Metal 3, same protocol "
+                 "(docs/PERFORMANCE-MLS-MPM.md:622-636); CPU reference 4.643 ms per step, one JS thread "
+                 "(docs/PERFORMANCE-MLS-MPM.md:778-790)"}
+    return out
 
 
 def batched_leg(pkg, steps: int):
@@ -277,6 +377,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=480)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=5)
+    ap.add_argument("--no-small", action="store_true", help="skip the Tait tank and Gate 4 leg")
     ap.add_argument("--tuning", default="auto",
                     help='launch shape: "auto" (autotune before timing), "default", or '
                          '"p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
@@ -346,18 +447,27 @@ def main():
 
     batched = None if args.no_batched or args.cells != 1000 else batched_leg(pkg, args.batched_steps)
     adjoint = None if args.no_adjoint else adjoint_leg(pkg, args.adjoint_reps)
+    small = None if args.no_small else small_scenes_leg(pkg)
 
     cpu = None
     if not args.no_cpu:
+        threads = baseline_threads()
+        log(f"[bench] CPU baseline: FP64 oracle step, {threads} threads")
+        rate, steps, el = cpu_baseline(pkg, arr, config, budget_s=8.0, threads=threads)
         log("[bench] CPU baseline: FP64 oracle step, one thread")
-        rate, steps, el = cpu_baseline(pkg, arr, config)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{steps} FP64 oracle microsteps of the full {n}-particle scene ({el:.1f} s, after 1 "
-                         "warm-up step)"}
+        rate1, steps1, el1 = cpu_baseline(pkg, arr, config)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{steps} FP64 oracle microsteps of the full {n}-particle scene on {threads} OpenMP "
+                         f"threads ({el:.1f} s, after 1 warm-up step)",
+               "single_thread": {"value": rate1, "cores": 1,
+                                 "sample": f"{steps1} FP64 oracle microsteps, one thread ({el1:.1f} s)"},
+               "gpu_over_cpu": None}
         if adjoint is not None:
             log("[bench] CPU baseline: FP64 oracle adjoint of the elastic block, one thread")
             cpu["adjoint"] = adjoint_cpu(pkg)
 
+    if cpu is not None:
+        cpu["gpu_over_cpu"] = r["rate"] / cpu["value"]
     line = {
         "metric": METRIC,
         "value": r["rate"],
@@ -386,6 +496,7 @@ def main():
         "tuning": tunings,
         "batched": batched,
         "adjoint": adjoint,
+        "small_scenes": small,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1997,7 +1997,8 @@ __device__ __forceinline__ void copy_slot(const SlotBuf& b, size_t from, size_t 
     b.plastic[to] = b.plastic[from];
 }
 
-constexpr int BODY_THREADS = 256;
+constexpr int BODY_THREADS = 1024;  // 16 waves: the body passes are chains of dependent loads
+constexpr int FRACTURE_THREADS = 256;  // the endpoint stress cache needs more than 128 registers
 constexpr uint32_t MAX_BODIES = 1024;  // bodies per scene (checked on the host)
 
 // After G2P: applyTriangleContactImpulses, enforceBondedTriangleConstraints,
@@ -2017,16 +2018,22 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     const uint32_t* in_slot = A.slot[in_buf];
     uint32_t* out_slot = A.slot[out_buf];
     const uint32_t nb = A.nbodies, tid = threadIdx.x;
-    __shared__ float s_red[BODY_THREADS][6];
     __shared__ double s_tri[MAX_BODIES][6];
     __shared__ float s_vel[MAX_BODIES][2];
     __shared__ uint32_t s_cnt[MAX_BODIES];
+    __shared__ uint8_t s_moved[3 * MAX_BODIES];  // vertex left its bucket: re-slot it
+    __shared__ uint32_t s_part_n[BODY_THREADS];  // per-thread partial contact counts
+    __shared__ float s_part_i[BODY_THREADS][2];  // per-thread partial contact impulses
     uint32_t err = 0u, err_index = NO_AUX;
     const double dx = P.dx, band = (double)P.dx * TRIANGLE_FIELD_BAND_CELLS;
     const float inv_mass = 1.0f / P.particle_mass;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    constexpr uint32_t BODY_WAVES = BODY_THREADS / 64;
 
-    // field impulses on each body, barycentric-weighted, summed in a fixed order
-    for (uint32_t b = 0; b < nb; ++b) {
+    // field impulses on each body, barycentric-weighted: one wave a body, each
+    // lane summing every 64th node of the body's box, then the wave's lanes
+    // in a fixed tree, so the sum's order is fixed
+    for (uint32_t b = wave; b < nb; b += BODY_WAVES) {
         double t[6];
         load_triangle(in, in_slot, b, t);
         const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
@@ -2038,7 +2045,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
         float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (x1 >= x0 && y1 >= y0) {
             const uint32_t w = (uint32_t)(x1 - x0 + 1), count = w * (uint32_t)(y1 - y0 + 1);
-            for (uint32_t i = tid; i < count; i += BODY_THREADS) {
+            for (uint32_t i = lane; i < count; i += 64u) {
                 const uint32_t node = (uint32_t)(y0 + (int)(i / w) + 1) * P.node_w + (uint32_t)(x0 + (int)(i % w) + 1);
                 if (A.field_body[node] != b + 1u) continue;
                 const FieldNode fn = A.field[node];
@@ -2050,28 +2057,27 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
                 acc[5] += fn.b2 * fn.iy;
             }
         }
-        for (int k = 0; k < 6; ++k) s_red[tid][k] = acc[k];
-        __syncthreads();
-        for (uint32_t stride = BODY_THREADS / 2; stride > 0; stride >>= 1) {
-            if (tid < stride)
-                for (int k = 0; k < 6; ++k) s_red[tid][k] += s_red[tid + stride][k];
-            __syncthreads();
-        }
-        if (tid == 0) {
-            for (int v = 0; v < 3; ++v) {
-                const uint32_t sl = out_slot[3u * b + (uint32_t)v];
-                float2 vel = out.vel[sl];
-                vel.x = vel.x + s_red[0][2 * v] * inv_mass;
-                vel.y = vel.y + s_red[0][2 * v + 1] * inv_mass;
-                if (!finite2(vel)) {
-                    err |= ERR_TRIANGLE_CONTACT;
-                    err_index = min(err_index, A.aux_pid[3u * b]);
-                }
-                out.vel[sl] = vel;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] += __shfl_down(acc[k], off, 64);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] = __shfl(acc[k], 0, 64);
+        if (lane < 3u) {  // lane v updates vertex v
+            const uint32_t sl = out_slot[3u * b + lane];
+            float2 vel = out.vel[sl];
+            const float ix = lane == 0u ? acc[0] : (lane == 1u ? acc[2] : acc[4]);
+            const float iy = lane == 0u ? acc[1] : (lane == 1u ? acc[3] : acc[5]);
+            vel.x = vel.x + ix * inv_mass;
+            vel.y = vel.y + iy * inv_mass;
+            if (!finite2(vel)) {
+                err |= ERR_TRIANGLE_CONTACT;
+                err_index = min(err_index, A.aux_pid[3u * b]);
             }
+            out.vel[sl] = vel;
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     // two Gauss-Seidel passes over the edges of each triangle, in FP64 offsets
     // from the first vertex (exact differences of the float32 positions), so
@@ -2143,22 +2149,38 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
         }
         __syncthreads();
         const double skin = (double)P.dx * TRIANGLE_CONTACT_SKIN_CELLS;
-        for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
+        // a team of threads per body, each testing every team-th other body;
+        // the team's partial counts and impulses are then added in thread order
+        const uint32_t team = max(1u, (uint32_t)BODY_THREADS / nb);
+        const uint32_t teams = (uint32_t)BODY_THREADS / team;
+        const uint32_t my_team = tid / team, t_in = tid % team;
+        for (uint32_t b0 = 0; b0 < nb; b0 += teams) {
+            const uint32_t b = b0 + my_team;
             uint32_t c = 0u;
-            for (uint32_t j = 0; j < nb; ++j) {
-                if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
-                double nx, ny;
-                if (b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny) : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny))
-                    ++c;
+            if (my_team < teams && b < nb) {
+                for (uint32_t j = t_in; j < nb; j += team) {
+                    if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
+                    double nx, ny;
+                    if (b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
+                              : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny))
+                        ++c;
+                }
             }
-            s_cnt[b] = c;
+            s_part_n[tid] = c;
+            __syncthreads();
+            if (my_team < teams && b < nb && t_in == 0u) {
+                uint32_t sum = 0u;
+                for (uint32_t q = 0; q < team; ++q) sum += s_part_n[tid + q];
+                s_cnt[b] = sum;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         const float body_mass = 3.0f * P.particle_mass;
-        for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
+        for (uint32_t b0 = 0; b0 < nb; b0 += teams) {
+            const uint32_t b = b0 + my_team;
             float ix = 0.0f, iy = 0.0f;
-            if (s_cnt[b] != 0u) {
-                for (uint32_t j = 0; j < nb; ++j) {
+            if (my_team < teams && b < nb && s_cnt[b] != 0u) {
+                for (uint32_t j = t_in; j < nb; j += team) {
                     if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
                     double nx, ny;
                     const bool hit = b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
@@ -2178,16 +2200,26 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
                     }
                 }
             }
-            for (int v = 0; v < 3; ++v) {
-                const uint32_t sl = out_slot[3u * b + (uint32_t)v];
-                const float2 p = out.pos[sl], vel = out.vel[sl];
-                const float2 bx = confine_axis(p.x, vel.x + ix, P.dom_w);
-                const float2 by = confine_axis(p.y, vel.y + iy, P.dom_h);
-                out.pos[sl] = make_float2(bx.x, by.x);
-                out.vel[sl] = make_float2(bx.y, by.y);
+            s_part_i[tid][0] = ix;
+            s_part_i[tid][1] = iy;
+            __syncthreads();
+            if (my_team < teams && b < nb && t_in < 3u) {  // the team's first threads: the three vertices
+                float tx = 0.0f, ty = 0.0f;
+                for (uint32_t q = 0; q < team; ++q) {
+                    tx += s_part_i[tid - t_in + q][0];
+                    ty += s_part_i[tid - t_in + q][1];
+                }
+                for (uint32_t v = t_in; v < 3u; v += team) {
+                    const uint32_t sl = out_slot[3u * b + v];
+                    const float2 p = out.pos[sl], vel = out.vel[sl];
+                    const float2 bx = confine_axis(p.x, vel.x + tx, P.dom_w);
+                    const float2 by = confine_axis(p.y, vel.y + ty, P.dom_h);
+                    out.pos[sl] = make_float2(bx.x, by.x);
+                    out.vel[sl] = make_float2(bx.y, by.y);
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
     // CFL check of the vertices (their velocities changed after G2P)
@@ -2208,10 +2240,20 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     if (err) raise_step_at(ctl, err, step, err_index);
     __syncthreads();
 
-    // a vertex moved by the contacts keeps the bucket invariant: re-slot it
+    // a vertex moved by the contacts keeps the bucket invariant: re-slot it.
+    // Every thread tests its vertices; one thread then moves the few that
+    // left their bucket, in vertex order.
+    const uint32_t cap = S.cap;
+    for (uint32_t a = tid; a < 3u * nb; a += BODY_THREADS) {
+        const uint32_t src = out_slot[a];
+        Stencil st;
+        const bool fits = make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st);
+        s_moved[a] = (!fits || bucket_of(st, S.bw) != src / cap) ? 1u : 0u;
+    }
+    __syncthreads();
     if (tid == 0) {
-        const uint32_t cap = S.cap;
         for (uint32_t a = 0; a < 3u * nb; ++a) {
+            if (!s_moved[a]) continue;
             const uint32_t src = out_slot[a];
             Stencil st;
             if (!make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st)) {
@@ -2245,7 +2287,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
 // reference/solid-bonds.ts:68-108): bond damage from the tensile strain of
 // the output positions, then each endpoint's damage = max(its damage, its
 // bonds'), and the endpoint's stress cache with that damage.  One block.
-__global__ __launch_bounds__(BODY_THREADS) void k_fracture(const SceneDesc* __restrict__ scenes, uint32_t step) {
+__global__ __launch_bounds__(FRACTURE_THREADS) void k_fracture(const SceneDesc* __restrict__ scenes, uint32_t step) {
     const SceneDesc& S = scenes[0];
     Control* ctl = S.ctl;
     if (block_status_set(ctl)) return;
@@ -2255,7 +2297,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_fracture(const SceneDesc* __re
     const uint32_t* out_slot = A.slot[out_buf];
     const uint32_t tid = threadIdx.x;
     uint32_t err = 0u, err_index = NO_AUX;
-    for (uint32_t k = tid; k < A.nbonds; k += BODY_THREADS) {
+    for (uint32_t k = tid; k < A.nbonds; k += FRACTURE_THREADS) {
         Bond bd = A.bonds[k];
         if (bd.broken != 0u) {
             bd.damage = 1.0f;
@@ -2278,7 +2320,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_fracture(const SceneDesc* __re
         A.bonds[k] = bd;
     }
     __syncthreads();
-    for (uint32_t a = A.frac_aux0 + tid; a < A.naux; a += BODY_THREADS) {
+    for (uint32_t a = A.frac_aux0 + tid; a < A.naux; a += FRACTURE_THREADS) {
         float dmg = fminf(fmaxf(A.damage[in_buf][a], 0.0f), 1.0f);
         for (uint32_t q = A.bond_off[a]; q < A.bond_off[a + 1]; ++q) dmg = fmaxf(dmg, A.bonds[A.bond_inc[q]].damage);
         A.damage[out_buf][a] = dmg;
@@ -2526,7 +2568,7 @@ hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure) {
         if (aux && sb.aux_bodies)
             MPM_LAUNCH("bodies", k_bodies, dim3(1), dim3(BODY_THREADS), 0, lc.stream, sb.d_scenes, s);
         if (aux && sb.aux_bonds)
-            MPM_LAUNCH("fracture", k_fracture, dim3(1), dim3(BODY_THREADS), 0, lc.stream, sb.d_scenes, s);
+            MPM_LAUNCH("fracture", k_fracture, dim3(1), dim3(FRACTURE_THREADS), 0, lc.stream, sb.d_scenes, s);
     }
     MPM_LAUNCH("commit_last", k_commit_last, dim3(ns), dim3(64), 0, lc.stream, sb.d_scenes, steps);
     const uint32_t fb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(sb.max_slots, 256), ns == 1 ? 2048u : 64u));
