@@ -238,7 +238,10 @@ def batched_leg(pkg, steps: int):
     total = sum(len(a) for a, _, _ in tanks)
     dt = pkg.scenes.BATCH_DT
     log(f"[bench] batched: {len(tanks)} tanks, {total} particles, {steps} microsteps")
-    batch = pkg.MpmMlsGpuBatch(scenes, device=0)
+    t0 = time.perf_counter()
+    tuned = pkg.autotune.autotune_batch(scenes, dt, gravity, device=0)
+    log(f"[bench] batched: autotuned {tuned['tuning']} in {time.perf_counter() - t0:.1f} s")
+    batch = pkg.MpmMlsGpuBatch(scenes, device=0, tuning=tuned["tuning"])
     batch.step_batch(8, dt, gravity)
     batch.scene(0).ctx.read_compact()
     t0 = time.perf_counter()
@@ -266,6 +269,7 @@ def batched_leg(pkg, steps: int):
     return {"value": rate, "unit": "particle-steps/s", "ms_per_step": 1e3 * t_batch / steps, "scenes": len(tanks),
             "particles": total, "steps": steps, "sequential_value": rate_seq,
             "sequential_ms_per_step": 1e3 * t_seq / steps, "speedup_vs_sequential": rate / rate_seq,
+            "tuning": tuned,
             "workload": f"{len(tanks)} SI-material Tait tanks, {total} particles (log-uniform 500-50,000 per tank), "
                         "64-512 cell grids, dam-break blocks, random gravity directions, float32 gather P2G"}
 

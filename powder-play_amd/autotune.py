@@ -23,6 +23,9 @@ stepped), with the library's per-kernel HIP-event timing:
    time (the two kernels share no knob);
 3. bucket capacities around the derived one, ranked by the whole step.
 
+``autotune_batch`` does the same for a batch of scenes (one launch covers
+every scene), ranking each shape by the batch's microstep time.
+
 Each candidate is timed twice, interleaved, and keeps its faster pass.  The
 result is cached as JSON keyed by (GPU architecture and compute units, scene
 class); ``tuning_for`` returns a cached entry or tunes and stores one.
@@ -254,6 +257,40 @@ def autotune(particles, config: dict, p2g_mode: int = MPM_P2G_GATHER, *, dt: flo
                 "candidates": log, "when": _dt.datetime.now(_dt.timezone.utc).isoformat(timespec="seconds")}
     finally:
         trial.close()
+
+
+def autotune_batch(scenes, dt, gravity, p2g_mode: int = MPM_P2G_GATHER, *, device: int = 0, warmup: int = 2,
+                   steps: int = 8, p2g_tiles=P2G_TILES, g2p_shapes=G2P_SHAPES) -> dict:
+    """Launch shape of a batch of scenes (MpmMlsGpuBatch): every compiled
+    shape is timed on a scratch batch built from the same scenes, by the wall
+    time of `steps` microsteps of all scenes (a batch launch covers them all,
+    so the two kernels are ranked together), each shape twice, interleaved,
+    keeping its faster pass.  The scenes keep their bucket capacities.
+    Returns {"tuning": {...}, "candidates": {shape: microstep microseconds}}."""
+    from .solver import MpmMlsGpuBatch
+    batch = MpmMlsGpuBatch(scenes, p2g_mode=p2g_mode, device=device)
+    try:
+        times: dict[str, float] = {}
+        effective: dict[str, dict] = {}
+        for _ in range(2):
+            for t in p2g_tiles:
+                for tb, th in g2p_shapes:
+                    key = f"{t}/{tb}x{th}"
+                    batch.native.set_tuning(_tuning(t, tb, th, 0))
+                    batch.step_batch(warmup, dt, gravity)
+                    batch.scene(0).ctx.read_compact()  # the scenes share one stream
+                    t0 = time.perf_counter()
+                    batch.step_batch(steps, dt, gravity)
+                    batch.scene(0).ctx.read_compact()
+                    us = 1e6 * (time.perf_counter() - t0) / steps
+                    times[key] = min(times.get(key, us), us)
+                    effective[key] = batch.native.get_tuning().as_dict()
+        for i in range(len(batch)):
+            batch.scene(i).check_numerical_status()
+        best = min(times, key=times.get)
+        return {"tuning": effective[best], "candidates": {k: round(v, 2) for k, v in times.items()}}
+    finally:
+        batch.dispose()
 
 
 def tuning_for(particles, config: dict, p2g_mode: int = MPM_P2G_GATHER, *, cache: pathlib.Path | str | None = None,

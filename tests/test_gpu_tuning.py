@@ -205,3 +205,12 @@ def test_autotuner_picks_a_compiled_shape_and_caches_it(pkg, tmp_path, monkeypat
                                                                           t["g2p_threads"])
     finally:
         s.dispose()
+
+
+def test_batch_autotuner_picks_a_compiled_shape(pkg):
+    members = [pkg.scenes.water_oil_column(cells=40), mixed_scene(pkg, seed=5), pkg.scenes.water_oil_column(cells=24)]
+    r = pkg.autotune.autotune_batch(members, 2e-4, GRAVITY, device=0, steps=2, warmup=1)
+    t = r["tuning"]
+    assert pkg.autotune.shape_supported(t["p2g_tile"], t["g2p_tile"], t["g2p_threads"])
+    assert len(r["candidates"]) == len(pkg.autotune.P2G_TILES) * len(pkg.autotune.G2P_SHAPES)
+    assert all(v > 0 for v in r["candidates"].values())
