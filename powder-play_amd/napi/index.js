@@ -53,6 +53,25 @@ class MpmNativeRuntime {
 	readDiagnostics() {
 		return addon.readDiagnostics(this.ctx);
 	}
+
+	// config.fractureBonds as 6-word records (solver.ts:929-935), and their state
+	setFractureBonds(records) {
+		addon.setFractureBonds(this.ctx, records);
+	}
+
+	readFractureBonds(count) {
+		return addon.readFractureBonds(this.ctx, count);
+	}
+
+	// launch shape and bucket capacity ({ p2gTile, g2pTile, g2pThreads,
+	// bucketCapacity? }); results do not depend on them
+	setTuning(tuning) {
+		addon.setTuning(this.ctx, tuning);
+	}
+
+	getTuning() {
+		return addon.getTuning(this.ctx);
+	}
 }
 
 module.exports = { addon, MpmNativeRuntime };

@@ -387,6 +387,86 @@ static napi_value read_diagnostics(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* setFractureBonds(ctx, Uint32Array(6 * n)): the 24-byte bond records of
+ * config.fractureBonds (solver.ts:929-935) */
+static napi_value set_fracture_bonds(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    void* p;
+    size_t words;
+    if (!typed_arg(env, argv[1], napi_uint32_array, &p, &words, 0)) return NULL;
+    if (words % 6 != 0) {
+        napi_throw_range_error(env, "MPM_EINVAL", "bond records are 6 words each");
+        return NULL;
+    }
+    const int code = mpm_set_fracture_bonds(ctx, p, (uint32_t)(words / 6));
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+/* readFractureBonds(ctx, count) -> Uint32Array(6 * count) */
+static napi_value read_fracture_bonds(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2], ab, arr;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    uint32_t count = 0;
+    if (napi_get_value_uint32(env, argv[1], &count) != napi_ok) {
+        napi_throw_type_error(env, NULL, "count must be a number");
+        return NULL;
+    }
+    void* data;
+    CHECK(napi_create_arraybuffer(env, (size_t)count * 24, &data, &ab));
+    const int code = mpm_read_fracture_bonds(ctx, data, count);
+    if (code != MPM_OK) return throw_code(env, code, ctx);
+    CHECK(napi_create_typedarray(env, napi_uint32_array, (size_t)count * 6, ab, 0, &arr));
+    return arr;
+}
+
+/* setTuning(ctx, {p2gTile, g2pTile, g2pThreads, bucketCapacity}) and
+ * getTuning(ctx) -> the same object (mpm_tuning; no reference counterpart) */
+static napi_value set_tuning(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    double v[4] = {0, 0, 0, 0};
+    const char* keys[4] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity"};
+    for (int k = 0; k < 4; ++k)
+        if (!get_number(env, argv[1], keys[k], &v[k], k < 3)) return NULL;
+    for (int k = 0; k < 4; ++k)
+        if (!(v[k] >= 0 && v[k] <= 4294967295.0) || v[k] != (double)(uint32_t)v[k]) {
+            napi_throw_range_error(env, "MPM_EINVAL", keys[k]);
+            return NULL;
+        }
+    mpm_tuning t = {(uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]};
+    const int code = mpm_set_tuning(ctx, &t);
+    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+}
+
+static napi_value get_tuning(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], out, x;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!ctx) return NULL;
+    mpm_tuning t;
+    const int code = mpm_get_tuning(ctx, &t);
+    if (code != MPM_OK) return throw_code(env, code, ctx);
+    CHECK(napi_create_object(env, &out));
+    const uint32_t vals[4] = {t.p2g_tile, t.g2p_tile, t.g2p_threads, t.bucket_capacity};
+    const char* keys[4] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity"};
+    for (int k = 0; k < 4; ++k) {
+        CHECK(napi_create_uint32(env, vals[k], &x));
+        napi_set_named_property(env, out, keys[k], x);
+    }
+    return out;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor p[] = {
         {"create", 0, create, 0, 0, 0, napi_enumerable, 0},
@@ -400,6 +480,10 @@ static napi_value init(napi_env env, napi_value exports) {
         {"setInteraction", 0, set_interaction, 0, 0, 0, napi_enumerable, 0},
         {"setDiagnostics", 0, set_diagnostics, 0, 0, 0, napi_enumerable, 0},
         {"readDiagnostics", 0, read_diagnostics, 0, 0, 0, napi_enumerable, 0},
+        {"setFractureBonds", 0, set_fracture_bonds, 0, 0, 0, napi_enumerable, 0},
+        {"readFractureBonds", 0, read_fracture_bonds, 0, 0, 0, napi_enumerable, 0},
+        {"setTuning", 0, set_tuning, 0, 0, 0, napi_enumerable, 0},
+        {"getTuning", 0, get_tuning, 0, 0, 0, napi_enumerable, 0},
     };
     if (napi_define_properties(env, exports, sizeof p / sizeof p[0], p) != napi_ok) return NULL;
     return exports;

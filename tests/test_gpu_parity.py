@@ -370,7 +370,8 @@ def test_fp64_conservation_1m_column(pkg):
 def test_node_addon_steps_like_the_python_path(pkg, tmp_path):
     """The Node-API binding (the runtime a TypeScript MpmMlsGpuSolver would
     call) runs the five-material fixture: 8 microsteps through stepBatch and
-    the Promise of readCompact, bit for bit the state of the Python path."""
+    the Promise of readCompact, with a launch shape set through setTuning,
+    bit for bit the state of the Python path (default shape)."""
     import json
     import shutil
     import subprocess
@@ -396,11 +397,13 @@ function u32(file) {
 const rt = new MpmNativeRuntime({cellsX: 48, cellsY: 16, cellSize: 1, particleMass: 0.1, particleVolume: 0.05,
                                  device: 0}, u32(dir + '/packed.bin'), null, false);
 const params = new Uint8Array(fs.readFileSync(dir + '/params.bin'));
+rt.setTuning({p2gTile: 12, g2pTile: 4, g2pThreads: 128});
+const tuning = rt.getTuning();
 rt.stepBatch(params, 8, true);
 rt.readCompact().then((c) => {
   const r = rt.readParticles();
   fs.writeFileSync(dir + '/out.bin', Buffer.from(r.records.buffer));
-  console.log(JSON.stringify({ completed: c.completed, status: Array.from(c.status), maxSpeed: c.maxSpeed }));
+  console.log(JSON.stringify({ completed: c.completed, status: Array.from(c.status), maxSpeed: c.maxSpeed, tuning }));
 });
 """
     print("node addon: 8 microsteps of the five-material fixture", flush=True)
@@ -415,6 +418,8 @@ rt.readCompact().then((c) => {
         recs, _ = s.read_particles(packed=True)
     finally:
         s.dispose()
+    # the addon ran another launch shape than the Python path's default: same words
+    assert compact["tuning"]["p2gTile"] == 12 and compact["tuning"]["g2pThreads"] == 128
     assert compact["completed"] == done == 8 and compact["status"][0] == status[0] == 0
     assert compact["maxSpeed"] == speed
     assert np.array_equal(node_recs, recs)

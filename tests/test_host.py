@@ -233,7 +233,8 @@ def test_fixed_point_division_by_reciprocal_matches_ieee(tmp_path):
 
 
 NODE_FUNCTIONS = ("create", "stepBatch", "readCompact", "readCompactSync", "clearStatus", "counts", "readParticles",
-                  "appendParticles", "setInteraction", "setDiagnostics", "readDiagnostics")
+                  "appendParticles", "setInteraction", "setDiagnostics", "readDiagnostics", "setFractureBonds",
+                  "readFractureBonds", "setTuning", "getTuning")
 
 
 def node_addon():
@@ -272,6 +273,7 @@ try { addon.create({cellsX: 0, cellsY: 4, cellSize: 1, particleMass: 1, particle
 try { addon.create({cellsX: 4, cellsY: 4, cellSize: 1, particleMass: 1, particleVolume: 1},
                    new Uint32Array(21), null, false); } catch (e) { res.words = [e.constructor.name, e.code]; }
 try { addon.stepBatch({}, new Uint8Array(144), 1, false); } catch (e) { res.ctx = e.constructor.name; }
+try { addon.setTuning({}, {p2gTile: 20, g2pTile: 8, g2pThreads: 256}); } catch (e) { res.tctx = e.constructor.name; }
 console.log(JSON.stringify(res));
 """)
     import json
@@ -280,6 +282,7 @@ console.log(JSON.stringify(res));
     assert res["bad"] == ["RangeError", "MPM_EINVAL"]
     assert res["words"] == ["RangeError", "MPM_EINVAL"]
     assert res["ctx"] == "TypeError"
+    assert res["tctx"] == "TypeError"
 
 
 EMITTER_BASE = {"count": 24, "seed": 0x51EED, "center": {"x": 24, "y": 24}, "shape": "circle", "spacing": 1,
