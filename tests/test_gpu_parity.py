@@ -524,3 +524,50 @@ def test_emitter_burst_appended_then_stepped_matches_oracle(pkg, orc):
     rel_v = float(np.linalg.norm(have[:, 2:4] - want[:, 2:4]) / np.linalg.norm(want[:, 2:4]))
     assert rel_v <= 1e-4
     assert float(np.max(np.abs(have[:, :2] - want[:, :2]))) <= 1e-4
+
+
+def _collapsing_tait_block(pkg):
+    """Water with Tait exponent 400 whose velocities converge on the block's
+    centre: after one step the liquid projection leaves det F = 0.5, and the
+    stress cache of the new state, 0.5 * lambda * (2^400 - 1), overflows
+    float32.  That failure belongs to the next microstep (it is deferred)."""
+    g = (np.arange(12) + 0.5) / 2.0 + 5.0  # 2 x 2 particles per cell over cells 5..10
+    x, y = np.meshgrid(g, g)
+    x, y = x.ravel(), y.ravel()
+    n = x.size
+    arr = pkg.ParticleArrays(x=x, y=y, vx=-30.0 * (x - 8.0), vy=-30.0 * (y - 8.0), model=np.full(n, 2, np.uint32),
+                             lambda_=np.full(n, 10.0), mu=np.zeros(n), density=np.ones(n), volume=np.full(n, 0.25),
+                             param=np.full(n, 400.0), object_id=np.zeros(n, np.uint32))
+    config = {"grid": {"cellsX": 16, "cellsY": 16, "cellSize": 1.0}, "particleMass": 0.25, "particleVolume": 0.25,
+              "material": {"model": "water", "lambda": 10.0, "mu": 0.0, "density": 1.0,
+                           "equationOfState": {"model": "tait", "exponent": 400.0}}, "capacity": n}
+    return arr, config
+
+
+def test_deferred_stress_failure_is_raised_in_every_batch_scene(pkg):
+    """A stress-cache failure computed by G2P is raised by the next
+    microstep's P2G, from the first tile of the scene it belongs to: alone,
+    and as the second scene of a batch (whose first tile is not the launch's
+    first block).  The step fails there and commits the one before."""
+    bad, bad_cfg = _collapsing_tait_block(pkg)
+    s = pkg.create_mpm_mls_gpu_solver(bad, bad_cfg, device=0)
+    try:
+        s.step_batch(2, 0.01, {"x": 0.0, "y": 0.0})
+        _, done_alone, status_alone = s.ctx.read_compact()
+    finally:
+        s.dispose()
+    assert status_alone[0] != 0 and done_alone == 1
+    # the same block at rest with exponent 7: a rest state stays at rest
+    calm = pkg.ParticleArrays(x=bad.x, y=bad.y, vx=np.zeros(len(bad)), vy=np.zeros(len(bad)), model=bad.model,
+                              lambda_=bad.lambda_, mu=bad.mu, density=bad.density, volume=bad.volume,
+                              param=np.full(len(bad), 7.0), object_id=bad.object_id)
+    calm_cfg = dict(bad_cfg, material=dict(bad_cfg["material"], equationOfState={"model": "tait", "exponent": 7.0}))
+    b = pkg.MpmMlsGpuBatch([(calm, calm_cfg), (bad, bad_cfg)], device=0)
+    try:
+        b.step_batch(2, 0.01, {"x": 0.0, "y": 0.0})
+        _, done0, status0 = b.scene(0).ctx.read_compact()
+        _, done1, status1 = b.scene(1).ctx.read_compact()
+    finally:
+        b.dispose()
+    assert status0[0] == 0 and done0 == 2
+    assert status1[0] == status_alone[0] and done1 == 1
