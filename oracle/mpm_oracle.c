@@ -1190,6 +1190,23 @@ int orc_step_bodies_f64(const orc_particle* in, int32_t n, const uint32_t* objec
 #define ERR_FIXED_POINT 65536u
 
 static float wf(const uint32_t* r, int k) { float f; memcpy(&f, &r[k], 4); return f; }
+
+/* b^e as the GPU's stress cache evaluates it: square-and-multiply in FP64 for
+ * a whole exponent in [1, 64], pow() otherwise (the same operation sequence,
+ * so the float32 results agree bit for bit). */
+static double tait_pow_f64(double b, float e) {
+    if (e >= 1.0f && e <= 64.0f && rintf(e) == e) {
+        uint32_t n = (uint32_t)e;
+        double acc = 1.0, x = b;
+        while (n != 0u) {
+            if (n & 1u) acc *= x;
+            x *= x;
+            n >>= 1;
+        }
+        return acc;
+    }
+    return pow(b, (double)e);
+}
 static int ff(float v) { return isfinite(v); }
 
 /* determinant2x2 (shaders.ts:261-266): compensated product difference. */
@@ -1232,7 +1249,7 @@ static uint32_t prepare_one(const uint32_t* r, const orc_params_f32* prm, float 
             return ERR_P2G | ERR_INVALID_MATERIAL;
         if (param > 0.0f) {
             float inv = 1.0f / det;
-            float pw = (float)pow((double)inv, (double)param);
+            float pw = (float)tait_pow_f64((double)inv, param);
             float pressure = lambda * (pw - 1.0f);
             float stress = -det * pressure;
             S[0] = stress; S[3] = stress;

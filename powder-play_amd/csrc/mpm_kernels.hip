@@ -225,6 +225,26 @@ __device__ bool solid_stress(uint32_t model, float4 Ff, float lambda, float mu, 
 // checks.  Liquids in f32 (tension-free linear EOS; Tait via FP64 pow);
 // solids in FP64, rounded once.  GENERAL = false admits linear liquids only.
 // Returns the status bits to raise.
+// b^e in FP64 for the Tait power J^-n.  A whole exponent up to 64 (the Tait
+// exponent is 7 in practice) takes square-and-multiply: a handful of FP64
+// products, each within an ulp, so the result is within a few FP64 ulps of
+// the exact power, far below the float32 rounding that follows.  Other
+// exponents take pow().  The oracle's float32 restatement of the stress
+// cache evaluates the same sequence.
+__device__ __forceinline__ double tait_pow(double b, float e) {
+    if (e >= 1.0f && e <= 64.0f && rintf(e) == e) {
+        uint32_t n = (uint32_t)e;
+        double acc = 1.0, x = b;
+        while (n != 0u) {
+            if (n & 1u) acc *= x;
+            x *= x;
+            n >>= 1;
+        }
+        return acc;
+    }
+    return pow(b, (double)e);
+}
+
 template <bool GENERAL>
 __device__ uint32_t particle_stress(const Material& mt, float4 F, float pv, float det, float4& S) {
     S = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -239,7 +259,7 @@ __device__ uint32_t particle_stress(const Material& mt, float4 F, float pv, floa
                 return ERR_P2G | ERR_INVALID_MATERIAL;
             } else {
                 const float inv = 1.0f / det;
-                const float pw = (float)pow((double)inv, (double)mt.param);
+                const float pw = (float)tait_pow((double)inv, mt.param);
                 const float pressure = mt.lambda * (pw - 1.0f);
                 const float stress = -det * pressure;
                 S.x = stress; S.w = stress;
@@ -1332,7 +1352,7 @@ __device__ float wave_speed_bound(const Params& P, uint32_t model, float4 F, flo
             if constexpr (!GENERAL) return FLT_MAX;
             float det = det_comp(F);
             if (!finitef(param) || param <= 0.0f || !finitef(det) || det <= 0.0f) return FLT_MAX;
-            ws2 = param * ws2 * (float)pow((double)(1.0f / det), (double)(param - 1.0f));
+            ws2 = param * ws2 * (float)tait_pow((double)(1.0f / det), param - 1.0f);
         }
         if (!finitef(ws2) || ws2 < 0.0f) return FLT_MAX;
         return sqrtf(ws2);
