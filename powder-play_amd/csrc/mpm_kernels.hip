@@ -245,7 +245,9 @@ __device__ __forceinline__ double tait_pow(double b, float e) {
     return pow(b, (double)e);
 }
 
-template <bool GENERAL>
+// SOLIDS = false compiles out the solid models (scenes whose flags list no
+// solid): fewer registers for the Tait-liquid kernels.
+template <bool GENERAL, bool SOLIDS = true>
 __device__ uint32_t particle_stress(const Material& mt, float4 F, float pv, float det, float4& S) {
     S = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!finitef(det) || det <= 0.0f || !finitef(mt.volume) || mt.volume <= 0.0f) return ERR_DEFORMATION;
@@ -270,7 +272,7 @@ __device__ uint32_t particle_stress(const Material& mt, float4 F, float pv, floa
             S.x = pressure; S.w = pressure;
         }
     } else {
-        if constexpr (!GENERAL) {
+        if constexpr (!GENERAL || !SOLIDS) {
             return ERR_P2G | ERR_INVALID_MATERIAL;
         } else {
             if (model == MODEL_NEO_HOOKEAN) {
@@ -1342,7 +1344,7 @@ __device__ __forceinline__ float4 project_liquid(float4 m) {
 }
 
 // particleWaveSpeedBound (shaders.ts:268-353)
-template <bool GENERAL>
+template <bool GENERAL, bool SOLIDS = true>
 __device__ float wave_speed_bound(const Params& P, uint32_t model, float4 F, float lambda, float mu, float density,
                                   float volume, float param) {
     if (model == MODEL_WATER || model == MODEL_OIL) {
@@ -1357,7 +1359,7 @@ __device__ float wave_speed_bound(const Params& P, uint32_t model, float4 F, flo
         if (!finitef(ws2) || ws2 < 0.0f) return FLT_MAX;
         return sqrtf(ws2);
     }
-    if constexpr (!GENERAL) return FLT_MAX;
+    if constexpr (!GENERAL || !SOLIDS) return FLT_MAX;
     if (model != MODEL_NEO_HOOKEAN) return FLT_MAX;
     if (!finitef(param)) return FLT_MAX;
     float remaining = 1.0f - fminf(fmaxf(param, 0.0f), 1.0f);
@@ -1443,7 +1445,7 @@ __device__ float2 project_out_of_triangles(const Params& P, const AuxData& A, fl
     return make_float2((float)((double)p.x + corr * nx), (float)((double)p.y + corr * ny));
 }
 
-template <int TN, bool GENERAL, bool AUX = false>
+template <int TN, bool GENERAL, bool AUX = false, bool SOLIDS = true>
 __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
                              float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
                              uint32_t step, G2POut& o, const AuxData* A = nullptr, bool cfl_here = false) {
@@ -1551,7 +1553,7 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     float pvn = pv;
     if (liquid) {
         Fn = project_liquid(trial);
-    } else if constexpr (!GENERAL) {
+    } else if constexpr (!GENERAL || !SOLIDS) {
         raise_step(ctl, ERR_G2P, step);
         return false;
     } else if (model == MODEL_SNOW) {
@@ -1642,7 +1644,8 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     o.plastic = pvn;
     if (P.cfl_on >= 0.5f && (P.body_count == 0u || cfl_here)) {
         const float speed = sqrtf(o.vel.x * o.vel.x + o.vel.y * o.vel.y);
-        const float ws = wave_speed_bound<GENERAL>(P, model, Fn, mt.lambda, mt.mu, mt.density, mt.volume, mt.param);
+        const float ws = wave_speed_bound<GENERAL, SOLIDS>(P, model, Fn, mt.lambda, mt.mu, mt.density, mt.volume,
+                                                           mt.param);
         if (!finite2(o.vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
             record_failure(ctl, 3u, pid, speed, ws, P.vel_bound, P.wave_bound);
             raise_step(ctl, ERR_CFL_BOUND, step);
@@ -1658,8 +1661,10 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 // current one is processed, so their latency overlaps the current G2P and
 // slot reservation.  Compiled shapes: 8x8 buckets with 128, 256 or 512
 // threads, 4x4 buckets with 128 or 256.
-template <int G_TB, int G_THREADS, bool GENERAL, bool AUX = false>
-__global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__ scenes,
+template <int G_TB, int G_THREADS, bool GENERAL, bool AUX = false, bool SOLIDS = true>
+__global__ __launch_bounds__(G_THREADS)
+__attribute__((amdgpu_waves_per_eu(GENERAL && !SOLIDS ? 4 : 1, 8)))  // Tait liquids: 4 waves a SIMD
+void k_g2p(const SceneDesc* __restrict__ scenes,
                                                  const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                                                  uint32_t step) {
     using T = G2PTile<G_TB>;
@@ -1774,7 +1779,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
                 ok = false;
             } else {
                 const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u ? s_vo : s_vs);
-                ok = g2p_particle<G_TN, GENERAL, AUX>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
+                ok = g2p_particle<G_TN, GENERAL, AUX, SOLIDS>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
                                                 Ax, cfl_here);
             }
             if (ok) {
@@ -1790,7 +1795,7 @@ __global__ __launch_bounds__(G_THREADS) void k_g2p(const SceneDesc* __restrict__
                 if (ok) {
                     // stress cache of the new state; failures belong to the
                     // microstep that will start from it
-                    const uint32_t pf = particle_stress<true>(mt, o.F, o.plastic, det_comp(o.F), S);
+                    const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
                     if (pf) {
                         if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
                         atomicOr(&ctl->deferred[out_buf], pf);
@@ -2532,19 +2537,24 @@ StepChoice pick_p2g(const StepBatch& sb, bool general, bool aux) {
     }
 }
 
+// G2P variant: plain liquids; general scenes without solids (Tait liquids);
+// general scenes with solids
 template <int TB, int THREADS>
-StepChoice g2p_choice(bool general) {
-    return {general ? k_g2p<TB, THREADS, true> : k_g2p<TB, THREADS, false>, (uint32_t)THREADS};
+StepChoice g2p_choice(bool general, bool solids) {
+    StepKernel fn = !general ? k_g2p<TB, THREADS, false>
+                             : (solids ? k_g2p<TB, THREADS, true> : k_g2p<TB, THREADS, true, false, false>);
+    return {fn, (uint32_t)THREADS};
 }
 
 StepChoice pick_g2p(const StepBatch& sb, bool general, bool aux) {
     if (aux) return {k_g2p<8, 256, true, true>, 256u};
-    if (sb.shape.g2p_tile == 4u) return sb.shape.g2p_threads == 128u ? g2p_choice<4, 128>(general)
-                                                                     : g2p_choice<4, 256>(general);
+    const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
+    if (sb.shape.g2p_tile == 4u) return sb.shape.g2p_threads == 128u ? g2p_choice<4, 128>(general, solids)
+                                                                     : g2p_choice<4, 256>(general, solids);
     switch (sb.shape.g2p_threads) {
-        case 128u: return g2p_choice<8, 128>(general);
-        case 512u: return g2p_choice<8, 512>(general);
-        default: return g2p_choice<8, 256>(general);
+        case 128u: return g2p_choice<8, 128>(general, solids);
+        case 512u: return g2p_choice<8, 512>(general, solids);
+        default: return g2p_choice<8, 256>(general, solids);
     }
 }
 
