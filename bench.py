@@ -228,7 +228,7 @@ def small_scenes_leg(pkg, macrosteps: int = 12):
     return out
 
 
-def batched_leg(pkg, steps: int):
+def batched_leg(pkg, steps: int, how: str = "auto"):
     """256 independent Tait tanks (scenes.batched_tanks): `steps` microsteps
     of all of them in one batch, against the same scenes as separate solvers
     stepped one after another (each synchronised before the next starts)."""
@@ -238,9 +238,13 @@ def batched_leg(pkg, steps: int):
     total = sum(len(a) for a, _, _ in tanks)
     dt = pkg.scenes.BATCH_DT
     log(f"[bench] batched: {len(tanks)} tanks, {total} particles, {steps} microsteps")
-    t0 = time.perf_counter()
-    tuned = pkg.autotune.autotune_batch(scenes, dt, gravity, device=0)
-    log(f"[bench] batched: autotuned {tuned['tuning']} in {time.perf_counter() - t0:.1f} s")
+    if how == "auto":
+        t0 = time.perf_counter()
+        tuned = pkg.autotune.autotune_batch(scenes, dt, gravity, device=0)
+        log(f"[bench] batched: autotuned {tuned['tuning']} in {time.perf_counter() - t0:.1f} s")
+    else:
+        keys = ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity")
+        tuned = {"tuning": dict(zip(keys, (int(v) for v in how.split(",")))), "candidates": None}
     batch = pkg.MpmMlsGpuBatch(scenes, device=0, tuning=tuned["tuning"])
     batch.step_batch(8, dt, gravity)
     batch.scene(0).ctx.read_compact()
@@ -382,6 +386,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--no-small", action="store_true", help="skip the Tait tank and Gate 4 leg")
+    ap.add_argument("--batch-tuning", default="auto",
+                    help='launch shape of the batched leg: "auto" or "p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
     ap.add_argument("--tuning", default="auto",
                     help='launch shape: "auto" (autotune before timing), "default", or '
                          '"p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
@@ -449,7 +455,8 @@ def main():
     step_bytes = PARTICLE_BYTES * n + NODE_PHASE_BYTES * phases * nodes
     step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
 
-    batched = None if args.no_batched or args.cells != 1000 else batched_leg(pkg, args.batched_steps)
+    batched = (None if args.no_batched or args.cells != 1000
+               else batched_leg(pkg, args.batched_steps, args.batch_tuning))
     adjoint = None if args.no_adjoint else adjoint_leg(pkg, args.adjoint_reps)
     small = None if args.no_small else small_scenes_leg(pkg)
 
