@@ -79,6 +79,24 @@ struct Control {
 };
 constexpr uint32_t CONTROL_HISTORY = 64;
 
+// Pointer fields of the structures the kernels read out of a scene
+// descriptor in device memory.  In the device pass they carry the global
+// address space: the compiler cannot infer it for a pointer loaded from
+// memory, and a generic pointer compiles to flat memory instructions, which
+// take 64-bit addresses in vector registers and are counted together with
+// the LDS operations (every wait for an LDS result then also waits for the
+// memory loads in flight).  Both passes see 64-bit pointers, so the layouts
+// agree; kernel parameters stay generic, so kernel names agree too.  Only
+// the kernels' translation unit asks for it (MPM_GLOBAL_POINTERS): the device
+// pass also checks host code, which fills these fields from host pointers.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(MPM_GLOBAL_POINTERS)
+#define MPM_GLOBAL __attribute__((address_space(1)))
+#else
+#define MPM_GLOBAL
+#endif
+template <class T>
+using dptr = T MPM_GLOBAL*;
+
 // One material row: particles reference it by index (material id).  Rows are
 // the distinct (model, lambda, mu, density, volume, param) of the scene.
 struct Material {
@@ -91,34 +109,34 @@ static_assert(sizeof(Material) == 32, "Material row is 32 bytes");
 
 // Particle state stored by bucket: slot = bucket * cap + k, k < cnt[bucket].
 struct SlotBuf {
-    float2* pos;
-    float2* vel;
-    float4* C;
-    float4* F;
-    float4* S;        // P*F^T stress cache (scenes with solids or Tait liquids)
-    float* plastic;   // snow plastic volume (scenes with solids)
-    uint32_t* pid;    // particle index in the caller's order
-    uint32_t* mat;    // material row
-    uint32_t* cnt;    // per bucket
+    dptr<float2> pos;
+    dptr<float2> vel;
+    dptr<float4> C;
+    dptr<float4> F;
+    dptr<float4> S;        // P*F^T stress cache (scenes with solids or Tait liquids)
+    dptr<float> plastic;   // snow plastic volume (scenes with solids)
+    dptr<uint32_t> pid;    // particle index in the caller's order
+    dptr<uint32_t> mat;    // material row
+    dptr<uint32_t> cnt;    // per bucket
 };
 
 // Particle arrays in the caller's index order (upload, readback, removed).
 struct Home {
-    float2* pos;
-    float2* vel;
-    float4* C;
-    float4* F;
-    float* plastic;
-    uint32_t* model;
-    uint32_t* mat;
+    dptr<float2> pos;
+    dptr<float2> vel;
+    dptr<float4> C;
+    dptr<float4> F;
+    dptr<float> plastic;
+    dptr<uint32_t> model;
+    dptr<uint32_t> mat;
 };
 
 struct Grid {
-    float2* v_solid;
-    float2* v_water;
-    float2* v_oil;
-    float2* liquid;     // (water+oil mass, water+oil volume)
-    int32_t* fixed;     // node_count*11 or nullptr
+    dptr<float2> v_solid;
+    dptr<float2> v_water;
+    dptr<float2> v_oil;
+    dptr<float2> liquid;     // (water+oil mass, water+oil volume)
+    dptr<int32_t> fixed;     // node_count*11 or nullptr
 };
 
 // Scene content flags, host-derived from the particle models.  SCENE_AUX:
@@ -146,16 +164,16 @@ static_assert(sizeof(Bond) == 24, "fracture bond record is 24 bytes");
 // endpoint in ascending particle index.  Both slot buffers keep where each
 // aux particle sits; the bucket and G2P kernels write it.
 struct AuxData {
-    uint32_t* aux_of;      // per particle index: aux index or NO_AUX
-    uint32_t* aux_pid;     // per aux: particle index
-    uint32_t* slot[2];     // per aux and slot buffer: slot
-    float* damage[2];      // per aux and slot buffer: neo-Hookean damage (fracture endpoints)
-    uint32_t* body_oid;    // per body: object id
-    FieldNode* field;      // per node
-    uint32_t* field_body;  // per node: body index + 1, 0 = no triangle within the band
-    Bond* bonds;
-    uint32_t* bond_off;    // per aux + 1: CSR offsets into bond_inc
-    uint32_t* bond_inc;    // incident bond indices
+    dptr<uint32_t> aux_of;      // per particle index: aux index or NO_AUX
+    dptr<uint32_t> aux_pid;     // per aux: particle index
+    dptr<uint32_t> slot[2];     // per aux and slot buffer: slot
+    dptr<float> damage[2];      // per aux and slot buffer: neo-Hookean damage (fracture endpoints)
+    dptr<uint32_t> body_oid;    // per body: object id
+    dptr<FieldNode> field;      // per node
+    dptr<uint32_t> field_body;  // per node: body index + 1, 0 = no triangle within the band
+    dptr<Bond> bonds;
+    dptr<uint32_t> bond_off;    // per aux + 1: CSR offsets into bond_inc
+    dptr<uint32_t> bond_inc;    // incident bond indices
     uint32_t nbodies, naux, nbonds, frac_aux0;  // frac_aux0 = 3 * nbodies
 };
 constexpr uint32_t NO_AUX = 0xffffffffu;
@@ -226,13 +244,13 @@ inline uint32_t g2p_tiles_for(uint32_t bw, uint32_t bh, uint32_t tile) {
 // tile -> scene map and reads everything scene-specific from here.
 struct SceneDesc {
     Params P;           // this scene's step parameters
-    Control* ctl;
+    dptr<Control> ctl;
     SlotBuf buf[2];
-    const Material* mats;
+    dptr<const Material> mats;
     Grid grid;
-    double* diag_grid;  // per-tile grid sums of the last P2G (diagnostics on), else nullptr
-    double* diag_part;  // particle partial sums
-    double* diag;       // the nine reported sums
+    dptr<double> diag_grid;  // per-tile grid sums of the last P2G (diagnostics on), else nullptr
+    dptr<double> diag_part;  // particle partial sums
+    dptr<double> diag;       // the nine reported sums
     uint32_t nmat, bw, bh, nb, cap;
     uint32_t ptx, pty, gtx, gty;  // P2G and G2P tile grids
     uint32_t p2g_base, g2p_base;  // first tile of this scene in the P2G / G2P launches

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 
+#define MPM_GLOBAL_POINTERS  // scene-descriptor pointers are global in device code
 #include "mpm_internal.h"
 #include "mpm_timing.h"
 
@@ -540,9 +541,12 @@ __device__ __forceinline__ Material material_at(const Material* s_mat, const Mat
     if (m < (uint32_t)LDS_MATS) {
         lo = reinterpret_cast<const uint4*>(s_mat)[2 * m];
         hi = reinterpret_cast<const uint4*>(s_mat)[2 * m + 1];
-    } else {
-        lo = reinterpret_cast<const uint4*>(mats)[2 * m];
-        hi = reinterpret_cast<const uint4*>(mats)[2 * m + 1];
+    } else {  // a different load form, so the two branches are not merged into one flat load
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(mats) + 2 * m);
+        const u32x4 b = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(mats) + 2 * m + 1);
+        lo = make_uint4(a.x, a.y, a.z, a.w);
+        hi = make_uint4(b.x, b.y, b.z, b.w);
     }
     Material r;
     r.model = lo.x;
