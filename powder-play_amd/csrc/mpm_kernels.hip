@@ -322,6 +322,18 @@ __device__ __forceinline__ float qweight(float f, int o) {
     return o == 0 ? w0 : (o == 1 ? w1 : w2);
 }
 
+// qweight with the component chosen by bit masks, without the branches a
+// select chain may compile to.  The outer weights share one form, (0.5 t) t
+// with t = 1.5 - f or f - 0.5, so one product serves both and every value
+// is rounded exactly as in qweight.
+__device__ __forceinline__ float qweight_masked(float f, int o) {
+    const uint32_t m0 = 0u - (uint32_t)(o == 0), m1 = 0u - (uint32_t)(o == 1);
+    const float t = __uint_as_float((__float_as_uint(1.5f - f) & m0) | (__float_as_uint(f - 0.5f) & ~m0));
+    const float c = f - 1.0f;
+    const float wt = (0.5f * t) * t, w1 = 0.75f - c * c;
+    return __uint_as_float((__float_as_uint(w1) & m1) | (__float_as_uint(wt) & ~m1));
+}
+
 // Raise status bits from microstep `step` of the batch; the batch commits
 // the microsteps before the first one that raised (commitParticles,
 // shaders.ts:3067-3080).
@@ -1051,25 +1063,35 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 n[dy] = s_cell[row + 3] - q0[dy];
             }
             const uint32_t n01 = n[0] + n[1], total = n01 + n[2];
+            // pair k lies in row range dy = (k >= n[0]) + (k >= n01), at
+            // q = k + base[dy]; the choices below are bit masks, so lanes at
+            // different rows or columns do not diverge, and the four LDS reads
+            // of a pair are issued together
+            const uint32_t base0 = q0[0], base1 = q0[1] - n[0], base2 = q0[2] - n01;
 #pragma unroll 1
             for (uint32_t k = 0; k < total; ++k) {
-                const int dy = k < n[0] ? 0 : (k < n01 ? 1 : 2);
-                const uint32_t q = dy == 0 ? q0[0] + k : (dy == 1 ? q0[1] + (k - n[0]) : q0[2] + (k - n01));
+                const uint32_t in0 = 0u - (uint32_t)(k < n[0]), in01 = 0u - (uint32_t)(k < n01);
+                const uint32_t q = k + ((base0 & in0) | (base1 & in01 & ~in0) | (base2 & ~in01));
                 const uint32_t tag = s_tag[q];
-                if (tag == 0xffffu) continue;
-                const uint32_t ph = tag >> 8;
-                const int ox = lx + 3 - (int)(tag & 0xffu);
-                const int oy = 2 - dy;
                 const float4 r0 = s_r0[q], r1 = s_r1[q];
                 const float2 r2 = s_r2[q];
+                // the empty asm takes one word of each read, so the four reads
+                // are issued back to back and waited for once (left alone, the
+                // scheduler waits for the tag before issuing the others, and
+                // defers the mass and volume into the phase branches)
+                asm volatile("" ::"v"(tag), "v"(r0.x), "v"(r1.x), "v"(r2.x));
+                const uint32_t ph = tag >> 8;  // 0xff: the particle contributes nothing
+                const int ox = lx + 3 - (int)(tag & 0xffu);
+                const int oy = 2 - (int)((1u & ~in0) + (1u & ~in01));
                 const float fx = r0.x, fy = r0.y;
-                const float w = qweight(fx, ox) * qweight(fy, oy);
+                const float w = qweight_masked(fx, ox) * qweight_masked(fy, oy);
                 const float dpx = ((float)ox - fx) * dx;
                 const float dpy = ((float)oy - fy) * dx;
                 const float qx = (r0.z + r1.x * dpx) + r1.y * dpy;
                 const float qy = (r0.w + r1.z * dpx) + r1.w * dpy;
-                if (!finitef(qx) || !finitef(qy)) {
-                    flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
+                // one test for both skips
+                if (ph > 2u || !finitef(qx) || !finitef(qy)) {
+                    if (ph <= 2u) flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
                     continue;
                 }
                 const float wm = w * r2.x, wqx = w * qx, wqy = w * qy;
