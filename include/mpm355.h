@@ -194,7 +194,11 @@ const char* mpm_batch_last_error(mpm_batch* batch);
  *
  * mpm_tuning fields:
  *   p2g_tile         P2G tile side in grid nodes: 28, 20 (default) or 12.
- *   g2p_tile         G2P tile side in 4x4-cell buckets: 8 (default) or 4.
+ *   g2p_tile         G2P tile side in 4x4-cell buckets: 8 (default) or 4
+ *                    (default for a grid, or a batch's grids together,
+ *                    with fewer 8-bucket tiles than the device has compute
+ *                    units: those launches are latency-bound, and the
+ *                    smaller tile gives each thread fewer particles).
  *   g2p_threads      threads of a G2P block: 128, 256 (default) or 512 for
  *                    8-bucket tiles; 128 or 256 for 4-bucket tiles.  Particles
  *                    per wavefront = the tile's particles / (g2p_threads/64).

@@ -46,6 +46,8 @@ from . import layout
 from .native import MPM_P2G_FIXED_POINT, MPM_P2G_GATHER, MpmTuning, device_arch
 
 CACHE_VERSION = 1
+# the library's default shape on a grid with at least one 8x8-bucket G2P tile
+# per compute unit (smaller grids default to 4x4-bucket tiles)
 DEFAULT_TUNING = {"p2g_tile": 20, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
 BODIES_TUNING = {"p2g_tile": 28, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
 P2G_TILES = (28, 20, 12)

@@ -493,6 +493,14 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
 
 int settle(mpm_ctx* c);
 
+// The default launch shape for `g2p_tiles8` 8x8-bucket G2P tiles on `device`.
+Shape default_shape(uint32_t g2p_tiles8, int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return DEFAULT_SHAPE;
+    return g2p_tiles8 < (uint32_t)cus ? SMALL_GRID_SHAPE : DEFAULT_SHAPE;
+}
+
 // Shape a step of this context runs with: scenes with bodies or bonds run
 // AUX_SHAPE only, and a P2G tile whose gather chunk is smaller than the bucket
 // capacity falls back to the 28-node tile (a bucket must fit a chunk).
@@ -729,6 +737,7 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     if ((e = hipGetDevice(&ctx->device)) != hipSuccess) return bail(hip_fail(ctx, e, "hipGetDevice"));
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(hip_fail(ctx, e, "hipStreamCreateWithFlags"));
+    ctx->shape = default_shape(g2p_tiles_for(lc.bw, lc.bh, 8u), ctx->device);
 
     const size_t cap = capacity ? capacity : 1;
     const size_t nodes = lc.node_count;
@@ -1345,6 +1354,11 @@ int mpm_batch_create(const mpm_config* configs, const uint32_t* const* packed20,
     }
     hipError_t e;
     b->device = b->scenes[0]->device;
+    {
+        uint32_t tiles8 = 0;
+        for (const mpm_ctx* c : b->scenes) tiles8 += g2p_tiles_for(c->lc.bw, c->lc.bh, 8u);
+        b->shape = default_shape(tiles8, b->device);
+    }
     if ((e = hipSetDevice(b->device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming)) != hipSuccess) {

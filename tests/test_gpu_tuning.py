@@ -79,12 +79,23 @@ def run(pkg, mode, particles, config, tuning=None, steps=12, dt=None, capacity_c
         s.dispose()
 
 
+def default_shape(pkg, configs):
+    """The library's default (include/mpm355.h, "kernel tuning"): 20-node P2G
+    tiles, and 8x8-bucket G2P tiles of 256 threads unless the grids have
+    fewer of those than the device has compute units, then 4x4-bucket ones."""
+    tiles8 = 0
+    for c in configs:
+        bw, bh = c["grid"]["cellsX"] // 4 + 1, c["grid"]["cellsY"] // 4 + 1
+        tiles8 += -(-bw // 8) * -(-bh // 8)
+    return (20, 8 if tiles8 >= pkg.native.device_arch(0)[1] else 4, 256)
+
+
 @pytest.mark.parametrize("mode", ["gather", "fixed"])
 @pytest.mark.parametrize("scene", ["liquid", "mixed"])
 def test_every_shape_is_bit_identical(pkg, mode, scene):
     particles, config = scenes(pkg)[scene]
     ref, dt, eff = run(pkg, mode, particles, config)
-    assert eff["p2g_tile"] == 20 and eff["g2p_tile"] == 8 and eff["g2p_threads"] == 256
+    assert (eff["p2g_tile"], eff["g2p_tile"], eff["g2p_threads"]) == default_shape(pkg, [config])
     for tile in pkg.autotune.P2G_TILES:
         for tb, th in pkg.autotune.G2P_SHAPES:
             want = {"p2g_tile": tile, "g2p_tile": tb, "g2p_threads": th, "bucket_capacity": 0}
@@ -101,6 +112,28 @@ def test_bucket_capacity_change_mid_run(pkg, mode):
     got, _, eff2 = run(pkg, mode, particles, config, steps=10, dt=dt, capacity_change_at=4)
     assert eff2["bucket_capacity"] == 2 * eff["bucket_capacity"]
     assert np.array_equal(got, ref)
+
+
+def test_default_shape_follows_the_grid_size(pkg):
+    particles, config = pkg.scenes.water_oil_column(cells=32)
+    big = dict(config, grid=dict(config["grid"], cellsX=1024, cellsY=1024))
+    for cfg, want_g2p in ((config, 4), (big, 8)):
+        assert default_shape(pkg, [cfg])[1] == want_g2p
+        s = pkg.create_mpm_mls_gpu_solver(particles, cfg, device=0)
+        try:
+            eff = s.ctx.get_tuning().as_dict()
+            assert (eff["p2g_tile"], eff["g2p_tile"], eff["g2p_threads"]) == (20, want_g2p, 256)
+            s.step_batch(4, 1e-4, GRAVITY)
+            s.check_numerical_status()
+        finally:
+            s.dispose()
+    # a batch counts the tiles of all its grids
+    b = pkg.MpmMlsGpuBatch([(particles, config)] * 3, device=0)
+    try:
+        eff = b.native.get_tuning().as_dict()
+        assert (eff["p2g_tile"], eff["g2p_tile"], eff["g2p_threads"]) == default_shape(pkg, [config] * 3)
+    finally:
+        b.dispose()
 
 
 def test_invalid_tuning_is_rejected_without_state_change(pkg):
