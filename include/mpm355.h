@@ -208,9 +208,10 @@ const char* mpm_batch_last_error(mpm_batch* batch);
  *   MPM_EINVAL with no state change; the scene of a batch takes the batch's
  *   shape (mpm_batch_set_tuning) and returns MPM_EINVAL for another shape.
  * mpm_get_tuning: the shape the next step runs and the current capacity.
- *   Scenes with bonded triangle bodies or fracture bonds run the shape
- *   {28, 8, 256}; a P2G tile whose gather chunk (448 particles for 12, 960 for 20)
- *   is below the bucket capacity runs the 28-node tile.
+ *   Scenes with bonded triangle bodies or fracture bonds run 28-node P2G
+ *   tiles and 256-thread G2P blocks, with the requested G2P tile side; a P2G
+ *   tile whose gather chunk (448 particles for 12, 960 for 20) is below the
+ *   bucket capacity runs the 28-node tile.
  * mpm_device_arch: the architecture name (e.g. "gfx950:sramecc+:xnack-")
  *   and compute-unit count of a device, the autotuner's cache key. */
 typedef struct {

@@ -50,6 +50,7 @@ CACHE_VERSION = 1
 # per compute unit (smaller grids default to 4x4-bucket tiles)
 DEFAULT_TUNING = {"p2g_tile": 20, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
 BODIES_TUNING = {"p2g_tile": 28, "g2p_tile": 8, "g2p_threads": 256, "bucket_capacity": 0}
+BODIES_G2P_TILES = (8, 4)  # scenes with bodies or bonds: 28-node P2G, 256-thread G2P of either tile
 P2G_TILES = (28, 20, 12)
 G2P_SHAPES = ((8, 256), (8, 512), (8, 128), (4, 256), (4, 128))
 # particles one float-gather chunk stages for each P2G tile side: a bucket
@@ -233,7 +234,11 @@ def autotune(particles, config: dict, p2g_mode: int = MPM_P2G_GATHER, *, dt: flo
                 table[key] = r
 
         if state.bonded_body_count > 0 or state.fracture_bonds is not None:
-            best = dict(BODIES_TUNING)  # bodies and bonds run this shape only
+            # bodies and bonds run 28-node P2G tiles and 256-thread G2P blocks
+            for _ in range(2):
+                for tb in BODIES_G2P_TILES:
+                    timed(log["g2p"], f"{tb}x256", _tuning(28, tb, 256), "g2p_us")
+            best = dict(BODIES_TUNING, g2p_tile=int(_best(log["g2p"], "g2p_us").split("x")[0]))
         else:
             for _ in range(2):
                 for t in p2g_tiles:

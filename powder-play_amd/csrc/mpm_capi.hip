@@ -502,10 +502,11 @@ Shape default_shape(uint32_t g2p_tiles8, int device) {
 }
 
 // Shape a step of this context runs with: scenes with bodies or bonds run
-// AUX_SHAPE only, and a P2G tile whose gather chunk is smaller than the bucket
-// capacity falls back to the 28-node tile (a bucket must fit a chunk).
+// AUX_SHAPE's P2G tile and G2P threads with the requested G2P tile, and a
+// P2G tile whose gather chunk is smaller than the bucket capacity falls back
+// to the 28-node tile (a bucket must fit a chunk).
 Shape effective_shape(const mpm_ctx* c, Shape s) {
-    if (c->lc.scene_flags & SCENE_AUX) return AUX_SHAPE;
+    if (c->lc.scene_flags & SCENE_AUX) return {AUX_SHAPE.p2g_tile, s.g2p_tile == 4u ? 4u : 8u, AUX_SHAPE.g2p_threads};
     if (c->lc.cap > p2g_chunk_for(s.p2g_tile)) s.p2g_tile = 28u;
     return s;
 }

@@ -225,8 +225,9 @@ constexpr Shape DEFAULT_SHAPE = {20u, 8u, 256u};
 // quarter of the particles a block) finishes sooner (the 1,600-particle Tait
 // tank: 10.8 us against 24.9 us per G2P launch on MI355X).
 constexpr Shape SMALL_GRID_SHAPE = {20u, 4u, 256u};
-// Scenes with bonded bodies or fracture bonds run one shape: the largest P2G
-// tile, whose gather chunk holds any bucket.
+// Scenes with bonded bodies or fracture bonds run the largest P2G tile, whose
+// gather chunk holds any bucket, and 256-thread G2P blocks of either tile
+// side (the G2P tile of the requested shape; 8 here).
 constexpr Shape AUX_SHAPE = {28u, 8u, 256u};
 constexpr uint32_t MIN_P2G_TILE = 12u;  // the most P2G tiles a grid can have
 // Particles the float-gather P2G stages per chunk, for each tile side: the

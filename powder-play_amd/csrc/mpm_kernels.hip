@@ -2573,7 +2573,8 @@ StepChoice g2p_choice(bool general, bool solids) {
 }
 
 StepChoice pick_g2p(const StepBatch& sb, bool general, bool aux) {
-    if (aux) return {k_g2p<8, 256, true, true>, 256u};
+    if (aux) return sb.shape.g2p_tile == 4u ? StepChoice{k_g2p<4, 256, true, true>, 256u}
+                                            : StepChoice{k_g2p<8, 256, true, true>, 256u};
     const bool solids = (sb.scene_flags & SCENE_SOLID) != 0u;
     if (sb.shape.g2p_tile == 4u) return sb.shape.g2p_threads == 128u ? g2p_choice<4, 128>(general, solids)
                                                                      : g2p_choice<4, 256>(general, solids);

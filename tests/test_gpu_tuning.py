@@ -169,14 +169,21 @@ def test_small_tile_falls_back_when_a_bucket_exceeds_its_chunk(pkg):
         s.dispose()
 
 
-def test_scene_with_bodies_runs_its_one_shape(pkg):
-    particles, config, gravity = pkg.scenes.gate4_scene()
-    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
-    try:
-        eff = pkg.autotune.apply(s, {"p2g_tile": 20, "g2p_tile": 4, "g2p_threads": 128, "bucket_capacity": 0})
-        assert (eff["p2g_tile"], eff["g2p_tile"], eff["g2p_threads"]) == (28, 8, 256)
-    finally:
-        s.dispose()
+def test_scene_with_bodies_runs_28_node_p2g_and_both_g2p_tiles(pkg):
+    particles, config, gravity = pkg.scenes.gate4_scene(mpm_grid=False, liquid_area=100.0)
+    out = {}
+    for tb in (8, 4):
+        s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+        try:
+            eff = pkg.autotune.apply(s, {"p2g_tile": 20, "g2p_tile": tb, "g2p_threads": 128, "bucket_capacity": 0})
+            assert (eff["p2g_tile"], eff["g2p_tile"], eff["g2p_threads"]) == (28, tb, 256)
+            print(f"[tuning] gate4 g2p tile {tb}", flush=True)
+            s.step_batch(6, 1.0 / 240.0, gravity)
+            s.check_numerical_status()
+            out[tb] = s.read_particles(packed=True)[0]
+        finally:
+            s.dispose()
+    assert np.array_equal(out[8], out[4])
 
 
 @pytest.mark.parametrize("mode", ["gather", "fixed"])
