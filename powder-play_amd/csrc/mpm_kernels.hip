@@ -2072,7 +2072,10 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     __shared__ double s_tri[MAX_BODIES][6];
     __shared__ float s_vel[MAX_BODIES][2];
     __shared__ uint32_t s_cnt[MAX_BODIES];
-    __shared__ uint8_t s_moved[3 * MAX_BODIES];  // vertex left its bucket: re-slot it
+    __shared__ uint32_t s_oid[MAX_BODIES];
+    __shared__ uint32_t s_islot[3 * MAX_BODIES];  // vertex slots in the input and output buffers
+    __shared__ uint32_t s_oslot[3 * MAX_BODIES];  // (the output ones change only in the final re-slot)
+    __shared__ uint32_t s_moved[(3 * MAX_BODIES + 31) / 32];  // bit a: vertex a left its bucket
     __shared__ uint32_t s_part_n[BODY_THREADS];  // per-thread partial contact counts
     __shared__ float s_part_i[BODY_THREADS][2];  // per-thread partial contact impulses
     uint32_t err = 0u, err_index = NO_AUX;
@@ -2080,32 +2083,63 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     const float inv_mass = 1.0f / P.particle_mass;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     constexpr uint32_t BODY_WAVES = BODY_THREADS / 64;
+    for (uint32_t a = tid; a < 3u * nb; a += BODY_THREADS) {
+        s_islot[a] = in_slot[a];
+        s_oslot[a] = out_slot[a];
+    }
+    __syncthreads();
 
     // field impulses on each body, barycentric-weighted: one wave a body, each
     // lane summing every 64th node of the body's box, then the wave's lanes
     // in a fixed tree, so the sum's order is fixed
     for (uint32_t b = wave; b < nb; b += BODY_WAVES) {
         double t[6];
-        load_triangle(in, in_slot, b, t);
+        load_triangle(in, s_islot, b, t);
         const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
         const double mny = fmin(t[1], fmin(t[3], t[5])), mxy = fmax(t[1], fmax(t[3], t[5]));
         const int x0 = (int)fmax(-1.0, floor((mnx - band) / dx) - 1.0);
         const int x1 = (int)fmin((double)P.cells_x + 1.0, ceil((mxx + band) / dx) + 1.0);
         const int y0 = (int)fmax(-1.0, floor((mny - band) / dx) - 1.0);
         const int y1 = (int)fmin((double)P.cells_y + 1.0, ceil((mxy + band) / dx) + 1.0);
+        // lane v < 3 updates vertex v: its slot and velocity load during the walk
+        uint32_t sl = 0u;
+        float2 vel = make_float2(0.f, 0.f);
+        if (lane < 3u) {
+            sl = s_oslot[3u * b + lane];
+            vel = out.vel[sl];
+        }
         float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (x1 >= x0 && y1 >= y0) {
             const uint32_t w = (uint32_t)(x1 - x0 + 1), count = w * (uint32_t)(y1 - y0 + 1);
-            for (uint32_t i = lane; i < count; i += 64u) {
-                const uint32_t node = (uint32_t)(y0 + (int)(i / w) + 1) * P.node_w + (uint32_t)(x0 + (int)(i % w) + 1);
-                if (A.field_body[node] != b + 1u) continue;
-                const FieldNode fn = A.field[node];
-                acc[0] += fn.b0 * fn.ix;
-                acc[1] += fn.b0 * fn.iy;
-                acc[2] += fn.b1 * fn.ix;
-                acc[3] += fn.b1 * fn.iy;
-                acc[4] += fn.b2 * fn.ix;
-                acc[5] += fn.b2 * fn.iy;
+            // four nodes a lane in flight, both loads of a node issued
+            // together; the sums still run in node order
+            constexpr uint32_t U = 4u;
+            for (uint32_t i0 = lane; i0 < count; i0 += 64u * U) {
+                uint32_t fb[U];
+                FieldNode fn[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t i = i0 + 64u * u;
+                    fb[u] = 0u;  // no body: the node adds nothing
+                    fn[u] = FieldNode{};
+                    if (i < count) {
+                        const uint32_t node =
+                            (uint32_t)(y0 + (int)(i / w) + 1) * P.node_w + (uint32_t)(x0 + (int)(i % w) + 1);
+                        fb[u] = A.field_body[node];
+                        fn[u] = A.field[node];
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    if (fb[u] == b + 1u) {
+                        acc[0] += fn[u].b0 * fn[u].ix;
+                        acc[1] += fn[u].b0 * fn[u].iy;
+                        acc[2] += fn[u].b1 * fn[u].ix;
+                        acc[3] += fn[u].b1 * fn[u].iy;
+                        acc[4] += fn[u].b2 * fn[u].ix;
+                        acc[5] += fn[u].b2 * fn[u].iy;
+                    }
+                }
             }
         }
 #pragma unroll
@@ -2114,9 +2148,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
             for (int k = 0; k < 6; ++k) acc[k] += __shfl_down(acc[k], off, 64);
 #pragma unroll
         for (int k = 0; k < 6; ++k) acc[k] = __shfl(acc[k], 0, 64);
-        if (lane < 3u) {  // lane v updates vertex v
-            const uint32_t sl = out_slot[3u * b + lane];
-            float2 vel = out.vel[sl];
+        if (lane < 3u) {
             const float ix = lane == 0u ? acc[0] : (lane == 1u ? acc[2] : acc[4]);
             const float iy = lane == 0u ? acc[1] : (lane == 1u ? acc[3] : acc[5]);
             vel.x = vel.x + ix * inv_mass;
@@ -2143,7 +2175,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
         float2 orig[3];
         double ox[3], oy[3], px[3], py[3];
         for (int v = 0; v < 3; ++v) {
-            sl[v] = out_slot[3u * b + (uint32_t)v];
+            sl[v] = s_oslot[3u * b + (uint32_t)v];
             orig[v] = out.pos[sl[v]];
         }
         for (int v = 0; v < 3; ++v) {
@@ -2187,11 +2219,12 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     if (nb >= 2u) {
         for (uint32_t b = tid; b < nb; b += BODY_THREADS) {
             double t[6];
-            load_triangle(in, in_slot, b, t);
+            load_triangle(in, s_islot, b, t);
             for (int q = 0; q < 6; ++q) s_tri[b][q] = t[q];
+            s_oid[b] = A.body_oid[b];
             float vx = 0.0f, vy = 0.0f;
             for (int v = 0; v < 3; ++v) {
-                const float2 vel = out.vel[out_slot[3u * b + (uint32_t)v]];
+                const float2 vel = out.vel[s_oslot[3u * b + (uint32_t)v]];
                 vx += vel.x;
                 vy += vel.y;
             }
@@ -2210,7 +2243,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
             uint32_t c = 0u;
             if (my_team < teams && b < nb) {
                 for (uint32_t j = t_in; j < nb; j += team) {
-                    if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
+                    if (j == b || s_oid[j] == s_oid[b]) continue;
                     double nx, ny;
                     if (b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
                               : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny))
@@ -2232,7 +2265,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
             float ix = 0.0f, iy = 0.0f;
             if (my_team < teams && b < nb && s_cnt[b] != 0u) {
                 for (uint32_t j = t_in; j < nb; j += team) {
-                    if (j == b || A.body_oid[j] == A.body_oid[b]) continue;
+                    if (j == b || s_oid[j] == s_oid[b]) continue;
                     double nx, ny;
                     const bool hit = b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
                                            : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny);
@@ -2261,7 +2294,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
                     ty += s_part_i[tid - t_in + q][1];
                 }
                 for (uint32_t v = t_in; v < 3u; v += team) {
-                    const uint32_t sl = out_slot[3u * b + v];
+                    const uint32_t sl = s_oslot[3u * b + v];
                     const float2 p = out.pos[sl], vel = out.vel[sl];
                     const float2 bx = confine_axis(p.x, vel.x + tx, P.dom_w);
                     const float2 by = confine_axis(p.y, vel.y + ty, P.dom_h);
@@ -2276,7 +2309,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     // CFL check of the vertices (their velocities changed after G2P)
     if (P.cfl_on >= 0.5f) {
         for (uint32_t a = tid; a < 3u * nb; a += BODY_THREADS) {
-            const uint32_t sl = out_slot[a];
+            const uint32_t sl = s_oslot[a];
             const float2 vel = out.vel[sl];
             const Material mt = S.mats[out.mat[sl]];
             const float speed = sqrtf(vel.x * vel.x + vel.y * vel.y);
@@ -2295,41 +2328,50 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     // Every thread tests its vertices; one thread then moves the few that
     // left their bucket, in vertex order.
     const uint32_t cap = S.cap;
+    const uint32_t words = (3u * nb + 31u) / 32u;
+    for (uint32_t q = tid; q < words; q += BODY_THREADS) s_moved[q] = 0u;
+    __syncthreads();
     for (uint32_t a = tid; a < 3u * nb; a += BODY_THREADS) {
-        const uint32_t src = out_slot[a];
+        const uint32_t src = s_oslot[a];
         Stencil st;
         const bool fits = make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st);
-        s_moved[a] = (!fits || bucket_of(st, S.bw) != src / cap) ? 1u : 0u;
+        if (!fits || bucket_of(st, S.bw) != src / cap) atomicOr(&s_moved[a >> 5], 1u << (a & 31u));
     }
     __syncthreads();
     if (tid == 0) {
-        for (uint32_t a = 0; a < 3u * nb; ++a) {
-            if (!s_moved[a]) continue;
-            const uint32_t src = out_slot[a];
-            Stencil st;
-            if (!make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st)) {
-                raise_step(ctl, ERR_STENCIL, step);
-                break;
+        // the set bits in ascending vertex order; a failure ends the walk
+        bool stop = false;
+        for (uint32_t q = 0; q < words && !stop; ++q) {
+            for (uint32_t bits = s_moved[q]; bits && !stop; bits &= bits - 1u) {
+                const uint32_t a = 32u * q + (uint32_t)__builtin_ctz(bits);
+                const uint32_t src = out_slot[a];
+                Stencil st;
+                if (!make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st)) {
+                    raise_step(ctl, ERR_STENCIL, step);
+                    stop = true;
+                    continue;
+                }
+                const uint32_t nbk = bucket_of(st, S.bw), obk = src / cap;
+                if (nbk == obk) continue;
+                const uint32_t kk = out.cnt[nbk];
+                if (kk >= cap) {
+                    raise_step(ctl, ERR_INTERNAL_REBUCKET, step);
+                    stop = true;
+                    continue;
+                }
+                const uint32_t dst = nbk * cap + kk;
+                out.cnt[nbk] = kk + 1u;
+                copy_slot(out, src, dst);
+                const uint32_t last = out.cnt[obk] - 1u;
+                out.cnt[obk] = last;
+                const uint32_t L = obk * cap + last;
+                if (L != src) {
+                    copy_slot(out, L, src);
+                    const uint32_t moved = A.aux_of[out.pid[src]];
+                    if (moved != NO_AUX) out_slot[moved] = src;
+                }
+                out_slot[a] = dst;
             }
-            const uint32_t nbk = bucket_of(st, S.bw), obk = src / cap;
-            if (nbk == obk) continue;
-            const uint32_t kk = out.cnt[nbk];
-            if (kk >= cap) {
-                raise_step(ctl, ERR_INTERNAL_REBUCKET, step);
-                break;
-            }
-            const uint32_t dst = nbk * cap + kk;
-            out.cnt[nbk] = kk + 1u;
-            copy_slot(out, src, dst);
-            const uint32_t last = out.cnt[obk] - 1u;
-            out.cnt[obk] = last;
-            const uint32_t L = obk * cap + last;
-            if (L != src) {
-                copy_slot(out, L, src);
-                const uint32_t moved = A.aux_of[out.pid[src]];
-                if (moved != NO_AUX) out_slot[moved] = src;
-            }
-            out_slot[a] = dst;
         }
     }
 }
