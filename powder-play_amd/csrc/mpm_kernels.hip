@@ -1935,6 +1935,8 @@ __global__ __launch_bounds__(FIELD_THREADS) void k_tri_field(const SceneDesc* __
     const SlotBuf in = S.buf[in_buf];
     __shared__ double s_tri[FIELD_CHUNK][6];
     __shared__ double s_box[FIELD_CHUNK][4];
+    __shared__ uint32_t s_near[FIELD_CHUNK];  // bodies of the chunk whose band box meets this block's rows
+    __shared__ uint32_t s_nnear;
     const uint32_t node = blockIdx.x * FIELD_THREADS + threadIdx.x;
     const bool live = node < P.node_count;
     const double dx = P.dx, band = (double)P.dx * TRIANGLE_FIELD_BAND_CELLS;
@@ -1953,9 +1955,29 @@ __global__ __launch_bounds__(FIELD_THREADS) void k_tri_field(const SceneDesc* __
             s_box[b][3] = fmax(t[1], fmax(t[3], t[5])) + band;
         }
         __syncthreads();
-        if (!live) continue;
         const uint32_t nchunk = min((uint32_t)FIELD_CHUNK, A.nbodies - c0);
-        for (uint32_t b = 0; b < nchunk; ++b) {
+        // the block's nodes lie on storage rows [r0, r1]: one wave lists the
+        // chunk's bodies whose box meets those rows, in ascending order
+        // (ties keep the first body)
+        if (threadIdx.x < 64u) {
+            const uint32_t first = blockIdx.x * FIELD_THREADS;
+            const uint32_t last = min(first + FIELD_THREADS, P.node_count) - 1u;
+            const double y0 = (double)((int)(first / P.node_w) - 1) * dx, y1 = (double)((int)(last / P.node_w) - 1) * dx;
+            uint32_t n = 0u;
+            for (uint32_t b0 = 0; b0 < nchunk; b0 += 64u) {
+                const uint32_t b = b0 + threadIdx.x;
+                const bool meets = b < nchunk && !(y1 < s_box[b][2] || y0 > s_box[b][3]);
+                const unsigned long long m = __ballot(meets);
+                if (meets) s_near[n + (uint32_t)__popcll(m & ((1ull << threadIdx.x) - 1ull))] = b;
+                n += (uint32_t)__popcll(m);
+            }
+            if (threadIdx.x == 0u) s_nnear = n;
+        }
+        __syncthreads();
+        if (!live) continue;
+        const uint32_t nnear = s_nnear;
+        for (uint32_t k = 0; k < nnear; ++k) {
+            const uint32_t b = s_near[k];
             if (px < s_box[b][0] || px > s_box[b][1] || py < s_box[b][2] || py > s_box[b][3]) continue;
             double sd, nx, ny, bary[3];
             if (!tri_sample64(px, py, s_tri[b], sd, nx, ny, bary)) {
