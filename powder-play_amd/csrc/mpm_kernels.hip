@@ -2073,6 +2073,8 @@ __device__ __forceinline__ void copy_slot(const SlotBuf& b, size_t from, size_t 
 constexpr int BODY_THREADS = 1024;  // 16 waves: the body passes are chains of dependent loads
 constexpr int FRACTURE_THREADS = 256;  // the endpoint stress cache needs more than 128 registers
 constexpr uint32_t MAX_BODIES = 1024;  // bodies per scene (checked on the host)
+constexpr uint32_t PAIR_CACHE_BODIES = 64;  // body counts whose pair tests are cached in LDS
+constexpr uint32_t PAIR_CACHE_PAIRS = PAIR_CACHE_BODIES * (PAIR_CACHE_BODIES - 1) / 2;
 
 // After G2P: applyTriangleContactImpulses, enforceBondedTriangleConstraints,
 // resolveBondedTriangleContacts (reference/solver.ts:1592-1802;
@@ -2095,6 +2097,11 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     __shared__ float s_vel[MAX_BODIES][2];
     __shared__ uint32_t s_cnt[MAX_BODIES];
     __shared__ uint32_t s_oid[MAX_BODIES];
+    // up to PAIR_CACHE_BODIES bodies, each unordered pair (i < j) is tested
+    // once: contact flag and the normal from i to j, as the float the
+    // impulse uses
+    __shared__ uint32_t s_pair_hit[PAIR_CACHE_PAIRS];
+    __shared__ float s_pair_n[PAIR_CACHE_PAIRS][2];
     __shared__ uint32_t s_islot[3 * MAX_BODIES];  // vertex slots in the input and output buffers
     __shared__ uint32_t s_oslot[3 * MAX_BODIES];  // (the output ones change only in the final re-slot)
     __shared__ uint32_t s_moved[(3 * MAX_BODIES + 31) / 32];  // bit a: vertex a left its bucket
@@ -2255,6 +2262,29 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
         }
         __syncthreads();
         const double skin = (double)P.dx * TRIANGLE_CONTACT_SKIN_CELLS;
+        const bool cached = nb <= PAIR_CACHE_BODIES;
+        if (cached) {
+            const uint32_t npairs = nb * (nb - 1u) / 2u;
+            for (uint32_t pi = tid; pi < npairs; pi += BODY_THREADS) {
+                uint32_t i = 0u, rest_p = pi;
+                while (rest_p >= nb - 1u - i) {
+                    rest_p -= nb - 1u - i;
+                    ++i;
+                }
+                const uint32_t j = i + 1u + rest_p;
+                double nx = 0.0, ny = 0.0;
+                const bool hit = s_oid[i] != s_oid[j] && pair_contact64(s_tri[i], s_tri[j], skin, nx, ny);
+                s_pair_hit[pi] = hit ? 1u : 0u;
+                s_pair_n[pi][0] = (float)nx;
+                s_pair_n[pi][1] = (float)ny;
+            }
+            __syncthreads();
+        }
+        // index of the unordered pair {b, j} (b != j) in the cache
+        auto pair_index = [nb](uint32_t b, uint32_t j) {
+            const uint32_t i = min(b, j), jj = max(b, j);
+            return i * (2u * nb - i - 1u) / 2u + (jj - i - 1u);
+        };
         // a team of threads per body, each testing every team-th other body;
         // the team's partial counts and impulses are then added in thread order
         const uint32_t team = max(1u, (uint32_t)BODY_THREADS / nb);
@@ -2267,8 +2297,9 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
                 for (uint32_t j = t_in; j < nb; j += team) {
                     if (j == b || s_oid[j] == s_oid[b]) continue;
                     double nx, ny;
-                    if (b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
-                              : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny))
+                    if (cached ? s_pair_hit[pair_index(b, j)] != 0u
+                               : (b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
+                                        : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny)))
                         ++c;
                 }
             }
@@ -2288,16 +2319,27 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
             if (my_team < teams && b < nb && s_cnt[b] != 0u) {
                 for (uint32_t j = t_in; j < nb; j += team) {
                     if (j == b || s_oid[j] == s_oid[b]) continue;
-                    double nx, ny;
-                    const bool hit = b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
-                                           : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny);
-                    if (!hit) continue;
-                    if (b > j) {
-                        nx = -nx;
-                        ny = -ny;
+                    float fnx, fny;
+                    if (cached) {
+                        const uint32_t pi = pair_index(b, j);
+                        if (s_pair_hit[pi] == 0u) continue;
+                        fnx = s_pair_n[pi][0];
+                        fny = s_pair_n[pi][1];
+                    } else {
+                        double nx, ny;
+                        const bool hit = b < j ? pair_contact64(s_tri[b], s_tri[j], skin, nx, ny)
+                                               : pair_contact64(s_tri[j], s_tri[b], skin, nx, ny);
+                        if (!hit) continue;
+                        fnx = (float)nx;
+                        fny = (float)ny;
                     }
+                    if (b > j) {  // the normal from b to j (negation is exact in either precision)
+                        fnx = -fnx;
+                        fny = -fny;
+                    }
+                    const float nx = fnx, ny = fny;
                     const float w = 1.0f / (float)max(s_cnt[b], s_cnt[j]);
-                    const float rn = (s_vel[b][0] - s_vel[j][0]) * (float)nx + (s_vel[b][1] - s_vel[j][1]) * (float)ny;
+                    const float rn = (s_vel[b][0] - s_vel[j][0]) * nx + (s_vel[b][1] - s_vel[j][1]) * ny;
                     if (rn < 0.0f) {
                         const float imp = -rn / (2.0f / body_mass);
                         const float iw = w * TRIANGLE_CONTACT_IMPULSE_RELAXATION;
