@@ -154,6 +154,51 @@ def test_contact_scene_matches_oracle_step_by_step(pkg, orc, factory, steps):
         s.dispose()
 
 
+def many_bodies_scene(pkg, nx=10, ny=7):
+    """nx x ny pebbles (each its own object) on a lattice of pitch 115 < edge
+    132, so neighbours in a row overlap and touch, in a sparse water/oil
+    field: more bodies than the kernel caches pair tests for (64)."""
+    zero = {"m00": 0.0, "m01": 0.0, "m10": 0.0, "m11": 0.0}
+    ident = {"m00": 1.0, "m01": 0.0, "m10": 0.0, "m11": 1.0}
+    nh = {"lambda": 20.0, "mu": 10.0, "model": "neo-hookean"}
+    rng = np.random.default_rng(11)
+    s3 = math.sqrt(3)
+    parts = []
+    for i in range(300):
+        mat = pkg.scenes.GATE4_WATER if i % 2 == 0 else pkg.scenes.GATE4_OIL
+        parts.append({"position": {"x": 60 + 1200 * rng.random(), "y": 60 + 900 * rng.random()},
+                      "velocity": {"x": 6 * rng.standard_normal(), "y": 6 * rng.standard_normal()},
+                      "C": dict(zero), "F": dict(ident), "objectId": 1, "volume": 400.0, "material": dict(mat)})
+    k = 0
+    for r in range(ny):
+        for c in range(nx):
+            cx, cy = 120.0 + 115.0 * c, 120.0 + 125.0 * r
+            vx, vy = 10.0 * rng.standard_normal(), 10.0 * rng.standard_normal()
+            for v, (x, y) in enumerate(((cx - 66, cy - 66 / s3), (cx + 66, cy - 66 / s3), (cx, cy + 132 / s3))):
+                parts.append({"position": {"x": x, "y": y}, "velocity": {"x": vx, "y": vy + 2.0 * v},
+                              "C": dict(zero), "F": dict(ident), "objectId": 2 + k, "bondedTriangleVertex": v,
+                              "volume": 100.0, "material": dict(nh)})
+            k += 1
+    config = {"grid": {"cellsX": 132, "cellsY": 100, "cellSize": 10.0}, "particleMass": 4.0,
+              "particleVolume": 100.0, "bondRestLength": 132.0, "bondReferenceDt": 1.0 / 240.0,
+              "material": dict(nh), "capacity": len(parts)}
+    return parts, config
+
+
+def test_many_bodies_scene_matches_oracle_step_by_step(pkg, orc):
+    # 70 bodies: the pair contacts run without the LDS pair cache
+    parts, config = many_bodies_scene(pkg)
+    gravity = {"x": 0.0, "y": 0.0}
+    dt = 1.0 / 240.0 / 8
+    s = pkg.create_mpm_mls_gpu_solver(parts, config, device=0)
+    try:
+        assert s.state.bonded_body_count == 70
+        for _ in range(3):
+            compare_step(pkg, orc, s, config, dt, gravity)
+    finally:
+        s.dispose()
+
+
 def test_gate4_scene_microsteps_match_oracle(pkg, orc):
     parts, config, gravity = pkg.scenes.gate4_scene()
     dt = 1.0 / 240.0 / 16
