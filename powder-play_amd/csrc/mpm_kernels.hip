@@ -2118,68 +2118,74 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     }
     __syncthreads();
 
-    // field impulses on each body, barycentric-weighted: one wave a body, each
-    // lane summing every 64th node of the body's box, then the wave's lanes
-    // in a fixed tree, so the sum's order is fixed
-    for (uint32_t b = wave; b < nb; b += BODY_WAVES) {
-        double t[6];
-        load_triangle(in, s_islot, b, t);
-        const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
-        const double mny = fmin(t[1], fmin(t[3], t[5])), mxy = fmax(t[1], fmax(t[3], t[5]));
-        const int x0 = (int)fmax(-1.0, floor((mnx - band) / dx) - 1.0);
-        const int x1 = (int)fmin((double)P.cells_x + 1.0, ceil((mxx + band) / dx) + 1.0);
-        const int y0 = (int)fmax(-1.0, floor((mny - band) / dx) - 1.0);
-        const int y1 = (int)fmin((double)P.cells_y + 1.0, ceil((mxy + band) / dx) + 1.0);
-        // lane v < 3 updates vertex v: its slot and velocity load during the walk
+    // field impulses on each body, barycentric-weighted: two bodies a wave,
+    // 32 lanes each, every lane summing every 32nd node of its body's box,
+    // then the 32 lanes in a fixed tree, so the sum's order is fixed
+    const uint32_t half = lane >> 5, hl = lane & 31u;
+    for (uint32_t b0 = 2u * wave; b0 < nb; b0 += 2u * BODY_WAVES) {
+        const uint32_t b = b0 + half;
+        const bool live = b < nb;
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        // lane v < 3 of a half updates vertex v: its slot and velocity load during the walk
         uint32_t sl = 0u;
         float2 vel = make_float2(0.f, 0.f);
-        if (lane < 3u) {
-            sl = s_oslot[3u * b + lane];
-            vel = out.vel[sl];
-        }
-        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (x1 >= x0 && y1 >= y0) {
-            const uint32_t w = (uint32_t)(x1 - x0 + 1), count = w * (uint32_t)(y1 - y0 + 1);
-            // four nodes a lane in flight, both loads of a node issued
-            // together; the sums still run in node order
-            constexpr uint32_t U = 4u;
-            for (uint32_t i0 = lane; i0 < count; i0 += 64u * U) {
-                uint32_t fb[U];
-                FieldNode fn[U];
+        if (live) {
+            double t[6];
+            load_triangle(in, s_islot, b, t);
+            const double mnx = fmin(t[0], fmin(t[2], t[4])), mxx = fmax(t[0], fmax(t[2], t[4]));
+            const double mny = fmin(t[1], fmin(t[3], t[5])), mxy = fmax(t[1], fmax(t[3], t[5]));
+            const int x0 = (int)fmax(-1.0, floor((mnx - band) / dx) - 1.0);
+            const int x1 = (int)fmin((double)P.cells_x + 1.0, ceil((mxx + band) / dx) + 1.0);
+            const int y0 = (int)fmax(-1.0, floor((mny - band) / dx) - 1.0);
+            const int y1 = (int)fmin((double)P.cells_y + 1.0, ceil((mxy + band) / dx) + 1.0);
+            if (hl < 3u) {
+                sl = s_oslot[3u * b + hl];
+                vel = out.vel[sl];
+            }
+            if (x1 >= x0 && y1 >= y0) {
+                const uint32_t w = (uint32_t)(x1 - x0 + 1), count = w * (uint32_t)(y1 - y0 + 1);
+                // four nodes a lane in flight, both loads of a node issued
+                // together; the sums still run in node order
+                constexpr uint32_t U = 4u;
+                for (uint32_t i0 = hl; i0 < count; i0 += 32u * U) {
+                    uint32_t fb[U];
+                    FieldNode fn[U];
 #pragma unroll
-                for (uint32_t u = 0; u < U; ++u) {
-                    const uint32_t i = i0 + 64u * u;
-                    fb[u] = 0u;  // no body: the node adds nothing
-                    fn[u] = FieldNode{};
-                    if (i < count) {
-                        const uint32_t node =
-                            (uint32_t)(y0 + (int)(i / w) + 1) * P.node_w + (uint32_t)(x0 + (int)(i % w) + 1);
-                        fb[u] = A.field_body[node];
-                        fn[u] = A.field[node];
+                    for (uint32_t u = 0; u < U; ++u) {
+                        const uint32_t i = i0 + 32u * u;
+                        fb[u] = 0u;  // no body: the node adds nothing
+                        fn[u] = FieldNode{};
+                        if (i < count) {
+                            const uint32_t node =
+                                (uint32_t)(y0 + (int)(i / w) + 1) * P.node_w + (uint32_t)(x0 + (int)(i % w) + 1);
+                            fb[u] = A.field_body[node];
+                            fn[u] = A.field[node];
+                        }
                     }
-                }
 #pragma unroll
-                for (uint32_t u = 0; u < U; ++u) {
-                    if (fb[u] == b + 1u) {
-                        acc[0] += fn[u].b0 * fn[u].ix;
-                        acc[1] += fn[u].b0 * fn[u].iy;
-                        acc[2] += fn[u].b1 * fn[u].ix;
-                        acc[3] += fn[u].b1 * fn[u].iy;
-                        acc[4] += fn[u].b2 * fn[u].ix;
-                        acc[5] += fn[u].b2 * fn[u].iy;
+                    for (uint32_t u = 0; u < U; ++u) {
+                        if (fb[u] == b + 1u) {
+                            acc[0] += fn[u].b0 * fn[u].ix;
+                            acc[1] += fn[u].b0 * fn[u].iy;
+                            acc[2] += fn[u].b1 * fn[u].ix;
+                            acc[3] += fn[u].b1 * fn[u].iy;
+                            acc[4] += fn[u].b2 * fn[u].ix;
+                            acc[5] += fn[u].b2 * fn[u].iy;
+                        }
                     }
                 }
             }
         }
+        // each half's 32 lanes, in a fixed tree (every lane of the wave takes part)
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
+        for (int off = 16; off > 0; off >>= 1)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc[k] += __shfl_down(acc[k], off, 64);
+            for (int k = 0; k < 6; ++k) acc[k] += __shfl_down(acc[k], off, 32);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) acc[k] = __shfl(acc[k], 0, 64);
-        if (lane < 3u) {
-            const float ix = lane == 0u ? acc[0] : (lane == 1u ? acc[2] : acc[4]);
-            const float iy = lane == 0u ? acc[1] : (lane == 1u ? acc[3] : acc[5]);
+        for (int k = 0; k < 6; ++k) acc[k] = __shfl(acc[k], 0, 32);
+        if (live && hl < 3u) {
+            const float ix = hl == 0u ? acc[0] : (hl == 1u ? acc[2] : acc[4]);
+            const float iy = hl == 0u ? acc[1] : (hl == 1u ? acc[3] : acc[5]);
             vel.x = vel.x + ix * inv_mass;
             vel.y = vel.y + iy * inv_mass;
             if (!finite2(vel)) {
