@@ -282,7 +282,7 @@ struct StepBatch {
     uint32_t fixed_point;
     uint32_t aux_bodies, aux_bonds;  // SCENE_AUX (one scene): body and bond counts
     uint32_t node_count;             // SCENE_AUX: the scene's grid nodes
-    Shape shape;                     // launch shape (AUX_SHAPE for SCENE_AUX)
+    Shape shape;                     // launch shape (28-node P2G, 256-thread G2P for SCENE_AUX)
     bool diag;                   // some scene has diagnostics on
     KernelTimes* timing;
 };

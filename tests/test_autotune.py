@@ -34,7 +34,7 @@ def test_scene_class_tracks_what_the_kernels_depend_on(pkg):
                                volume=np.ones(n), param=np.zeros(n), object_id=np.zeros(n, np.uint32))
     ks = at.scene_class(pkg.layout.validate_input(solid, cfg), 0)
     assert "phases=solid|" in ks
-    # bonded bodies and fracture bonds are flagged (they run the default shape)
+    # bonded bodies and fracture bonds are flagged (they run 28-node P2G tiles and 256-thread G2P)
     particles, gcfg, _ = pkg.scenes.gate4_scene()
     kg = at.scene_class(pkg.layout.validate_input(particles, gcfg), 0)
     assert "bodies=1" in kg and "phases=solid+water+oil" in kg
