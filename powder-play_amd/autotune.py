@@ -266,13 +266,16 @@ def autotune(particles, config: dict, p2g_mode: int = MPM_P2G_GATHER, *, dt: flo
         trial.close()
 
 
-def autotune_batch(scenes, dt, gravity, p2g_mode: int = MPM_P2G_GATHER, *, device: int = 0, warmup: int = 2,
-                   steps: int = 8, p2g_tiles=P2G_TILES, g2p_shapes=G2P_SHAPES) -> dict:
+def autotune_batch(scenes, dt, gravity, p2g_mode: int = MPM_P2G_GATHER, *, device: int = 0, warmup: int = 4,
+                   steps: int = 48, p2g_tiles=P2G_TILES, g2p_shapes=G2P_SHAPES) -> dict:
     """Launch shape of a batch of scenes (MpmMlsGpuBatch): every compiled
     shape is timed on a scratch batch built from the same scenes, by the wall
     time of `steps` microsteps of all scenes (a batch launch covers them all,
     so the two kernels are ranked together), each shape twice, interleaved,
-    keeping its faster pass.  The scenes keep their bucket capacities.
+    keeping its faster pass.  A step_batch call also pays host work per scene
+    (256 tanks: about a quarter of a millisecond a microstep over 8 steps),
+    so `steps` is long enough for the kernels to decide.  The scenes keep
+    their bucket capacities.
     Returns {"tuning": {...}, "candidates": {shape: microstep microseconds}}."""
     from .solver import MpmMlsGpuBatch
     batch = MpmMlsGpuBatch(scenes, p2g_mode=p2g_mode, device=device)
