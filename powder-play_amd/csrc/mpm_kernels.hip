@@ -1011,13 +1011,9 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             if (cell[k] == 0xffffffffu) continue;
-            const uint32_t rc = cell[k] >> 16;
-            const uint32_t q0 = s_cell[rc], q1 = s_cell[rc + 1];
-            uint32_t q = q0;
-            for (uint32_t t = q0; t < q1; ++t) q += s_pid[t] < pid[k] ? 1u : 0u;
+            // the inputs load while the rank is counted
             const uint32_t g = gi[k];
-            Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
-            make_stencil(P.dx, P.cells_x, P.cells_y, in.pos[g], sc);
+            const float2 pos = in.pos[g];
             const uint32_t mat = in.mat[g];
             const float2 v = in.vel[g];
             const float4 C = in.C[g];
@@ -1028,6 +1024,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 S = in.S[g];
                 pv = in.plastic[g];
             }
+            const uint32_t rc = cell[k] >> 16;
+            const uint32_t q0 = s_cell[rc], q1 = s_cell[rc + 1];
+            uint32_t q = q0;
+            for (uint32_t t = q0; t < q1; ++t) q += s_pid[t] < pid[k] ? 1u : 0u;
+            Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
+            make_stencil(P.dx, P.cells_x, P.cells_y, pos, sc);
             const Material mt = material_at(s_mat, mats, mat);
             const float det = det_comp(F);
             uint32_t pf = 0;
