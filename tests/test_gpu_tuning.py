@@ -127,6 +127,9 @@ def test_default_shape_follows_the_grid_size(pkg):
             s.check_numerical_status()
         finally:
             s.dispose()
+    # the autotuner's fallback when it may not tune follows the same rule
+    for cfg, want_g2p in ((config, 4), (big, 8)):
+        assert pkg.autotune.default_tuning(cfg, 0)["g2p_tile"] == want_g2p
     # a batch counts the tiles of all its grids
     b = pkg.MpmMlsGpuBatch([(particles, config)] * 3, device=0)
     try:
