@@ -470,6 +470,19 @@ static int grid_node(const orc_particle* in, const orc_config* c, double* G, lon
         if (mass <= 0) { vel[ph][0] = vel[ph][1] = 0; continue; }
         vel[ph][0] = nd[mi[ph] + 1] / mass + c->dt * c->gravity_x;
         vel[ph][1] = nd[mi[ph] + 2] / mass + c->dt * c->gravity_y;
+        /* drag spring-damper of finishGridVelocity (kernels/shaders.ts:1748-1777):
+         * nodes inside the domain within the radius of the target */
+        if (c->drag_radius > 0 && xi >= 0 && yi >= 0 && xi <= c->cells_x && yi <= c->cells_y) {
+            const double ddx = c->drag_x - (double)xi * dx, ddy = c->drag_y - (double)yi * dx;
+            const double dist = hypot(ddx, ddy);
+            if (dist < c->drag_radius) {
+                const double falloff = 1 - dist / c->drag_radius;
+                const double ax = ddx * c->drag_stiffness - vel[ph][0] * c->drag_damping;
+                const double ay = ddy * c->drag_stiffness - vel[ph][1] * c->drag_damping;
+                vel[ph][0] += c->dt * falloff * ax;
+                vel[ph][1] += c->dt * falloff * ay;
+            }
+        }
     }
     /* solveLiquidGridContact + volumeGradient (solver.ts:1293-1409) */
     if (nd[N_WM] > 0 && nd[N_OM] > 0 && nd[N_WV] > 0 && nd[N_OV] > 0 && xi >= 0 && xi <= c->cells_x &&

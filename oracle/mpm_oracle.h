@@ -82,6 +82,11 @@ typedef struct {
     double particle_mass, particle_volume;
     double gravity_x, gravity_y;
     double wall_friction;
+    /* Drag interaction (the uniform of writeInteractionParameters,
+     * kernels/solver.ts:1389-1401; applied by finishGridVelocity,
+     * kernels/shaders.ts:1748-1777): target point, radius (0 = off),
+     * spring stiffness and damping.  The CPU reference step has no drag. */
+    double drag_x, drag_y, drag_radius, drag_stiffness, drag_damping;
 } orc_config;
 
 /* inputMass, gridMass, outputMass, inputMomentum(x,y), gridMomentum(x,y),

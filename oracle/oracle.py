@@ -13,6 +13,7 @@ solid and its hand-written reverse pass (the checker of the GPU adjoint).
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 import subprocess
 
@@ -20,6 +21,10 @@ import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libmpm_oracle.so"
+# A deliberately broken copy of the oracle, built by tests/test_oracle_mutants.py
+# to show that the closed-form tests catch it, is loaded in its place when this
+# names it.
+_LIB_OVERRIDE = os.environ.get("MPM_ORACLE_LIB")
 
 PARTICLE_DTYPE = np.dtype(
     [
@@ -47,6 +52,8 @@ class Config(ctypes.Structure):
         ("cell_size", ctypes.c_double), ("particle_mass", ctypes.c_double),
         ("particle_volume", ctypes.c_double), ("gravity_x", ctypes.c_double),
         ("gravity_y", ctypes.c_double), ("wall_friction", ctypes.c_double),
+        ("drag_x", ctypes.c_double), ("drag_y", ctypes.c_double), ("drag_radius", ctypes.c_double),
+        ("drag_stiffness", ctypes.c_double), ("drag_damping", ctypes.c_double),
     ]
 
 
@@ -85,9 +92,12 @@ def build() -> pathlib.Path:
 def lib():
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            build()
-        _lib = ctypes.CDLL(str(LIB_PATH))
+        if _LIB_OVERRIDE:
+            _lib = ctypes.CDLL(_LIB_OVERRIDE)
+        else:
+            if not LIB_PATH.exists():
+                build()
+            _lib = ctypes.CDLL(str(LIB_PATH))
         P = ctypes.c_void_p
         _lib.orc_step_f64.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Config), P, ctypes.POINTER(Diagnostics)]
         _lib.orc_run_f64.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Config), ctypes.c_int32]
@@ -107,9 +117,11 @@ def _ptr(a: np.ndarray):
 
 
 def make_config(dt, cells_x, cells_y, cell_size, particle_mass, particle_volume,
-                gravity=(0.0, 0.0), wall_friction=0.0) -> Config:
+                gravity=(0.0, 0.0), wall_friction=0.0, drag=None) -> Config:
+    """``drag`` = (target x, target y, radius, stiffness, damping) or None."""
+    d = (0.0, 0.0, 0.0, 0.0, 0.0) if drag is None else tuple(float(v) for v in drag)
     return Config(dt, cells_x, cells_y, cell_size, particle_mass, particle_volume,
-                  gravity[0], gravity[1], wall_friction)
+                  gravity[0], gravity[1], wall_friction, *d)
 
 
 def particles_from_packed(packed20: np.ndarray) -> np.ndarray:
