@@ -74,16 +74,18 @@ def default_cache_path() -> pathlib.Path:
     return pathlib.Path.home() / ".cache" / "mpm355" / "tuning.json"
 
 
-def default_tuning(config: dict, device: int = 0) -> dict:
+def default_tuning(config: dict, device: int = 0, bodies: bool = False) -> dict:
     """The shape the library runs a scene with when it is given none
     (include/mpm355.h, "kernel tuning"): DEFAULT_TUNING, with 4x4-bucket G2P
     tiles when the grid has fewer 8x8-bucket tiles than the device has
-    compute units."""
+    compute units.  Scenes with bonded bodies or fracture bonds (``bodies``)
+    run BODIES_TUNING's 28-node P2G tile and 256-thread G2P blocks, with the
+    G2P tile side from the same grid-size rule."""
     bw = int(config["grid"]["cellsX"]) // 4 + 1
     bh = int(config["grid"]["cellsY"]) // 4 + 1
     tiles8 = -(-bw // 8) * -(-bh // 8)
     _, cus = device_arch(max(device, 0))
-    return dict(DEFAULT_TUNING, g2p_tile=8 if tiles8 >= cus else 4)
+    return dict(BODIES_TUNING if bodies else DEFAULT_TUNING, g2p_tile=8 if tiles8 >= cus else 4)
 
 
 def device_key(device: int = 0) -> str:
@@ -326,7 +328,7 @@ def tuning_for(particles, config: dict, p2g_mode: int = MPM_P2G_GATHER, *, cache
     if hit is not None:
         return dict(hit["tuning"])
     if not tune_if_missing:
-        return default_tuning(config, device)
+        return default_tuning(config, device, bodies=state.bonded_body_count > 0 or state.fracture_bonds is not None)
     entry = autotune(particles, config, p2g_mode, device=device, **kw)
     store_entry(dev, klass, entry, cache)
     return dict(entry["tuning"])

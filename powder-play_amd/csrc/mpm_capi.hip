@@ -596,6 +596,20 @@ int settle(mpm_ctx* c) {
     }
 }
 
+// Give a scene the FP64 stress cache (SCENE_GENERAL) after it has stepped:
+// its committed state goes back to the caller-order arrays and is bucketed
+// again into slot buffers that carry the cache.
+int make_general(mpm_ctx* o) {
+    int s = settle(o);
+    if (s) return s;
+    HIP_TRY(o, launch_unbucket(o->lc, o->cur));
+    HIP_TRY(o, hipStreamSynchronize(o->stream));
+    o->lc.scene_flags |= SCENE_GENERAL;
+    int a = alloc_slots(o, o->lc.cap);
+    if (a) return a;
+    return rebucket_all(o, o->lc.cap);
+}
+
 }  // namespace
 
 // RN(1/b) for the fixed-point kernel's division by particleMass and
@@ -925,13 +939,24 @@ int mpm_append_particles(mpm_ctx* c, const uint32_t* packed20, const uint32_t* o
         int a = alloc_slots(c, c->lc.cap);
         if (a) return a;
     }
-    return rebucket_all(c, c->lc.cap);
+    int r = rebucket_all(c, c->lc.cap);
+    if (r || !c->owner || was_general || !scene_general(c->lc.scene_flags)) return r;
+    // One kernel variant serves a batch's launch, so when this scene now
+    // needs the stress cache every scene of its batch carries it, as
+    // mpm_batch_create arranges for scenes that need it from the start.
+    for (mpm_ctx* o : c->owner->scenes) {
+        if (o == c || scene_general(o->lc.scene_flags)) continue;
+        r = make_general(o);
+        if (r) return fail(c, r, "batch scene upgrade: %s", o->error.c_str());
+    }
+    return MPM_OK;
 }
 
 // validateMpmSolidBondGraph (reference/solid-bonds.ts:11-66) on the packed
 // 24-byte bonds (layout.ts:527-548), then the device tables.
 int mpm_set_fracture_bonds(mpm_ctx* c, const void* bonds24, uint32_t count) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (c->owner) return fail(c, MPM_EINVAL, "fracture bonds cannot be set on a scene of a batch");
     if (count > 0 && !bonds24) return fail(c, MPM_EINVAL, "bonds must not be NULL");
     if (c->stepped) return fail(c, MPM_ESTATE, "fracture bonds must be set before the first step");
     std::vector<Bond> bonds(count);

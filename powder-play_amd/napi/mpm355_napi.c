@@ -6,8 +6,16 @@
  * device work (solver.ts:1414-1441) and device failures become Error with a
  * `code` property.  readCompact returns a Promise resolved from a worker
  * thread, like the reference's mapAsync readback (solver.ts:2388-2512).
+ *
+ * The C ABI asks callers to serialise the calls on one context.  Each context
+ * object therefore carries a mutex: every call holds it for the duration of
+ * its C-ABI call, and so does the readCompact worker while it waits for the
+ * GPU.  JS may keep submitting work while a readback is pending (the
+ * reference allows that); such a call waits for the readback's wait to end.
  */
 #include <node_api.h>
+#include <pthread.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -32,14 +40,23 @@ static napi_value throw_code(napi_env env, int code, mpm_ctx* ctx) {
     return NULL;
 }
 
+/* A solver context and the lock that serialises the calls on it. */
+typedef struct {
+    mpm_ctx* ctx;
+    pthread_mutex_t lock;
+} ctx_box;
+
 static void finalize_ctx(napi_env env, void* data, void* hint) {
     (void)env;
     (void)hint;
-    mpm_destroy((mpm_ctx*)data);
+    ctx_box* b = (ctx_box*)data;
+    mpm_destroy(b->ctx);
+    pthread_mutex_destroy(&b->lock);
+    free(b);
 }
 
 /* The context of argument 0, or NULL after a thrown TypeError. */
-static mpm_ctx* ctx_arg(napi_env env, napi_value v) {
+static ctx_box* ctx_arg(napi_env env, napi_value v) {
     napi_valuetype t;
     void* p = NULL;
     if (napi_typeof(env, v, &t) != napi_ok || t != napi_external || napi_get_value_external(env, v, &p) != napi_ok ||
@@ -47,8 +64,25 @@ static mpm_ctx* ctx_arg(napi_env env, napi_value v) {
         napi_throw_type_error(env, NULL, "expected a solver context");
         return NULL;
     }
-    return (mpm_ctx*)p;
+    return (ctx_box*)p;
 }
+
+/* Throw for a call that failed under the context's lock, then release it
+ * (the message is read while the lock is still held). */
+static napi_value fail_unlock(napi_env env, int code, ctx_box* b) {
+    throw_code(env, code, b->ctx);
+    pthread_mutex_unlock(&b->lock);
+    return NULL;
+}
+
+/* Run one C-ABI call under the context's lock; throw on failure. */
+#define LOCKED_CALL(box, call)                                     \
+    do {                                                           \
+        pthread_mutex_lock(&(box)->lock);                          \
+        const int code_ = (call);                                  \
+        if (code_ != MPM_OK) return fail_unlock(env, code_, box);  \
+        pthread_mutex_unlock(&(box)->lock);                        \
+    } while (0)
 
 static int get_number(napi_env env, napi_value obj, const char* key, double* out, int required) {
     bool has = false;
@@ -146,8 +180,20 @@ static napi_value create(napi_env env, napi_callback_info info) {
     mpm_ctx* ctx = NULL;
     const int code = mpm_create(&cfg, (const uint32_t*)packed, (const uint32_t*)ids, n, &ctx);
     if (code != MPM_OK) return throw_code(env, code, NULL);
+    ctx_box* box = (ctx_box*)calloc(1, sizeof *box);
+    if (!box) {
+        mpm_destroy(ctx);
+        napi_throw_error(env, NULL, "out of memory");
+        return NULL;
+    }
+    box->ctx = ctx;
+    pthread_mutex_init(&box->lock, NULL);
     napi_value ext;
-    CHECK(napi_create_external(env, ctx, finalize_ctx, NULL, &ext));
+    if (napi_create_external(env, box, finalize_ctx, NULL, &ext) != napi_ok) {
+        finalize_ctx(env, box, NULL);
+        napi_throw_error(env, NULL, "Node-API call failed: napi_create_external");
+        return NULL;
+    }
     return ext;
 }
 
@@ -156,8 +202,8 @@ static napi_value step_batch(napi_env env, napi_callback_info info) {
     size_t argc = 4;
     napi_value argv[4];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 4 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 4 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     void* params = NULL;
     size_t len = 0;
     bool is_ab = false;
@@ -175,8 +221,8 @@ static napi_value step_batch(napi_env env, napi_callback_info info) {
     bool measure = false;
     CHECK(napi_get_value_uint32(env, argv[2], &steps));
     CHECK(napi_get_value_bool(env, argv[3], &measure));
-    const int code = mpm_step_batch(ctx, params, steps, measure ? 1u : 0u);
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_step_batch(box->ctx, params, steps, measure ? 1u : 0u));
+    return NULL;
 }
 
 static napi_value compact_object(napi_env env, float speed, uint32_t completed, const uint32_t status[8]) {
@@ -199,12 +245,11 @@ static napi_value read_compact_sync(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     float speed = 0;
     uint32_t completed = 0, status[8];
-    const int code = mpm_read_compact(ctx, &speed, &completed, status);
-    if (code != MPM_OK) return throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_read_compact(box->ctx, &speed, &completed, status));
     return compact_object(env, speed, completed, status);
 }
 
@@ -212,16 +257,20 @@ typedef struct {
     napi_async_work work;
     napi_deferred deferred;
     napi_ref ctx_ref; /* keeps the context alive while the worker waits */
-    mpm_ctx* ctx;
+    ctx_box* box;
     float speed;
     uint32_t completed, status[8];
     int code;
+    char message[512]; /* the library's message of a failed read */
 } compact_job;
 
 static void compact_exec(napi_env env, void* p) {
     (void)env;
     compact_job* j = (compact_job*)p;
-    j->code = mpm_read_compact(j->ctx, &j->speed, &j->completed, j->status);
+    pthread_mutex_lock(&j->box->lock);
+    j->code = mpm_read_compact(j->box->ctx, &j->speed, &j->completed, j->status);
+    if (j->code != MPM_OK) snprintf(j->message, sizeof j->message, "%s", mpm_last_error(j->box->ctx));
+    pthread_mutex_unlock(&j->box->lock);
 }
 
 static void compact_done(napi_env env, napi_status s, void* p) {
@@ -229,7 +278,7 @@ static void compact_done(napi_env env, napi_status s, void* p) {
     (void)s;
     if (j->code != MPM_OK) {
         napi_value msg, err, code;
-        napi_create_string_utf8(env, mpm_last_error(j->ctx), NAPI_AUTO_LENGTH, &msg);
+        napi_create_string_utf8(env, j->message, NAPI_AUTO_LENGTH, &msg);
         napi_create_string_utf8(env, j->code == MPM_EINVAL ? "MPM_EINVAL" : "MPM_EDEVICE", NAPI_AUTO_LENGTH, &code);
         napi_create_error(env, code, msg, &err);
         napi_reject_deferred(env, j->deferred, err);
@@ -246,14 +295,14 @@ static napi_value read_compact(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1], promise, name;
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     compact_job* j = (compact_job*)calloc(1, sizeof *j);
     if (!j) {
         napi_throw_error(env, NULL, "out of memory");
         return NULL;
     }
-    j->ctx = ctx;
+    j->box = box;
     CHECK(napi_create_reference(env, argv[0], 1, &j->ctx_ref));
     CHECK(napi_create_promise(env, &j->deferred, &promise));
     CHECK(napi_create_string_utf8(env, "mpm_read_compact", NAPI_AUTO_LENGTH, &name));
@@ -267,10 +316,10 @@ static napi_value clear_status(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
-    const int code = mpm_clear_status(ctx);
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
+    LOCKED_CALL(box, mpm_clear_status(box->ctx));
+    return NULL;
 }
 
 /* counts(ctx) -> { particleCount, particleCapacity, nodeCount } */
@@ -278,11 +327,10 @@ static napi_value counts(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1], out, v;
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     uint32_t n = 0, cap = 0, nodes = 0;
-    const int code = mpm_get_counts(ctx, &n, &cap, &nodes);
-    if (code != MPM_OK) return throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_get_counts(box->ctx, &n, &cap, &nodes));
     CHECK(napi_create_object(env, &out));
     napi_create_uint32(env, n, &v);
     napi_set_named_property(env, out, "particleCount", v);
@@ -298,22 +346,28 @@ static napi_value read_particles(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2], out, rec_ab, ids_ab, rec, ids;
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     void* idx = NULL;
     size_t nidx = 0;
     if (argc >= 2 && !typed_arg(env, argv[1], napi_uint32_array, &idx, &nidx, 1)) return NULL;
     uint32_t n = (uint32_t)nidx;
+    pthread_mutex_lock(&box->lock);
     if (!idx) {
         uint32_t cap = 0, nodes = 0;
-        const int code = mpm_get_counts(ctx, &n, &cap, &nodes);
-        if (code != MPM_OK) return throw_code(env, code, ctx);
+        const int code = mpm_get_counts(box->ctx, &n, &cap, &nodes);
+        if (code != MPM_OK) return fail_unlock(env, code, box);
     }
     void *rd, *id;
-    CHECK(napi_create_arraybuffer(env, (size_t)n * MPM_PARTICLE_WORDS * 4, &rd, &rec_ab));
-    CHECK(napi_create_arraybuffer(env, (size_t)n * 4, &id, &ids_ab));
-    const int code = mpm_read_particles(ctx, (const uint32_t*)idx, n, (uint32_t*)rd, (uint32_t*)id);
-    if (code != MPM_OK) return throw_code(env, code, ctx);
+    if (napi_create_arraybuffer(env, (size_t)n * MPM_PARTICLE_WORDS * 4, &rd, &rec_ab) != napi_ok ||
+        napi_create_arraybuffer(env, (size_t)n * 4, &id, &ids_ab) != napi_ok) {
+        pthread_mutex_unlock(&box->lock);
+        napi_throw_error(env, NULL, "Node-API call failed: napi_create_arraybuffer");
+        return NULL;
+    }
+    const int code = mpm_read_particles(box->ctx, (const uint32_t*)idx, n, (uint32_t*)rd, (uint32_t*)id);
+    if (code != MPM_OK) return fail_unlock(env, code, box);
+    pthread_mutex_unlock(&box->lock);
     CHECK(napi_create_typedarray(env, napi_uint32_array, (size_t)n * MPM_PARTICLE_WORDS, rec_ab, 0, &rec));
     CHECK(napi_create_typedarray(env, napi_uint32_array, n, ids_ab, 0, &ids));
     CHECK(napi_create_object(env, &out));
@@ -327,8 +381,8 @@ static napi_value append_particles(napi_env env, napi_callback_info info) {
     size_t argc = 3;
     napi_value argv[3];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 3 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 3 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     void *packed, *ids;
     size_t words, nid;
     if (!typed_arg(env, argv[1], napi_uint32_array, &packed, &words, 0)) return NULL;
@@ -337,9 +391,9 @@ static napi_value append_particles(napi_env env, napi_callback_info info) {
         napi_throw_range_error(env, "MPM_EINVAL", "packed particles and objectIds disagree");
         return NULL;
     }
-    const int code = mpm_append_particles(ctx, (const uint32_t*)packed, (const uint32_t*)ids,
-                                          (uint32_t)(words / MPM_PARTICLE_WORDS));
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_append_particles(box->ctx, (const uint32_t*)packed, (const uint32_t*)ids,
+                                          (uint32_t)(words / MPM_PARTICLE_WORDS)));
+    return NULL;
 }
 
 /* setInteraction(ctx, Float32Array(8)) */
@@ -347,8 +401,8 @@ static napi_value set_interaction(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     void* p;
     size_t n;
     if (!typed_arg(env, argv[1], napi_float32_array, &p, &n, 0)) return NULL;
@@ -356,8 +410,8 @@ static napi_value set_interaction(napi_env env, napi_callback_info info) {
         napi_throw_range_error(env, "MPM_EINVAL", "interaction uniform must be 8 floats");
         return NULL;
     }
-    const int code = mpm_set_interaction(ctx, (const float*)p);
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_set_interaction(box->ctx, (const float*)p));
+    return NULL;
 }
 
 /* setDiagnostics(ctx, enable) / readDiagnostics(ctx) -> Float64Array(9) */
@@ -365,24 +419,23 @@ static napi_value set_diagnostics(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     bool on = false;
     CHECK(napi_get_value_bool(env, argv[1], &on));
-    const int code = mpm_set_diagnostics(ctx, on ? 1u : 0u);
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_set_diagnostics(box->ctx, on ? 1u : 0u));
+    return NULL;
 }
 
 static napi_value read_diagnostics(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1], ab, arr;
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     void* data;
     CHECK(napi_create_arraybuffer(env, 9 * sizeof(double), &data, &ab));
-    const int code = mpm_read_diagnostics(ctx, (double*)data);
-    if (code != MPM_OK) return throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_read_diagnostics(box->ctx, (double*)data));
     CHECK(napi_create_typedarray(env, napi_float64_array, 9, ab, 0, &arr));
     return arr;
 }
@@ -393,8 +446,8 @@ static napi_value set_fracture_bonds(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     void* p;
     size_t words;
     if (!typed_arg(env, argv[1], napi_uint32_array, &p, &words, 0)) return NULL;
@@ -402,8 +455,8 @@ static napi_value set_fracture_bonds(napi_env env, napi_callback_info info) {
         napi_throw_range_error(env, "MPM_EINVAL", "bond records are 6 words each");
         return NULL;
     }
-    const int code = mpm_set_fracture_bonds(ctx, p, (uint32_t)(words / 6));
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_set_fracture_bonds(box->ctx, p, (uint32_t)(words / 6)));
+    return NULL;
 }
 
 /* readFractureBonds(ctx, count) -> Uint32Array(6 * count) */
@@ -411,8 +464,8 @@ static napi_value read_fracture_bonds(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2], ab, arr;
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     uint32_t count = 0;
     if (napi_get_value_uint32(env, argv[1], &count) != napi_ok) {
         napi_throw_type_error(env, NULL, "count must be a number");
@@ -420,8 +473,7 @@ static napi_value read_fracture_bonds(napi_env env, napi_callback_info info) {
     }
     void* data;
     CHECK(napi_create_arraybuffer(env, (size_t)count * 24, &data, &ab));
-    const int code = mpm_read_fracture_bonds(ctx, data, count);
-    if (code != MPM_OK) return throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_read_fracture_bonds(box->ctx, data, count));
     CHECK(napi_create_typedarray(env, napi_uint32_array, (size_t)count * 6, ab, 0, &arr));
     return arr;
 }
@@ -432,8 +484,8 @@ static napi_value set_tuning(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     double v[4] = {0, 0, 0, 0};
     const char* keys[4] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity"};
     for (int k = 0; k < 4; ++k)
@@ -444,19 +496,18 @@ static napi_value set_tuning(napi_env env, napi_callback_info info) {
             return NULL;
         }
     mpm_tuning t = {(uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]};
-    const int code = mpm_set_tuning(ctx, &t);
-    return code == MPM_OK ? NULL : throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_set_tuning(box->ctx, &t));
+    return NULL;
 }
 
 static napi_value get_tuning(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1], out, x;
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-    mpm_ctx* ctx = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
-    if (!ctx) return NULL;
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
     mpm_tuning t;
-    const int code = mpm_get_tuning(ctx, &t);
-    if (code != MPM_OK) return throw_code(env, code, ctx);
+    LOCKED_CALL(box, mpm_get_tuning(box->ctx, &t));
     CHECK(napi_create_object(env, &out));
     const uint32_t vals[4] = {t.p2g_tile, t.g2p_tile, t.g2p_threads, t.bucket_capacity};
     const char* keys[4] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity"};

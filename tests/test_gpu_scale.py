@@ -1,0 +1,96 @@
+"""GPU: the 1M-particle water/oil column over a whole 1/240 s macrostep, in
+both P2G modes, against the oracle.
+
+* Float32 state: at microsteps 1, 20 and 40 of the macrostep (40 microsteps
+  of 1/9600 s), the oracle takes one FP64 step from the GPU's own state and
+  the GPU takes the same step.  By microstep 20 the particles carry an
+  affine C and a deformed F, so the stress and affine terms of P2G are
+  compared at full size.  Bars: relative L2 error of the velocities 1e-4 and
+  of the positions 1e-7 (the stated float32 tolerance of the step).
+* Fixed-point P2G: at microsteps 20, 21 and 22 the 11-word int32 grid of the
+  GPU's P2G (44 MB) equals the oracle's integer restatement of the WGSL
+  scatter (kernels/shaders.ts:971-1070) word for word; the SHA-256 of both
+  grids is printed per microstep.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("x", "y", "vx", "vy", "c00", "c01", "c10", "c11", "f00", "f01", "f10", "f11")
+CELLS = 1000
+DT = 1.0 / 9600.0
+
+
+def compare_with_oracle_step(pkg, orc, s, mode, k):
+    before, _ = s.read_particles(packed=True)
+    s.step(DT, pkg.scenes.WATER_OIL_GRAVITY)
+    s.check_numerical_status()
+    got, _ = s.read_particles(packed=True)
+    cfg = orc.make_config(float(np.float32(DT)), CELLS, CELLS, 1.0, 0.02, 1.0, (0.0, 90.0))
+    exp, diag = orc.step_f64(orc.particles_from_packed(before), cfg)
+    want = np.stack([exp[n] for n in STATE], axis=1)
+    have = got.view(np.float32)[:, :12].astype(np.float64)
+    inp = before.view(np.float32)[:, 4:12].astype(np.float64)
+    rel_v = float(np.linalg.norm(have[:, 2:4] - want[:, 2:4]) / np.linalg.norm(want[:, 2:4]))
+    rel_x = float(np.linalg.norm(have[:, :2] - want[:, :2]) / np.linalg.norm(want[:, :2]))
+    rel_c = float(np.linalg.norm(have[:, 4:8] - want[:, 4:8]) / np.linalg.norm(want[:, 4:8]))
+    c_in = float(np.abs(inp[:, 0:4]).max())
+    f_dev = float(np.abs(inp[:, 4:8] - np.array([1.0, 0.0, 0.0, 1.0])).max())
+    print(f"1M column {mode}, microstep {k}: input max |C| {c_in:.2e}, max |F - I| {f_dev:.2e}; "
+          f"rel L2 v {rel_v:.2e}, x {rel_x:.2e}, C {rel_c:.2e}", flush=True)
+    assert rel_v <= 1e-4 and rel_x <= 1e-7
+    assert diag.grid_mass == pytest.approx(diag.input_mass, rel=1e-12)
+    if k > 1:
+        assert c_in > 1e-2 and f_dev > 1e-5  # stress and affine terms are live
+    return rel_v, rel_x
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+def test_column_1m_macrostep_against_the_oracle(pkg, orc, mode):
+    arr, config = pkg.scenes.water_oil_column(cells=CELLS)
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    s = factory(arr, config, device=0, keep_fixed_grid=(mode == "fixed"))
+    try:
+        k = 1
+        errors = {}
+        while k <= 40:
+            if k in (1, 20, 40):
+                errors[k] = compare_with_oracle_step(pkg, orc, s, mode, k)
+                k += 1
+            else:
+                nxt = min(m for m in (20, 40, 41) if m > k)
+                s.step_batch(nxt - k, DT, pkg.scenes.WATER_OIL_GRAVITY)
+                k = nxt
+        s.check_numerical_status()
+        assert sorted(errors) == [1, 20, 40]
+    finally:
+        s.dispose()
+
+
+def test_column_1m_fixed_point_words_bit_exact(pkg, orc):
+    arr, config = pkg.scenes.water_oil_column(cells=CELLS)
+    s = pkg.create_experimental_mpm_mls_atomic_gpu_solver(arr, config, device=0, keep_fixed_grid=True)
+    prm = orc.params_f32(CELLS, CELLS, 1.0, 0.02, 1.0, DT, len(arr))
+    try:
+        s.step_batch(19, DT, pkg.scenes.WATER_OIL_GRAVITY)
+        s.check_numerical_status()
+        for k in (20, 21, 22):
+            before, _ = s.read_particles(packed=True)
+            s.step(DT, pkg.scenes.WATER_OIL_GRAVITY)
+            s.check_numerical_status()
+            got = s.read_fixed_grid()
+            want, flags = orc.fixed_p2g_f32(before, prm)
+            hg = hashlib.sha256(np.ascontiguousarray(got).tobytes()).hexdigest()[:16]
+            hw = hashlib.sha256(np.ascontiguousarray(want).tobytes()).hexdigest()[:16]
+            mism = int(np.count_nonzero(got != want))
+            print(f"microstep {k}: {got.size} int32 words, sha256 GPU {hg} / CPU {hw}, {mism} differ", flush=True)
+            assert flags == 0
+            assert mism == 0 and hg == hw
+            assert np.count_nonzero(got[:, 1:3]) > 0  # momenta present
+    finally:
+        s.dispose()

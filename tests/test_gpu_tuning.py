@@ -12,6 +12,9 @@ scene class), and the cache answers the second call.
 """
 from __future__ import annotations
 
+import pathlib
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -130,6 +133,17 @@ def test_default_shape_follows_the_grid_size(pkg):
     # the autotuner's fallback when it may not tune follows the same rule
     for cfg, want_g2p in ((config, 4), (big, 8)):
         assert pkg.autotune.default_tuning(cfg, 0)["g2p_tile"] == want_g2p
+    # and for a scene with bodies reports the shape the library runs it with
+    gp, gc, _ = pkg.scenes.gate4_scene()
+    s = pkg.create_mpm_mls_gpu_solver(gp, gc, device=0)
+    try:
+        eff = s.ctx.get_tuning().as_dict()
+    finally:
+        s.dispose()
+    fallback = pkg.autotune.tuning_for(gp, gc, tune_if_missing=False, device=0,
+                                       cache=pathlib.Path(tempfile.mkdtemp()) / "none.json")
+    assert {k: fallback[k] for k in ("p2g_tile", "g2p_tile", "g2p_threads")} == \
+        {k: eff[k] for k in ("p2g_tile", "g2p_tile", "g2p_threads")}
     # a batch counts the tiles of all its grids
     b = pkg.MpmMlsGpuBatch([(particles, config)] * 3, device=0)
     try:
