@@ -76,7 +76,11 @@ def test_drag_matches_the_oracle(pkg, orc, mode):
     assert 0 < dragged.sum() < len(keys)
     err = np.abs(have_v - want_v).max()
     print(f"drag ({mode}): {dragged.sum()} of {len(keys)} nodes dragged; max |dv_node| {err:.2e}")
-    assert err <= 1e-5 * max(1.0, float(np.abs(want_v).max()))
+    # float32 sums; the fixed-point path adds the 2^-24 quantum of each
+    # contribution (relative to particleMass), divided by node masses as
+    # small as 1/64 of a particle's
+    tol = 1e-5 if mode == "gather" else 1e-4
+    assert err <= tol * max(1.0, float(np.abs(want_v).max()))
     # the drag is visible: the dragged nodes moved away from the undragged velocity
     plain = orc.grid_f64(inp, orc.make_config(float(F32(dt)), 48, 48, 1.0, 0.25, 0.25))
     moved = np.abs(want_v - plain[touched][:, [G.index("vx"), G.index("vy")]]).max(axis=1)
@@ -84,7 +88,7 @@ def test_drag_matches_the_oracle(pkg, orc, mode):
     exp, _ = orc.step_f64(inp, cfg)
     have = got.view(np.float32)[:, :4].astype(np.float64)
     want = np.stack([exp[k] for k in ("x", "y", "vx", "vy")], axis=1)
-    assert np.abs(have - want).max() <= 1e-5
+    assert np.abs(have - want).max() <= tol
 
 
 # ------------------------------------------------------------- max speed
