@@ -664,7 +664,7 @@ int prepare_params(mpm_ctx* c, const void* params144, Params& p) {
     p.drag_pad[0] = p.drag_pad[1] = 0.0f;
     p.inv_mass_ref = reciprocal_for_division(p.particle_mass);
     p.inv_volume_ref = reciprocal_for_division(p.particle_volume);
-    p.wall_pad = 0.0f;
+    p.inv_dx = exact_reciprocal(p.dx);
     p.drag_target_x = c->interaction[0];
     p.drag_target_y = c->interaction[1];
     p.drag_radius = c->interaction[2];

@@ -58,13 +58,24 @@ struct Params {
     uint32_t fracture[4];                                      // fracture
     float wall_friction;                                       // wall
     float inv_mass_ref, inv_volume_ref;  // RN(1/particleMass), RN(1/particleVolume) or 0 (host-set)
-    float wall_pad;
+    float inv_dx;  // 1/dx when dx is a power of two (x/dx == x*inv_dx exactly), else 0 (host-set)
     float drag_target_x, drag_target_y, drag_radius, drag_stiffness; // interaction
     float drag_damping;
     uint32_t diag_on;  // FP64 mass/momentum reductions of the batch (host-set)
     float drag_pad[2];
 };
 static_assert(sizeof(Params) == 176, "Params must mirror the 144+32 byte blocks");
+
+// 1/x when x is a power of two whose reciprocal is a normal float (then
+// y / x and y * (1/x) are the same correctly rounded value for every y), else
+// 0: the kernels divide by x.
+__host__ __device__ inline float exact_reciprocal(float x) {
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    const uint32_t e = (u >> 23) & 0xffu;
+    if ((u & 0x807fffffu) != 0u || e < 2u || e > 252u) return 0.0f;
+    return 1.0f / x;
+}
 
 // Control words: status (shaders.ts:157-172) plus batch bookkeeping.
 struct Control {

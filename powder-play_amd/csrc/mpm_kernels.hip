@@ -302,10 +302,13 @@ __device__ __forceinline__ float4 affine_of(const Params& P, const Material& mt,
                        mt.mass * C.w + ss * S.w);
 }
 
-// Stencil base and fraction (shaders.ts:667-676, :1023-1028).
+// Stencil base and fraction (shaders.ts:667-676, :1023-1028).  inv_dx is
+// exact_reciprocal(dx): for a power-of-two cell size the scaling is one
+// exact multiply instead of a division, with the same result.
 struct Stencil { int bx, by; float fx, fy; };
-__device__ __forceinline__ bool make_stencil(float dx, uint32_t cells_x, uint32_t cells_y, float2 p, Stencil& s) {
-    const float gx = p.x / dx, gy = p.y / dx;
+__device__ __forceinline__ bool make_stencil(float dx, float inv_dx, uint32_t cells_x, uint32_t cells_y, float2 p,
+                                             Stencil& s) {
+    const float gx = inv_dx != 0.0f ? p.x * inv_dx : p.x / dx, gy = inv_dx != 0.0f ? p.y * inv_dx : p.y / dx;
     if (!finitef(gx) || !finitef(gy)) return false;
     s.bx = (int)floorf(gx - 0.5f);
     s.by = (int)floorf(gy - 0.5f);
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(256) void k_bucket(Control* ctl, Home home, const M
     float2 p = make_float2(0.f, 0.f);
     if (active) {
         p = home.pos[i];
-        if (!make_stencil(dx, cells_x, cells_y, p, s)) {
+        if (!make_stencil(dx, exact_reciprocal(dx), cells_x, cells_y, p, s)) {
             raise_step(ctl, ERR_STENCIL, 0u);
             active = false;
         }
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(256) void k_keys(Home home, uint32_t* keys, uint32_
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Stencil s;
-    if (home.model[i] == MODEL_REMOVED || !make_stencil(dx, cells_x, cells_y, home.pos[i], s)) {
+    if (home.model[i] == MODEL_REMOVED || !make_stencil(dx, exact_reciprocal(dx), cells_x, cells_y, home.pos[i], s)) {
         keys[i] = INVALID_KEY;
         return;
     }
@@ -508,16 +511,20 @@ __device__ __forceinline__ float2 finish_velocity(const Params& P, float m, floa
     }
     return v;
 }
-// applyWallCushion / applyWallTangentialFriction (shaders.ts:1438-1457)
+// applyWallCushion / applyWallTangentialFriction (shaders.ts:1438-1457).
+// At dist >= cushion the clamped ratio is exactly 1 (division is monotone
+// and cushion / cushion = 1), so the factor is exact without the division.
 __device__ __forceinline__ float wall_cushion(float v, int coord, int extent) {
     float cushion = fminf(WALL_CUSHION_CELLS, (float)extent / 4.0f);
     float dist = v < 0.0f ? (float)coord : (float)(extent - coord);
+    if (dist >= cushion) return v;
     return v * fminf(fmaxf(dist / cushion, 0.0f), 1.0f);
 }
 __device__ __forceinline__ float wall_friction(float v, int coord, int extent, float friction) {
     if (friction <= 0.0f) return v;
     float dist = fmaxf(0.0f, fminf((float)coord, (float)(extent - coord)));
     float cushion = fminf(WALL_CUSHION_CELLS, (float)extent / 4.0f);
+    if (dist >= cushion) return v;
     float infl = 1.0f - fminf(fmaxf(dist / cushion, 0.0f), 1.0f);
     return v * (1.0f - friction * infl);
 }
@@ -570,18 +577,6 @@ __device__ __forceinline__ Material material_at(const Material* s_mat, const Mat
     r.mass = __uint_as_float(hi.z);
     r.phase = hi.w;
     return r;
-}
-
-// All six quadratic weights of a stencil fraction finite and non-negative
-// (the weight test of p2g, shaders.ts:1153-1157, for every support node).
-__device__ __forceinline__ bool weights_valid(float fx, float fy) {
-    bool ok = true;
-#pragma unroll
-    for (int o = 0; o < 3; ++o) {
-        const float a = qweight(fx, o), b = qweight(fy, o);
-        ok = ok && finitef(a) && a >= 0.0f && finitef(b) && b >= 0.0f;
-    }
-    return ok;
 }
 
 // Block sum of three doubles in a fixed order (wave shuffles, then the
@@ -989,7 +984,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             const float2 p = in.pos[gi[k]];
             pid[k] = in.pid[gi[k]];
             Stencil sc;
-            if (make_stencil(P.dx, P.cells_x, P.cells_y, p, sc) && ((sc.bx + 1) >> 2) == gbx &&
+            if (make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, p, sc) && ((sc.bx + 1) >> 2) == gbx &&
                 ((sc.by + 1) >> 2) == gby) {
                 const uint32_t rc = (uint32_t)((sc.by + 1 - cy0) * T::RC + (sc.bx + 1 - cx0));
                 cell[k] = rc << 16 | atomicAdd(&s_cell[rc], 1u);
@@ -1029,7 +1024,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             uint32_t q = q0;
             for (uint32_t t = q0; t < q1; ++t) q += s_pid[t] < pid[k] ? 1u : 0u;
             Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
-            make_stencil(P.dx, P.cells_x, P.cells_y, pos, sc);
+            make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, pos, sc);
             const Material mt = material_at(s_mat, mats, mat);
             const float det = det_comp(F);
             uint32_t pf = 0;
@@ -1040,9 +1035,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             }
             const float4 A = affine_of(P, mt, C, S);
             if (!pf && !finite4(A)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
-            // every stencil weight of the particle is used by some node: the
-            // per-contribution weight test of the reference, done once
-            if (!pf && !weights_valid(sc.fx, sc.fy)) pf = ERR_P2G | ERR_NONFINITE_PARTICLE;
+            // The reference tests every stencil weight for finiteness and
+            // sign (shaders.ts:1153-1157).  A stencil that make_stencil
+            // accepted has |x/dx| < 2^17, so x/dx - 0.5, its floor and the
+            // fraction are exact and the fraction lies in [0.5, 1.5): all
+            // three quadratic weights are then finite and non-negative, and
+            // the test cannot fail.
             if (pf) {
                 if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
                 flags |= pf;
@@ -1228,7 +1226,7 @@ void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restric
         for (int k = 0; k < 2; ++k) {
             if (!have[k]) continue;
             Stencil sc;
-            if (!make_stencil(P.dx, P.cells_x, P.cells_y, p[k], sc) || ((sc.bx + 1) >> 2) != gbx[k] ||
+            if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, p[k], sc) || ((sc.bx + 1) >> 2) != gbx[k] ||
                 ((sc.by + 1) >> 2) != gby[k]) {
                 flags |= ERR_STENCIL;
                 continue;
@@ -1801,7 +1799,7 @@ void k_g2p(const SceneDesc* __restrict__ scenes,
                 cfl_here = aux == NO_AUX || aux >= Ax->frac_aux0;
             }
             Stencil s;
-            if (!make_stencil(P.dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
+            if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
                 ((s.by + 1) >> 2) != c_gby) {
                 raise_step(ctl, ERR_STENCIL, step);
                 ok = false;
@@ -1812,7 +1810,7 @@ void k_g2p(const SceneDesc* __restrict__ scenes,
             }
             if (ok) {
                 Stencil ns;
-                if (!make_stencil(P.dx, P.cells_x, P.cells_y, o.pos, ns)) {
+                if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns)) {
                     raise_step(ctl, ERR_STENCIL, step);
                     ok = false;
                 } else {
@@ -2406,7 +2404,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
     for (uint32_t a = tid; a < 3u * nb; a += BODY_THREADS) {
         const uint32_t src = s_oslot[a];
         Stencil st;
-        const bool fits = make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st);
+        const bool fits = make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, out.pos[src], st);
         if (!fits || bucket_of(st, S.bw) != src / cap) atomicOr(&s_moved[a >> 5], 1u << (a & 31u));
     }
     __syncthreads();
@@ -2418,7 +2416,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_bodies(const SceneDesc* __rest
                 const uint32_t a = 32u * q + (uint32_t)__builtin_ctz(bits);
                 const uint32_t src = out_slot[a];
                 Stencil st;
-                if (!make_stencil(P.dx, P.cells_x, P.cells_y, out.pos[src], st)) {
+                if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, out.pos[src], st)) {
                     raise_step(ctl, ERR_STENCIL, step);
                     stop = true;
                     continue;
