@@ -125,6 +125,31 @@ int mpm_read_fracture_bonds(mpm_ctx* ctx, void* out24, uint32_t count);
  * p[0..3] = target x, target y, radius (0 = off), stiffness; p[4] = damping. */
 int mpm_set_interaction(mpm_ctx* ctx, const float p[8]);
 
+/* removeParticlesInOutlet (solver.ts:2280-2386; removeParticlesInOutlet,
+ * shaders.ts:3041-3065): every water and oil particle whose position lies in
+ * the closed box bounds = {minX, minY, maxX, maxY} becomes a removed
+ * tombstone (record word 13 = 0xFFFFFFFF; its slot stays allocated and
+ * mpm_read_particles still returns it).  *removed (may be NULL) receives the
+ * count.  MPM_EINVAL for non-finite bounds or min > max. */
+int mpm_remove_in_outlet(mpm_ctx* ctx, const float bounds[4], uint32_t* removed);
+
+/* pick (solver.ts:2733-2823; pickParticle, shaders.ts:3082-3118), as a
+ * parallel arg-min over the committed particles: the non-removed particle
+ * whose float32 squared distance (x - px)^2 + (y - py)^2 is at most
+ * radius^2 and smallest, the lowest index on a tie (the reference's serial
+ * scan order).  out = {index or 0xFFFFFFFF, f32 bits of its x, of its y,
+ * its object id}: the reference's 16-byte pick result. */
+int mpm_pick(mpm_ctx* ctx, float x, float y, float radius, uint32_t out[4]);
+
+/* clone (solver.ts:2852-2948; cloneObject, shaders.ts:3120-3147): copies the
+ * particles [start, start + count) after the active slots with their
+ * positions translated by the offset and clamped to the domain, and object
+ * id `object_id`; bonded triangle bodies inside the range are cloned as
+ * bodies.  MPM_ECAPACITY when the copies do not fit; MPM_ESTATE (the
+ * reference returns false) after an outlet removal or when the range holds
+ * fracture-bond endpoints; MPM_EINVAL for a bad range or offset. */
+int mpm_clone(mpm_ctx* ctx, uint32_t start, uint32_t count, float offset_x, float offset_y, uint32_t object_id);
+
 /* Number of active particle slots and the capacity. */
 int mpm_get_counts(mpm_ctx* ctx, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count);
 

@@ -102,6 +102,8 @@ struct mpm_ctx {
     std::vector<uint32_t> aux_pid;     // particle of each aux entry
     DeviceArrays aux_mem;
     bool stepped = false;              // a step has been launched
+    uint32_t removed = 0;              // particles removed by outlets (tombstones in their slots)
+    uint64_t* d_small = nullptr;       // device: pick key and position, outlet counter
 };
 
 namespace {
@@ -778,6 +780,7 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     if (keep_fixed) ALLOC(lc.grid.fixed, nodes * 11 * sizeof(int32_t));
     ALLOC(ctx->d_ctl, sizeof(Control));
     ALLOC(ctx->d_scene, sizeof(SceneDesc));
+    ALLOC(ctx->d_small, 4 * sizeof(uint64_t));
     // tile maps sized for the most tiles any shape makes
     const uint32_t max_p2g = p2g_tiles_for(lc.node_w, lc.node_h, MIN_P2G_TILE);
     const uint32_t max_g2p = g2p_tiles_for(lc.bw, lc.bh, 4u);
@@ -1019,6 +1022,112 @@ int mpm_set_interaction(mpm_ctx* c, const float p[8]) {
         return fail(c, MPM_EINVAL, "drag radius, stiffness and damping must be non-negative");
     memcpy(c->interaction, p, sizeof c->interaction);
     return MPM_OK;
+}
+
+// removeParticlesInOutlet (solver.ts:2280-2386; shaders.ts:3041-3065).
+int mpm_remove_in_outlet(mpm_ctx* c, const float bounds[4], uint32_t* removed) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!bounds) return fail(c, MPM_EINVAL, "outlet bounds must not be NULL");
+    for (int k = 0; k < 4; ++k)
+        if (!isfinite(bounds[k])) return fail(c, MPM_EINVAL, "outlet bounds must be finite");
+    if (bounds[0] > bounds[2] || bounds[1] > bounds[3])
+        return fail(c, MPM_EINVAL, "outlet minimum coordinates must not exceed the maximum.");
+    if (removed) *removed = 0;
+    if (c->n == 0) return MPM_OK;
+    int s = settle(c);
+    if (s) return s;
+    uint32_t* d_count = reinterpret_cast<uint32_t*>(c->d_small + 2);
+    HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+    HIP_TRY(c, hipMemsetAsync(d_count, 0, sizeof(uint32_t), c->stream));
+    HIP_TRY(c, launch_outlet(c->lc, make_float4(bounds[0], bounds[1], bounds[2], bounds[3]), d_count));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t count = 0;
+    HIP_TRY(c, hipMemcpy(&count, d_count, sizeof count, hipMemcpyDeviceToHost));
+    if (count) {
+        HIP_TRY(c, hipMemcpy(c->st.model.data(), c->lc.home.model, (size_t)c->n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+        c->removed += count;
+        int r = rebucket_all(c, c->lc.cap);
+        if (r) return r;
+    }
+    if (removed) *removed = count;
+    return MPM_OK;
+}
+
+// pick (solver.ts:2733-2823; pickParticle, shaders.ts:3082-3118).
+int mpm_pick(mpm_ctx* c, float x, float y, float radius, uint32_t out[4]) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!out) return fail(c, MPM_EINVAL, "pick result must not be NULL");
+    if (!isfinite(x) || !isfinite(y) || !isfinite(radius) || radius < 0.0f)
+        return fail(c, MPM_EINVAL, "pick point and radius must be finite, the radius non-negative");
+    out[0] = 0xffffffffu;
+    out[1] = out[2] = out[3] = 0u;
+    int s = settle(c);
+    if (s) return s;
+    unsigned long long* d_best = reinterpret_cast<unsigned long long*>(c->d_small);
+    float2* d_where = reinterpret_cast<float2*>(c->d_small + 1);
+    HIP_TRY(c, hipMemsetAsync(d_best, 0xff, sizeof(unsigned long long), c->stream));
+    HIP_TRY(c, launch_pick(c->lc, c->cur, x, y, radius * radius, d_best, d_where));
+    uint64_t h[2] = {0, 0};
+    HIP_TRY(c, hipMemcpyAsync(h, c->d_small, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (h[0] == ~0ull) return MPM_OK;
+    const uint32_t index = (uint32_t)h[0];
+    if (index >= c->n) return fail(c, MPM_EDEVICE, "pick returned a particle index outside the active range");
+    out[0] = index;
+    out[1] = (uint32_t)h[1];
+    out[2] = (uint32_t)(h[1] >> 32);
+    out[3] = c->st.object_id[index];
+    return MPM_OK;
+}
+
+// clone (solver.ts:2852-2948; cloneObject, shaders.ts:3120-3147).
+int mpm_clone(mpm_ctx* c, uint32_t start, uint32_t count, float offset_x, float offset_y, uint32_t object_id) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (count == 0 || start >= c->n || count > c->n - start)
+        return fail(c, MPM_EINVAL, "clone range [%u, %u + %u) must lie inside the %u particle slots", start, start,
+                    count, c->n);
+    if (!isfinite(offset_x) || !isfinite(offset_y)) return fail(c, MPM_EINVAL, "clone offset must be finite");
+    if (c->removed) return fail(c, MPM_ESTATE, "objects cannot be cloned after outlet removal");
+    for (const Bond& b : c->bonds)
+        if ((b.first >= start && b.first < start + count) || (b.second >= start && b.second < start + count))
+            return fail(c, MPM_ESTATE, "objects with fracture bonds cannot be cloned");
+    if ((uint64_t)c->n + count > c->cfg.capacity)
+        return fail(c, MPM_ECAPACITY, "cloning %u particles exceeds the capacity of %u", count, c->cfg.capacity);
+    std::vector<uint32_t> new_bodies;
+    for (uint32_t b : c->body_start) {
+        const bool first_in = b >= start && b < start + count, last_in = b + 2 >= start && b + 2 < start + count;
+        if (first_in != last_in) return fail(c, MPM_EINVAL, "the clone range must hold whole bonded triangles");
+        if (first_in) new_bodies.push_back(b - start + c->n);
+    }
+    if (c->body_start.size() + new_bodies.size() > 1024u)
+        return fail(c, MPM_ECAPACITY, "at most 1024 bonded triangle bodies are supported per scene");
+    int s = settle(c);
+    if (s) return s;
+    HIP_TRY(c, launch_unbucket(c->lc, c->cur));
+    const float dom_w = (float)c->cfg.cells_x * c->cfg.cell_size, dom_h = (float)c->cfg.cells_y * c->cfg.cell_size;
+    HIP_TRY(c, launch_clone(c->lc, start, count, c->n, offset_x, offset_y, dom_w, dom_h));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HostStatic& st = c->st;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t from = start + k, to = c->n + k;
+        st.model[to] = st.model[from];
+        st.lambda[to] = st.lambda[from];
+        st.mu[to] = st.mu[from];
+        st.density[to] = st.density[from];
+        st.volume[to] = st.volume[from];
+        st.param[to] = st.param[from];
+        st.vertex[to] = st.vertex[from];
+        st.object_id[to] = object_id;
+    }
+    c->n += count;
+    c->lc.n = c->n;
+    if (!new_bodies.empty()) {
+        c->body_start.insert(c->body_start.end(), new_bodies.begin(), new_bodies.end());
+        int b = build_aux(c);
+        if (b) return b;
+    }
+    return rebucket_all(c, c->lc.cap);
 }
 
 int mpm_get_counts(mpm_ctx* c, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count) {

@@ -307,5 +307,17 @@ hipError_t launch_clear_counts(const LaunchCtx& lc, int b);
 hipError_t launch_unbucket(const LaunchCtx& lc, int b);
 // Stencil-base keys of the home positions (debug/parity).
 hipError_t launch_keys(const LaunchCtx& lc, uint32_t* keys);
+// Interaction operations on the caller-order arrays (home) or the committed
+// slot buffer b.  outlet: water and oil particles inside the closed box
+// (min x, min y, max x, max y) become removed, *removed counts them.  pick:
+// *best receives the smallest (float32 bits of the squared distance << 32 |
+// particle index) over the particles within sqrt(r2) of (px, py), and *where
+// the position of that particle.  clone: home particles [src, src + count)
+// are copied to [dst, dst + count), translated and clamped to the domain.
+hipError_t launch_outlet(const LaunchCtx& lc, float4 box, uint32_t* removed);
+hipError_t launch_pick(const LaunchCtx& lc, int b, float px, float py, float r2, unsigned long long* best,
+                       float2* where);
+hipError_t launch_clone(const LaunchCtx& lc, uint32_t src, uint32_t count, uint32_t dst, float ox, float oy,
+                        float dom_w, float dom_h);
 
 }  // namespace mpm

@@ -2777,6 +2777,111 @@ hipError_t launch_unbucket(const LaunchCtx& lc, int b) {
     return hipSuccess;
 }
 
+// ------------------------------------------------- interaction operations
+namespace {
+
+// removeParticlesInOutlet (shaders.ts:3041-3065) on the caller-order arrays:
+// a water or oil particle whose position lies in the closed box becomes a
+// removed tombstone.  One counter add per wave.
+__global__ __launch_bounds__(256) void k_outlet(Home home, uint32_t n, float4 box, uint32_t* removed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (i < n) {
+        const uint32_t m = home.model[i];
+        if (m == MODEL_WATER || m == MODEL_OIL) {
+            const float2 p = home.pos[i];
+            hit = p.x >= box.x && p.y >= box.y && p.x <= box.z && p.y <= box.w;
+            if (hit) home.model[i] = MODEL_REMOVED;
+        }
+    }
+    const uint64_t mask = __ballot(hit);
+    if ((threadIdx.x & 63u) == 0u && mask != 0u) atomicAdd(removed, (uint32_t)__popcll(mask));
+}
+
+// pickParticle (shaders.ts:3082-3118) as a parallel arg-min over the
+// committed slots: the key (squared distance bits, particle index) orders
+// candidates exactly as the reference's serial scan (smaller distance first,
+// the lower index on a tie; non-negative float bits order like the floats).
+__global__ __launch_bounds__(256) void k_pick(SlotBuf b, uint32_t nb, uint32_t cap, float px, float py, float r2,
+                                              unsigned long long* best) {
+    unsigned long long key = ~0ull;
+    const size_t total = (size_t)nb * cap;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        if ((uint32_t)(g % cap) >= b.cnt[g / cap]) continue;
+        const float2 p = b.pos[g];
+        const float ox = p.x - px, oy = p.y - py;
+        const float d2 = ox * ox + oy * oy;
+        if (d2 <= r2) {
+            const unsigned long long k = ((unsigned long long)__float_as_uint(d2) << 32) | b.pid[g];
+            key = k < key ? k : key;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_down((uint32_t)key, off, 64), hi = __shfl_down((uint32_t)(key >> 32), off, 64);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        key = o < key ? o : key;
+    }
+    if ((threadIdx.x & 63u) == 0u && key != ~0ull) atomicMin(best, key);
+}
+
+// Position of the picked particle (the one committed slot holding its index).
+__global__ __launch_bounds__(256) void k_pick_where(SlotBuf b, uint32_t nb, uint32_t cap,
+                                                    const unsigned long long* best, float2* where) {
+    const unsigned long long key = *best;
+    if (key == ~0ull) return;
+    const uint32_t pid = (uint32_t)key;
+    const size_t total = (size_t)nb * cap;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        if ((uint32_t)(g % cap) >= b.cnt[g / cap]) continue;
+        if (b.pid[g] == pid) *where = b.pos[g];
+    }
+}
+
+// cloneObject (shaders.ts:3120-3147) on the caller-order arrays: the copy of
+// particle src + k goes to dst + k with its position translated and clamped
+// to the domain (clamp(p + offset, 0, domain)).
+__global__ __launch_bounds__(256) void k_clone(Home home, uint32_t src, uint32_t count, uint32_t dst, float ox,
+                                               float oy, float dom_w, float dom_h) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t s = src + k, d = dst + k;
+    const float2 p = home.pos[s];
+    home.pos[d] = make_float2(fminf(fmaxf(p.x + ox, 0.0f), dom_w), fminf(fmaxf(p.y + oy, 0.0f), dom_h));
+    home.vel[d] = home.vel[s];
+    home.C[d] = home.C[s];
+    home.F[d] = home.F[s];
+    home.plastic[d] = home.plastic[s];
+    home.model[d] = home.model[s];
+    home.mat[d] = home.mat[s];
+}
+
+}  // namespace
+
+hipError_t launch_outlet(const LaunchCtx& lc, float4 box, uint32_t* removed) {
+    if (lc.n == 0) return hipSuccess;
+    MPM_LAUNCH("outlet", k_outlet, dim3(blocks_for(lc.n, 256)), dim3(256), 0, lc.stream, lc.home, lc.n, box, removed);
+    return hipSuccess;
+}
+
+hipError_t launch_pick(const LaunchCtx& lc, int b, float px, float py, float r2, unsigned long long* best,
+                       float2* where) {
+    const size_t slots = (size_t)lc.nb * lc.cap;
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(slots, 256), 2048u));
+    MPM_LAUNCH("pick", k_pick, dim3(blocks), dim3(256), 0, lc.stream, lc.buf[b], lc.nb, lc.cap, px, py, r2, best);
+    MPM_LAUNCH("pick_where", k_pick_where, dim3(blocks), dim3(256), 0, lc.stream, lc.buf[b], lc.nb, lc.cap, best,
+               where);
+    return hipSuccess;
+}
+
+hipError_t launch_clone(const LaunchCtx& lc, uint32_t src, uint32_t count, uint32_t dst, float ox, float oy,
+                        float dom_w, float dom_h) {
+    if (count == 0) return hipSuccess;
+    MPM_LAUNCH("clone", k_clone, dim3(blocks_for(count, 256)), dim3(256), 0, lc.stream, lc.home, src, count, dst, ox,
+               oy, dom_w, dom_h);
+    return hipSuccess;
+}
+
 hipError_t launch_keys(const LaunchCtx& lc, uint32_t* keys) {
     if (lc.n == 0) return hipSuccess;
     MPM_LAUNCH("keys", k_keys, dim3(blocks_for(lc.n, 256)), dim3(256), 0, lc.stream, lc.home, keys, lc.n,

@@ -538,13 +538,15 @@ def pack_step_parameters(state: ValidatedInput, dt: float, gravity: dict, partic
 
 
 def unpack_particles(records: np.ndarray, object_ids: np.ndarray | None = None) -> list[dict]:
-    """Decode 20-word records back to MpmParticle-shaped dicts (readParticles,
-    solver.ts:2594-2731)."""
+    """Decode 20-word records back to MpmParticle-shaped dicts (unpackParticles,
+    solver.ts:3148-3226): removed tombstones are skipped."""
     rec = np.asarray(records, np.uint32).reshape(-1, 20)
     fv = rec.view(np.float32).astype(np.float64)
     out = []
     for i in range(rec.shape[0]):
         model = int(rec[i, 13])
+        if model == MODEL_REMOVED:
+            continue
         name = MODEL_NAMES.get(model, "removed")
         mat = {"lambda": fv[i, 14], "mu": fv[i, 15], "model": name}
         p = {

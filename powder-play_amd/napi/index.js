@@ -46,6 +46,21 @@ class MpmNativeRuntime {
 		addon.setInteraction(this.ctx, uniform);
 	}
 
+	// removeParticlesInOutlet (solver.ts:2280-2386): Float32Array [minX, minY, maxX, maxY] -> count
+	removeParticlesInOutlet(bounds) {
+		return addon.removeParticlesInOutlet(this.ctx, bounds);
+	}
+
+	// pick (solver.ts:2733-2823): Uint32Array [index | 0xFFFFFFFF, x bits, y bits, objectId]
+	pick(x, y, radius) {
+		return addon.pick(this.ctx, x, y, radius);
+	}
+
+	// clone (solver.ts:2852-2948): false where the reference refuses
+	clone(start, count, offsetX, offsetY, objectId) {
+		return addon.clone(this.ctx, start, count, offsetX, offsetY, objectId);
+	}
+
 	setDiagnostics(enable) {
 		addon.setDiagnostics(this.ctx, Boolean(enable));
 	}

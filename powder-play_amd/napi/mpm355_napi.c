@@ -414,6 +414,70 @@ static napi_value set_interaction(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+/* removeParticlesInOutlet(ctx, Float32Array(4) {minX, minY, maxX, maxY}) -> removed count */
+static napi_value remove_in_outlet(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2], out;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
+    void* p;
+    size_t n;
+    if (!typed_arg(env, argv[1], napi_float32_array, &p, &n, 0)) return NULL;
+    if (n != 4) {
+        napi_throw_range_error(env, "MPM_EINVAL", "outlet bounds must be 4 floats");
+        return NULL;
+    }
+    uint32_t removed = 0;
+    LOCKED_CALL(box, mpm_remove_in_outlet(box->ctx, (const float*)p, &removed));
+    CHECK(napi_create_uint32(env, removed, &out));
+    return out;
+}
+
+/* pick(ctx, x, y, radius) -> Uint32Array(4) {index | 0xFFFFFFFF, x bits, y bits, objectId} */
+static napi_value pick(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value argv[4], ab, arr;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    ctx_box* box = argc >= 4 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
+    double v[3];
+    for (int k = 0; k < 3; ++k)
+        if (napi_get_value_double(env, argv[k + 1], &v[k]) != napi_ok) {
+            napi_throw_type_error(env, NULL, "pick point and radius must be numbers");
+            return NULL;
+        }
+    void* data;
+    CHECK(napi_create_arraybuffer(env, 16, &data, &ab));
+    LOCKED_CALL(box, mpm_pick(box->ctx, (float)v[0], (float)v[1], (float)v[2], (uint32_t*)data));
+    CHECK(napi_create_typedarray(env, napi_uint32_array, 4, ab, 0, &arr));
+    return arr;
+}
+
+/* clone(ctx, start, count, offsetX, offsetY, objectId) -> boolean (false when
+ * the reference refuses: capacity, outlet removal, fracture bonds) */
+static napi_value clone_object(napi_env env, napi_callback_info info) {
+    size_t argc = 6;
+    napi_value argv[6], out;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    ctx_box* box = argc >= 6 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
+    uint32_t start = 0, count = 0, oid = 0;
+    double ox = 0, oy = 0;
+    if (napi_get_value_uint32(env, argv[1], &start) != napi_ok || napi_get_value_uint32(env, argv[2], &count) != napi_ok ||
+        napi_get_value_double(env, argv[3], &ox) != napi_ok || napi_get_value_double(env, argv[4], &oy) != napi_ok ||
+        napi_get_value_uint32(env, argv[5], &oid) != napi_ok) {
+        napi_throw_type_error(env, NULL, "clone(ctx, start, count, offsetX, offsetY, objectId)");
+        return NULL;
+    }
+    pthread_mutex_lock(&box->lock);
+    const int code = mpm_clone(box->ctx, start, count, (float)ox, (float)oy, oid);
+    if (code != MPM_OK && code != MPM_ECAPACITY && code != MPM_ESTATE) return fail_unlock(env, code, box);
+    pthread_mutex_unlock(&box->lock);
+    CHECK(napi_get_boolean(env, code == MPM_OK, &out));
+    return out;
+}
+
 /* setDiagnostics(ctx, enable) / readDiagnostics(ctx) -> Float64Array(9) */
 static napi_value set_diagnostics(napi_env env, napi_callback_info info) {
     size_t argc = 2;
@@ -529,6 +593,9 @@ static napi_value init(napi_env env, napi_value exports) {
         {"readParticles", 0, read_particles, 0, 0, 0, napi_enumerable, 0},
         {"appendParticles", 0, append_particles, 0, 0, 0, napi_enumerable, 0},
         {"setInteraction", 0, set_interaction, 0, 0, 0, napi_enumerable, 0},
+        {"removeParticlesInOutlet", 0, remove_in_outlet, 0, 0, 0, napi_enumerable, 0},
+        {"pick", 0, pick, 0, 0, 0, napi_enumerable, 0},
+        {"clone", 0, clone_object, 0, 0, 0, napi_enumerable, 0},
         {"setDiagnostics", 0, set_diagnostics, 0, 0, 0, napi_enumerable, 0},
         {"readDiagnostics", 0, read_diagnostics, 0, 0, 0, napi_enumerable, 0},
         {"setFractureBonds", 0, set_fracture_bonds, 0, 0, 0, napi_enumerable, 0},

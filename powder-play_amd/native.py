@@ -33,6 +33,9 @@ SIGNATURES = {
     "mpm_set_fracture_bonds": (ctypes.c_int, [_P, _P, _U32]),
     "mpm_read_fracture_bonds": (ctypes.c_int, [_P, _P, _U32]),
     "mpm_get_counts": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mpm_remove_in_outlet": (ctypes.c_int, [_P, _P, _P]),
+    "mpm_pick": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+    "mpm_clone": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _U32]),
     "mpm_read_fixed_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_read_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_read_bins": (ctypes.c_int, [_P, _P, _U32, _P, ctypes.c_size_t]),
@@ -210,6 +213,21 @@ class Context:
         arr = np.zeros(8, np.float32)
         arr[: len(values)] = values
         self._check(self.lib.mpm_set_interaction(self.handle, _ptr(arr)))
+
+    def remove_in_outlet(self, bounds) -> int:
+        b = np.ascontiguousarray(bounds, np.float32).reshape(4)
+        out = np.zeros(1, np.uint32)
+        self._check(self.lib.mpm_remove_in_outlet(self.handle, _ptr(b), _ptr(out)))
+        return int(out[0])
+
+    def pick(self, x: float, y: float, radius: float) -> np.ndarray:
+        """The 16-byte pick result: index (0xFFFFFFFF = none), x bits, y bits, object id."""
+        out = np.zeros(4, np.uint32)
+        self._check(self.lib.mpm_pick(self.handle, x, y, radius, _ptr(out)))
+        return out
+
+    def clone(self, start: int, count: int, offset_x: float, offset_y: float, object_id: int):
+        self._check(self.lib.mpm_clone(self.handle, start, count, offset_x, offset_y, object_id))
 
     def read_fixed_grid(self) -> np.ndarray:
         nodes = self.counts()[2]

@@ -21,7 +21,8 @@ import numpy as np
 from . import cfl as cflmod
 from . import layout
 from .layout import RangeError
-from .native import MPM_FLAG_KEEP_FIXED_GRID, MPM_P2G_FIXED_POINT, MPM_P2G_GATHER, Context, MpmConfig
+from .native import (MPM_ECAPACITY, MPM_ESTATE, MPM_FLAG_KEEP_FIXED_GRID, MPM_P2G_FIXED_POINT, MPM_P2G_GATHER,
+                     Context, MpmConfig, NativeError)
 
 ERROR_DEFORMATION = 2
 ERROR_DEFORMATION_SUBSTEP_LIMIT = 32
@@ -114,7 +115,56 @@ class CflBatchMeasurement:
     remainingDuration: float
 
 
-MPM_MLS_GPU_DRAG_DAMPING = 12.0
+MPM_MLS_GPU_DRAG_STIFFNESS = 20.0   # solver.ts:379-380
+MPM_MLS_GPU_DRAG_DAMPING = 8.0
+MAX_U32_OBJECT_ID = 0xFFFFFFFF
+
+
+def _finite_interaction_float32(value, name: str) -> float:
+    """finiteInteractionFloat32 (solver.ts:3271-3280)."""
+    if isinstance(value, bool) or not isinstance(value, (int, float, np.floating, np.integer)) or \
+            not math.isfinite(float(value)):
+        raise TypeError(f"{name} must be a finite number.")
+    with np.errstate(over="ignore"):
+        result = float(np.float32(value))
+    if not math.isfinite(result):
+        raise RangeError(f"{name} is outside the supported float32 range.")
+    return result
+
+
+def validate_world_point(point, state, name: str) -> dict:
+    """validateMpmWorldPoint (solver.ts:3228-3242): a float32 point inside the domain."""
+    if not isinstance(point, dict):
+        raise TypeError(f"{name} must be a vector.")
+    x = _finite_interaction_float32(point.get("x"), f"{name}.x")
+    y = _finite_interaction_float32(point.get("y"), f"{name}.y")
+    if x < 0 or x > state.domain_width_f32 or y < 0 or y > state.domain_height_f32:
+        raise RangeError(f"{name} must be inside the MLS-MPM domain.")
+    return {"x": x, "y": y}
+
+
+def validate_pick_radius(radius, state) -> float:
+    """validateMpmPickRadius (solver.ts:3244-3250)."""
+    value = _finite_interaction_float32(radius, "pick radius")
+    if value < 0 or value > max(state.domain_width_f32, state.domain_height_f32):
+        raise RangeError("pick radius must be non-negative and within the MLS-MPM domain extent.")
+    return value
+
+
+def validate_drag_radius(radius, state) -> float:
+    """validateMpmDragRadius (solver.ts:3252-3256)."""
+    value = validate_pick_radius(radius, state)
+    if value <= 0:
+        raise RangeError("drag radius must be greater than zero.")
+    return value
+
+
+def validate_clone_offset(offset) -> dict:
+    """validateMpmCloneOffset (solver.ts:3258-3266)."""
+    if not isinstance(offset, dict):
+        raise TypeError("clone offset must be a vector.")
+    return {"x": _finite_interaction_float32(offset.get("x"), "clone offset.x"),
+            "y": _finite_interaction_float32(offset.get("y"), "clone offset.y")}
 
 
 class MpmMlsGpuSolver:
@@ -148,6 +198,11 @@ class MpmMlsGpuSolver:
                     raise RangeError('tuning must be None, a dict or "auto"')
                 tuning = autotune.tuning_for(particles, config, p2g_mode, device=max(device, 0))
             autotune.apply(self, tuning)
+        self._init_host_state()
+
+    def _init_host_state(self):
+        """Host bookkeeping of createGpuSolver (solver.ts:1242-1270): active
+        count, object ranges and the next object id, removal and drag state."""
         self._disposed = False
         self._cfl_state_usable = True
         self._speed_measurement_submitted = False
@@ -155,6 +210,23 @@ class MpmMlsGpuSolver:
         self._latest_sequence = 0
         self._latest_microsteps = 0
         self._drag = None
+        packed = self.state.packed
+        self._active_count = int(np.count_nonzero(packed[:, 13] != layout.MODEL_REMOVED)) if len(packed) else 0
+        self._has_removed = self._active_count != len(packed)
+        self._object_ranges = {r["objectId"]: dict(r) for r in self.state.object_ranges}
+        self._next_object_id = max((r["objectId"] + 1 for r in self.state.object_ranges), default=0)
+        oid = self.state.object_ids
+        self._body_count = self.state.bonded_body_count
+        self._object_bodies = {}
+        first_vertex = packed[:, 19] == 1 if len(packed) else np.zeros(0, bool)
+        for r in self.state.object_ranges:
+            k = int(np.count_nonzero(first_vertex[r["start"]:r["start"] + r["count"]]))
+            if k:
+                self._object_bodies[r["objectId"]] = k
+        self._bonded_objects = set()
+        if self.state.fracture_bonds is not None:
+            for b in self.state.fracture_bonds:
+                self._bonded_objects.add(int(oid[int(b[0])]))
 
     @classmethod
     def _over_context(cls, state, cfl_bounds, ctx, p2g_mode):
@@ -164,19 +236,14 @@ class MpmMlsGpuSolver:
         self.cfl_bounds = cfl_bounds
         self.ctx = ctx
         self.p2g_mode = p2g_mode
-        self._disposed = False
-        self._cfl_state_usable = True
-        self._speed_measurement_submitted = False
-        self._last_scheduled_velocity_bound = None
-        self._latest_sequence = 0
-        self._latest_microsteps = 0
-        self._drag = None
+        self._init_host_state()
         return self
 
     # ----------------------------------------------------------- properties
     @property
     def particle_count(self) -> int:
-        return self.ctx.counts()[0]
+        """Active (non-removed) particles (activeParticleCount, solver.ts:1242)."""
+        return self._active_count
 
     @property
     def allocated_particle_count(self) -> int:
@@ -219,8 +286,8 @@ class MpmMlsGpuSolver:
             if not math.isfinite(cfl_velocity_bound) or cfl_velocity_bound < 0:
                 raise RangeError("velocityBound must be finite and non-negative.")
             guard = {"waveSpeedBound": self.cfl_bounds.waveSpeedBound, "velocityBound": cfl_velocity_bound}
-        params = layout.pack_step_parameters(self.state, dt, gravity, self.allocated_particle_count, None, guard,
-                                             bond_reference_dt)
+        params = layout.pack_step_parameters(self.state, dt, gravity, self.allocated_particle_count,
+                                             self._body_count, guard, bond_reference_dt)
         try:
             self.ctx.step_batch(params, step_count, measure_speed or guard is not None)
         except Exception as cause:
@@ -463,11 +530,25 @@ class MpmMlsGpuSolver:
             return {"requested": 0, "accepted": 0, "refused": 0, "dropped": 0}
         cfg = dict(self.state.config)
         cfg["capacity"] = max(requested, 1)
+        cfg.pop("fractureBonds", None)
+        if self._next_object_id > MAX_U32_OBJECT_ID:
+            return {"requested": requested, "accepted": 0, "refused": requested, "dropped": 0,
+                    "reason": "no object ids remain for a new emitter group"}
+        # every emitted group is one new object (solver.ts:2154-2165)
+        object_id = self._next_object_id
+        if isinstance(particles, layout.ParticleArrays):
+            particles = replace(particles, object_id=np.full(len(particles), object_id, np.uint32))
+        else:
+            particles = [dict(p, objectId=object_id) for p in particles]
         add = layout.validate_input(particles, cfg)
         free = self.particle_capacity - self.allocated_particle_count
         accepted = min(free, requested)
         if accepted > 0:
+            start = self.allocated_particle_count
             self.ctx.append(add.packed[:accepted], add.object_ids[:accepted])
+            self._object_ranges[object_id] = {"objectId": object_id, "start": start, "count": accepted}
+            self._active_count += accepted
+            self._next_object_id += 1
             if self.cfl_bounds is not None:
                 extra = cflmod.derive_cfl_bounds_packed(add.packed[:accepted], self.state.particle_mass_f32)
                 if extra is None:
@@ -481,21 +562,128 @@ class MpmMlsGpuSolver:
                 **({} if accepted == requested else {"reason": "capacity"})}
 
     # ------------------------------------------------------------ interaction
-    def begin_drag(self, target: dict, radius: float, stiffness: float = 40.0):
-        if not math.isfinite(radius) or radius <= 0:
-            raise RangeError("drag radius must be positive and finite.")
-        self._drag = (target["x"], target["y"], radius, stiffness)
-        self.ctx.set_interaction([target["x"], target["y"], radius, stiffness, MPM_MLS_GPU_DRAG_DAMPING])
+    def _resolve_selection(self, selection) -> dict:
+        """resolveSelection (solver.ts:1363-1387): the object range of a
+        selection {particleIndex, objectId, anchor}."""
+        if not isinstance(selection, dict):
+            raise TypeError("selection must identify an MPM particle.")
+        idx, oid = selection.get("particleIndex"), selection.get("objectId")
+        if not isinstance(idx, (int, np.integer)) or isinstance(idx, bool) or idx < 0 or \
+                idx >= self.allocated_particle_count or not isinstance(oid, (int, np.integer)) or \
+                isinstance(oid, bool) or oid < 0 or oid > MAX_U32_OBJECT_ID:
+            raise RangeError("selection must identify an active MPM particle.")
+        validate_world_point(selection.get("anchor"), self.state, "selection anchor")
+        r = self._object_ranges.get(int(oid))
+        if r is None or idx < r["start"] or idx >= r["start"] + r["count"]:
+            raise RangeError("selection objectId does not match the selected particle.")
+        return r
+
+    def _write_interaction(self, target: dict, radius: float):
+        """writeInteractionParameters (solver.ts:1389-1401)."""
+        self.ctx.set_interaction([target["x"], target["y"], radius, MPM_MLS_GPU_DRAG_STIFFNESS,
+                                  MPM_MLS_GPU_DRAG_DAMPING, 0.0, 0.0, 0.0])
+
+    def begin_drag(self, selection: dict, target: dict, radius: float):
+        """beginDrag (solver.ts:2825-2836): a spring-damper on the grid
+        nodes within `radius` of the target (finishGridVelocity)."""
+        self._assert_active()
+        self._resolve_selection(selection)
+        t = validate_world_point(target, self.state, "drag target")
+        r = validate_drag_radius(radius, self.state)
+        self._write_interaction(t, r)
+        self._drag = r
 
     def update_drag(self, target: dict):
+        """updateDrag (solver.ts:2838-2843)."""
+        self._assert_active()
         if self._drag is None:
             return
-        _, _, r, k = self._drag
-        self.begin_drag(target, r, k)
+        self._write_interaction(validate_world_point(target, self.state, "drag target"), self._drag)
 
     def end_drag(self):
+        """endDrag (solver.ts:2845-2850)."""
+        if self._drag is None:
+            return
         self._drag = None
-        self.ctx.set_interaction([0.0] * 8)
+        if self._disposed:
+            return
+        self._write_interaction({"x": 0.0, "y": 0.0}, 0.0)
+
+    def pick(self, point: dict, radius: float):
+        """pick (solver.ts:2733-2823): the active particle nearest to the
+        point within the radius, as a selection {particleIndex, objectId,
+        anchor}, or None."""
+        self._assert_active()
+        q = validate_world_point(point, self.state, "pick point")
+        r = validate_pick_radius(radius, self.state)
+        if self._active_count == 0:
+            return None
+        out = self.ctx.pick(q["x"], q["y"], r)
+        index = int(out[0])
+        if index == MAX_U32:
+            return None
+        anchor = {"x": float(out[1:2].view(np.float32)[0]), "y": float(out[2:3].view(np.float32)[0])}
+        oid = int(out[3])
+        rng = self._object_ranges.get(oid)
+        if rng is None or index < rng["start"] or index >= rng["start"] + rng["count"]:
+            raise RuntimeError("GPU returned an object id inconsistent with the selected particle.")
+        return {"particleIndex": index, "objectId": oid,
+                "anchor": validate_world_point(anchor, self.state, "GPU selection anchor")}
+
+    def remove_particles_in_outlet(self, bounds: dict) -> int:
+        """removeParticlesInOutlet (solver.ts:2280-2386): water and oil
+        particles inside the box become removed; returns how many."""
+        self._assert_active()
+        if not isinstance(bounds, dict):
+            raise TypeError("outlet bounds must be an object.")
+        lo = validate_world_point({"x": bounds.get("minX"), "y": bounds.get("minY")}, self.state, "outlet minimum")
+        hi = validate_world_point({"x": bounds.get("maxX"), "y": bounds.get("maxY")}, self.state, "outlet maximum")
+        if lo["x"] > hi["x"] or lo["y"] > hi["y"]:
+            raise RangeError("outlet minimum coordinates must not exceed the maximum.")
+        if self.allocated_particle_count == 0 or self._active_count == 0:
+            return 0
+        try:
+            removed = self.ctx.remove_in_outlet([lo["x"], lo["y"], hi["x"], hi["y"]])
+        except Exception as cause:
+            self._cfl_state_usable = False
+            raise RuntimeError("Failed to remove liquid particles from the GPU outlet region.") from cause
+        if removed > self._active_count:
+            raise RuntimeError("GPU reported an outlet removal count above the active particle count.")
+        self._active_count -= removed
+        self._has_removed = self._has_removed or removed > 0
+        return removed
+
+    def clone(self, selection: dict, offset: dict) -> bool:
+        """clone (solver.ts:2852-2948): copy the selected object, translated
+        by the offset and clamped to the domain, as a new object after the
+        active slots.  False when the reference refuses (after an outlet
+        removal, for an object with fracture bonds, or without capacity)."""
+        self._assert_active()
+        src = self._resolve_selection(selection)
+        t = validate_clone_offset(offset)
+        if self._has_removed or src["objectId"] in self._bonded_objects:
+            return False
+        if src["count"] > self.particle_capacity - self.allocated_particle_count:
+            return False
+        if self._next_object_id > MAX_U32_OBJECT_ID:
+            raise RangeError("No unsigned 32-bit object ids remain for MLS-MPM cloning.")
+        self.end_drag()
+        start = self.allocated_particle_count
+        object_id = self._next_object_id
+        try:
+            self.ctx.clone(src["start"], src["count"], t["x"], t["y"], object_id)
+        except NativeError as err:
+            if err.code in (MPM_ECAPACITY, MPM_ESTATE):
+                return False
+            raise RuntimeError("Failed to clone the selected MLS-MPM object on the GPU.") from err
+        self._object_ranges[object_id] = {"objectId": object_id, "start": start, "count": src["count"]}
+        bodies = self._object_bodies.get(src["objectId"], 0)
+        if bodies:
+            self._object_bodies[object_id] = bodies
+            self._body_count += bodies
+        self._active_count += src["count"]
+        self._next_object_id += 1
+        return True
 
     def read_fracture_bonds(self) -> list[dict]:
         """The committed state of the fracture bonds (damage, broken), as the
