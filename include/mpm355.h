@@ -150,6 +150,15 @@ int mpm_pick(mpm_ctx* ctx, float x, float y, float radius, uint32_t out[4]);
  * fracture-bond endpoints; MPM_EINVAL for a bad range or offset. */
 int mpm_clone(mpm_ctx* ctx, uint32_t start, uint32_t count, float offset_x, float offset_y, uint32_t object_id);
 
+/* gatherSurfaceOccupancy (shaders.ts:710-806; CPU form
+ * reference/occupancy.ts:70-140): the material-separated surface field of
+ * the committed state, out[2 * node] = (water, oil) = the display area
+ * 2 V det F of the liquid particles, B-spline weighted onto the node and
+ * divided by the cell area (node_count * 2 floats, gridIndex order).  The
+ * sums follow the P2G gather's fixed order (the reference's follow its
+ * linked lists).  MPM_ESTATE while the numerical status is set. */
+int mpm_gather_occupancy(mpm_ctx* ctx, float* out, size_t floats);
+
 /* Number of active particle slots and the capacity. */
 int mpm_get_counts(mpm_ctx* ctx, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count);
 

@@ -104,6 +104,7 @@ struct mpm_ctx {
     bool stepped = false;              // a step has been launched
     uint32_t removed = 0;              // particles removed by outlets (tombstones in their slots)
     uint64_t* d_small = nullptr;       // device: pick key and position, outlet counter
+    float2* d_occ = nullptr;           // device: surface occupancy per node (first mpm_gather_occupancy)
 };
 
 namespace {
@@ -1128,6 +1129,46 @@ int mpm_clone(mpm_ctx* c, uint32_t start, uint32_t count, float offset_x, float 
         if (b) return b;
     }
     return rebucket_all(c, c->lc.cap);
+}
+
+// gatherSurfaceOccupancy (shaders.ts:710-806) of the committed state.
+int mpm_gather_occupancy(mpm_ctx* c, float* out, size_t floats) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!out || floats != (size_t)c->lc.node_count * 2)
+        return fail(c, MPM_EINVAL, "output must hold node_count*2 floats");
+    int s = settle(c);
+    if (s) return s;
+    if (c->h_ctl->status[0] != 0u)
+        return fail(c, MPM_ESTATE, "the numerical status is set: clear it before gathering the surface occupancy");
+    if (!c->d_occ) {
+        hipError_t e = c->mem.alloc((void**)&c->d_occ, (size_t)c->lc.node_count * sizeof(float2));
+        if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(occupancy)");
+    }
+    // the grid of this context as a parameter block (no step: dt 0, no gravity)
+    Params p{};
+    p.cells_x = c->cfg.cells_x;
+    p.cells_y = c->cfg.cells_y;
+    p.node_w = c->lc.node_w;
+    p.node_h = c->lc.node_h;
+    p.particle_count = c->n;
+    p.node_count = c->lc.node_count;
+    p.dx = c->cfg.cell_size;
+    p.inv_dx = exact_reciprocal(p.dx);
+    p.particle_mass = c->cfg.particle_mass;
+    p.particle_volume = c->cfg.particle_volume;
+    p.grad_scale = 4.0f / (p.dx * p.dx);
+    p.dom_w = (float)c->cfg.cells_x * c->cfg.cell_size;
+    p.dom_h = (float)c->cfg.cells_y * c->cfg.cell_size;
+    const Shape shape = {28u, 8u, 256u};
+    SceneDesc d = scene_desc(c, p, 0u, 0u, c->seq, shape);
+    d.grid.occupancy = c->d_occ;
+    d.diag_grid = nullptr;
+    HIP_TRY(c, hipMemcpy(c->d_scene, &d, sizeof d, hipMemcpyHostToDevice));
+    HIP_TRY(c, launch_occupancy(c->stream, c->d_scene, p2g_tiles_for(c->lc.node_w, c->lc.node_h, 28u),
+                                scene_general(c->lc.scene_flags), c->times));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->d_occ, floats * sizeof(float), hipMemcpyDeviceToHost));
+    return MPM_OK;
 }
 
 int mpm_get_counts(mpm_ctx* c, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count) {

@@ -148,6 +148,7 @@ struct Grid {
     dptr<float2> v_oil;
     dptr<float2> liquid;     // (water+oil mass, water+oil volume)
     dptr<int32_t> fixed;     // node_count*11 or nullptr
+    dptr<float2> occupancy;  // (water, oil) surface occupancy per node, written by the occupancy gather only
 };
 
 // Scene content flags, host-derived from the particle models.  SCENE_AUX:
@@ -315,6 +316,11 @@ hipError_t launch_keys(const LaunchCtx& lc, uint32_t* keys);
 // the position of that particle.  clone: home particles [src, src + count)
 // are copied to [dst, dst + count), translated and clamped to the domain.
 hipError_t launch_outlet(const LaunchCtx& lc, float4 box, uint32_t* removed);
+// Surface occupancy of the committed state (gatherSurfaceOccupancy,
+// shaders.ts:710-806) by the float gather over 28-node tiles: d_scene holds
+// one descriptor whose Params carry the grid, grid.occupancy the output.
+hipError_t launch_occupancy(hipStream_t stream, const SceneDesc* d_scene, uint32_t tiles, bool general,
+                            KernelTimes* timing);
 hipError_t launch_pick(const LaunchCtx& lc, int b, float px, float py, float r2, unsigned long long* best,
                        float2* where);
 hipError_t launch_clone(const LaunchCtx& lc, uint32_t src, uint32_t count, uint32_t dst, float ox, float oy,

@@ -751,7 +751,7 @@ __device__ __forceinline__ uint32_t fluid_triangle_node(const Params& P, const A
 // clearing the next bucket counts this tile owns, and the staged offsets of
 // the 9x9 region buckets (s_boff, one wave, two buckets a lane).  Returns
 // false when a flag is already raised (the whole block returns).
-template <class T>
+template <class T, bool OCC = false>
 __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32_t* cnt_next, uint32_t tile,
                                          uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap, uint32_t step,
                                          int in_buf, uint32_t* s_boff) {
@@ -760,7 +760,8 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
     // After a failed microstep the output buffer of this one holds the
     // committed state: leave its counts and deferred flags untouched.
     if (block_status_set(ctl)) return false;
-    if (tile == 0u && tid == 0) {
+    // the occupancy gather only reads the committed state
+    if (!OCC && tile == 0u && tid == 0) {
         // stress-cache failures of the state this microstep starts from (the
         // scene's first tile; in a batch every scene has one)
         const uint32_t d = ctl->deferred[in_buf];
@@ -773,7 +774,7 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
     }
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     // this tile owns buckets [OWN tx, OWN tx + OWN) x [OWN ty, OWN ty + OWN): clear their next counts
-    if (tid < T::OWN * T::OWN) {
+    if (!OCC && tid < T::OWN * T::OWN) {
         const int obx = T::OWN * tx + tid % T::OWN, oby = T::OWN * ty + tid / T::OWN;
         if (obx < (int)bw && oby < (int)bh) cnt_next[(uint32_t)oby * bw + (uint32_t)obx] = 0u;
     }
@@ -888,7 +889,14 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 // CAP is split into single buckets, and then a node whose stencil row spans
 // two buckets of different chunks adds them bucket by bucket.  The sums carry
 // across chunks.
-template <int TI, bool GENERAL, bool AUX = false>
+//
+// OCC = true is the surface occupancy gather (gatherSurfaceOccupancy,
+// shaders.ts:710-806) of the committed state: the same staging and gather,
+// no grid update, no bucket bookkeeping and no flags; every node receives
+// (water, oil) = 2 (sum w V det F) / dx^2, the display area of the liquid
+// phases over the cell area.  The reference scales each contribution,
+// ((V det F) 2) w / dx^2; here the sum is scaled once.
+template <int TI, bool GENERAL, bool AUX = false, bool OCC = false>
 __global__ __launch_bounds__(P2GTile<TI>::THREADS)
 __attribute__((amdgpu_waves_per_eu(P2GTile<TI>::GATHER_WAVES_PER_EU, 8)))
 void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
@@ -921,7 +929,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     __shared__ uint32_t s_pid[G_CAP];  // particle indices at provisional places
 
     stage_materials(s_mat, mats, nmat);
-    if (!p2g_head<T>(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
+    if (!p2g_head<T, OCC>(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
     const int tid = threadIdx.x;
     const Params& P = *Pp;
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
@@ -931,9 +939,14 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
     const bool node_ok = lx < T::TC && ly < T::TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
+    const bool interior = node_ok && lx >= 1 && lx <= TI && ly >= 1 && ly <= TI;
     __syncthreads();
     if (s_boff[T::NB] == 0u) {
-        p2g_empty_tile<TI>(P, grid, 0u, diag_grid, tile, X0, Y0);
+        if constexpr (OCC) {
+            if (interior) grid.occupancy[(uint32_t)sy * (uint32_t)W + (uint32_t)sx] = make_float2(0.f, 0.f);
+        } else {
+            p2g_empty_tile<TI>(P, grid, 0u, diag_grid, tile, X0, Y0);
+        }
         return;
     }
     if (tid == 0) {
@@ -1041,7 +1054,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             // fraction are exact and the fraction lies in [0.5, 1.5): all
             // three quadratic weights are then finite and non-negative, and
             // the test cannot fail.
-            if (pf) {
+            if (pf && !OCC) {
                 if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
                 flags |= pf;
             }
@@ -1107,6 +1120,14 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
         __syncthreads();
     }
 
+    if constexpr (OCC) {
+        if (interior) {
+            const float cell_area = P.dx * P.dx;
+            grid.occupancy[(uint32_t)sy * (uint32_t)W + (uint32_t)sx] =
+                make_float2((acc[6] * 2.0f) / cell_area, (acc[10] * 2.0f) / cell_area);
+        }
+        return;
+    }
     // ring volumes for the gradient (the staging arrays are free now)
     float* s_vw = reinterpret_cast<float*>(s_r0);
     float* s_vo = s_vw + G_CAP;
@@ -2857,6 +2878,17 @@ __global__ __launch_bounds__(256) void k_clone(Home home, uint32_t src, uint32_t
 }
 
 }  // namespace
+
+hipError_t launch_occupancy(hipStream_t stream, const SceneDesc* d_scene, uint32_t tiles, bool general,
+                            KernelTimes* timing) {
+    struct {
+        hipStream_t stream;
+        KernelTimes* timing;
+    } lc = {stream, timing};
+    auto fn = general ? k_p2g_gather<28, true, false, true> : k_p2g_gather<28, false, false, true>;
+    MPM_LAUNCH("occupancy", fn, dim3(tiles), dim3(P2GTile<28>::THREADS), 0, lc.stream, d_scene, nullptr, 1u, 0u);
+    return hipSuccess;
+}
 
 hipError_t launch_outlet(const LaunchCtx& lc, float4 box, uint32_t* removed) {
     if (lc.n == 0) return hipSuccess;

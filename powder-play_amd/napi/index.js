@@ -51,6 +51,11 @@ class MpmNativeRuntime {
 		return addon.removeParticlesInOutlet(this.ctx, bounds);
 	}
 
+	// gatherSurfaceOccupancy (shaders.ts:710-806): Float32Array (water, oil) per node
+	gatherOccupancy() {
+		return addon.gatherOccupancy(this.ctx);
+	}
+
 	// pick (solver.ts:2733-2823): Uint32Array [index | 0xFFFFFFFF, x bits, y bits, objectId]
 	pick(x, y, radius) {
 		return addon.pick(this.ctx, x, y, radius);

@@ -434,6 +434,22 @@ static napi_value remove_in_outlet(napi_env env, napi_callback_info info) {
     return out;
 }
 
+/* gatherOccupancy(ctx) -> Float32Array(2 * nodeCount): (water, oil) per node */
+static napi_value gather_occupancy(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], ab, arr;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    ctx_box* box = argc >= 1 ? ctx_arg(env, argv[0]) : NULL;
+    if (!box) return NULL;
+    uint32_t n = 0, cap = 0, nodes = 0;
+    LOCKED_CALL(box, mpm_get_counts(box->ctx, &n, &cap, &nodes));
+    void* data;
+    CHECK(napi_create_arraybuffer(env, (size_t)nodes * 8, &data, &ab));
+    LOCKED_CALL(box, mpm_gather_occupancy(box->ctx, (float*)data, (size_t)nodes * 2));
+    CHECK(napi_create_typedarray(env, napi_float32_array, (size_t)nodes * 2, ab, 0, &arr));
+    return arr;
+}
+
 /* pick(ctx, x, y, radius) -> Uint32Array(4) {index | 0xFFFFFFFF, x bits, y bits, objectId} */
 static napi_value pick(napi_env env, napi_callback_info info) {
     size_t argc = 4;
@@ -595,6 +611,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"setInteraction", 0, set_interaction, 0, 0, 0, napi_enumerable, 0},
         {"removeParticlesInOutlet", 0, remove_in_outlet, 0, 0, 0, napi_enumerable, 0},
         {"pick", 0, pick, 0, 0, 0, napi_enumerable, 0},
+        {"gatherOccupancy", 0, gather_occupancy, 0, 0, 0, napi_enumerable, 0},
         {"clone", 0, clone_object, 0, 0, 0, napi_enumerable, 0},
         {"setDiagnostics", 0, set_diagnostics, 0, 0, 0, napi_enumerable, 0},
         {"readDiagnostics", 0, read_diagnostics, 0, 0, 0, napi_enumerable, 0},

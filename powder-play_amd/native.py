@@ -34,6 +34,7 @@ SIGNATURES = {
     "mpm_read_fracture_bonds": (ctypes.c_int, [_P, _P, _U32]),
     "mpm_get_counts": (ctypes.c_int, [_P, _P, _P, _P]),
     "mpm_remove_in_outlet": (ctypes.c_int, [_P, _P, _P]),
+    "mpm_gather_occupancy": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_pick": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
     "mpm_clone": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _U32]),
     "mpm_read_fixed_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
@@ -213,6 +214,13 @@ class Context:
         arr = np.zeros(8, np.float32)
         arr[: len(values)] = values
         self._check(self.lib.mpm_set_interaction(self.handle, _ptr(arr)))
+
+    def gather_occupancy(self) -> np.ndarray:
+        """(node_count, 2) float32: water and oil surface occupancy per node."""
+        nodes = self.counts()[2]
+        out = np.zeros((nodes, 2), np.float32)
+        self._check(self.lib.mpm_gather_occupancy(self.handle, _ptr(out), out.size))
+        return out
 
     def remove_in_outlet(self, bounds) -> int:
         b = np.ascontiguousarray(bounds, np.float32).reshape(4)

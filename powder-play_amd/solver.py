@@ -609,6 +609,14 @@ class MpmMlsGpuSolver:
             return
         self._write_interaction({"x": 0.0, "y": 0.0}, 0.0)
 
+    def gather_surface_occupancy(self) -> np.ndarray:
+        """The surface field the reference's renderer gathers from its bins
+        (gatherSurfaceOccupancy, shaders.ts:710-806): per grid node (gridIndex
+        order) the water and oil display area 2 V det F, weighted by the
+        quadratic B-spline and divided by the cell area.  (node_count, 2)."""
+        self._assert_active()
+        return self.ctx.gather_occupancy()
+
     def pick(self, point: dict, radius: float):
         """pick (solver.ts:2733-2823): the active particle nearest to the
         point within the radius, as a selection {particleIndex, objectId,
