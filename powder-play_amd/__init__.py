@@ -7,7 +7,7 @@ scheduling and scene generators, and the differentiable rollout of a
 fixed-corotated solid (``adjoint``).  Import it as ``powder_play_amd`` (the
 directory name carries a hyphen; see ``load_package`` in ``__graft_entry__``).
 """
-from . import adjoint, autotune, cfl, layout, native, scenes  # noqa: F401
+from . import adjoint, autotune, cfl, layout, native, scenes, snapshot  # noqa: F401
 from .layout import ParticleArrays, RangeError  # noqa: F401
 from .solver import (  # noqa: F401
     MpmMlsGpuBatch,
@@ -18,6 +18,6 @@ from .solver import (  # noqa: F401
 )
 
 __all__ = [
-    "adjoint", "autotune", "cfl", "layout", "native", "scenes", "ParticleArrays", "RangeError", "MpmMlsGpuNumericalError",
+    "adjoint", "autotune", "cfl", "layout", "native", "scenes", "snapshot", "ParticleArrays", "RangeError", "MpmMlsGpuNumericalError",
     "MpmMlsGpuSolver", "MpmMlsGpuBatch", "create_mpm_mls_gpu_solver", "create_experimental_mpm_mls_atomic_gpu_solver",
 ]

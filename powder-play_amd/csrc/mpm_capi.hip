@@ -157,12 +157,18 @@ int hip_fail(mpm_ctx* c, hipError_t e, const char* what) {
 // validateMpmMlsGpuInput per-record checks at the float32 level
 // (layout.ts:300-437): model code, finite state, positive volume/density,
 // damage and plastic-volume ranges, domain bounds and a positive packed det.
+// A removed tombstone (model 0xFFFFFFFF, left by an outlet and kept by a
+// restored snapshot) is not checked: no kernel reads it.
 int validate_records(mpm_ctx* c, const mpm_config& cfg, const uint32_t* rec, uint32_t n) {
     const float dom_w = (float)cfg.cells_x * cfg.cell_size;
     const float dom_h = (float)cfg.cells_y * cfg.cell_size;
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t* r = rec + 20ull * i;
         const uint32_t model = r[13];
+        if (model == MODEL_REMOVED) {
+            if (r[19] != 0u) return fail(c, MPM_EINVAL, "particles[%u] is removed but marked as a bonded vertex", i);
+            continue;
+        }
         if (model > 4u)
             return fail(c, MPM_EINVAL, "particles[%u].model code %u is not a supported material", i, model);
         if (r[19] > 3u)
@@ -282,7 +288,7 @@ int write_home(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n,
         F[i] = make_float4(u2f(r[8]), u2f(r[9]), u2f(r[10]), u2f(r[11]));
         pl[i] = u2f(r[12]);
         model[i] = r[13];
-        mat[i] = material_row(c, r);
+        mat[i] = r[13] == MODEL_REMOVED ? 0u : material_row(c, r);
         const uint32_t j = offset + i;
         c->st.model[j] = r[13];
         c->st.lambda[j] = u2f(r[14]);
@@ -293,6 +299,10 @@ int write_home(mpm_ctx* c, const uint32_t* rec, const uint32_t* ids, uint32_t n,
         c->st.object_id[j] = ids ? ids[i] : 0u;
         c->st.vertex[j] = r[19];
         const uint32_t m = r[13];
+        if (m == MODEL_REMOVED) {
+            ++c->removed;
+            continue;
+        }
         c->lc.scene_flags |= m == 2u ? SCENE_WATER : (m == 3u ? SCENE_OIL : SCENE_SOLID);
         if ((m == 2u || m == 3u) && u2f(r[18]) > 0.0f) c->lc.scene_flags |= SCENE_TAIT;
     }

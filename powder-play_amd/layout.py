@@ -426,6 +426,31 @@ def validate_input(particles, config: dict) -> ValidatedInput:
                           gradient_scale, grad_f32, ranges, bodies, bonds_packed, incidence)
 
 
+def validated_from_records(records: np.ndarray, object_ids: np.ndarray, config: dict) -> ValidatedInput:
+    """A ValidatedInput holding exactly the given 20-word records (a restored
+    snapshot): the records go through validate_input as particle columns,
+    and must pack back to themselves word for word.  Removed tombstones
+    (word 13 = 0xFFFFFFFF, left by outlets) are validated as the liquids they
+    were and keep their place and their tombstone word."""
+    rec = np.ascontiguousarray(records, np.uint32).reshape(-1, 20)
+    n = rec.shape[0]
+    fv = rec.view(np.float32).astype(np.float64)
+    removed = rec[:, 13] == MODEL_REMOVED
+    model = np.where(removed, MODEL_CODES["water"], rec[:, 13]).astype(np.uint32)
+    vertex = rec[:, 19].astype(np.int64) - 1
+    arr = ParticleArrays(x=fv[:, 0], y=fv[:, 1], vx=fv[:, 2], vy=fv[:, 3], model=model, lambda_=fv[:, 14],
+                         mu=fv[:, 15], density=fv[:, 16], volume=fv[:, 17], param=fv[:, 18], C=fv[:, 4:8],
+                         F=fv[:, 8:12], plastic_volume=fv[:, 12], object_id=np.asarray(object_ids, np.uint32),
+                         bonded_vertex=vertex if np.any(vertex >= 0) else None)
+    state = validate_input(arr, config)
+    state.packed[removed, 13] = MODEL_REMOVED
+    if state.packed.shape != rec.shape or not np.array_equal(state.packed, rec):
+        bad = int(np.flatnonzero(np.any(state.packed != rec, axis=1))[0]) if state.packed.shape == rec.shape else -1
+        raise RangeError(f"snapshot record {bad} does not pack back to itself")
+    state.particle_count = n
+    return state
+
+
 def pack_fracture_bonds(bonds, packed: np.ndarray, oid: np.ndarray) -> np.ndarray:
     """validateMpmSolidBondGraph (reference/solid-bonds.ts:11-66) and the
     24-byte bond records (layout.ts:497-548): u32 first, u32 second,

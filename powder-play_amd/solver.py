@@ -176,8 +176,15 @@ class MpmMlsGpuSolver:
         """tuning: None runs the default launch shape; a dict sets one
         (autotune.DEFAULT_TUNING's keys); "auto" takes the cached tuning of
         this scene's class, tuning and caching it first when there is none
-        (autotune.tuning_for)."""
-        self.state = layout.validate_input(particles, config)
+        (autotune.tuning_for).  particles may also be a ValidatedInput (a
+        restored snapshot, layout.validated_from_records), which is taken as
+        it is."""
+        if isinstance(particles, layout.ValidatedInput):
+            if isinstance(tuning, str):
+                raise RangeError("a validated particle set takes an explicit tuning or none")
+            self.state = particles
+        else:
+            self.state = layout.validate_input(particles, config)
         derived = cfl_bounds
         if derived is None:
             derived = cflmod.derive_cfl_bounds_packed(self.state.packed, self.state.particle_mass_f32)
