@@ -392,3 +392,24 @@ def calculate_cfl_step_batch(currentVelocityBound, gravityMagnitude, oscillation
             not math.isfinite(gpu_dt) or gpu_dt <= 0 or not math.isfinite(gpu_damp) or gpu_dt * gpu_damp > 1):
         raise RangeError("CFL microDt is too large for the explicit drag damping update to remain dissipative.")
     return batch, bound
+
+
+HORIZONTAL_OSCILLATION_FREQUENCY_HZ = 0.16  # kernels/oscillation.ts:5
+
+
+def mean_sinusoidal_acceleration(startSeconds, durationSeconds, amplitude,
+                                 frequencyHz=HORIZONTAL_OSCILLATION_FREQUENCY_HZ) -> float:
+    """meanSinusoidalAcceleration (kernels/oscillation.ts:7-32): the mean of
+    amplitude * sin(2 pi f t) over [start, start + duration]."""
+    for name, v in (("startSeconds", startSeconds), ("durationSeconds", durationSeconds),
+                    ("amplitude", amplitude), ("frequencyHz", frequencyHz)):
+        if not isinstance(v, (int, float)) or not math.isfinite(v):
+            raise RangeError("Oscillation inputs must be finite, with non-negative duration and frequency.")
+    if durationSeconds < 0 or frequencyHz < 0:
+        raise RangeError("Oscillation inputs must be finite, with non-negative duration and frequency.")
+    if amplitude == 0 or frequencyHz == 0:
+        return 0.0
+    w = 2 * math.pi * frequencyHz
+    half = (w * durationSeconds) / 2
+    sinc = 1.0 if half == 0 else math.sin(half) / half
+    return amplitude * math.sin(w * (startSeconds + durationSeconds / 2)) * sinc
