@@ -159,6 +159,23 @@ int mpm_clone(mpm_ctx* ctx, uint32_t start, uint32_t count, float offset_x, floa
  * linked lists).  MPM_ESTATE while the numerical status is set. */
 int mpm_gather_occupancy(mpm_ctx* ctx, float* out, size_t floats);
 
+/* Grid nodes the allocation-free occupancy gather can serve (8 MiB of
+ * float2 scratch per GPU). */
+#define MPM_STATIC_SCRATCH_NODES 1048576u
+
+/* No counterpart in the reference.  mpm_gather_occupancy for callers that
+ * cannot allocate device memory (tight render loops, hosts that forbid
+ * hipMalloc): the same output, bit for bit, with the per-node scratch in a
+ * static device array of MPM_STATIC_SCRATCH_NODES float2, one copy per GPU,
+ * instead of a buffer the library allocates.  It calls no hipMalloc or
+ * hipFree.  MPM_EINVAL for a grid of more than MPM_STATIC_SCRATCH_NODES
+ * nodes; MPM_ESTATE while the numerical status is set or while queued steps
+ * await a bucket reallocation (any readback settles those first).  Calls on
+ * one GPU share its array; the library serialises them with a per-device
+ * lock, and each call rewrites every node it returns.  Kernel timing
+ * (mpm_set_timing) does not record this gather. */
+int mpm_gather_occupancy_static(mpm_ctx* ctx, float* out, size_t floats);
+
 /* Number of active particle slots and the capacity. */
 int mpm_get_counts(mpm_ctx* ctx, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count);
 

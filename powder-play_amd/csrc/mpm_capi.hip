@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -1141,19 +1142,15 @@ int mpm_clone(mpm_ctx* c, uint32_t start, uint32_t count, float offset_x, float 
     return rebucket_all(c, c->lc.cap);
 }
 
-// gatherSurfaceOccupancy (shaders.ts:710-806) of the committed state.
-int mpm_gather_occupancy(mpm_ctx* c, float* out, size_t floats) {
-    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
-    if (!out || floats != (size_t)c->lc.node_count * 2)
-        return fail(c, MPM_EINVAL, "output must hold node_count*2 floats");
-    int s = settle(c);
-    if (s) return s;
-    if (c->h_ctl->status[0] != 0u)
-        return fail(c, MPM_ESTATE, "the numerical status is set: clear it before gathering the surface occupancy");
-    if (!c->d_occ) {
-        hipError_t e = c->mem.alloc((void**)&c->d_occ, (size_t)c->lc.node_count * sizeof(float2));
-        if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(occupancy)");
-    }
+}  // extern "C"
+
+namespace {
+
+// The surface occupancy gather of the committed state into dst (node_count
+// float2 of device memory), then copied to the caller's out.  Every node of
+// the grid is the interior of exactly one tile, so each launch writes all of
+// dst and nothing in it needs clearing first.
+int run_occupancy(mpm_ctx* c, float2* dst, float* out, size_t floats, KernelTimes* times) {
     // the grid of this context as a parameter block (no step: dt 0, no gravity)
     Params p{};
     p.cells_x = c->cfg.cells_x;
@@ -1171,14 +1168,83 @@ int mpm_gather_occupancy(mpm_ctx* c, float* out, size_t floats) {
     p.dom_h = (float)c->cfg.cells_y * c->cfg.cell_size;
     const Shape shape = {28u, 8u, 256u};
     SceneDesc d = scene_desc(c, p, 0u, 0u, c->seq, shape);
-    d.grid.occupancy = c->d_occ;
+    d.grid.occupancy = dst;
     d.diag_grid = nullptr;
     HIP_TRY(c, hipMemcpy(c->d_scene, &d, sizeof d, hipMemcpyHostToDevice));
     HIP_TRY(c, launch_occupancy(c->stream, c->d_scene, p2g_tiles_for(c->lc.node_w, c->lc.node_h, 28u),
-                                scene_general(c->lc.scene_flags), c->times));
+                                scene_general(c->lc.scene_flags), times));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    HIP_TRY(c, hipMemcpy(out, c->d_occ, floats * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(out, dst, floats * sizeof(float), hipMemcpyDeviceToHost));
     return MPM_OK;
+}
+
+// Scratch of the allocation-free occupancy gather: one float2 per grid node,
+// MPM_STATIC_SCRATCH_NODES of them (8 MiB).  Each GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in; the device address comes from hipGetSymbolAddress,
+// once per device, and only that address is handed to kernels and copies.
+static __device__ __align__(256) float2 scratch[MPM_STATIC_SCRATCH_NODES];
+
+constexpr int STATIC_SCRATCH_DEVICES = 64;
+
+struct StaticScratch {
+    std::mutex lock;        // held for a whole call: calls on one GPU share the array
+    float2* ptr = nullptr;  // device address of this device's copy
+};
+StaticScratch g_static_scratch[STATIC_SCRATCH_DEVICES];
+
+}  // namespace
+
+extern "C" {
+
+// gatherSurfaceOccupancy (shaders.ts:710-806) of the committed state.
+int mpm_gather_occupancy(mpm_ctx* c, float* out, size_t floats) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!out || floats != (size_t)c->lc.node_count * 2)
+        return fail(c, MPM_EINVAL, "output must hold node_count*2 floats");
+    int s = settle(c);
+    if (s) return s;
+    if (c->h_ctl->status[0] != 0u)
+        return fail(c, MPM_ESTATE, "the numerical status is set: clear it before gathering the surface occupancy");
+    if (!c->d_occ) {
+        hipError_t e = c->mem.alloc((void**)&c->d_occ, (size_t)c->lc.node_count * sizeof(float2));
+        if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(occupancy)");
+    }
+    return run_occupancy(c, c->d_occ, out, floats, c->times);
+}
+
+// mpm_gather_occupancy without any allocation: the per-node scratch is this
+// device's copy of the static array.  Queued steps are waited for, but a
+// batch that ran out of bucket slots (whose replay reallocates the buckets)
+// is refused instead of replayed.  Kernel timing is not recorded.
+int mpm_gather_occupancy_static(mpm_ctx* c, float* out, size_t floats) {
+    if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
+    if (!out || floats != (size_t)c->lc.node_count * 2)
+        return fail(c, MPM_EINVAL, "output must hold node_count*2 floats");
+    if (c->lc.node_count > MPM_STATIC_SCRATCH_NODES)
+        return fail(c, MPM_EINVAL, "the grid has %u nodes; the static scratch holds at most %u", c->lc.node_count,
+                    (uint32_t)MPM_STATIC_SCRATCH_NODES);
+    if (c->device < 0 || c->device >= STATIC_SCRATCH_DEVICES)
+        return fail(c, MPM_EDEVICE, "device %d is outside the %d devices the static scratch serves", c->device,
+                    STATIC_SCRATCH_DEVICES);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(c->h_ctl, c->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+    if (c->h_ctl->status[0] & ERR_INTERNAL_REBUCKET)
+        return fail(c, MPM_ESTATE, "queued steps ran out of bucket slots and must be replayed with larger buckets, "
+                                   "which allocates: read the status or particles first");
+    c->pending.clear();
+    if (c->h_ctl->status[0] != 0u)
+        return fail(c, MPM_ESTATE, "the numerical status is set: clear it before gathering the surface occupancy");
+    StaticScratch& st = g_static_scratch[c->device];
+    std::lock_guard<std::mutex> hold(st.lock);
+    if (!st.ptr) {
+        void* ptr = nullptr;
+        HIP_TRY(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(c, MPM_EDEVICE, "hipGetSymbolAddress(scratch) returned no address");
+        st.ptr = static_cast<float2*>(ptr);
+    }
+    return run_occupancy(c, st.ptr, out, floats, nullptr);
 }
 
 int mpm_get_counts(mpm_ctx* c, uint32_t* particle_count, uint32_t* capacity, uint32_t* node_count) {

@@ -35,6 +35,7 @@ SIGNATURES = {
     "mpm_get_counts": (ctypes.c_int, [_P, _P, _P, _P]),
     "mpm_remove_in_outlet": (ctypes.c_int, [_P, _P, _P]),
     "mpm_gather_occupancy": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "mpm_gather_occupancy_static": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "mpm_pick": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
     "mpm_clone": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _U32]),
     "mpm_read_fixed_grid": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
@@ -220,6 +221,14 @@ class Context:
         nodes = self.counts()[2]
         out = np.zeros((nodes, 2), np.float32)
         self._check(self.lib.mpm_gather_occupancy(self.handle, _ptr(out), out.size))
+        return out
+
+    def gather_occupancy_static(self) -> np.ndarray:
+        """gather_occupancy through the allocation-free entry point (the
+        per-node scratch is a static device array of the library)."""
+        nodes = self.counts()[2]
+        out = np.zeros((nodes, 2), np.float32)
+        self._check(self.lib.mpm_gather_occupancy_static(self.handle, _ptr(out), out.size))
         return out
 
     def remove_in_outlet(self, bounds) -> int:
