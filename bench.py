@@ -10,9 +10,11 @@ inputs resident in HBM, the timed region bracketed by stream synchronisation.
 Printed JSON line (one): metric/value/unit of the float32 gather path plus
 the fixed-point path's rate, the roofline of the dominant kernel (HIP events
 on the solver's stream; algorithmic bytes per launch), the batched-scenes and
-differentiable-rollout legs, and the CPU baseline (the FP64 oracle step,
-single thread, on a bounded sample of the same scene; and the FP64 oracle
-adjoint of the elastic block).
+differentiable-rollout legs, the CPU baseline (the FP64 oracle step on a
+bounded sample of the same scene, on every host CPU the process may use and
+on one thread; and the FP64 oracle adjoint of the elastic block), and the
+measured error of every configuration against the FP64 oracle ("parity",
+the oracle as checker beside the CPU baseline).
 """
 from __future__ import annotations
 
@@ -110,13 +112,36 @@ def timed(pkg, solver, plan, limit, steps, warmup):
     return dt, speed
 
 
-def baseline_threads() -> int:
-    """Host threads of the all-core CPU baseline: OMP_NUM_THREADS when set
-    (the GPU host's CPU share), else the CPU count, at most 16."""
-    env = os.environ.get("OMP_NUM_THREADS", "")
+def host_cpus() -> dict:
+    """The host CPUs this process may use: the affinity mask, nproc, the
+    lscpu model name, and the OpenMP thread budget of the environment."""
+    import subprocess
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        info["nproc"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        fields = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        info["model"] = fields.get("Model name", "").strip() or None
+        info["sockets"] = fields.get("Socket(s)", "").strip() or None
+        info["threads_per_core"] = fields.get("Thread(s) per core", "").strip() or None
+    except (OSError, subprocess.SubprocessError):
+        info["model"] = None
+    return info
+
+
+def baseline_threads(info: dict) -> int:
+    """Threads of the all-core CPU baseline: every CPU in the affinity mask,
+    capped by OMP_NUM_THREADS when it is set (the CPU share the host grants
+    one GPU's jobs)."""
+    n = info["affinity_cpus"]
+    env = info.get("omp_num_threads") or ""
     if env.isdigit() and int(env) > 0:
-        return int(env)
-    return max(1, min(16, os.cpu_count() or 1))
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def cpu_baseline(pkg, arr, config, budget_s=12.0, threads=1):
@@ -170,8 +195,9 @@ def cfl_macrosteps(pkg, solver, macro_dt, gravity, count):
 def small_scenes_leg(pkg, macrosteps: int = 12):
     """The reference's own scenes, which are launch-bound at this size: the SI
     Tait tank (1,600 particles, 1/120 s macrosteps, both P2G modes) and the
-    Gate 4 scene (3,674 particles with bonded pebbles and a fracture solid,
-    1/240 s macrosteps).  Each is stepped through the adaptive CFL loop with
+    Gate 4 scene (3,674 particles with bonded pebbles and a fracture solid),
+    on the reference's benchmark grid with fixed 1/240 s steps and on the
+    MPM page's 220 x 600 grid with CFL macrosteps.  Each is stepped through the adaptive CFL loop with
     its host work and status readback per macrostep, after one warm-up
     macrostep; rate = particles x microsteps / wall seconds."""
     out = {}
@@ -218,6 +244,25 @@ def small_scenes_leg(pkg, macrosteps: int = 12):
                     "protocol": "220x150 grid, cell 10, liquid area 100, dt 1/240, gravity (0, 90); 64 warm-up "
                                 "steps, 12 batches of 8 steps, wall clock per batch including its synchronisation"}
     log(f"[bench] Gate 4: {1e3 * med:.4f} ms per step (median of 12 batches of 8), {n / med:.3e} particle-steps/s")
+    # Gate 4 as the MPM page runs it: the 220 x 600 grid of cell size 10,
+    # liquid area 1,600, 1/240 s macrosteps through the adaptive CFL loop
+    particles, config, gravity = pkg.scenes.gate4_scene()
+    s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
+    try:
+        n = s.particle_count
+        cfl_macrosteps(pkg, s, 1.0 / 240.0, gravity, 1)
+        t0 = time.perf_counter()
+        micro = cfl_macrosteps(pkg, s, 1.0 / 240.0, gravity, macrosteps)
+        el = time.perf_counter() - t0
+    finally:
+        s.dispose()
+    out["gate4_mpm_grid"] = {"value": n * micro / el, "unit": "particle-steps/s", "particles": n,
+                             "macrosteps": macrosteps, "microsteps": micro, "ms_per_microstep": 1e3 * el / micro,
+                             "ms_per_macrostep": 1e3 * el / macrosteps,
+                             "protocol": "220x600 grid, cell 10, liquid area 1,600, gravity (0, 90); 1/240 s "
+                                         "macrosteps through the adaptive CFL loop after one warm-up macrostep, "
+                                         "wall clock including host planning and status readbacks"}
+    log(f"[bench] Gate 4 (220x600): {micro} microsteps in {1e3 * el:.1f} ms, {n * micro / el:.3e} particle-steps/s")
     out["reference_published"] = {
         "tait_tank": "23.89e6 particle-steps/s, 1,600 particles, 423-microstep batch, Apple Metal 3 "
                      "(docs/PERFORMANCE-MLS-MPM.md:837-851); ~4.8e7 with fixed-point P2G "
@@ -253,9 +298,23 @@ def batched_leg(pkg, steps: int, how: str = "auto"):
     batch.scene(0).ctx.read_compact()  # waits for the batch's stream
     t_batch = time.perf_counter() - t0
     bad = [i for i in range(len(batch)) if batch.scene(i).ctx.read_compact()[2][0] != 0]
-    batch.dispose()
     if bad:
+        batch.dispose()
         raise RuntimeError(f"batched scenes {bad[:8]} raised numerical flags")
+    # the same batch with the FP64 mass/momentum reductions of every scene on
+    for i in range(len(batch)):
+        batch.scene(i).set_diagnostics(True)
+    batch.step_batch(8, dt, gravity)
+    batch.scene(0).ctx.read_compact()
+    t0 = time.perf_counter()
+    batch.step_batch(steps, dt, gravity)
+    batch.scene(0).ctx.read_compact()
+    t_diag = time.perf_counter() - t0
+    drift = 0.0
+    for i in range(len(batch)):
+        d = batch.scene(i).read_diagnostics()
+        drift = max(drift, abs(d["outputMass"] - d["inputMass"]) / d["inputMass"])
+    batch.dispose()
     solvers = [pkg.create_mpm_mls_gpu_solver(a, c, device=0) for a, c in scenes]
     for s, g in zip(solvers, gravity):
         s.step_batch(8, dt, g)
@@ -269,10 +328,13 @@ def batched_leg(pkg, steps: int, how: str = "auto"):
         s.dispose()
     rate, rate_seq = total * steps / t_batch, total * steps / t_seq
     log(f"[bench] batched: {1e3 * t_batch / steps:.4f} ms per microstep of all scenes ({rate:.3e} particle-steps/s); "
+        f"with FP64 reductions {1e3 * t_diag / steps:.4f} ms (mass drift {drift:.1e}); "
         f"sequential {1e3 * t_seq / steps:.4f} ms ({rate_seq:.3e}); x{rate / rate_seq:.1f}")
     return {"value": rate, "unit": "particle-steps/s", "ms_per_step": 1e3 * t_batch / steps, "scenes": len(tanks),
             "particles": total, "steps": steps, "sequential_value": rate_seq,
             "sequential_ms_per_step": 1e3 * t_seq / steps, "speedup_vs_sequential": rate / rate_seq,
+            "fp64_reductions": {"value": total * steps / t_diag, "ms_per_step": 1e3 * t_diag / steps,
+                                "max_relative_mass_drift": drift},
             "tuning": tuned,
             "workload": f"{len(tanks)} SI-material Tait tanks, {total} particles (log-uniform 500-50,000 per tank), "
                         "64-512 cell grids, dam-break blocks, random gravity directions, float32 gather P2G"}
@@ -315,7 +377,8 @@ def adjoint_leg(pkg, reps: int):
 
 
 def adjoint_cpu(pkg):
-    """FP64 oracle forward + reverse of the same elastic block, one thread."""
+    """FP64 oracle forward + reverse of the same elastic block, one thread.
+    Returns (report, (final, bar, dL/dstate0, dL/dlambda, dL/dmu))."""
     from oracle import oracle as orc
     state, lam, mu, config, target = pkg.scenes.elastic_block()
     cfg = orc.diff_config(config)
@@ -323,12 +386,195 @@ def adjoint_cpu(pkg):
     final = orc.diff_forward(state, lam, mu, cfg)
     t1 = time.perf_counter()
     _, bar = orc.com_loss(final, target)
-    orc.diff_backward(state, lam, mu, cfg, bar)
+    b0, gl, gm, _ = orc.diff_backward(state, lam, mu, cfg, bar)
     t2 = time.perf_counter()
     n, T = len(lam), config["steps"]
-    return {"forward_value": n * T / (t1 - t0), "backward_value": n * T / (t2 - t1), "unit": "particle-steps/s",
-            "cores": 1, "kind": "port", "sample": f"one FP64 forward and reverse of the {n}-particle block "
-                                                   f"({t1 - t0:.2f} s + {t2 - t1:.2f} s; the reverse re-runs the forward)"}
+    return ({"forward_value": n * T / (t1 - t0), "backward_value": n * T / (t2 - t1), "unit": "particle-steps/s",
+             "cores": 1, "kind": "port", "sample": f"one FP64 forward and reverse of the {n}-particle block "
+                                                   f"({t1 - t0:.2f} s + {t2 - t1:.2f} s; the reverse re-runs the "
+                                                   "forward)"},
+            (final, bar, b0, gl, gm))
+
+
+def _rel_l2(a, b) -> float:
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _oracle_config(orc, config, dt, gravity):
+    g = config["grid"]
+    return orc.make_config(float(np.float32(dt)), g["cellsX"], g["cellsY"], g["cellSize"], config["particleMass"],
+                           config["particleVolume"], (gravity["x"], gravity["y"]), config.get("wallFriction", 0.0))
+
+
+def _step_error(orc, solver, config, dt, gravity, bodies=False):
+    """One microstep of `solver` against one FP64 oracle microstep from the
+    same (the GPU's own) input state: relative L2 error of the velocities and
+    positions and the largest position error."""
+    before, oid = solver.read_particles(packed=True)
+    cfg = _oracle_config(orc, config, dt, gravity)
+    if bodies:
+        bc = orc.BodyConfig(float(np.float32(config.get("bondRestLength", 0.0))),
+                            float(np.float32(config.get("bondStiffness", 0.35))),
+                            float(np.float32(config.get("bondReferenceDt", dt))))
+        bonds = None
+        if solver.state.fracture_bonds is not None:
+            b = solver.read_fracture_bonds()
+            bonds = np.zeros(len(b), orc.BOND_DTYPE)
+            for k, d in enumerate(b):
+                bonds[k] = (d["firstParticle"], d["secondParticle"], d["restLength"], d["failureStrain"],
+                            d["damage"], 1 if d["broken"] else 0, 0)
+        exp, _, _ = orc.step_bodies_f64(orc.particles_from_packed(before), oid, cfg, bc, bonds)
+    else:
+        exp, _ = orc.step_f64(orc.particles_from_packed(before), cfg)
+    solver.step(dt, gravity)
+    solver.check_numerical_status()
+    got, _ = solver.read_particles(packed=True)
+    have = got.view(np.float32)[:, :4].astype(np.float64)
+    want = np.stack([exp["x"], exp["y"], exp["vx"], exp["vy"]], axis=1)
+    return {"rel_l2_v": _rel_l2(have[:, 2:4], want[:, 2:4]), "rel_l2_x": _rel_l2(have[:, :2], want[:, :2]),
+            "max_abs_x": float(np.abs(have[:, :2] - want[:, :2]).max())}
+
+
+def parity_leg(pkg, arr, config, fp64_adjoint):
+    """The measured error of every configuration against the FP64 oracle (the
+    oracle as checker, on the host, beside the CPU baseline).  Per microstep
+    the oracle takes one FP64 step from the GPU's own input state; the bar of
+    the float32 gather path is a relative L2 velocity error of 1e-4 per
+    microstep, and the fixed-point grid words must be bit-exact."""
+    import hashlib
+    from oracle import oracle as orc
+    t_start = time.perf_counter()
+    out = {"bar": "per microstep from a common state: relative L2 velocity error <= 1e-4 (float32 paths); "
+                  "int32 fixed-point grid words bit-exact; adjoint gradients relative L2 <= 1e-3"}
+
+    # C1: SI Tait tank, first microstep and one whole 1/120 s macrostep
+    particles, config1, macro_dt, _ = pkg.scenes.tait_tank()
+    g1 = pkg.scenes.TAIT_TANK_GRAVITY
+    c1 = {}
+    for mode in ("gather", "fixed"):
+        factory = (pkg.create_mpm_mls_gpu_solver if mode == "gather"
+                   else pkg.create_experimental_mpm_mls_atomic_gpu_solver)
+        s = factory(particles, config1, device=0)
+        try:
+            plan = pkg.cfl.calculate_microsteps(macro_dt, config1["grid"]["cellSize"], s.cfl_bounds.waveSpeedBound,
+                                                pkg.cfl.calculate_velocity_bound(s.cfl_bounds.velocityBound, macro_dt,
+                                                                                 9.81), 1024)
+            first = _step_error(orc, s, config1, plan.microDt, g1)
+            s.step_batch(plan.stepCount - 1, plan.microDt, g1)
+            s.check_numerical_status()
+            got, _ = s.read_particles(packed=True)
+        finally:
+            s.dispose()
+        ref = orc.run_f64(orc.particles_from_packed(pkg.layout.validate_input(particles, config1).packed),
+                          _oracle_config(orc, config1, plan.microDt, g1), plan.stepCount)
+        have = got.view(np.float32)[:, :4].astype(np.float64)
+        want = np.stack([ref["x"], ref["y"], ref["vx"], ref["vy"]], axis=1)
+        c1[mode] = {"microstep_1": first,
+                    "macrostep": {"microsteps": plan.stepCount,
+                                  "max_abs_x_m": float(np.abs(have[:, :2] - want[:, :2]).max()),
+                                  "max_abs_v_ms": float(np.abs(have[:, 2:4] - want[:, 2:4]).max()),
+                                  "rel_l2_v": _rel_l2(have[:, 2:4], want[:, 2:4])}}
+    out["C1_tait_tank"] = c1
+    log(f"[bench] parity C1: {c1}")
+
+    # C2: Gate 4 on the 220 x 600 grid (bodies, contact, fracture bonds)
+    particles, config2, gravity2 = pkg.scenes.gate4_scene()
+    s = pkg.create_mpm_mls_gpu_solver(particles, config2, device=0)
+    c2 = {}
+    try:
+        dt2 = 1.0 / 2400.0
+        c2["microstep_1"] = _step_error(orc, s, config2, dt2, gravity2, bodies=True)
+        s.step_batch(8, dt2, gravity2)
+        c2["microstep_10"] = _step_error(orc, s, config2, dt2, gravity2, bodies=True)
+        c2["dt"] = dt2
+        c2["note"] = ("max_abs_x is led by liquid particles that start inside the pebbles: the projection out of "
+                      "the triangles moves them by up to ~54 units along a normal interpolated from the float32 "
+                      "state, so their position error scales with that displacement")
+    finally:
+        s.dispose()
+    out["C2_gate4"] = c2
+    log(f"[bench] parity C2: {c2}")
+
+    # C3: the 1M column at microsteps 1, 20 and 40 of a 1/240 s macrostep,
+    # both modes; fixed-point grid words at microstep 20
+    dt3, g3 = 1.0 / 9600.0, pkg.scenes.WATER_OIL_GRAVITY
+    cells = config["grid"]["cellsX"]
+    c3 = {}
+    for mode in ("gather", "fixed"):
+        factory = (pkg.create_mpm_mls_gpu_solver if mode == "gather"
+                   else pkg.create_experimental_mpm_mls_atomic_gpu_solver)
+        s = factory(arr, config, device=0, keep_fixed_grid=(mode == "fixed"))
+        rows = {}
+        try:
+            done = 0
+            for k in (1, 20, 40):
+                if k - 1 > done:
+                    s.step_batch(k - 1 - done, dt3, g3)
+                    done = k - 1
+                if mode == "fixed" and k == 20:
+                    before, _ = s.read_particles(packed=True)
+                    prm = orc.params_f32(cells, cells, 1.0, 0.02, 1.0, dt3, len(arr))
+                    want, _ = orc.fixed_p2g_f32(before, prm)
+                rows[f"microstep_{k}"] = _step_error(orc, s, config, dt3, g3)
+                done = k
+                if mode == "fixed" and k == 20:
+                    got = s.read_fixed_grid()
+                    rows["fixed_words_microstep_20"] = {
+                        "words": int(got.size), "differ": int(np.count_nonzero(got != want)),
+                        "sha256_gpu": hashlib.sha256(np.ascontiguousarray(got).tobytes()).hexdigest()[:16],
+                        "sha256_cpu": hashlib.sha256(np.ascontiguousarray(want).tobytes()).hexdigest()[:16]}
+        finally:
+            s.dispose()
+        c3[mode] = rows
+    out["C3_column_1m"] = c3
+    log(f"[bench] parity C3: {c3}")
+
+    # C4: the 256-tank batch; after 8 batched microsteps, the smallest, a
+    # median and the largest tank against one oracle microstep each
+    tanks = pkg.scenes.batched_tanks()
+    sizes = np.array([len(a) for a, _, _ in tanks])
+    order = np.argsort(sizes)
+    pick = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    batch = pkg.MpmMlsGpuBatch([(a, c) for a, c, _ in tanks], device=0)
+    gs = [g for _, _, g in tanks]
+    c4 = {}
+    try:
+        batch.step_batch(8, pkg.scenes.BATCH_DT, gs)
+        before = {i: batch.scene(i).read_particles(packed=True)[0] for i in pick}
+        batch.step_batch(1, pkg.scenes.BATCH_DT, gs)
+        for i in pick:
+            sc = batch.scene(i)
+            sc.check_numerical_status()
+            got = sc.read_particles(packed=True)[0]
+            exp, _ = orc.step_f64(orc.particles_from_packed(before[i]),
+                                  _oracle_config(orc, tanks[i][1], pkg.scenes.BATCH_DT, gs[i]))
+            have = got.view(np.float32)[:, :4].astype(np.float64)
+            c4[f"tank_{i}_{int(sizes[i])}_particles"] = {
+                "rel_l2_v": _rel_l2(have[:, 2:4], np.stack([exp["vx"], exp["vy"]], 1)),
+                "rel_l2_x": _rel_l2(have[:, :2], np.stack([exp["x"], exp["y"]], 1))}
+    finally:
+        batch.dispose()
+    out["C4_batched"] = c4
+    log(f"[bench] parity C4: {c4}")
+
+    # C5: GPU forward and adjoint of the elastic block against the FP64 ones
+    if fp64_adjoint is not None:
+        final64, bar, b0, gl, gm = fp64_adjoint
+        state, lam, mu, config5, _ = pkg.scenes.elastic_block()
+        r = pkg.adjoint.ElasticRollout(len(lam), config5, segment_steps=20, device=0)
+        try:
+            final = r.forward(state, lam, mu)
+            g0, ggl, ggm = r.backward(bar.astype(np.float32))
+        finally:
+            r.close()
+        out["C5_adjoint"] = {"max_abs_x_final": float(np.abs(final[:, :2] - final64[:, :2]).max()),
+                             "rel_l2_grad_x0": _rel_l2(g0[:, 0:2], b0[:, 0:2]),
+                             "rel_l2_grad_v0": _rel_l2(g0[:, 2:4], b0[:, 2:4]),
+                             "rel_l2_grad_lambda": _rel_l2(ggl, gl), "rel_l2_grad_mu": _rel_l2(ggm, gm)}
+        log(f"[bench] parity C5: {out['C5_adjoint']}")
+    out["seconds"] = round(time.perf_counter() - t_start, 1)
+    return out
 
 
 def choose_tuning(pkg, arr, config, mode, how):
@@ -386,6 +632,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--no-small", action="store_true", help="skip the Tait tank and Gate 4 leg")
+    ap.add_argument("--no-parity", action="store_true", help="skip the error-vs-FP64 checks of the CPU leg")
     ap.add_argument("--batch-tuning", default="auto",
                     help='launch shape of the batched leg: "auto" or "p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
     ap.add_argument("--tuning", default="auto",
@@ -461,21 +708,27 @@ def main():
     small = None if args.no_small else small_scenes_leg(pkg)
 
     cpu = None
+    parity = None
     if not args.no_cpu:
-        threads = baseline_threads()
-        log(f"[bench] CPU baseline: FP64 oracle step, {threads} threads")
+        hosts = host_cpus()
+        threads = baseline_threads(hosts)
+        log(f"[bench] CPU baseline: FP64 oracle step, {threads} threads (host: {hosts})")
         rate, steps, el = cpu_baseline(pkg, arr, config, budget_s=8.0, threads=threads)
         log("[bench] CPU baseline: FP64 oracle step, one thread")
         rate1, steps1, el1 = cpu_baseline(pkg, arr, config)
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"{steps} FP64 oracle microsteps of the full {n}-particle scene on {threads} OpenMP "
                          f"threads ({el:.1f} s, after 1 warm-up step)",
+               "host": hosts,
                "single_thread": {"value": rate1, "cores": 1,
                                  "sample": f"{steps1} FP64 oracle microsteps, one thread ({el1:.1f} s)"},
                "gpu_over_cpu": None}
+        fp64_adjoint = None
         if adjoint is not None:
             log("[bench] CPU baseline: FP64 oracle adjoint of the elastic block, one thread")
-            cpu["adjoint"] = adjoint_cpu(pkg)
+            cpu["adjoint"], fp64_adjoint = adjoint_cpu(pkg)
+        if not args.no_parity:
+            parity = parity_leg(pkg, arr, config, fp64_adjoint)
 
     if cpu is not None:
         cpu["gpu_over_cpu"] = r["rate"] / cpu["value"]
@@ -509,6 +762,7 @@ def main():
         "adjoint": adjoint,
         "small_scenes": small,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(line), flush=True)
 
