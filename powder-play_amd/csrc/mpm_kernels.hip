@@ -329,6 +329,21 @@ __device__ __forceinline__ float qweight(float f, int o) {
 // select chain may compile to.  The outer weights share one form, (0.5 t) t
 // with t = 1.5 - f or f - 0.5, so one product serves both and every value
 // is rounded exactly as in qweight.
+// The quadratic B-spline at d = o - f, o the stencil offset and f in
+// [0.5, 1.5) the fraction: 0.75 - d^2 at the middle offset, else
+// (1.5 - |d|)^2 / 2.  For o = 0 and 1 this is bit for bit quadraticWeights
+// (shaders.ts:500-509); for o = 2, 1.5 - |2 - f| equals its f - 0.5 but for
+// the rounding of 2 - f when f < 1 (at most 2^-24 in t, below 2^-25 in the
+// weight).  d is the reference's dpos / dx, so the gather forms it once.
+// Both branches are computed and one is kept by a bit mask, so lanes of a
+// wave at different offsets do not diverge.
+__device__ __forceinline__ float bspline_of_offset(float d, bool middle) {
+    const uint32_t m = 0u - (uint32_t)middle;
+    const float t = 1.5f - fabsf(d);
+    const float mid = 0.75f - d * d, side = (0.5f * t) * t;
+    return __uint_as_float((__float_as_uint(mid) & m) | (__float_as_uint(side) & ~m));
+}
+
 __device__ __forceinline__ float qweight_masked(float f, int o) {
     const uint32_t m0 = 0u - (uint32_t)(o == 0), m1 = 0u - (uint32_t)(o == 1);
     const float t = __uint_as_float((__float_as_uint(1.5f - f) & m0) | (__float_as_uint(f - 0.5f) & ~m0));
@@ -973,7 +988,6 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = 0.0f;
     uint32_t flags = 0;
-    const float dx = P.dx;
     const uint32_t nchunks = s_nchunks;
     const int cx0 = rbx0 * BUCKET_CELLS, cy0 = rby0 * BUCKET_CELLS;  // first region cell (stencil-base coords)
     constexpr int PER = G_CAP / T::THREADS + (G_CAP % T::THREADS ? 1 : 0);  // staged particles a thread
@@ -1059,7 +1073,10 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 flags |= pf;
             }
             s_r0[q] = make_float4(sc.fx, sc.fy, mt.mass * v.x, mt.mass * v.y);
-            s_r1[q] = A;
+            // the affine scaled by the cell size: a contribution's momentum
+            // is then m v + (A dx) d with d = offset - fraction, two fused
+            // multiply-adds per component
+            s_r1[q] = make_float4(A.x * P.dx, A.y * P.dx, A.z * P.dx, A.w * P.dx);
             s_r2[q] = make_float2(mt.mass, mt.volume * det);
             s_tag[q] = pf ? 0xffffu : ((rc % (uint32_t)T::RC) | (mt.phase << 8));
         }
@@ -1077,14 +1094,14 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             }
             const uint32_t n01 = n[0] + n[1], total = n01 + n[2];
             // pair k lies in row range dy = (k >= n[0]) + (k >= n01), at
-            // q = k + base[dy]; the choices below are bit masks, so lanes at
+            // q = k + base[dy]; the choices below are selects, so lanes at
             // different rows or columns do not diverge, and the four LDS reads
             // of a pair are issued together
             const uint32_t base0 = q0[0], base1 = q0[1] - n[0], base2 = q0[2] - n01;
 #pragma unroll 1
             for (uint32_t k = 0; k < total; ++k) {
-                const uint32_t in0 = 0u - (uint32_t)(k < n[0]), in01 = 0u - (uint32_t)(k < n01);
-                const uint32_t q = k + ((base0 & in0) | (base1 & in01 & ~in0) | (base2 & ~in01));
+                const bool row2 = k < n[0], row1 = !row2 && k < n01;  // oy = 2, oy = 1
+                const uint32_t q = k + (row2 ? base0 : (row1 ? base1 : base2));
                 const uint32_t tag = s_tag[q];
                 const float4 r0 = s_r0[q], r1 = s_r1[q];
                 const float2 r2 = s_r2[q];
@@ -1095,13 +1112,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
                 asm volatile("" ::"v"(tag), "v"(r0.x), "v"(r1.x), "v"(r2.x));
                 const uint32_t ph = tag >> 8;  // 0xff: the particle contributes nothing
                 const int ox = lx + 3 - (int)(tag & 0xffu);
-                const int oy = 2 - (int)((1u & ~in0) + (1u & ~in01));
-                const float fx = r0.x, fy = r0.y;
-                const float w = qweight_masked(fx, ox) * qweight_masked(fy, oy);
-                const float dpx = ((float)ox - fx) * dx;
-                const float dpy = ((float)oy - fy) * dx;
-                const float qx = (r0.z + r1.x * dpx) + r1.y * dpy;
-                const float qy = (r0.w + r1.z * dpx) + r1.w * dpy;
+                // d = offset - fraction per axis, the reference's dpos / dx
+                const float ddx = (float)ox - r0.x;
+                const float ddy = (row2 ? 2.0f : (row1 ? 1.0f : 0.0f)) - r0.y;
+                const float w = bspline_of_offset(ddx, ox == 1) * bspline_of_offset(ddy, row1);
+                const float qx = fmaf(r1.y, ddy, fmaf(r1.x, ddx, r0.z));
+                const float qy = fmaf(r1.w, ddy, fmaf(r1.z, ddx, r0.w));
                 // one test for both skips
                 if (ph > 2u || !finitef(qx) || !finitef(qy)) {
                     if (ph <= 2u) flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
