@@ -507,10 +507,21 @@ __global__ __launch_bounds__(256) void k_keys(Home home, uint32_t* keys, uint32_
 }
 
 // ------------------------------------------------------------ P2G + grid
+// a / b for finite non-zero b in the grid update: the hardware reciprocal,
+// then one correction of the quotient (Markstein), within an ulp of the
+// IEEE quotient in four instructions instead of the ten of the correctly
+// rounded sequence.  The WGSL quotient it stands for carries 2.5 ulp.  The
+// binning and the fixed-point quotients keep their exact forms.
+__device__ __forceinline__ float quot(float a, float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    return fmaf(fmaf(-q, b, a), r, q);
+}
+
 // finishGridVelocity (shaders.ts:1748-1777)
 __device__ __forceinline__ float2 finish_velocity(const Params& P, float m, float qx, float qy, int nx, int ny) {
     if (m <= 0.0f) return make_float2(0.f, 0.f);
-    float2 v = make_float2(qx / m + P.dt * P.gx, qy / m + P.dt * P.gy);
+    float2 v = make_float2(quot(qx, m) + P.dt * P.gx, quot(qy, m) + P.dt * P.gy);
     if (P.drag_radius > 0.0f && nx >= 0 && ny >= 0 && nx <= (int)P.cells_x && ny <= (int)P.cells_y) {
         float px = (float)nx * P.dx, py = (float)ny * P.dx;
         float dxv = P.drag_target_x - px, dyv = P.drag_target_y - py;
@@ -655,30 +666,32 @@ __device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& gri
         const float wgx = nvol(0, 1, 0) - nvol(0, -1, 0), wgy = nvol(0, 0, 1) - nvol(0, 0, -1);
         const float ogx = nvol(1, 1, 0) - nvol(1, -1, 0), ogy = nvol(1, 0, 1) - nvol(1, 0, -1);
         const float sepx = ogx - wgx, sepy = ogy - wgy;
-        const float len = sqrtf(sepx * sepx + sepy * sepy);
+        // (the hardware square root, within an ulp, where WGSL allows 2.5)
+        const float len = __builtin_amdgcn_sqrtf(sepx * sepx + sepy * sepy);
         if (finitef(len) && len > 1e-8f) {
-            float nrx = sepx / len, nry = sepy / len;
-            const float wd = f_wm / f_wv, od = f_om / f_ov;
+            float nrx = quot(sepx, len), nry = quot(sepy, len);
+            const float wd = quot(f_wm, f_wv), od = quot(f_om, f_ov);
             const float dscale = fmaxf(wd, od);
-            const float contrast = (od - wd) / dscale;
-            const float gmag = sqrtf(P.gx * P.gx + P.gy * P.gy);
+            const float contrast = quot(od - wd, dscale);
+            const float gmag = __builtin_amdgcn_sqrtf(P.gx * P.gx + P.gy * P.gy);
             float align = 0.0f;
             if (!finitef(wd) || wd <= 0.0f || !finitef(od) || od <= 0.0f || !finitef(contrast) || !finitef(gmag)) {
                 flags |= ERR_GRID;
             } else {
-                if (gmag > 0.0f) align = (nrx * P.gx + nry * P.gy) / gmag;
+                if (gmag > 0.0f) align = quot(nrx * P.gx + nry * P.gy, gmag);
                 if (!finitef(align)) flags |= ERR_GRID;
                 if (contrast * align < -LIQUID_STABLE_ORDER_EPSILON) { nrx = -nrx; nry = -nry; }
                 const float rel = (vo.x - vw.x) * nrx + (vo.y - vw.y) * nry;
-                const float overlap = fminf(fmaxf(2.0f * fminf(f_wv, f_ov) / (f_wv + f_ov), 0.0f), 1.0f);
+                const float overlap = fminf(fmaxf(quot(2.0f * fminf(f_wv, f_ov), f_wv + f_ov), 0.0f), 1.0f);
                 const float minsep =
-                    fmaxf(LIQUID_CONTACT_SEPARATION_FRACTION * overlap * P.dx / LIQUID_CONTACT_RELAXATION_SECONDS,
+                    fmaxf(quot(LIQUID_CONTACT_SEPARATION_FRACTION * overlap * P.dx, LIQUID_CONTACT_RELAXATION_SECONDS),
                           fmaxf(-rel, 0.0f) * LIQUID_CONTACT_APPROACH_SPEED_MULTIPLIER);
                 if (rel < minsep) {
-                    const float imp = (minsep - rel) / (1.0f / f_wm + 1.0f / f_om);
+                    const float iwm = quot(1.0f, f_wm), iom = quot(1.0f, f_om);
+                    const float imp = quot(minsep - rel, iwm + iom);
                     const float ix = imp * nrx, iy = imp * nry;
-                    vw.x = vw.x - ix / f_wm; vw.y = vw.y - iy / f_wm;
-                    vo.x = vo.x + ix / f_om; vo.y = vo.y + iy / f_om;
+                    vw.x = vw.x - ix * iwm; vw.y = vw.y - iy * iwm;
+                    vo.x = vo.x + ix * iom; vo.y = vo.y + iy * iom;
                 }
             }
         }
@@ -687,13 +700,19 @@ __device__ __forceinline__ uint32_t update_node(const Params& P, const Grid& gri
     flags |= tri(node, f_wm, f_om, vw, vo);
     if (!finite2(vs) || !finite2(vw) || !finite2(vo)) return flags | ERR_GRID | ERR_TRIANGLE_CONTACT;
     const int cxn = (int)P.cells_x, cyn = (int)P.cells_y;
-    vs.x = wall_cushion(vs.x, nx, cxn); vs.y = wall_cushion(vs.y, ny, cyn);
-    vw.x = wall_cushion(vw.x, nx, cxn); vw.y = wall_cushion(vw.y, ny, cyn);
-    vo.x = wall_cushion(vo.x, nx, cxn); vo.y = wall_cushion(vo.y, ny, cyn);
-    const float fr = P.wall_friction;
-    vs.x = wall_friction(vs.x, ny, cyn, fr); vs.y = wall_friction(vs.y, nx, cxn, fr);
-    vw.x = wall_friction(vw.x, ny, cyn, fr); vw.y = wall_friction(vw.y, nx, cxn, fr);
-    vo.x = wall_friction(vo.x, ny, cyn, fr); vo.y = wall_friction(vo.y, nx, cxn, fr);
+    // A node at least a cushion away from every wall keeps its velocities:
+    // each cushion and friction distance below is then >= its cushion.
+    const float cush_x = fminf(WALL_CUSHION_CELLS, (float)cxn / 4.0f);
+    const float cush_y = fminf(WALL_CUSHION_CELLS, (float)cyn / 4.0f);
+    if (!((float)nx >= cush_x && (float)(cxn - nx) >= cush_x && (float)ny >= cush_y && (float)(cyn - ny) >= cush_y)) {
+        vs.x = wall_cushion(vs.x, nx, cxn); vs.y = wall_cushion(vs.y, ny, cyn);
+        vw.x = wall_cushion(vw.x, nx, cxn); vw.y = wall_cushion(vw.y, ny, cyn);
+        vo.x = wall_cushion(vo.x, nx, cxn); vo.y = wall_cushion(vo.y, ny, cyn);
+        const float fr = P.wall_friction;
+        vs.x = wall_friction(vs.x, ny, cyn, fr); vs.y = wall_friction(vs.y, nx, cxn, fr);
+        vw.x = wall_friction(vw.x, ny, cyn, fr); vw.y = wall_friction(vw.y, nx, cxn, fr);
+        vo.x = wall_friction(vo.x, ny, cyn, fr); vo.y = wall_friction(vo.y, nx, cxn, fr);
+    }
     if (scene_flags & SCENE_SOLID) grid.v_solid[node] = vs;
     if (scene_flags & SCENE_WATER) grid.v_water[node] = vw;
     if (scene_flags & SCENE_OIL) grid.v_oil[node] = vo;
@@ -939,9 +958,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     // consecutive 16-byte records, which keeps the banks apart)
     __shared__ float4 s_r0[G_CAP];     // stencil fraction fx, fy; momentum m vx, m vy
     __shared__ float4 s_r1[G_CAP];     // affine A (row-major)
-    __shared__ float2 s_r2[G_CAP];     // mass, volume x det F
-    __shared__ uint32_t s_tag[G_CAP];  // region column | phase << 8; 0xffff = contributes nothing
-    __shared__ uint32_t s_pid[G_CAP];  // particle indices at provisional places
+    // mass, volume x det F, the tag (region column | phase << 8, 0xffff =
+    // contributes nothing) as bits; the fourth word holds the particle
+    // indices at their provisional places, which the ranking reads while the
+    // first three words are written
+    __shared__ float4 s_r2[G_CAP];
+    uint32_t* const s_pid = reinterpret_cast<uint32_t*>(s_r2) + 3;  // stride 4 words
 
     stage_materials(s_mat, mats, nmat);
     if (!p2g_head<T, OCC>(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
@@ -1025,7 +1047,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
         // (C) particle indices at their provisional places
 #pragma unroll
         for (int k = 0; k < PER; ++k)
-            if (cell[k] != 0xffffffffu) s_pid[s_cell[cell[k] >> 16] + (cell[k] & 0xffffu)] = pid[k];
+            if (cell[k] != 0xffffffffu) s_pid[4u * (s_cell[cell[k] >> 16] + (cell[k] & 0xffffu))] = pid[k];
         __syncthreads();
         // (D) rank by particle index inside the cell (the summation order),
         // then stage the prepared affine (prepareParticle, shaders.ts:808-963)
@@ -1049,7 +1071,7 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             const uint32_t rc = cell[k] >> 16;
             const uint32_t q0 = s_cell[rc], q1 = s_cell[rc + 1];
             uint32_t q = q0;
-            for (uint32_t t = q0; t < q1; ++t) q += s_pid[t] < pid[k] ? 1u : 0u;
+            for (uint32_t t = q0; t < q1; ++t) q += s_pid[4u * t] < pid[k] ? 1u : 0u;
             Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
             make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, pos, sc);
             const Material mt = material_at(s_mat, mats, mat);
@@ -1077,8 +1099,8 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             // is then m v + (A dx) d with d = offset - fraction, two fused
             // multiply-adds per component
             s_r1[q] = make_float4(A.x * P.dx, A.y * P.dx, A.z * P.dx, A.w * P.dx);
-            s_r2[q] = make_float2(mt.mass, mt.volume * det);
-            s_tag[q] = pf ? 0xffffu : ((rc % (uint32_t)T::RC) | (mt.phase << 8));
+            const uint32_t tag = pf ? 0xffffu : ((rc % (uint32_t)T::RC) | (mt.phase << 8));
+            *reinterpret_cast<float3*>(&s_r2[q]) = make_float3(mt.mass, mt.volume * det, __uint_as_float(tag));
         }
         __syncthreads();
         // (E) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region.
@@ -1095,21 +1117,20 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
             const uint32_t n01 = n[0] + n[1], total = n01 + n[2];
             // pair k lies in row range dy = (k >= n[0]) + (k >= n01), at
             // q = k + base[dy]; the choices below are selects, so lanes at
-            // different rows or columns do not diverge, and the four LDS reads
-            // of a pair are issued together
+            // different rows or columns do not diverge, and the three LDS reads
+            // of a pair are issued together (one index scales to all three)
             const uint32_t base0 = q0[0], base1 = q0[1] - n[0], base2 = q0[2] - n01;
 #pragma unroll 1
             for (uint32_t k = 0; k < total; ++k) {
                 const bool row2 = k < n[0], row1 = !row2 && k < n01;  // oy = 2, oy = 1
                 const uint32_t q = k + (row2 ? base0 : (row1 ? base1 : base2));
-                const uint32_t tag = s_tag[q];
-                const float4 r0 = s_r0[q], r1 = s_r1[q];
-                const float2 r2 = s_r2[q];
-                // the empty asm takes one word of each read, so the four reads
+                const float4 r0 = s_r0[q], r1 = s_r1[q], r2 = s_r2[q];
+                const uint32_t tag = __float_as_uint(r2.z);
+                // the empty asm takes one word of each read, so the three reads
                 // are issued back to back and waited for once (left alone, the
                 // scheduler waits for the tag before issuing the others, and
                 // defers the mass and volume into the phase branches)
-                asm volatile("" ::"v"(tag), "v"(r0.x), "v"(r1.x), "v"(r2.x));
+                asm volatile("" ::"v"(r0.x), "v"(r1.x), "v"(r2.x));
                 const uint32_t ph = tag >> 8;  // 0xff: the particle contributes nothing
                 const int ox = lx + 3 - (int)(tag & 0xffu);
                 // d = offset - fraction per axis, the reference's dpos / dx
