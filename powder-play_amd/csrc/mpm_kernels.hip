@@ -1561,10 +1561,11 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
             const float wgx = w * g.x, wgy = w * g.y;
             vx = vx + wgx;
             vy = vy + wgy;
-            a0 = a0 + wgx * dpx[ox];
-            a1 = a1 + wgx * dpy[oy];
-            a2 = a2 + wgy * dpx[ox];
-            a3 = a3 + wgy * dpy[oy];
+            // fused: one rounding per term where the WGSL sum may take two
+            a0 = fmaf(wgx, dpx[ox], a0);
+            a1 = fmaf(wgx, dpy[oy], a1);
+            a2 = fmaf(wgy, dpx[ox], a2);
+            a3 = fmaf(wgy, dpy[oy], a3);
         }
     }
     if (liquid) {
@@ -1574,8 +1575,8 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
             for (int ox = 0; ox < 3; ++ox) {
                 const float w = wx[ox] * wy[oy];
                 const float2 lq = llq[row0 + oy * TN + ox];
-                pm = pm + w * lq.x;
-                pvol = pvol + w * lq.y;
+                pm = fmaf(w, lq.x, pm);
+                pvol = fmaf(w, lq.y, pvol);
             }
         }
     }
@@ -1585,9 +1586,10 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
             raise_step_at(ctl, ERR_G2P, step, pid);
             return false;
         }
-        const float pd = mt.density / pdet;
-        const float ld = pm / pvol;
-        const float rel = (pd - ld) / pd;
+        // (quotients of finite positive values: quot, within an ulp)
+        const float pd = quot(mt.density, pdet);
+        const float ld = quot(pm, pvol);
+        const float rel = quot(pd - ld, pd);
         vx = vx + P.dt * (rel * P.gx);
         vy = vy + P.dt * (rel * P.gy);
         if (!finitef(pd) || pd <= 0.0f || !finitef(ld) || ld <= 0.0f || !finitef(rel) || !finitef(vx) ||
