@@ -255,6 +255,14 @@ const char* mpm_batch_last_error(mpm_batch* batch);
  *                    per wavefront = the tile's particles / (g2p_threads/64).
  *   bucket_capacity  slots per bucket: 0 keeps the current capacity, else a
  *                    multiple of 16 in [32, 1024]; it still grows on demand.
+ *   tile_queue       0 (automatic), 1 (on) or 2 (off).  On: a compaction
+ *                    kernel lists each microstep the tiles that hold
+ *                    particles, and P2G and G2P run as the blocks the device
+ *                    holds at once, taking tiles from those lists, so the
+ *                    empty tiles of a sparse grid cost no blocks.
+ *                    Automatic: on when the grid nodes (of a batch, all of
+ *                    them) exceed twice the particles.  Never used for a
+ *                    retained fixed-point grid or bonded bodies.
  * mpm_set_tuning: a shape that is not compiled, or a bad capacity, returns
  *   MPM_EINVAL with no state change; the scene of a batch takes the batch's
  *   shape (mpm_batch_set_tuning) and returns MPM_EINVAL for another shape.
@@ -270,6 +278,7 @@ typedef struct {
     uint32_t g2p_tile;
     uint32_t g2p_threads;
     uint32_t bucket_capacity;
+    uint32_t tile_queue;
 } mpm_tuning;
 int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* tuning);
 int mpm_get_tuning(mpm_ctx* ctx, mpm_tuning* out);

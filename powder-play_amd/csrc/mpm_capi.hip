@@ -32,6 +32,13 @@ thread_local std::string g_create_error = "no error";
 uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
+// mpm_tuning.tile_queue: 1 on, 2 off, 0 on when most tiles of the grid hold
+// no particle (more than two grid nodes per particle).
+bool queue_wanted(uint32_t mode, uint64_t nodes, uint64_t particles) {
+    if (mode != 0u) return mode == 1u;
+    return nodes > 2u * particles;
+}
+
 struct DeviceArrays {
     std::vector<void*> ptrs;
     hipError_t alloc(void** p, size_t bytes) {
@@ -77,6 +84,9 @@ struct mpm_ctx {
     SceneDesc* d_scene = nullptr;     // device: this context as a one-scene step launch
     uint32_t* d_p2g_map = nullptr;    // device: tile -> scene (all 0; a one-scene launch does not read it)
     uint32_t* d_g2p_map = nullptr;
+    uint32_t* d_queue = nullptr;      // device: tile queue counters and lists (sized for the most tiles)
+    uint32_t queue_p2g = 0, queue_g2p = 0;  // list entries it holds for each kind
+    uint32_t tile_queue = 0;          // mpm_tuning.tile_queue
     Shape shape = DEFAULT_SHAPE;      // requested launch shape (mpm_set_tuning)
     hipEvent_t ring_events[PARAM_RING] = {};
     bool ring_used[PARAM_RING] = {};
@@ -128,6 +138,8 @@ struct mpm_batch {
     SceneDesc* h_scenes = nullptr;   // pinned staging
     uint32_t* d_p2g_map = nullptr;   // device: scene of every tile
     uint32_t* d_g2p_map = nullptr;
+    uint32_t* d_queue = nullptr;     // device: tile queue counters and lists of the current tile maps
+    uint32_t tile_queue = 0;         // mpm_tuning.tile_queue
     hipEvent_t staged = nullptr;     // the staging buffer's last copy
     bool staged_used = false;
     std::string error = "no error";
@@ -560,6 +572,9 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     sb.aux_bodies = c->lc.aux.nbodies;
     sb.aux_bonds = c->lc.aux.nbonds;
     sb.node_count = c->lc.node_count;
+    if (!c->lc.grid.fixed && queue_wanted(c->tile_queue, c->lc.node_count, c->n))
+        sb.queue = {c->d_queue, {c->d_queue + 2 * QUEUE_SET_WORDS, c->d_queue + 2 * QUEUE_SET_WORDS + c->queue_p2g},
+                    {sb.p2g_tiles, sb.g2p_tiles}};
     c->stepped = true;
     HIP_TRY(c, launch_steps(sb, steps, measure));
     c->pending.push_back({p, steps, measure, seq, done_before});
@@ -798,6 +813,9 @@ int mpm_create(const mpm_config* cfg, const uint32_t* packed20, const uint32_t* 
     const uint32_t max_g2p = g2p_tiles_for(lc.bw, lc.bh, 4u);
     ALLOC(ctx->d_p2g_map, sizeof(uint32_t) * max_p2g);
     ALLOC(ctx->d_g2p_map, sizeof(uint32_t) * max_g2p);
+    ALLOC(ctx->d_queue, sizeof(uint32_t) * (2 * QUEUE_SET_WORDS + max_p2g + max_g2p));
+    ctx->queue_p2g = max_p2g;
+    ctx->queue_g2p = max_g2p;
 #undef ALLOC
     if ((e = hipMemset(ctx->d_p2g_map, 0, sizeof(uint32_t) * max_p2g)) != hipSuccess ||
         (e = hipMemset(ctx->d_g2p_map, 0, sizeof(uint32_t) * max_g2p)) != hipSuccess)
@@ -1326,6 +1344,10 @@ bool tuning_valid(const mpm_tuning* t, std::string& why) {
               std::to_string(t->g2p_tile) + ", g2p_threads " + std::to_string(t->g2p_threads);
         return false;
     }
+    if (t->tile_queue > 2u) {
+        why = "tile_queue must be 0 (automatic), 1 (on) or 2 (off)";
+        return false;
+    }
     const uint32_t cap = t->bucket_capacity;
     if (cap != 0u && (cap % 16u != 0u || cap < MIN_BUCKET_CAP || cap > MAX_BUCKET_CAP)) {
         why = "bucket_capacity must be 0 or a multiple of 16 in [" + std::to_string(MIN_BUCKET_CAP) + ", " +
@@ -1348,11 +1370,12 @@ int set_bucket_capacity(mpm_ctx* c, uint32_t cap) {
     return rebucket_all(c, cap);
 }
 
-void tuning_of(const Shape& s, uint32_t cap, mpm_tuning* out) {
+void tuning_of(const Shape& s, uint32_t cap, uint32_t queue, mpm_tuning* out) {
     out->p2g_tile = s.p2g_tile;
     out->g2p_tile = s.g2p_tile;
     out->g2p_threads = s.g2p_threads;
     out->bucket_capacity = cap;
+    out->tile_queue = queue;
 }
 
 }  // namespace
@@ -1374,13 +1397,15 @@ int mpm_set_tuning(mpm_ctx* c, const mpm_tuning* t) {
     const int r = set_bucket_capacity(c, t->bucket_capacity);
     if (r) return r;
     c->shape = s;
+    c->tile_queue = t->tile_queue;
     return MPM_OK;
 }
 
 int mpm_get_tuning(mpm_ctx* c, mpm_tuning* out) {
     if (!c) return fail(nullptr, MPM_EINVAL, "context must not be NULL");
     if (!out) return fail(c, MPM_EINVAL, "out must not be NULL");
-    tuning_of(c->owner ? batch_shape(c->owner) : effective_shape(c, c->shape), c->lc.cap, out);
+    tuning_of(c->owner ? batch_shape(c->owner) : effective_shape(c, c->shape), c->lc.cap,
+              c->owner ? c->owner->tile_queue : c->tile_queue, out);
     return MPM_OK;
 }
 
@@ -1504,6 +1529,7 @@ void mpm_batch_destroy(mpm_batch* b) {
     if (b->d_scenes) (void)hipFree(b->d_scenes);
     if (b->d_p2g_map) (void)hipFree(b->d_p2g_map);
     if (b->d_g2p_map) (void)hipFree(b->d_g2p_map);
+    if (b->d_queue) (void)hipFree(b->d_queue);
     if (b->h_scenes) (void)hipHostFree(b->h_scenes);
     if (b->staged) (void)hipEventDestroy(b->staged);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1556,6 +1582,11 @@ int batch_maps(mpm_batch* b, const Shape& shape) {
             return batch_fail(b, MPM_EDEVICE, "batch tile maps: %s", hipGetErrorString(e));
         b->map_capacity = need;
     }
+    if (b->d_queue) (void)hipFree(b->d_queue);
+    b->d_queue = nullptr;
+    if ((e = hipMalloc((void**)&b->d_queue, sizeof(uint32_t) * (2 * QUEUE_SET_WORDS + b->p2g_tiles + b->g2p_tiles))) !=
+        hipSuccess)
+        return batch_fail(b, MPM_EDEVICE, "batch tile queue: %s", hipGetErrorString(e));
     if ((e = hipMemcpy(b->d_p2g_map, pmap.data(), sizeof(uint32_t) * pmap.size(), hipMemcpyHostToDevice)) !=
             hipSuccess ||
         (e = hipMemcpy(b->d_g2p_map, gmap.data(), sizeof(uint32_t) * gmap.size(), hipMemcpyHostToDevice)) !=
@@ -1711,6 +1742,16 @@ int mpm_batch_step(mpm_batch* b, const void* params, uint32_t step_count, uint32
         sb.scene_flags |= c->lc.scene_flags;
         sb.diag = sb.diag || c->lc.diag != nullptr;
     }
+    uint64_t nodes = 0, particles = 0;
+    bool fixed_grid = false;
+    for (const mpm_ctx* c : b->scenes) {
+        nodes += c->lc.node_count;
+        particles += c->n;
+        fixed_grid = fixed_grid || c->lc.grid.fixed != nullptr;
+    }
+    if (!fixed_grid && queue_wanted(b->tile_queue, nodes, particles))
+        sb.queue = {b->d_queue, {b->d_queue + 2 * QUEUE_SET_WORDS, b->d_queue + 2 * QUEUE_SET_WORDS + b->p2g_tiles},
+                    {b->p2g_tiles, b->g2p_tiles}};
     hipError_t e;
     if ((e = hipMemcpyAsync(b->d_scenes, b->h_scenes, sizeof(SceneDesc) * n, hipMemcpyHostToDevice, b->stream)) !=
             hipSuccess ||
@@ -1733,6 +1774,7 @@ int mpm_batch_set_tuning(mpm_batch* b, const mpm_tuning* t) {
         if (r) return batch_fail(b, r, "scene %u: %s", i, b->scenes[i]->error.c_str());
     }
     b->shape = {t->p2g_tile, t->g2p_tile, t->g2p_threads};
+    b->tile_queue = t->tile_queue;
     return MPM_OK;
 }
 
@@ -1741,7 +1783,7 @@ int mpm_batch_get_tuning(mpm_batch* b, mpm_tuning* out) {
     if (!out) return batch_fail(b, MPM_EINVAL, "out must not be NULL");
     uint32_t cap = 0;
     for (const mpm_ctx* c : b->scenes) cap = std::max(cap, c->lc.cap);
-    tuning_of(batch_shape(b), cap, out);
+    tuning_of(batch_shape(b), cap, b->tile_queue, out);
     return MPM_OK;
 }
 

@@ -279,6 +279,29 @@ struct SceneDesc {
     uint32_t start_buf;           // slot buffer holding the state the batch starts from
 };
 
+// Active-tile queues of a step launch.  When `sets` is set, a compaction
+// kernel lists each microstep the P2G (kind 0) and G2P (kind 1) tiles that
+// hold particles, and the step kernels run as a fixed number of blocks that
+// share out those lists; otherwise every tile gets a block.  A list holds
+// the active tiles of an XCD's range of the launch (the ranges of xcd_tile)
+// from that range's start.  Two sets of counters, fill[kind][xcd] (tiles
+// listed), alternate by microstep parity.
+constexpr uint32_t QUEUE_XCDS = 8;
+constexpr uint32_t QUEUE_SET_WORDS = 2 * QUEUE_XCDS;
+struct TileQueue {
+    uint32_t* sets;      // device, 2 x QUEUE_SET_WORDS; nullptr: one block per tile
+    uint32_t* list[2];   // device: tiles[0] and tiles[1] entries
+    uint32_t tiles[2];   // tiles of the P2G and G2P launches
+};
+__host__ __device__ inline uint32_t xcd_range_start(uint32_t tiles, uint32_t x) {
+    const uint32_t per = tiles / QUEUE_XCDS, extra = tiles % QUEUE_XCDS;
+    return x * per + (x < extra ? x : extra);
+}
+__host__ __device__ inline uint32_t xcd_of_tile(uint32_t tiles, uint32_t t) {
+    const uint32_t per = tiles / QUEUE_XCDS, extra = tiles % QUEUE_XCDS, big = extra * (per + 1u);
+    return t < big ? t / (per + 1u) : extra + (t - big) / per;  // per > 0 whenever t >= big
+}
+
 // Host description of one step launch over `nscenes` scenes that share a
 // stream (each starts from its own slot buffer).
 struct StepBatch {
@@ -297,6 +320,7 @@ struct StepBatch {
     Shape shape;                     // launch shape (28-node P2G, 256-thread G2P for SCENE_AUX)
     bool diag;                   // some scene has diagnostics on
     KernelTimes* timing;
+    TileQueue queue;             // sets == nullptr: one block per tile
 };
 
 hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure);

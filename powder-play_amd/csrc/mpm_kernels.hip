@@ -325,10 +325,6 @@ __device__ __forceinline__ float qweight(float f, int o) {
     return o == 0 ? w0 : (o == 1 ? w1 : w2);
 }
 
-// qweight with the component chosen by bit masks, without the branches a
-// select chain may compile to.  The outer weights share one form, (0.5 t) t
-// with t = 1.5 - f or f - 0.5, so one product serves both and every value
-// is rounded exactly as in qweight.
 // The quadratic B-spline at d = o - f, o the stencil offset and f in
 // [0.5, 1.5) the fraction: 0.75 - d^2 at the middle offset, else
 // (1.5 - |d|)^2 / 2.  For o = 0 and 1 this is bit for bit quadraticWeights
@@ -342,14 +338,6 @@ __device__ __forceinline__ float bspline_of_offset(float d, bool middle) {
     const float t = 1.5f - fabsf(d);
     const float mid = 0.75f - d * d, side = (0.5f * t) * t;
     return __uint_as_float((__float_as_uint(mid) & m) | (__float_as_uint(side) & ~m));
-}
-
-__device__ __forceinline__ float qweight_masked(float f, int o) {
-    const uint32_t m0 = 0u - (uint32_t)(o == 0), m1 = 0u - (uint32_t)(o == 1);
-    const float t = __uint_as_float((__float_as_uint(1.5f - f) & m0) | (__float_as_uint(f - 0.5f) & ~m0));
-    const float c = f - 1.0f;
-    const float wt = (0.5f * t) * t, w1 = 0.75f - c * c;
-    return __uint_as_float((__float_as_uint(w1) & m1) | (__float_as_uint(wt) & ~m1));
 }
 
 // Raise status bits from microstep `step` of the batch; the batch commits
@@ -398,7 +386,7 @@ __device__ __forceinline__ uint32_t bucket_of(const Stencil& s, uint32_t bw) {
 // XCDs, each with its own L2; handing every XCD a contiguous band of tiles
 // (in tile order) keeps neighbouring tiles, which re-read each other's halo
 // particles and nodes, on the same L2.
-constexpr uint32_t XCDS = 8;
+constexpr uint32_t XCDS = QUEUE_XCDS;
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t tiles) {
     const uint32_t b = blockIdx.x, x = b % XCDS, k = b / XCDS;
     const uint32_t per = tiles / XCDS, extra = tiles % XCDS;
@@ -776,10 +764,43 @@ __device__ __forceinline__ uint32_t fluid_triangle_node(const Params& P, const A
 // are spread over the XCDs in bands of consecutive global tiles (xcd_tile),
 // and the tile -> scene map names the scene of each global tile (a
 // one-scene launch skips the lookup).
-#define SCENE_PROLOGUE(MAP, BASE)                                                    \
-    const uint32_t global_tile_ = xcd_tile(gridDim.x);                               \
+#define SCENE_AT(MAP, BASE, G)                                                       \
+    const uint32_t global_tile_ = (G);                                               \
     const SceneDesc& S = scenes[nscenes == 1u ? 0u : (MAP)[global_tile_]];           \
     const uint32_t tile = global_tile_ - S.BASE
+
+// The tiles of a step kernel's block: without a queue the one tile of the
+// block (XCD-aware order); with one, the block's share of the active-tile
+// list of its XCD: block k of the K blocks an XCD runs (the launch has 8K
+// blocks, dispatched to the XCDs in turn) takes entries k, k + K, ... of that
+// XCD's list, which holds the active tiles of the XCD's range of the launch,
+// so the XCD keeps the tiles (and the halos they share) the one-block-a-tile
+// launch would give it.  No atomics: every value is uniform over the block.
+template <class Body>
+__device__ __forceinline__ void run_tiles(const TileQueue& q, uint32_t kind, uint32_t step, Body body) {
+    if (q.sets == nullptr) {
+        body(xcd_tile(gridDim.x));
+        return;
+    }
+    const uint32_t x = blockIdx.x % QUEUE_XCDS, k = blockIdx.x / QUEUE_XCDS, K = gridDim.x / QUEUE_XCDS;
+    const uint32_t n = q.sets[(step & 1u) * QUEUE_SET_WORDS + kind * QUEUE_XCDS + x];
+    const uint32_t* list = q.list[kind] + xcd_range_start(q.tiles[kind], x);
+    for (uint32_t i = k; i < n; i += K) body(list[i]);
+}
+
+// Stress-cache failures of the state a microstep starts from, raised once a
+// microstep by the scene's first P2G tile (in a batch every scene has one),
+// and the deferred words of the microstep's output cleared.
+__device__ __forceinline__ void scene_deferred_flags(Control* ctl, int in_buf, uint32_t step) {
+    const int out_buf = in_buf ^ 1;
+    const uint32_t d = ctl->deferred[in_buf];
+    if (d) {
+        if (d & ERR_P2G) atomicMin(&ctl->status[1], ctl->deferred_index[in_buf]);
+        raise_step(ctl, d, step);
+    }
+    ctl->deferred[out_buf] = 0u;
+    ctl->deferred_index[out_buf] = 0xffffffffu;
+}
 
 // Shared head of both P2G kernels: status gate, deferred stress-cache flags,
 // clearing the next bucket counts this tile owns, and the staged offsets of
@@ -790,22 +811,11 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
                                          uint32_t tiles_x, uint32_t bw, uint32_t bh, uint32_t cap, uint32_t step,
                                          int in_buf, uint32_t* s_boff) {
     const int tid = threadIdx.x;
-    const int out_buf = in_buf ^ 1;
     // After a failed microstep the output buffer of this one holds the
     // committed state: leave its counts and deferred flags untouched.
     if (block_status_set(ctl)) return false;
     // the occupancy gather only reads the committed state
-    if (!OCC && tile == 0u && tid == 0) {
-        // stress-cache failures of the state this microstep starts from (the
-        // scene's first tile; in a batch every scene has one)
-        const uint32_t d = ctl->deferred[in_buf];
-        if (d) {
-            if (d & ERR_P2G) atomicMin(&ctl->status[1], ctl->deferred_index[in_buf]);
-            raise_step(ctl, d, step);
-        }
-        ctl->deferred[out_buf] = 0u;
-        ctl->deferred_index[out_buf] = 0xffffffffu;
-    }
+    if (!OCC && tile == 0u && tid == 0) scene_deferred_flags(ctl, in_buf, step);
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     // this tile owns buckets [OWN tx, OWN tx + OWN) x [OWN ty, OWN ty + OWN): clear their next counts
     if (!OCC && tid < T::OWN * T::OWN) {
@@ -930,14 +940,13 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 // (water, oil) = 2 (sum w V det F) / dx^2, the display area of the liquid
 // phases over the cell area.  The reference scales each contribution,
 // ((V det F) 2) w / dx^2; here the sum is scaled once.
-template <int TI, bool GENERAL, bool AUX = false, bool OCC = false>
-__global__ __launch_bounds__(P2GTile<TI>::THREADS)
-__attribute__((amdgpu_waves_per_eu(P2GTile<TI>::GATHER_WAVES_PER_EU, 8)))
-void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
-                  uint32_t step) {
+template <int TI, bool GENERAL, bool AUX, bool OCC>
+__device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ scenes,
+                                                const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                                                uint32_t step, uint32_t global_tile) {
     using T = P2GTile<TI>;
     constexpr int G_CAP = T::CAP;
-    SCENE_PROLOGUE(tile_scene, p2g_base);
+    SCENE_AT(tile_scene, p2g_base, global_tile);
     const int in_buf = (int)((S.start_buf + step) & 1u);
     const Params* __restrict__ Pp = &S.P;
     Control* ctl = S.ctl;
@@ -1200,6 +1209,15 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
     if (flags) raise_step(ctl, flags, step);
 }
 
+
+template <int TI, bool GENERAL, bool AUX = false, bool OCC = false>
+__global__ __launch_bounds__(P2GTile<TI>::THREADS)
+__attribute__((amdgpu_waves_per_eu(P2GTile<TI>::GATHER_WAVES_PER_EU, 8)))
+void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                  uint32_t step, TileQueue q) {
+    run_tiles(q, 0u, step, [&](uint32_t t) { p2g_gather_tile<TI, GENERAL, AUX, OCC>(scenes, tile_scene, nscenes, step, t); });
+}
+
 // Fixed-point P2G (prepareAndScatterParticles + gatherAtomicGrid,
 // shaders.ts:971-1070) fused with updateGrid.  Same tiles as the float
 // gather, but every particle of the region adds its rounded int32
@@ -1207,13 +1225,12 @@ void k_p2g_gather(const SceneDesc* __restrict__ scenes, const uint32_t* __restri
 // integer atomics: integer sums do not depend on the order, and an exact sum
 // outside int32 is the accumulator overflow of 65536.  Scenes without solids
 // keep only the eight liquid channels, so two tiles fit in one CU's LDS.
-template <int TI, bool GENERAL, bool SOLIDS, bool AUX = false>
-__global__ __launch_bounds__(P2GTile<TI>::THREADS)
-__attribute__((amdgpu_waves_per_eu((TI == 20 && SOLIDS) ? 6 : 8, 8)))  // LDS: 3 x 8 waves a CU for 20 with solids
-void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
-                 uint32_t step) {
+template <int TI, bool GENERAL, bool SOLIDS, bool AUX>
+__device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ scenes,
+                                               const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                                               uint32_t step, uint32_t global_tile) {
     using T = P2GTile<TI>;
-    SCENE_PROLOGUE(tile_scene, p2g_base);
+    SCENE_AT(tile_scene, p2g_base, global_tile);
     const int in_buf = (int)((S.start_buf + step) & 1u);
     const Params* __restrict__ Pp = &S.P;
     Control* ctl = S.ctl;
@@ -1399,6 +1416,15 @@ void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restric
                        diag_grid + 3 * tile);
     }
     if (flags) raise_step(ctl, flags, step);
+}
+
+
+template <int TI, bool GENERAL, bool SOLIDS, bool AUX = false>
+__global__ __launch_bounds__(P2GTile<TI>::THREADS)
+__attribute__((amdgpu_waves_per_eu((TI == 20 && SOLIDS) ? 6 : 8, 8)))  // LDS: 3 x 8 waves a CU for 20 with solids
+void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+                 uint32_t step, TileQueue q) {
+    run_tiles(q, 0u, step, [&](uint32_t t) { p2g_fixed_tile<TI, GENERAL, SOLIDS, AUX>(scenes, tile_scene, nscenes, step, t); });
 }
 
 // -------------------------------------------------------------------- G2P
@@ -1747,16 +1773,13 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
 // current one is processed, so their latency overlaps the current G2P and
 // slot reservation.  Compiled shapes: 8x8 buckets with 128, 256 or 512
 // threads, 4x4 buckets with 128 or 256.
-template <int G_TB, int G_THREADS, bool GENERAL, bool AUX = false, bool SOLIDS = true>
-__global__ __launch_bounds__(G_THREADS)
-__attribute__((amdgpu_waves_per_eu(GENERAL && !SOLIDS ? 4 : 1, 8)))  // Tait liquids: 4 waves a SIMD
-void k_g2p(const SceneDesc* __restrict__ scenes,
-                                                 const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
-                                                 uint32_t step) {
+template <int G_TB, int G_THREADS, bool GENERAL, bool AUX, bool SOLIDS>
+__device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                                         uint32_t nscenes, uint32_t step, uint32_t global_tile) {
     using T = G2PTile<G_TB>;
     constexpr int G_TN = T::TN, NBK = T::NBK;
     static_assert(G_THREADS >= 64, "the first wave scans the bucket counts");
-    SCENE_PROLOGUE(tile_scene, g2p_base);
+    SCENE_AT(tile_scene, g2p_base, global_tile);
     const int out_buf = (int)((S.start_buf + step + 1u) & 1u);
     const Params* __restrict__ Pp = &S.P;
     Control* ctl = S.ctl;
@@ -1913,6 +1936,15 @@ void k_g2p(const SceneDesc* __restrict__ scenes,
         c_p = n_p; c_F = n_F; c_pid = n_pid; c_mat = n_mat; c_pv = n_pv; c_gbx = n_gbx; c_gby = n_gby;
     }
 #undef G2P_FETCH
+}
+
+
+template <int G_TB, int G_THREADS, bool GENERAL, bool AUX = false, bool SOLIDS = true>
+__global__ __launch_bounds__(G_THREADS)
+__attribute__((amdgpu_waves_per_eu(GENERAL && !SOLIDS ? 4 : 1, 8)))  // Tait liquids: 4 waves a SIMD
+void k_g2p(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
+           uint32_t step, TileQueue q) {
+    run_tiles(q, 1u, step, [&](uint32_t t) { g2p_tile<G_TB, G_THREADS, GENERAL, AUX, SOLIDS>(scenes, tile_scene, nscenes, step, t); });
 }
 
 // ------------------------------------------ config-2 bodies and fracture
@@ -2706,7 +2738,115 @@ inline uint32_t blocks_for(size_t n, uint32_t t) { return (uint32_t)((n + t - 1)
 
 // Kernel and block size of the step kernels for a launch shape.  Scenes
 // with bodies or bonds (AUX) run the default shape only.
-using StepKernel = void (*)(const SceneDesc*, const uint32_t*, uint32_t, uint32_t);
+// The counters of both microstep sets, at the start of a step launch.
+__global__ __launch_bounds__(64) void k_queue_reset(TileQueue q) {
+    for (uint32_t i = threadIdx.x; i < 2u * QUEUE_SET_WORDS; i += blockDim.x) q.sets[i] = 0u;
+}
+
+// Active-tile lists of one microstep, for step kernels that take their tiles
+// from them: a P2G tile is active when a bucket of its region holds a
+// particle, a G2P tile when one of its own buckets does (the tiles the step
+// kernels would otherwise end at once).  An inactive P2G tile's duties are
+// done here: clearing the next counts of the buckets it owns, the scene's
+// deferred stress-cache flags when it is the scene's first tile, and its
+// diagnostic grid sums.  Scenes whose status is set are passed over, as the
+// step kernels pass over them.  The next microstep's counters are cleared.
+// A list's order depends on the order of the atomic adds; the results do
+// not, since every tile computes the same sums whichever block runs it.
+template <int TI, int TB>
+__global__ __launch_bounds__(256) void k_compact_tiles(const SceneDesc* __restrict__ scenes,
+                                                       const uint32_t* __restrict__ p2g_scene,
+                                                       const uint32_t* __restrict__ g2p_scene, uint32_t nscenes,
+                                                       uint32_t step, TileQueue q) {
+    using T = P2GTile<TI>;
+    __shared__ uint32_t s_n[2 * QUEUE_XCDS];     // this block's tiles per list, then their offset in it
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < QUEUE_SET_WORDS) q.sets[((step + 1u) & 1u) * QUEUE_SET_WORDS + g] = 0u;
+    if (threadIdx.x < 2u * QUEUE_XCDS) s_n[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t kind = 0u, t = 0u, slot = 0xffffffffu;  // slot: list entry of an active tile
+    if (g < q.tiles[0] + q.tiles[1]) {
+        kind = g < q.tiles[0] ? 0u : 1u;
+        t = kind ? g - q.tiles[0] : g;
+        const SceneDesc& S = scenes[nscenes == 1u ? 0u : (kind ? g2p_scene : p2g_scene)[t]];
+        Control* ctl = S.ctl;
+        if (load_status(ctl) == 0u) {
+            const uint32_t tile = t - (kind ? S.g2p_base : S.p2g_base);
+            const int in_buf = (int)((S.start_buf + step) & 1u);
+            const uint32_t* cnt = S.buf[in_buf].cnt;
+            const int bw = (int)S.bw, bh = (int)S.bh;
+            uint32_t any = 0u;  // every count is loaded, then they are tested together
+            if (kind == 0u) {
+                const int tx = (int)(tile % S.ptx), ty = (int)(tile / S.ptx);
+                const int rbx0 = tx * T::OWN - 1, rby0 = ty * T::OWN - 1;
+#pragma unroll
+                for (int b = 0; b < T::NB; ++b) {
+                    const int gbx = rbx0 + b % T::RB, gby = rby0 + b / T::RB;
+                    if (gbx >= 0 && gby >= 0 && gbx < bw && gby < bh) any |= cnt[gby * bw + gbx];
+                }
+                if (any == 0u) {
+                    uint32_t* cnt_next = S.buf[in_buf ^ 1].cnt;
+                    for (int k = 0; k < T::OWN * T::OWN; ++k) {
+                        const int obx = T::OWN * tx + k % T::OWN, oby = T::OWN * ty + k / T::OWN;
+                        if (obx < bw && oby < bh) cnt_next[oby * bw + obx] = 0u;
+                    }
+                    if (tile == 0u) scene_deferred_flags(ctl, in_buf, step);
+                    if (S.P.diag_on) {
+                        double* dg = S.diag_grid;
+                        dg[3 * tile] = 0.0;
+                        dg[3 * tile + 1] = 0.0;
+                        dg[3 * tile + 2] = 0.0;
+                    }
+                }
+            } else {
+                const int tbx = (int)(tile % S.gtx) * TB, tby = (int)(tile / S.gtx) * TB;
+#pragma unroll
+                for (int b = 0; b < TB * TB; ++b) {
+                    const int gbx = tbx + b % TB, gby = tby + b / TB;
+                    if (gbx < bw && gby < bh) any |= cnt[gby * bw + gbx];
+                }
+            }
+            if (any != 0u) slot = atomicAdd(&s_n[kind * QUEUE_XCDS + xcd_of_tile(q.tiles[kind], t)], 1u);
+        }
+    }
+    __syncthreads();
+    // one add per list and block to the microstep's counters
+    uint32_t* set = q.sets + (step & 1u) * QUEUE_SET_WORDS;
+    if (threadIdx.x < 2u * QUEUE_XCDS) {
+        const uint32_t c = s_n[threadIdx.x];
+        s_n[threadIdx.x] = c != 0u ? atomicAdd(&set[threadIdx.x], c) : 0u;
+    }
+    __syncthreads();
+    if (slot != 0xffffffffu) {
+        const uint32_t x = xcd_of_tile(q.tiles[kind], t);
+        q.list[kind][xcd_range_start(q.tiles[kind], x) + s_n[kind * QUEUE_XCDS + x] + slot] = t;
+    }
+}
+
+using CompactKernel = void (*)(const SceneDesc*, const uint32_t*, const uint32_t*, uint32_t, uint32_t, TileQueue);
+CompactKernel pick_compact(const Shape& shape) {
+    const bool small = shape.g2p_tile == 4u;
+    switch (shape.p2g_tile) {
+        case 20u: return small ? k_compact_tiles<20, 4> : k_compact_tiles<20, 8>;
+        case 12u: return small ? k_compact_tiles<12, 4> : k_compact_tiles<12, 8>;
+        default: return small ? k_compact_tiles<28, 4> : k_compact_tiles<28, 8>;
+    }
+}
+
+// Blocks of a queued step launch: K for each of the 8 XCDs, K the device's
+// compute units (eight blocks a compute unit, more than any step kernel holds
+// at once) or, for a small launch, the largest XCD range of tiles, so that
+// each block then takes at most one tile.
+uint32_t queued_blocks(uint32_t tiles) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 0u;
+    const uint32_t range = (tiles + QUEUE_XCDS - 1u) / QUEUE_XCDS;
+    return QUEUE_XCDS * std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)cus, range));
+}
+
+using StepKernel = void (*)(const SceneDesc*, const uint32_t*, uint32_t, uint32_t, TileQueue);
 struct StepChoice {
     StepKernel fn;
     uint32_t threads;
@@ -2786,14 +2926,29 @@ hipError_t launch_steps(const StepBatch& sb, uint32_t steps, bool measure) {
     const StepChoice p2g = pick_p2g(sb, general, aux);
     const StepChoice g2p = pick_g2p(sb, general, aux);
     const char* p2g_name = sb.fixed_point ? "p2g_grid_fixed" : "p2g_grid";
+    // with a tile queue the step kernels run as the blocks the device holds
+    // at once and take the active tiles from the queue
+    const TileQueue q = aux ? TileQueue{} : sb.queue;
+    uint32_t p2g_blocks = sb.p2g_tiles, g2p_blocks = sb.g2p_tiles;
+    CompactKernel compact = nullptr;
+    if (q.sets) {
+        p2g_blocks = queued_blocks(sb.p2g_tiles);
+        g2p_blocks = queued_blocks(sb.g2p_tiles);
+        if (p2g_blocks == 0u || g2p_blocks == 0u) return hipErrorInvalidValue;
+        compact = pick_compact(sb.shape);
+        MPM_LAUNCH("queue_reset", k_queue_reset, dim3(1), dim3(64), 0, lc.stream, q);
+    }
     for (uint32_t s = 0; s < steps; ++s) {
         if (aux && sb.aux_bodies)
             MPM_LAUNCH("tri_field", k_tri_field, dim3(blocks_for(sb.node_count, FIELD_THREADS)), dim3(FIELD_THREADS),
                        0, lc.stream, sb.d_scenes, s);
-        MPM_LAUNCH(p2g_name, p2g.fn, dim3(sb.p2g_tiles), dim3(p2g.threads), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
-                   sb.nscenes, s);
-        MPM_LAUNCH("g2p", g2p.fn, dim3(sb.g2p_tiles), dim3(g2p.threads), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
-                   sb.nscenes, s);
+        if (compact)
+            MPM_LAUNCH("compact_tiles", compact, dim3(blocks_for(sb.p2g_tiles + sb.g2p_tiles, 256)), dim3(256), 0,
+                       lc.stream, sb.d_scenes, sb.p2g_scene, sb.g2p_scene, sb.nscenes, s, q);
+        MPM_LAUNCH(p2g_name, p2g.fn, dim3(p2g_blocks), dim3(p2g.threads), 0, lc.stream, sb.d_scenes, sb.p2g_scene,
+                   sb.nscenes, s, q);
+        MPM_LAUNCH("g2p", g2p.fn, dim3(g2p_blocks), dim3(g2p.threads), 0, lc.stream, sb.d_scenes, sb.g2p_scene,
+                   sb.nscenes, s, q);
         if (aux && sb.aux_bodies)
             MPM_LAUNCH("bodies", k_bodies, dim3(1), dim3(BODY_THREADS), 0, lc.stream, sb.d_scenes, s);
         if (aux && sb.aux_bonds)
@@ -2925,7 +3080,7 @@ hipError_t launch_occupancy(hipStream_t stream, const SceneDesc* d_scene, uint32
         KernelTimes* timing;
     } lc = {stream, timing};
     auto fn = general ? k_p2g_gather<28, true, false, true> : k_p2g_gather<28, false, false, true>;
-    MPM_LAUNCH("occupancy", fn, dim3(tiles), dim3(P2GTile<28>::THREADS), 0, lc.stream, d_scene, nullptr, 1u, 0u);
+    MPM_LAUNCH("occupancy", fn, dim3(tiles), dim3(P2GTile<28>::THREADS), 0, lc.stream, d_scene, nullptr, 1u, 0u, TileQueue{});
     return hipSuccess;
 }
 

@@ -311,7 +311,8 @@ class Context:
 
 class MpmTuning(ctypes.Structure):
     """``mpm_tuning``: launch shape of the step kernels and bucket capacity."""
-    _fields_ = [("p2g_tile", _U32), ("g2p_tile", _U32), ("g2p_threads", _U32), ("bucket_capacity", _U32)]
+    _fields_ = [("p2g_tile", _U32), ("g2p_tile", _U32), ("g2p_threads", _U32), ("bucket_capacity", _U32),
+                ("tile_queue", _U32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

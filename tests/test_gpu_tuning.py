@@ -177,7 +177,7 @@ def test_small_tile_falls_back_when_a_bucket_exceeds_its_chunk(pkg):
     s = pkg.create_mpm_mls_gpu_solver(particles, config, device=0)
     try:
         eff = pkg.autotune.apply(s, {"p2g_tile": 12, "g2p_tile": 4, "g2p_threads": 128, "bucket_capacity": 512})
-        assert eff == {"p2g_tile": 28, "g2p_tile": 4, "g2p_threads": 128, "bucket_capacity": 512}
+        assert eff == {"p2g_tile": 28, "g2p_tile": 4, "g2p_threads": 128, "bucket_capacity": 512, "tile_queue": 0}
         eff = pkg.autotune.apply(s, {"p2g_tile": 12, "g2p_tile": 4, "g2p_threads": 128, "bucket_capacity": 64})
         assert eff["p2g_tile"] == 12 and eff["bucket_capacity"] == 64
         s.step_batch(3, 1e-4, GRAVITY)
