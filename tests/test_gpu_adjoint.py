@@ -138,3 +138,19 @@ def test_particle_outside_grid_is_reported(pkg, block):
             r.backward(np.zeros_like(state, dtype=np.float32))
     finally:
         r.close()
+
+
+def test_gpu_gradients_match_central_differences_on_the_full_block(pkg, orc):
+    """The GPU adjoint of the full differentiable block (its own float32
+    forward and loss) against central differences of the FP64 forward at the
+    sampled components of tests/test_adjoint_c5.py: 1e-3 relative, 1e-8
+    absolute at the differences' noise floor."""
+    from test_adjoint_c5 import c5_fd_samples, pick
+    (state, lam, mu, config, target, _), samples = c5_fd_samples(pkg, orc)
+    final, g0, gl, gm, status, *_ = run_gpu(pkg, (state, lam, mu, config, target))
+    assert status & ~pkg.native.MPM_DIFF_CONFINED == 0
+    for label, fd, index in samples:
+        g = pick((g0.astype(np.float64), gl.astype(np.float64), gm.astype(np.float64)), index)
+        print(f"{label}: FD {fd:+.6e} GPU {g:+.6e}", flush=True)
+        assert g == pytest.approx(fd, rel=1e-3, abs=1e-8), label
+
