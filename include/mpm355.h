@@ -261,8 +261,9 @@ const char* mpm_batch_last_error(mpm_batch* batch);
  *                    holds at once, taking tiles from those lists, so the
  *                    empty tiles of a sparse grid cost no blocks.
  *                    Automatic: on when the grid nodes (of a batch, all of
- *                    them) exceed twice the particles.  Never used for a
- *                    retained fixed-point grid or bonded bodies.
+ *                    them) exceed twice the particles and there are at
+ *                    least four P2G tiles per compute unit.  Never used for
+ *                    a retained fixed-point grid or bonded bodies.
  * mpm_set_tuning: a shape that is not compiled, or a bad capacity, returns
  *   MPM_EINVAL with no state change; the scene of a batch takes the batch's
  *   shape (mpm_batch_set_tuning) and returns MPM_EINVAL for another shape.

@@ -33,10 +33,16 @@ uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
 // mpm_tuning.tile_queue: 1 on, 2 off, 0 on when most tiles of the grid hold
-// no particle (more than two grid nodes per particle).
-bool queue_wanted(uint32_t mode, uint64_t nodes, uint64_t particles) {
+// no particle (more than two grid nodes per particle) and there are more
+// tiles than the device runs at once (four P2G tiles per compute unit); a
+// smaller launch runs all its tiles in one round, so skipping the empty ones
+// saves less than the compaction launch costs.
+bool queue_wanted(uint32_t mode, uint64_t nodes, uint64_t particles, uint32_t p2g_tiles, int device) {
     if (mode != 0u) return mode == 1u;
-    return nodes > 2u * particles;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return false;
+    return nodes > 2u * particles && p2g_tiles >= 4u * (uint32_t)cus;
 }
 
 struct DeviceArrays {
@@ -572,7 +578,7 @@ int launch(mpm_ctx* c, const Params& p, uint32_t steps, bool measure, uint32_t d
     sb.aux_bodies = c->lc.aux.nbodies;
     sb.aux_bonds = c->lc.aux.nbonds;
     sb.node_count = c->lc.node_count;
-    if (!c->lc.grid.fixed && queue_wanted(c->tile_queue, c->lc.node_count, c->n))
+    if (!c->lc.grid.fixed && queue_wanted(c->tile_queue, c->lc.node_count, c->n, sb.p2g_tiles, c->device))
         sb.queue = {c->d_queue, {c->d_queue + 2 * QUEUE_SET_WORDS, c->d_queue + 2 * QUEUE_SET_WORDS + c->queue_p2g},
                     {sb.p2g_tiles, sb.g2p_tiles}};
     c->stepped = true;
@@ -1749,7 +1755,7 @@ int mpm_batch_step(mpm_batch* b, const void* params, uint32_t step_count, uint32
         particles += c->n;
         fixed_grid = fixed_grid || c->lc.grid.fixed != nullptr;
     }
-    if (!fixed_grid && queue_wanted(b->tile_queue, nodes, particles))
+    if (!fixed_grid && queue_wanted(b->tile_queue, nodes, particles, b->p2g_tiles, b->device))
         sb.queue = {b->d_queue, {b->d_queue + 2 * QUEUE_SET_WORDS, b->d_queue + 2 * QUEUE_SET_WORDS + b->p2g_tiles},
                     {b->p2g_tiles, b->g2p_tiles}};
     hipError_t e;
