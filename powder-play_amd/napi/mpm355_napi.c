@@ -558,24 +558,26 @@ static napi_value read_fracture_bonds(napi_env env, napi_callback_info info) {
     return arr;
 }
 
-/* setTuning(ctx, {p2gTile, g2pTile, g2pThreads, bucketCapacity}) and
- * getTuning(ctx) -> the same object (mpm_tuning; no reference counterpart) */
+/* setTuning(ctx, {p2gTile, g2pTile, g2pThreads, bucketCapacity, tileQueue}) and
+ * getTuning(ctx) -> the same object (mpm_tuning; no reference counterpart).
+ * bucketCapacity and tileQueue may be left out (0: keep the capacity, the
+ * automatic tile queue). */
 static napi_value set_tuning(napi_env env, napi_callback_info info) {
     size_t argc = 2;
     napi_value argv[2];
     CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
     ctx_box* box = argc >= 2 ? ctx_arg(env, argv[0]) : NULL;
     if (!box) return NULL;
-    double v[4] = {0, 0, 0, 0};
-    const char* keys[4] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity"};
-    for (int k = 0; k < 4; ++k)
+    double v[5] = {0, 0, 0, 0, 0};
+    const char* keys[5] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity", "tileQueue"};
+    for (int k = 0; k < 5; ++k)
         if (!get_number(env, argv[1], keys[k], &v[k], k < 3)) return NULL;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 5; ++k)
         if (!(v[k] >= 0 && v[k] <= 4294967295.0) || v[k] != (double)(uint32_t)v[k]) {
             napi_throw_range_error(env, "MPM_EINVAL", keys[k]);
             return NULL;
         }
-    mpm_tuning t = {(uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]};
+    mpm_tuning t = {(uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3], (uint32_t)v[4]};
     LOCKED_CALL(box, mpm_set_tuning(box->ctx, &t));
     return NULL;
 }
@@ -589,9 +591,9 @@ static napi_value get_tuning(napi_env env, napi_callback_info info) {
     mpm_tuning t;
     LOCKED_CALL(box, mpm_get_tuning(box->ctx, &t));
     CHECK(napi_create_object(env, &out));
-    const uint32_t vals[4] = {t.p2g_tile, t.g2p_tile, t.g2p_threads, t.bucket_capacity};
-    const char* keys[4] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity"};
-    for (int k = 0; k < 4; ++k) {
+    const uint32_t vals[5] = {t.p2g_tile, t.g2p_tile, t.g2p_threads, t.bucket_capacity, t.tile_queue};
+    const char* keys[5] = {"p2gTile", "g2pTile", "g2pThreads", "bucketCapacity", "tileQueue"};
+    for (int k = 0; k < 5; ++k) {
         CHECK(napi_create_uint32(env, vals[k], &x));
         napi_set_named_property(env, out, keys[k], x);
     }
