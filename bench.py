@@ -273,7 +273,7 @@ def small_scenes_leg(pkg, macrosteps: int = 12):
     return out
 
 
-def batched_leg(pkg, steps: int, how: str = "auto"):
+def batched_leg(pkg, steps: int, how: str = "auto", sequential: bool = True):
     """256 independent Tait tanks (scenes.batched_tanks): `steps` microsteps
     of all of them in one batch, against the same scenes as separate solvers
     stepped one after another (each synchronised before the next starts)."""
@@ -315,24 +315,28 @@ def batched_leg(pkg, steps: int, how: str = "auto"):
         d = batch.scene(i).read_diagnostics()
         drift = max(drift, abs(d["outputMass"] - d["inputMass"]) / d["inputMass"])
     batch.dispose()
-    solvers = [pkg.create_mpm_mls_gpu_solver(a, c, device=0) for a, c in scenes]
-    for s, g in zip(solvers, gravity):
-        s.step_batch(8, dt, g)
-        s.ctx.read_compact()
-    t0 = time.perf_counter()
-    for s, g in zip(solvers, gravity):
-        s.step_batch(steps, dt, g)
-        s.ctx.read_compact()
-    t_seq = time.perf_counter() - t0
-    for s in solvers:
-        s.dispose()
-    rate, rate_seq = total * steps / t_batch, total * steps / t_seq
+    t_seq = float("nan")
+    if sequential:
+        solvers = [pkg.create_mpm_mls_gpu_solver(a, c, device=0) for a, c in scenes]
+        for s, g in zip(solvers, gravity):
+            s.step_batch(8, dt, g)
+            s.ctx.read_compact()
+        t0 = time.perf_counter()
+        for s, g in zip(solvers, gravity):
+            s.step_batch(steps, dt, g)
+            s.ctx.read_compact()
+        t_seq = time.perf_counter() - t0
+        for s in solvers:
+            s.dispose()
+    rate = total * steps / t_batch
+    rate_seq = total * steps / t_seq if sequential else None
     log(f"[bench] batched: {1e3 * t_batch / steps:.4f} ms per microstep of all scenes ({rate:.3e} particle-steps/s); "
-        f"with FP64 reductions {1e3 * t_diag / steps:.4f} ms (mass drift {drift:.1e}); "
-        f"sequential {1e3 * t_seq / steps:.4f} ms ({rate_seq:.3e}); x{rate / rate_seq:.1f}")
+        f"with FP64 reductions {1e3 * t_diag / steps:.4f} ms (mass drift {drift:.1e})"
+        + (f"; sequential {1e3 * t_seq / steps:.4f} ms ({rate_seq:.3e}); x{rate / rate_seq:.1f}" if sequential else ""))
     return {"value": rate, "unit": "particle-steps/s", "ms_per_step": 1e3 * t_batch / steps, "scenes": len(tanks),
             "particles": total, "steps": steps, "sequential_value": rate_seq,
-            "sequential_ms_per_step": 1e3 * t_seq / steps, "speedup_vs_sequential": rate / rate_seq,
+            "sequential_ms_per_step": 1e3 * t_seq / steps if sequential else None,
+            "speedup_vs_sequential": rate / rate_seq if sequential else None,
             "fp64_reductions": {"value": total * steps / t_diag, "ms_per_step": 1e3 * t_diag / steps,
                                 "max_relative_mass_drift": drift},
             "tuning": tuned,
@@ -629,6 +633,10 @@ def main():
     ap.add_argument("--mode", choices=["both", "gather", "fixed"], default="both")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batched-steps", type=int, default=480)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched leg without the scenes stepped one after another")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run the batched leg alone (a profiling workload); prints its JSON object")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--no-small", action="store_true", help="skip the Tait tank and Gate 4 leg")
@@ -648,6 +656,9 @@ def main():
         sys.exit(2)
 
     pkg = __graft_entry__.load_package()
+    if args.only_batched:
+        print(json.dumps(batched_leg(pkg, args.batched_steps, args.batch_tuning, not args.no_sequential)), flush=True)
+        return
     log(f"[bench] building scene: {args.cells}x{args.cells} cells")
     arr, config = pkg.scenes.water_oil_column(cells=args.cells)
     if args.pmc_sample:
@@ -703,7 +714,7 @@ def main():
     step_gbs = step_bytes / (r["ms_per_step"] * 1e-3) / 1e9
 
     batched = (None if args.no_batched or args.cells != 1000
-               else batched_leg(pkg, args.batched_steps, args.batch_tuning))
+               else batched_leg(pkg, args.batched_steps, args.batch_tuning, not args.no_sequential))
     adjoint = None if args.no_adjoint else adjoint_leg(pkg, args.adjoint_reps)
     small = None if args.no_small else small_scenes_leg(pkg)
 
