@@ -288,7 +288,7 @@ def batched_leg(pkg, steps: int, how: str = "auto", sequential: bool = True):
         tuned = pkg.autotune.autotune_batch(scenes, dt, gravity, device=0)
         log(f"[bench] batched: autotuned {tuned['tuning']} in {time.perf_counter() - t0:.1f} s")
     else:
-        keys = ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity")
+        keys = ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity", "tile_queue")
         tuned = {"tuning": dict(zip(keys, (int(v) for v in how.split(",")))), "candidates": None}
     batch = pkg.MpmMlsGpuBatch(scenes, device=0, tuning=tuned["tuning"])
     batch.step_batch(8, dt, gravity)
@@ -589,7 +589,7 @@ def choose_tuning(pkg, arr, config, mode, how):
     if how == "default":
         return None, {"source": "default", "tuning": dict(pkg.autotune.DEFAULT_TUNING)}
     if how != "auto":
-        keys = ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity")
+        keys = ("p2g_tile", "g2p_tile", "g2p_threads", "bucket_capacity", "tile_queue")
         t = dict(zip(keys, (int(v) for v in how.split(","))))
         return t, {"source": "explicit", "tuning": t}
     code = pkg.native.MPM_P2G_GATHER if mode == "gather" else pkg.native.MPM_P2G_FIXED_POINT
@@ -642,10 +642,11 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the Tait tank and Gate 4 leg")
     ap.add_argument("--no-parity", action="store_true", help="skip the error-vs-FP64 checks of the CPU leg")
     ap.add_argument("--batch-tuning", default="auto",
-                    help='launch shape of the batched leg: "auto" or "p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
+                    help='launch shape of the batched leg: "auto" or "p2g_tile,g2p_tile,g2p_threads,bucket_capacity'
+                         '[,tile_queue]"')
     ap.add_argument("--tuning", default="auto",
                     help='launch shape: "auto" (autotune before timing), "default", or '
-                         '"p2g_tile,g2p_tile,g2p_threads,bucket_capacity"')
+                         '"p2g_tile,g2p_tile,g2p_threads,bucket_capacity[,tile_queue]"')
     ap.add_argument("--pmc-sample", action="store_true",
                     help="counter-collection workload: one 256 MiB copy kernel (the HBM byte calibration), then "
                          "8 untimed microsteps per mode; prints nothing")
