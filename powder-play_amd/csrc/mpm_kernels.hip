@@ -340,6 +340,25 @@ __device__ __forceinline__ float bspline_of_offset(float d, bool middle) {
     return __uint_as_float((__float_as_uint(mid) & m) | (__float_as_uint(side) & ~m));
 }
 
+// Whether the momentum m v + (A dx) d of a staged particle is finite at all
+// nine stencil offsets, d = offset - fraction per axis, as the gather forms
+// it: fmaf(a01, dy, fmaf(a00, dx, m)).  |d| <= 1.5, so finite inputs whose
+// magnitudes bound every sum below 1e37 pass at once.  Otherwise the four
+// corners decide: with the fraction in [0.5, 1.5) no d of the outer offsets
+// 0 and 2 is zero, so a non-finite input makes every corner sum non-finite,
+// and with finite inputs each rounded fmaf is monotone in d and in its
+// addend, so the middle sums lie between corner sums.
+__device__ __forceinline__ bool contributions_finite(float fx, float fy, float mx, float my, float4 Ad) {
+    const float bx = fabsf(mx) + 2.0f * (fabsf(Ad.x) + fabsf(Ad.y));
+    const float by = fabsf(my) + 2.0f * (fabsf(Ad.z) + fabsf(Ad.w));
+    if (bx < 1e37f && by < 1e37f) return true;  // false for a NaN bound
+    const float dx0 = 0.0f - fx, dx2 = 2.0f - fx, dy0 = 0.0f - fy, dy2 = 2.0f - fy;
+    auto fin2 = [&](float ddx, float ddy) {
+        return finitef(fmaf(Ad.y, ddy, fmaf(Ad.x, ddx, mx))) && finitef(fmaf(Ad.w, ddy, fmaf(Ad.z, ddx, my)));
+    };
+    return fin2(dx0, dy0) && fin2(dx2, dy0) && fin2(dx0, dy2) && fin2(dx2, dy2);
+}
+
 // Raise status bits from microstep `step` of the batch; the batch commits
 // the microsteps before the first one that raised (commitParticles,
 // shaders.ts:3067-3080).
@@ -967,8 +986,8 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
     // consecutive 16-byte records, which keeps the banks apart)
     __shared__ float4 s_r0[G_CAP];     // stencil fraction fx, fy; momentum m vx, m vy
     __shared__ float4 s_r1[G_CAP];     // affine A (row-major)
-    // mass, volume x det F, the tag (region column | phase << 8, 0xffff =
-    // contributes nothing) as bits; the fourth word holds the particle
+    // mass, volume x det F, the tag (region column | phase << 8) as bits;
+    // the fourth word holds the particle
     // indices at their provisional places, which the ranking reads while the
     // first three words are written
     __shared__ float4 s_r2[G_CAP];
@@ -986,6 +1005,7 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
     const bool node_ok = lx < T::TC && ly < T::TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
     const bool interior = node_ok && lx >= 1 && lx <= TI && ly >= 1 && ly <= TI;
+    for (int e = tid; e <= T::CELLS; e += T::THREADS) s_cell[e] = 0u;  // the first chunk's cell counts
     __syncthreads();
     if (s_boff[T::NB] == 0u) {
         if constexpr (OCC) {
@@ -995,39 +1015,42 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
         }
         return;
     }
-    if (tid == 0) {
+    // one chunk unless the region holds more particles than the staging
+    // arrays (the total is the same for every thread, so is the branch)
+    const bool chunked = s_boff[T::NB] > (uint32_t)G_CAP;
+    if (chunked && tid == 0) {
         uint32_t n = 0;
         s_chunk[0] = 0u;
-        if (s_boff[T::NB] > (uint32_t)G_CAP) {
-            for (int r = 0; r < T::RB; ++r) {
-                const int lo = r * T::RB, hi = lo + T::RB;
-                if (s_boff[hi] - s_boff[s_chunk[n]] <= (uint32_t)G_CAP) continue;
-                if ((int)s_chunk[n] < lo) s_chunk[++n] = (uint32_t)lo;
-                if (s_boff[hi] - s_boff[lo] > (uint32_t)G_CAP) {
-                    for (int b = lo; b < hi; ++b)
-                        if (s_boff[b + 1] - s_boff[s_chunk[n]] > (uint32_t)G_CAP && b > (int)s_chunk[n])
-                            s_chunk[++n] = (uint32_t)b;
-                }
+        for (int r = 0; r < T::RB; ++r) {
+            const int lo = r * T::RB, hi = lo + T::RB;
+            if (s_boff[hi] - s_boff[s_chunk[n]] <= (uint32_t)G_CAP) continue;
+            if ((int)s_chunk[n] < lo) s_chunk[++n] = (uint32_t)lo;
+            if (s_boff[hi] - s_boff[lo] > (uint32_t)G_CAP) {
+                for (int b = lo; b < hi; ++b)
+                    if (s_boff[b + 1] - s_boff[s_chunk[n]] > (uint32_t)G_CAP && b > (int)s_chunk[n])
+                        s_chunk[++n] = (uint32_t)b;
             }
         }
         s_chunk[++n] = T::NB;
         s_nchunks = n;
     }
-    __syncthreads();
+    if (chunked) __syncthreads();
 
     float acc[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = 0.0f;
     uint32_t flags = 0;
-    const uint32_t nchunks = s_nchunks;
+    const uint32_t nchunks = chunked ? s_nchunks : 1u;
     const int cx0 = rbx0 * BUCKET_CELLS, cy0 = rby0 * BUCKET_CELLS;  // first region cell (stencil-base coords)
     constexpr int PER = G_CAP / T::THREADS + (G_CAP % T::THREADS ? 1 : 0);  // staged particles a thread
 
     for (uint32_t ch = 0; ch < nchunks; ++ch) {
-        const int b0 = (int)s_chunk[ch], b1 = (int)s_chunk[ch + 1];
+        const int b0 = chunked ? (int)s_chunk[ch] : 0, b1 = chunked ? (int)s_chunk[ch + 1] : T::NB;
         const uint32_t c0 = s_boff[b0], c1 = s_boff[b1];
-        for (int e = tid; e <= T::CELLS; e += T::THREADS) s_cell[e] = 0u;
-        __syncthreads();
+        if (ch > 0u) {  // the first chunk's counts were cleared before the head's barrier
+            for (int e = tid; e <= T::CELLS; e += T::THREADS) s_cell[e] = 0u;
+            __syncthreads();
+        }
         // (A) positions: region cell of each particle, counted per cell
         uint32_t cell[PER], pid[PER], gi[PER];  // cell: region cell << 16 | place in the cell
 #pragma unroll
@@ -1103,13 +1126,30 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 if (pf & ERR_P2G) atomicMin(&ctl->status[1], pid[k]);
                 flags |= pf;
             }
-            s_r0[q] = make_float4(sc.fx, sc.fy, mt.mass * v.x, mt.mass * v.y);
+            const float mvx = mt.mass * v.x, mvy = mt.mass * v.y;
             // the affine scaled by the cell size: a contribution's momentum
             // is then m v + (A dx) d with d = offset - fraction, two fused
             // multiply-adds per component
-            s_r1[q] = make_float4(A.x * P.dx, A.y * P.dx, A.z * P.dx, A.w * P.dx);
-            const uint32_t tag = pf ? 0xffffu : ((rc % (uint32_t)T::RC) | (mt.phase << 8));
-            *reinterpret_cast<float3*>(&s_r2[q]) = make_float3(mt.mass, mt.volume * det, __uint_as_float(tag));
+            const float4 Ad = make_float4(A.x * P.dx, A.y * P.dx, A.z * P.dx, A.w * P.dx);
+            // a non-finite contribution fails the microstep (shaders.ts:1153-1172);
+            // the nine momenta are tested here, once a staged particle, so the
+            // gather loop does not test each pair
+            bool drop = pf != 0u;
+            if constexpr (!OCC) {
+                if (!drop && !contributions_finite(sc.fx, sc.fy, mvx, mvy, Ad)) {
+                    flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
+                    drop = true;
+                }
+            }
+            // a failed particle is staged as a solid of zero mass, volume and
+            // momentum: its contributions add exact zeros
+            const uint32_t col = rc % (uint32_t)T::RC;
+            // word by word (a select of whole vectors compiles to a stack copy)
+            auto kept = [drop](float v) { return drop ? 0.0f : v; };
+            s_r0[q] = make_float4(sc.fx, sc.fy, kept(mvx), kept(mvy));
+            s_r1[q] = make_float4(kept(Ad.x), kept(Ad.y), kept(Ad.z), kept(Ad.w));
+            *reinterpret_cast<float3*>(&s_r2[q]) =
+                make_float3(kept(mt.mass), kept(mt.volume * det), __uint_as_float(col | (drop ? 0u : mt.phase << 8)));
         }
         __syncthreads();
         // (E) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region.
@@ -1140,7 +1180,7 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 // scheduler waits for the tag before issuing the others, and
                 // defers the mass and volume into the phase branches)
                 asm volatile("" ::"v"(r0.x), "v"(r1.x), "v"(r2.x));
-                const uint32_t ph = tag >> 8;  // 0xff: the particle contributes nothing
+                const uint32_t ph = tag >> 8;
                 const int ox = lx + 3 - (int)(tag & 0xffu);
                 // d = offset - fraction per axis, the reference's dpos / dx
                 const float ddx = (float)ox - r0.x;
@@ -1148,18 +1188,15 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 const float w = bspline_of_offset(ddx, ox == 1) * bspline_of_offset(ddy, row1);
                 const float qx = fmaf(r1.y, ddy, fmaf(r1.x, ddx, r0.z));
                 const float qy = fmaf(r1.w, ddy, fmaf(r1.z, ddx, r0.w));
-                // one test for both skips
-                if (ph > 2u || !finitef(qx) || !finitef(qy)) {
-                    if (ph <= 2u) flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
-                    continue;
-                }
-                const float wm = w * r2.x, wqx = w * qx, wqy = w * qy;
+                // each product is added with one rounding
                 if (ph == 0u) {
-                    acc[0] += wm; acc[1] += wqx; acc[2] += wqy;
+                    acc[0] = fmaf(w, r2.x, acc[0]); acc[1] = fmaf(w, qx, acc[1]); acc[2] = fmaf(w, qy, acc[2]);
+                } else if (ph == 1u) {
+                    acc[3] = fmaf(w, r2.x, acc[3]); acc[4] = fmaf(w, qx, acc[4]);
+                    acc[5] = fmaf(w, qy, acc[5]); acc[6] = fmaf(w, r2.y, acc[6]);
                 } else {
-                    const float wv = w * r2.y;
-                    if (ph == 1u) { acc[3] += wm; acc[4] += wqx; acc[5] += wqy; acc[6] += wv; }
-                    else { acc[7] += wm; acc[8] += wqx; acc[9] += wqy; acc[10] += wv; }
+                    acc[7] = fmaf(w, r2.x, acc[7]); acc[8] = fmaf(w, qx, acc[8]);
+                    acc[9] = fmaf(w, qy, acc[9]); acc[10] = fmaf(w, r2.y, acc[10]);
                 }
             }
         }
