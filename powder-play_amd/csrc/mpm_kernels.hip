@@ -1825,7 +1825,7 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
     const Material* __restrict__ mats = S.mats;
     const Grid grid = S.grid;
     const uint32_t nmat = S.nmat, tiles_x = S.gtx, bw = S.bw, bh = S.bh, cap = S.cap, scene_flags = S.scene_flags;
-    __shared__ float2 s_vs[G_TN * G_TN];
+    __shared__ float2 s_vs[GENERAL ? G_TN * G_TN : 1];  // scenes without solids (not GENERAL) never read it
     __shared__ float2 s_vw[G_TN * G_TN];
     __shared__ float2 s_vo[G_TN * G_TN];
     __shared__ float2 s_lq[G_TN * G_TN];
@@ -1860,12 +1860,12 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
         const uint32_t node = (uint32_t)ny * (uint32_t)W + (uint32_t)nx;
         float2 vs = make_float2(0.f, 0.f), vw = vs, vo = vs, lq = vs;
         if (in_grid) {
-            if (scene_flags & SCENE_SOLID) vs = grid.v_solid[node];
+            if (GENERAL && (scene_flags & SCENE_SOLID)) vs = grid.v_solid[node];
             if (scene_flags & SCENE_WATER) vw = grid.v_water[node];
             if (scene_flags & SCENE_OIL) vo = grid.v_oil[node];
             if (scene_flags & (SCENE_WATER | SCENE_OIL)) lq = grid.liquid[node];
         }
-        if (scene_flags & SCENE_SOLID) s_vs[e] = vs;
+        if (GENERAL && (scene_flags & SCENE_SOLID)) s_vs[e] = vs;
         if (scene_flags & SCENE_WATER) s_vw[e] = vw;
         if (scene_flags & SCENE_OIL) s_vo[e] = vo;
         if (scene_flags & (SCENE_WATER | SCENE_OIL)) s_lq[e] = lq;
@@ -1924,7 +1924,7 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                 raise_step(ctl, ERR_STENCIL, step);
                 ok = false;
             } else {
-                const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u ? s_vo : s_vs);
+                const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
                 ok = g2p_particle<G_TN, GENERAL, AUX, SOLIDS>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
                                                 Ax, cfl_here);
             }
