@@ -509,8 +509,8 @@ SceneDesc scene_desc(const mpm_ctx* c, const Params& p, uint32_t p2g_base, uint3
     d.bh = lc.bh;
     d.nb = lc.nb;
     d.cap = lc.cap;
-    d.ptx = (lc.node_w + shape.p2g_tile - 1) / shape.p2g_tile;
-    d.pty = (lc.node_h + shape.p2g_tile - 1) / shape.p2g_tile;
+    d.ptx = p2g_tiles_side(lc.node_w, shape.p2g_tile);
+    d.pty = p2g_tiles_side(lc.node_h, shape.p2g_tile);
     d.gtx = (lc.bw + shape.g2p_tile - 1) / shape.g2p_tile;
     d.gty = (lc.bh + shape.g2p_tile - 1) / shape.g2p_tile;
     d.p2g_base = p2g_base;

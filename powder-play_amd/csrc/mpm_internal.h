@@ -251,8 +251,12 @@ inline bool shape_supported(const Shape& s) {
                    (s.g2p_tile == 4u && (s.g2p_threads == 128u || s.g2p_threads == 256u));
     return p && g;
 }
+// P2G tiles a side: tile t covers interior nodes [tile t - 1, tile t + tile - 2]
+// (its region then starts on a bucket boundary), so n nodes need
+// ceil((n + 1) / tile) tiles.
+inline uint32_t p2g_tiles_side(uint32_t nodes, uint32_t tile) { return (nodes + tile) / tile; }
 inline uint32_t p2g_tiles_for(uint32_t node_w, uint32_t node_h, uint32_t tile) {
-    return ((node_w + tile - 1) / tile) * ((node_h + tile - 1) / tile);
+    return p2g_tiles_side(node_w, tile) * p2g_tiles_side(node_h, tile);
 }
 inline uint32_t g2p_tiles_for(uint32_t bw, uint32_t bh, uint32_t tile) {
     return ((bw + tile - 1) / tile) * ((bh + tile - 1) / tile);

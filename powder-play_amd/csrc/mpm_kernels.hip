@@ -2,11 +2,12 @@
 //
 // Particles live in buckets of 4x4 stencil-base cells (double-buffered slot
 // arrays, SlotBuf).  One microstep is two launches on one stream:
-//   p2g_grid<FIXED, GENERAL>  per 28x28-node tile: stage the particles of the
-//                             9x9 surrounding buckets in LDS (ordered by cell,
+//   p2g_grid<FIXED, GENERAL>  per 20x20-node tile (28 and 12 also compiled):
+//                             stage the particles of the 6x6 buckets whose
+//                             stencils reach the tile in LDS (ordered by cell,
 //                             then particle index), gather the quadratic
-//                             B-spline P2G for a 30x30 node patch, run the grid
-//                             update on the 28x28 interior, write node velocities
+//                             B-spline P2G for a 22x22 node patch, run the grid
+//                             update on the 20x20 interior, write node velocities
 //   g2p<GENERAL>              per 8x8-bucket tile: stage the 34x34 node patch in
 //                             LDS, G2P, F update, projections, CFL guard, and
 //                             write each particle into its next bucket
@@ -55,20 +56,25 @@ constexpr float TRIANGLE_CONTACT_IMPULSE_RELAXATION = 0.25f;
 constexpr float TRIANGLE_FLUID_TANGENTIAL_DRAG_RATE = 4.0f;
 constexpr int BONDED_TRIANGLE_SOLVER_ITERATIONS = 2;
 
-// P2G tile of TI x TI interior nodes (TI/4 buckets a side) and a one-node
-// ring for the liquid-contact volume gradient ((TI+2)^2 nodes computed), fed
-// by the (TI/4 + 2)^2 buckets around it.  Three sides are compiled:
+// P2G tile of TI x TI interior nodes and a one-node ring for the
+// liquid-contact volume gradient ((TI+2)^2 nodes computed).  A node receives
+// from the particles whose stencil-base cell c (= floor(x/dx - 0.5) + 1, the
+// cell of the bucket keys) lies in [node - 2, node], so the tile and its ring
+// need TI + 4 cells a side.  Tile t starts one node before a bucket boundary
+// (interior nodes [TI t - 1, TI t + TI - 2]), which makes those cells exactly
+// the TI/4 + 1 buckets [TI/4 t - 1, TI/4 t + TI/4 - 1]: no particle of the
+// region lies outside every stencil of the tile.  Three sides are compiled:
 //   28: 1024 threads, two tiles a CU (LDS), 32 node threads to a row
 //   20:  512 threads, three tiles a CU
 //   12:  256 threads, six tiles a CU
-// Smaller tiles re-read more region buckets (the ring of buckets around a
-// tile) but keep more blocks in flight per CU.
+// Smaller tiles re-read more region buckets (each bucket of a region is
+// staged by one to four tiles) but keep more blocks in flight per CU.
 template <int TI_>
 struct P2GTile {
     static constexpr int TI = TI_;                     // interior nodes per tile side
     static constexpr int TC = TI + 2;                  // with the one-node ring
     static constexpr int OWN = TI / BUCKET_CELLS;      // buckets a tile owns per side
-    static constexpr int RB = OWN + 2;                 // region buckets per side
+    static constexpr int RB = OWN + 1;                 // region buckets per side
     static constexpr int NB = RB * RB;
     static constexpr int RC = RB * BUCKET_CELLS;       // region cells per side
     static constexpr int CELLS = RC * RC;
@@ -823,7 +829,7 @@ __device__ __forceinline__ void scene_deferred_flags(Control* ctl, int in_buf, u
 
 // Shared head of both P2G kernels: status gate, deferred stress-cache flags,
 // clearing the next bucket counts this tile owns, and the staged offsets of
-// the 9x9 region buckets (s_boff, one wave, two buckets a lane).  Returns
+// the region buckets (s_boff, one wave, two buckets a lane).  Returns
 // false when a flag is already raised (the whole block returns).
 template <class T, bool OCC = false>
 __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32_t* cnt_next, uint32_t tile,
@@ -866,7 +872,7 @@ __device__ __forceinline__ bool p2g_head(Control* ctl, const SlotBuf& in, uint32
     return true;
 }
 
-// A P2G tile whose 9x9-bucket region holds no particle: its nodes receive
+// A P2G tile whose region holds no particle: its nodes receive
 // nothing, and no particle is within reach of them in this microstep's G2P,
 // so their velocities are not rewritten.  The kept fixed-point words and
 // the diagnostics partial still read zero.
@@ -876,7 +882,7 @@ __device__ void p2g_empty_tile(const Params& P, const Grid& grid, uint32_t keep_
     if (keep_fixed) {
         for (int e = threadIdx.x; e < TI * TI; e += blockDim.x) {
             const int sx = X0 + e % TI, sy = Y0 + e / TI;
-            if (sx < (int)P.node_w && sy < (int)P.node_h) {
+            if (sx >= 0 && sy >= 0 && sx < (int)P.node_w && sy < (int)P.node_h) {
                 const size_t node = (size_t)sy * P.node_w + (size_t)sx;
 #pragma unroll
                 for (int k = 0; k < 11; ++k) grid.fixed[node * 11 + k] = 0;
@@ -941,8 +947,8 @@ __device__ void block_exclusive_scan(uint32_t* s, int n, uint32_t* s_wave) {
 // tile and a one-node ring of complete sums, then updates the interior (the
 // ring supplies the volume gradient).
 //
-// The tile's particles are those of the region buckets around it (for TI =
-// 28, 9x9 buckets: a 36x36-cell region).  They are staged in LDS in
+// The tile's particles are those of its region buckets (for TI = 20, 6x6
+// buckets: a 24x24-cell region).  They are staged in LDS in
 // region-row-major cell order, ascending particle index inside a cell, so
 // each node sums its three stencil rows as three contiguous ranges in a fixed
 // order (rows, then cells, then particle index).  That order does not depend
@@ -999,9 +1005,9 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
     const Params& P = *Pp;
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
     const int lx = tid % T::ROW, ly = tid / T::ROW;
-    const int X0 = tx * TI, Y0 = ty * TI;
+    const int X0 = tx * TI - 1, Y0 = ty * TI - 1;  // first interior node
     const int W = (int)P.node_w, H = (int)P.node_h;
-    const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
+    const int rbx0 = tx * T::OWN - 1, rby0 = ty * T::OWN - 1;
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
     const bool node_ok = lx < T::TC && ly < T::TC && sx >= 0 && sx < W && sy >= 0 && sy < H;
     const bool interior = node_ok && lx >= 1 && lx <= TI && ly >= 1 && ly <= TI;
@@ -1152,14 +1158,14 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 make_float3(kept(mt.mass), kept(mt.volume * det), __uint_as_float(col | (drop ? 0u : mt.phase << 8)));
         }
         __syncthreads();
-        // (E) gather: stencil rows ly+1..ly+3, columns lx+1..lx+3 of the region.
+        // (E) gather: stencil rows ly..ly+2, columns lx..lx+2 of the region.
         // The three row ranges run as one loop, so a wave iterates over the
         // largest per-node pair count rather than the largest per cell.
         if (node_ok) {
             uint32_t q0[3], n[3];
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
-                const int row = (ly + 1 + dy) * T::RC + lx + 1;
+                const int row = (ly + dy) * T::RC + lx;
                 q0[dy] = s_cell[row];
                 n[dy] = s_cell[row + 3] - q0[dy];
             }
@@ -1181,7 +1187,7 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 // defers the mass and volume into the phase branches)
                 asm volatile("" ::"v"(r0.x), "v"(r1.x), "v"(r2.x));
                 const uint32_t ph = tag >> 8;
-                const int ox = lx + 3 - (int)(tag & 0xffu);
+                const int ox = lx + 2 - (int)(tag & 0xffu);
                 // d = offset - fraction per axis, the reference's dpos / dx
                 const float ddx = (float)ox - r0.x;
                 const float ddy = (row2 ? 2.0f : (row1 ? 1.0f : 0.0f)) - r0.y;
@@ -1290,9 +1296,9 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
     const int tid = threadIdx.x;
     const Params& P = *Pp;
     const int tx = (int)(tile % tiles_x), ty = (int)(tile / tiles_x);
-    const int X0 = tx * TI, Y0 = ty * TI;
+    const int X0 = tx * TI - 1, Y0 = ty * TI - 1;  // first interior node
     const int W = (int)P.node_w, H = (int)P.node_h;
-    const int rbx0 = X0 / BUCKET_CELLS - 1, rby0 = Y0 / BUCKET_CELLS - 1;
+    const int rbx0 = tx * T::OWN - 1, rby0 = ty * T::OWN - 1;
     if (s_boff[T::NB] == 0u) {
         p2g_empty_tile<TI>(P, grid, keep_fixed, diag_grid, tile, X0, Y0);
         return;
@@ -1422,7 +1428,7 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
     };
     const int lx = tid % T::ROW, ly = tid / T::ROW;
     const int sx = X0 - 1 + lx, sy = Y0 - 1 + ly;  // this thread's node (storage coords)
-    if (lx >= 1 && lx <= TI && ly >= 1 && ly <= TI && sx < W && sy < H) {
+    if (lx >= 1 && lx <= TI && ly >= 1 && ly <= TI && sx >= 0 && sy >= 0 && sx < W && sy < H) {
         const int ring = ly * T::TC + lx;
         const uint32_t node = (uint32_t)sy * (uint32_t)W + (uint32_t)sx;
         float f[11];
@@ -1445,7 +1451,7 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
         }
     }
     if (P.diag_on) {  // grid mass and momentum of this microstep (reference/solver.ts:481-492)
-        const bool mine = lx >= 1 && lx <= TI && ly >= 1 && ly <= TI && sx < W && sy < H;
+        const bool mine = lx >= 1 && lx <= TI && ly >= 1 && ly <= TI && sx >= 0 && sy >= 0 && sx < W && sy < H;
         const int ring = ly * T::TC + lx;
         block_sum3_f64(mine ? (double)decoded(0, ring) + (double)decoded(3, ring) + (double)decoded(7, ring) : 0.0,
                        mine ? (double)decoded(1, ring) + (double)decoded(4, ring) + (double)decoded(8, ring) : 0.0,
@@ -1592,10 +1598,20 @@ __device__ float2 project_out_of_triangles(const Params& P, const AuxData& A, fl
     return make_float2((float)((double)p.x + corr * nx), (float)((double)p.y + corr * ny));
 }
 
-template <int TN, bool GENERAL, bool AUX = false, bool SOLIDS = true>
-__device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
-                             float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
-                             uint32_t step, G2POut& o, const AuxData* A = nullptr, bool cfl_here = false) {
+// The G2P of one particle runs in three parts:
+//   g2p_motion:  velocity and affine from the node patch, the liquid
+//                density correction, and the unconfined new position;
+//   g2p_deform:  deformation update with its doubling retry and the
+//                constitutive projections;
+//   g2p_cfl_ok:  the CFL guard of the new state.
+// The new position, once confined, fixes the particle's next bucket, so a
+// caller without bonded bodies reserves the output slot after g2p_motion and
+// runs the other parts while that atomic is in flight.  Each part raises its
+// own flags and returns false when the particle failed.
+template <int TN>
+__device__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
+                           const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0, uint32_t step,
+                           float2& vel, float4& Cout, float2& np_out) {
     const uint32_t model = mt.model;
     const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
@@ -1668,6 +1684,18 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
         raise_step(ctl, ERR_G2P, step);
         return false;
     }
+    vel = make_float2(vx, vy);
+    Cout = C;
+    np_out = np;
+    return true;
+}
+
+template <bool GENERAL, bool SOLIDS = true>
+__device__ bool g2p_deform(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float4 F, float4 C,
+                           float pv, uint32_t step, float4& Fout, float& pv_out, double& ndet_out) {
+    const uint32_t model = mt.model;
+    const bool liquid = model == MODEL_WATER || model == MODEL_OIL;
+    const float dt = P.dt;
     const float rowsum = fmaxf(fabsf(C.x) + fabsf(C.y), fabsf(C.z) + fabsf(C.w));
     const float increment = dt * rowsum;
     const float required = ceilf(increment * (1.0f / MAX_DEFORMATION_INCREMENT));  // exact: a power of two
@@ -1770,10 +1798,45 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
         raise_step(ctl, ERR_PROJECTED_DEFORMATION, step);
         return false;
     }
+    Fout = Fn;
+    pv_out = pvn;
+    ndet_out = ndet;
+    return true;
+}
+
+// validateCflParticle (shaders.ts:2423-2454) of the new velocity and F.
+template <bool GENERAL, bool SOLIDS = true>
+__device__ bool g2p_cfl_ok(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 vel, float4 Fn,
+                           uint32_t step) {
+    const float speed = sqrtf(vel.x * vel.x + vel.y * vel.y);
+    const float ws = wave_speed_bound<GENERAL, SOLIDS>(P, mt.model, Fn, mt.lambda, mt.mu, mt.density, mt.volume,
+                                                       mt.param);
+    if (!finite2(vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
+        record_failure(ctl, 3u, pid, speed, ws, P.vel_bound, P.wave_bound);
+        raise_step(ctl, ERR_CFL_BOUND, step);
+        return false;
+    }
+    return true;
+}
+
+// The whole G2P of one particle in the reference's order, for scenes with
+// bonded bodies, whose liquid particles leave the triangles after the
+// deformation update (the projection radius needs the new det F).
+template <int TN, bool GENERAL, bool AUX = false, bool SOLIDS = true>
+__device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
+                             float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
+                             uint32_t step, G2POut& o, const AuxData* A = nullptr, bool cfl_here = false) {
+    float2 vel, np;
+    float4 C, Fn;
+    float pvn;
+    double ndet;
+    if (!g2p_motion<TN>(P, ctl, mt, pid, p, F, s, lv, llq, nx0, ny0, step, vel, C, np)) return false;
+    if (!g2p_deform<GENERAL, SOLIDS>(P, ctl, mt, pid, F, C, pv, step, Fn, pvn, ndet)) return false;
+    const bool liquid = mt.model == MODEL_WATER || mt.model == MODEL_OIL;
     // projectLiquidFromTriangles is the identity without bonded bodies; the
     // second confineAxis pass is then idempotent (shaders.ts:2376-2408).
-    float2 bxv = confine_axis(np.x, vx, P.dom_w);
-    float2 byv = confine_axis(np.y, vy, P.dom_h);
+    float2 bxv = confine_axis(np.x, vel.x, P.dom_w);
+    float2 byv = confine_axis(np.y, vel.y, P.dom_h);
     if constexpr (AUX) {
         if (liquid && A->nbodies != 0u) {
             const float r = (float)sqrt(((double)mt.volume * ndet) / 3.14159265358979323846);
@@ -1791,16 +1854,8 @@ __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, 
     o.C = C;
     o.F = Fn;
     o.plastic = pvn;
-    if (P.cfl_on >= 0.5f && (P.body_count == 0u || cfl_here)) {
-        const float speed = sqrtf(o.vel.x * o.vel.x + o.vel.y * o.vel.y);
-        const float ws = wave_speed_bound<GENERAL, SOLIDS>(P, model, Fn, mt.lambda, mt.mu, mt.density, mt.volume,
-                                                           mt.param);
-        if (!finite2(o.vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
-            record_failure(ctl, 3u, pid, speed, ws, P.vel_bound, P.wave_bound);
-            raise_step(ctl, ERR_CFL_BOUND, step);
-            return false;
-        }
-    }
+    if (P.cfl_on >= 0.5f && (P.body_count == 0u || cfl_here))
+        return g2p_cfl_ok<GENERAL, SOLIDS>(P, ctl, mt, pid, o.vel, Fn, step);
     return true;
 }
 
@@ -1854,23 +1909,6 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
     }
     __syncthreads();
     if (s_boff[NBK] == 0u) return;  // no particle in this tile
-    for (int e = tid; e < G_TN * G_TN; e += G_THREADS) {
-        const int nx = nx0 + e % G_TN, ny = ny0 + e / G_TN;
-        const bool in_grid = nx < W && ny < H;
-        const uint32_t node = (uint32_t)ny * (uint32_t)W + (uint32_t)nx;
-        float2 vs = make_float2(0.f, 0.f), vw = vs, vo = vs, lq = vs;
-        if (in_grid) {
-            if (GENERAL && (scene_flags & SCENE_SOLID)) vs = grid.v_solid[node];
-            if (scene_flags & SCENE_WATER) vw = grid.v_water[node];
-            if (scene_flags & SCENE_OIL) vo = grid.v_oil[node];
-            if (scene_flags & (SCENE_WATER | SCENE_OIL)) lq = grid.liquid[node];
-        }
-        if (GENERAL && (scene_flags & SCENE_SOLID)) s_vs[e] = vs;
-        if (scene_flags & SCENE_WATER) s_vw[e] = vw;
-        if (scene_flags & SCENE_OIL) s_vo[e] = vo;
-        if (scene_flags & (SCENE_WATER | SCENE_OIL)) s_lq[e] = lq;
-    }
-    __syncthreads();
     const uint32_t total = s_boff[NBK];
 
     // inputs of the thread's current (c_) and next (n_) particle
@@ -1898,9 +1936,50 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
             if constexpr (GENERAL) PFX##pv = in.plastic[g_];                           \
         }                                                                              \
     } while (0)
+    // The first particle's inputs load while the node patch is staged.  The
+    // patch loads of a thread are all issued before any is written to LDS
+    // (in groups of up to five nodes), so the staging costs one memory
+    // round trip per group instead of one per node.
+    G2P_FETCH((uint32_t)tid, c_);
+    {
+        constexpr int NODES = G_TN * G_TN, PER = (NODES + G_THREADS - 1) / G_THREADS, GROUP = PER < 5 ? PER : 5;
+        const bool has_s = GENERAL && (scene_flags & SCENE_SOLID), has_w = scene_flags & SCENE_WATER,
+                   has_o = scene_flags & SCENE_OIL, has_l = scene_flags & (SCENE_WATER | SCENE_OIL);
+#pragma unroll
+        for (int r0 = 0; r0 < PER; r0 += GROUP) {
+            float2 vs[GROUP], vw[GROUP], vo[GROUP], lq[GROUP];
+#pragma unroll
+            for (int r = 0; r < GROUP; ++r) {
+                const int e = tid + (r0 + r) * G_THREADS;
+                const int nx = nx0 + e % G_TN, ny = ny0 + e / G_TN;
+                vs[r] = make_float2(0.f, 0.f);
+                vw[r] = vs[r];
+                vo[r] = vs[r];
+                lq[r] = vs[r];
+                if (r0 + r < PER && e < NODES && nx < W && ny < H) {
+                    const uint32_t node = (uint32_t)ny * (uint32_t)W + (uint32_t)nx;
+                    if (has_s) vs[r] = grid.v_solid[node];
+                    if (has_w) vw[r] = grid.v_water[node];
+                    if (has_o) vo[r] = grid.v_oil[node];
+                    if (has_l) lq[r] = grid.liquid[node];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < GROUP; ++r) {
+                const int e = tid + (r0 + r) * G_THREADS;
+                if (r0 + r < PER && e < NODES) {
+                    if (has_s) s_vs[e] = vs[r];
+                    if (has_w) s_vw[e] = vw[r];
+                    if (has_o) s_vo[e] = vo[r];
+                    if (has_l) s_lq[e] = lq[r];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
     const AuxData* const Ax = &S.aux;
     uint32_t* const S_aux_slot_out = AUX ? S.aux.slot[out_buf] : nullptr;
-    G2P_FETCH((uint32_t)tid, c_);
     for (uint32_t base = 0; base < total; base += G_THREADS) {
         G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
         bool ok = base + (uint32_t)tid < total;
@@ -1908,48 +1987,104 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
         uint32_t dest = 0;
         float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t aux = NO_AUX;
-        if (ok) {
-            Material mt = material_at(s_mat, mats, c_mat);
-            bool cfl_here = false;
-            if constexpr (AUX) {
+        uint32_t k;
+        if constexpr (AUX) {
+            if (ok) {
+                Material mt = material_at(s_mat, mats, c_mat);
                 aux = Ax->aux_of[c_pid];
                 // fracture endpoints carry their own damage (the neo-Hookean parameter)
                 if (aux != NO_AUX && aux >= Ax->frac_aux0) mt.param = Ax->damage[out_buf ^ 1][aux];
                 // triangle vertices are checked after the body contacts (validateCflParticles)
-                cfl_here = aux == NO_AUX || aux >= Ax->frac_aux0;
-            }
-            Stencil s;
-            if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
-                ((s.by + 1) >> 2) != c_gby) {
-                raise_step(ctl, ERR_STENCIL, step);
-                ok = false;
-            } else {
-                const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
-                ok = g2p_particle<G_TN, GENERAL, AUX, SOLIDS>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0, ny0, step, o,
-                                                Ax, cfl_here);
-            }
-            if (ok) {
-                Stencil ns;
-                if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns)) {
+                const bool cfl_here = aux == NO_AUX || aux >= Ax->frac_aux0;
+                Stencil s;
+                if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
+                    ((s.by + 1) >> 2) != c_gby) {
                     raise_step(ctl, ERR_STENCIL, step);
                     ok = false;
                 } else {
-                    dest = bucket_of(ns, bw);
+                    const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
+                    ok = g2p_particle<G_TN, GENERAL, AUX, SOLIDS>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0,
+                                                                  ny0, step, o, Ax, cfl_here);
+                }
+                if (ok) {
+                    Stencil ns;
+                    if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns)) {
+                        raise_step(ctl, ERR_STENCIL, step);
+                        ok = false;
+                    } else {
+                        dest = bucket_of(ns, bw);
+                    }
+                }
+                if constexpr (GENERAL) {
+                    if (ok) {
+                        const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
+                        if (pf) {
+                            if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
+                            atomicOr(&ctl->deferred[out_buf], pf);
+                        }
+                    }
                 }
             }
-            if constexpr (GENERAL) {
-                if (ok) {
-                    // stress cache of the new state; failures belong to the
-                    // microstep that will start from it
-                    const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
-                    if (pf) {
-                        if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
-                        atomicOr(&ctl->deferred[out_buf], pf);
+            k = wave_alloc(out.cnt, dest, ok);
+        } else {
+            // The confined new position fixes the next bucket: its slot is
+            // reserved here and the atomic completes while the deformation
+            // update, projections and CFL guard run.  A particle that fails
+            // after the reservation fails the microstep, whose output buffer
+            // is then never read (rollback copies the committed buffer over
+            // it, counts included).  The flags are those of the reference's
+            // order: a new position outside the grid is reported only when
+            // the rest of the particle's update succeeded.
+            Material mt;
+            bool placed = false;
+            if (ok) {
+                mt = material_at(s_mat, mats, c_mat);
+                Stencil s;
+                if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
+                    ((s.by + 1) >> 2) != c_gby) {
+                    raise_step(ctl, ERR_STENCIL, step);
+                    ok = false;
+                } else {
+                    const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
+                    float2 np;
+                    ok = g2p_motion<G_TN>(P, ctl, mt, c_pid, c_p, c_F, s, lv, s_lq, nx0, ny0, step, o.vel, o.C, np);
+                    if (ok) {
+                        // projectLiquidFromTriangles is the identity without
+                        // bonded bodies; the second confineAxis pass is then
+                        // idempotent (shaders.ts:2376-2408)
+                        const float2 bxv = confine_axis(np.x, o.vel.x, P.dom_w);
+                        const float2 byv = confine_axis(np.y, o.vel.y, P.dom_h);
+                        o.pos = make_float2(bxv.x, byv.x);
+                        o.vel = make_float2(bxv.y, byv.y);
+                        Stencil ns;
+                        placed = make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns);
+                        if (placed) dest = bucket_of(ns, bw);
+                    }
+                }
+            }
+            k = wave_alloc(out.cnt, dest, ok && placed);
+            if (ok) {
+                double ndet;
+                ok = g2p_deform<GENERAL, SOLIDS>(P, ctl, mt, c_pid, c_F, o.C, c_pv, step, o.F, o.plastic, ndet);
+                if (ok && P.cfl_on >= 0.5f && P.body_count == 0u)
+                    ok = g2p_cfl_ok<GENERAL, SOLIDS>(P, ctl, mt, c_pid, o.vel, o.F, step);
+                if (ok && !placed) {
+                    raise_step(ctl, ERR_STENCIL, step);
+                    ok = false;
+                }
+                if constexpr (GENERAL) {
+                    if (ok) {
+                        // stress cache of the new state; failures belong to the
+                        // microstep that will start from it
+                        const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
+                        if (pf) {
+                            if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
+                            atomicOr(&ctl->deferred[out_buf], pf);
+                        }
                     }
                 }
             }
         }
-        const uint32_t k = wave_alloc(out.cnt, dest, ok);
         if (ok) {
             if (k >= cap) {
                 raise_step(ctl, ERR_INTERNAL_REBUCKET, step);
@@ -1978,7 +2113,7 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
 
 template <int G_TB, int G_THREADS, bool GENERAL, bool AUX = false, bool SOLIDS = true>
 __global__ __launch_bounds__(G_THREADS)
-__attribute__((amdgpu_waves_per_eu(GENERAL && !SOLIDS ? 4 : 1, 8)))  // Tait liquids: 4 waves a SIMD
+__attribute__((amdgpu_waves_per_eu(!GENERAL || !SOLIDS ? 4 : 1, 8)))  // liquid-only scenes: 4 waves a SIMD
 void k_g2p(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
            uint32_t step, TileQueue q) {
     run_tiles(q, 1u, step, [&](uint32_t t) { g2p_tile<G_TB, G_THREADS, GENERAL, AUX, SOLIDS>(scenes, tile_scene, nscenes, step, t); });
