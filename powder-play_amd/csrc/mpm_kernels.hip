@@ -382,25 +382,38 @@ __device__ __forceinline__ void raise_step_at(Control* c, uint32_t bits, uint32_
 // slots with one atomic, all groups in the same instruction, so the wave
 // waits for a single round trip.  Every lane of the wave must call it
 // (converged control flow).
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t* cnt, uint32_t b, bool active) {
+// The reservation in two halves: wave_alloc_issue groups the lanes and
+// issues the atomics; wave_alloc_finish hands each lane its slot once the
+// atomics have returned, so independent work can run between the two.
+struct WaveAlloc {
+    uint32_t base;  // the group leader's atomic result (valid in the leader)
+    int leader;
+    unsigned long long group;
+};
+__device__ __forceinline__ WaveAlloc wave_alloc_issue(uint32_t* cnt, uint32_t b, bool active) {
     const int lane = threadIdx.x & 63;
     unsigned long long pending = __ballot(active);
-    unsigned long long group = 0ull;
-    int leader = lane;
+    WaveAlloc a{0u, lane, 0ull};
     while (pending) {
         const int first = __ffsll((long long)pending) - 1;
         const uint32_t fb = __shfl(b, first, 64);
         const unsigned long long same = __ballot(active && b == fb);
         if (active && b == fb) {
-            group = same;
-            leader = first;
+            a.group = same;
+            a.leader = first;
         }
         pending &= ~same;
     }
-    uint32_t base = 0u;
-    if (active && lane == leader) base = atomicAdd(&cnt[b], (uint32_t)__popcll(group));
-    base = __shfl(base, leader, 64);
-    return active ? base + (uint32_t)__popcll(group & ((1ull << lane) - 1ull)) : 0xffffffffu;
+    if (active && lane == a.leader) a.base = atomicAdd(&cnt[b], (uint32_t)__popcll(a.group));
+    return a;
+}
+__device__ __forceinline__ uint32_t wave_alloc_finish(const WaveAlloc& a, bool active) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t base = __shfl(a.base, a.leader, 64);
+    return active ? base + (uint32_t)__popcll(a.group & ((1ull << lane) - 1ull)) : 0xffffffffu;
+}
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* cnt, uint32_t b, bool active) {
+    return wave_alloc_finish(wave_alloc_issue(cnt, b, active), active);
 }
 
 __device__ __forceinline__ uint32_t bucket_of(const Stencil& s, uint32_t bw) {
@@ -1608,10 +1621,12 @@ __device__ float2 project_out_of_triangles(const Params& P, const AuxData& A, fl
 // caller without bonded bodies reserves the output slot after g2p_motion and
 // runs the other parts while that atomic is in flight.  Each part raises its
 // own flags and returns false when the particle failed.
-template <int TN>
-__device__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
-                           const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0, uint32_t step,
-                           float2& vel, float4& Cout, float2& np_out) {
+// The node values come from `lv` (the particle's phase velocity) and `llq`
+// (liquid mass and volume), node (x, y) at (x - x0) + (y - y0) * stride.
+template <class NodePtr>
+__device__ __forceinline__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p,
+                                           float4 F, const Stencil& s, NodePtr lv, NodePtr llq, int stride, int x0,
+                                           int y0, uint32_t step, float2& vel, float4& Cout, float2& np_out) {
     const uint32_t model = mt.model;
     const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
@@ -1630,13 +1645,13 @@ __device__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, ui
         dpx[o] = ((float)o - s.fx) * dx;
         dpy[o] = ((float)o - s.fy) * dx;
     }
-    const int row0 = (s.by + 1 - ny0) * TN + (s.bx + 1 - nx0);
+    const int row0 = (s.by + 1 - y0) * stride + (s.bx + 1 - x0);  // stencil offset (0, 0)
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy) {
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
             const float w = wx[ox] * wy[oy];
-            const float2 g = lv[row0 + oy * TN + ox];
+            const float2 g = lv[row0 + oy * stride + ox];
             const float wgx = w * g.x, wgy = w * g.y;
             vx = vx + wgx;
             vy = vy + wgy;
@@ -1653,7 +1668,7 @@ __device__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, ui
 #pragma unroll
             for (int ox = 0; ox < 3; ++ox) {
                 const float w = wx[ox] * wy[oy];
-                const float2 lq = llq[row0 + oy * TN + ox];
+                const float2 lq = llq[row0 + oy * stride + ox];
                 pm = fmaf(w, lq.x, pm);
                 pvol = fmaf(w, lq.y, pvol);
             }
@@ -1822,15 +1837,15 @@ __device__ bool g2p_cfl_ok(const Params& P, Control* ctl, const Material& mt, ui
 // The whole G2P of one particle in the reference's order, for scenes with
 // bonded bodies, whose liquid particles leave the triangles after the
 // deformation update (the projection radius needs the new det F).
-template <int TN, bool GENERAL, bool AUX = false, bool SOLIDS = true>
+template <bool GENERAL, bool AUX, bool SOLIDS, class NodePtr>
 __device__ bool g2p_particle(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p, float4 F,
-                             float pv, const Stencil& s, const float2* lv, const float2* llq, int nx0, int ny0,
+                             float pv, const Stencil& s, NodePtr lv, NodePtr llq, int stride, int x0, int y0,
                              uint32_t step, G2POut& o, const AuxData* A = nullptr, bool cfl_here = false) {
     float2 vel, np;
     float4 C, Fn;
     float pvn;
     double ndet;
-    if (!g2p_motion<TN>(P, ctl, mt, pid, p, F, s, lv, llq, nx0, ny0, step, vel, C, np)) return false;
+    if (!g2p_motion(P, ctl, mt, pid, p, F, s, lv, llq, stride, x0, y0, step, vel, C, np)) return false;
     if (!g2p_deform<GENERAL, SOLIDS>(P, ctl, mt, pid, F, C, pv, step, Fn, pvn, ndet)) return false;
     const bool liquid = mt.model == MODEL_WATER || mt.model == MODEL_OIL;
     // projectLiquidFromTriangles is the identity without bonded bodies; the
@@ -1981,21 +1996,29 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
     const AuxData* const Ax = &S.aux;
     uint32_t* const S_aux_slot_out = AUX ? S.aux.slot[out_buf] : nullptr;
     for (uint32_t base = 0; base < total; base += G_THREADS) {
-        G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
         bool ok = base + (uint32_t)tid < total;
         G2POut o;
         uint32_t dest = 0;
         float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t aux = NO_AUX;
         uint32_t k;
-        if constexpr (AUX) {
+        if constexpr (AUX || GENERAL) {
+            // the reference's order: the slot is reserved once the particle
+            // is complete (with bonded bodies the projection out of the
+            // triangles moves the position after the deformation update; in
+            // scenes with the stress cache the early reservation would hold
+            // more registers than four waves a SIMD allow)
+            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
             if (ok) {
                 Material mt = material_at(s_mat, mats, c_mat);
-                aux = Ax->aux_of[c_pid];
-                // fracture endpoints carry their own damage (the neo-Hookean parameter)
-                if (aux != NO_AUX && aux >= Ax->frac_aux0) mt.param = Ax->damage[out_buf ^ 1][aux];
-                // triangle vertices are checked after the body contacts (validateCflParticles)
-                const bool cfl_here = aux == NO_AUX || aux >= Ax->frac_aux0;
+                bool cfl_here = false;
+                if constexpr (AUX) {
+                    aux = Ax->aux_of[c_pid];
+                    // fracture endpoints carry their own damage (the neo-Hookean parameter)
+                    if (aux != NO_AUX && aux >= Ax->frac_aux0) mt.param = Ax->damage[out_buf ^ 1][aux];
+                    // triangle vertices are checked after the body contacts (validateCflParticles)
+                    cfl_here = aux == NO_AUX || aux >= Ax->frac_aux0;
+                }
                 Stencil s;
                 if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
                     ((s.by + 1) >> 2) != c_gby) {
@@ -2003,8 +2026,9 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                     ok = false;
                 } else {
                     const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
-                    ok = g2p_particle<G_TN, GENERAL, AUX, SOLIDS>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv, s_lq, nx0,
-                                                                  ny0, step, o, Ax, cfl_here);
+                    ok = g2p_particle<GENERAL, AUX, SOLIDS>(P, ctl, mt, c_pid, c_p, c_F, c_pv, s, lv,
+                                                            (const float2*)s_lq, G_TN, nx0, ny0, step, o, Ax,
+                                                            cfl_here);
                 }
                 if (ok) {
                     Stencil ns;
@@ -2017,6 +2041,8 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                 }
                 if constexpr (GENERAL) {
                     if (ok) {
+                        // stress cache of the new state; failures belong to the
+                        // microstep that will start from it
                         const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
                         if (pf) {
                             if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
@@ -2028,7 +2054,8 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
             k = wave_alloc(out.cnt, dest, ok);
         } else {
             // The confined new position fixes the next bucket: its slot is
-            // reserved here and the atomic completes while the deformation
+            // reserved here, and the atomic completes while the next
+            // particle's inputs are fetched and this particle's deformation
             // update, projections and CFL guard run.  A particle that fails
             // after the reservation fails the microstep, whose output buffer
             // is then never read (rollback copies the committed buffer over
@@ -2047,7 +2074,8 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                 } else {
                     const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
                     float2 np;
-                    ok = g2p_motion<G_TN>(P, ctl, mt, c_pid, c_p, c_F, s, lv, s_lq, nx0, ny0, step, o.vel, o.C, np);
+                    ok = g2p_motion(P, ctl, mt, c_pid, c_p, c_F, s, lv, (const float2*)s_lq, G_TN, nx0, ny0, step, o.vel,
+                                    o.C, np);
                     if (ok) {
                         // projectLiquidFromTriangles is the identity without
                         // bonded bodies; the second confineAxis pass is then
@@ -2062,7 +2090,9 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                     }
                 }
             }
-            k = wave_alloc(out.cnt, dest, ok && placed);
+            const bool reserve = ok && placed;
+            const WaveAlloc wa = wave_alloc_issue(out.cnt, dest, reserve);
+            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
             if (ok) {
                 double ndet;
                 ok = g2p_deform<GENERAL, SOLIDS>(P, ctl, mt, c_pid, c_F, o.C, c_pv, step, o.F, o.plastic, ndet);
@@ -2072,18 +2102,8 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                     raise_step(ctl, ERR_STENCIL, step);
                     ok = false;
                 }
-                if constexpr (GENERAL) {
-                    if (ok) {
-                        // stress cache of the new state; failures belong to the
-                        // microstep that will start from it
-                        const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
-                        if (pf) {
-                            if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
-                            atomicOr(&ctl->deferred[out_buf], pf);
-                        }
-                    }
-                }
             }
+            k = wave_alloc_finish(wa, reserve);
         }
         if (ok) {
             if (k >= cap) {
