@@ -181,6 +181,84 @@ def grid_f64(particles: np.ndarray, cfg: Config) -> np.ndarray:
     return out
 
 
+# Rounding scale of the float32 node sums relative to their magnitude, used
+# to tell a threshold decision of the contact that float32 may take the other
+# way (the sums of up to ~20 rounded products each carry a few 1e-7).
+CONTACT_ROUNDING = 1e-6
+
+
+def contact_near_threshold(grid: np.ndarray, cfg: Config, rounding: float = CONTACT_ROUNDING):
+    """Nodes whose water/oil contact decision lies within float32 rounding of
+    its threshold, from the FP64 node sums of grid_f64 (the P2G of one
+    microstep).  Restates in FP64 the three decisions of solveLiquidGridContact
+    (src/lib/mpm/reference/solver.ts:1293-1409, the contact branch of
+    grid_node in mpm_oracle.c; WGSL kernels/shaders.ts:1527-1594):
+
+    * the length of the volume-gradient difference g = grad V_oil - grad V_water
+      (the float32 kernel keeps nodes above 1e-8 of the unscaled difference,
+      the reference above 1e-12 of g);
+    * the normal's orientation flip, contrast * align < -1e-3;
+    * the impulse, rel < minsep.
+
+    The normal n = g / |g| of nearly cancelling gradients is ill-conditioned:
+    float32 volumes with relative error `rounding` move it by about
+    rounding * (|grad V_w| + |grad V_o| + V_w + V_o) / |g|.  A node is flagged
+    when a decision's margin is within that perturbation of the quantities it
+    compares (or the normal is undetermined).  Returns (mask of shape
+    (cells_y + 3, cells_x + 3), number of contact nodes)."""
+    W, H = cfg.cells_x + 3, cfg.cells_y + 3
+    f = {n: grid[:, i].reshape(H, W) for i, n in enumerate(GRID_FIELDS)}
+    dx, dt = float(cfg.cell_size), float(cfg.dt)
+    gvx, gvy = float(cfg.gravity_x), float(cfg.gravity_y)
+    vs = 1.0 / (2 * dx * dx)
+
+    def grad(V, axis):  # central differences, zero volume outside the node grid
+        P = np.pad(V, 1)
+        return ((P[1:-1, 2:] - P[1:-1, :-2]) if axis == 0 else (P[2:, 1:-1] - P[:-2, 1:-1])) * vs
+
+    wV, oV, wm, om = f["wV"], f["oV"], f["wm"], f["om"]
+    wgx, wgy, ogx, ogy = grad(wV, 0), grad(wV, 1), grad(oV, 0), grad(oV, 1)
+    gx, gy = ogx - wgx, ogy - wgy
+    gl = np.hypot(gx, gy)
+    yi, xi = np.mgrid[0:H, 0:W] - 1
+    contact = ((wm > 0) & (om > 0) & (wV > 0) & (oV > 0) & (xi >= 0) & (xi <= cfg.cells_x) & (yi >= 0) &
+               (yi <= cfg.cells_y))
+    with np.errstate(all="ignore"):
+        nx, ny = gx / gl, gy / gl
+        wd, od = wm / wV, om / oV
+        contrast = (od - wd) / np.maximum(wd, od)
+        gm = np.hypot(gvx, gvy)
+        align = (nx * gvx + ny * gvy) / gm if gm > 0 else np.zeros_like(nx)
+        m_flip = contrast * align + 1e-3
+        side = np.where(m_flip < 0, -1.0, 1.0)
+        vwx, vwy = f["wqx"] / wm + dt * gvx, f["wqy"] / wm + dt * gvy
+        vox, voy = f["oqx"] / om + dt * gvx, f["oqy"] / om + dt * gvy
+        rel = ((vox - vwx) * nx + (voy - vwy) * ny) * side
+        overlap = np.minimum(1.0, 2 * np.minimum(wV, oV) / (wV + oV))
+        minsep = np.maximum(0.25 * overlap * dx / 0.05, np.maximum(-rel, 0.0))
+        m_imp = rel - minsep
+        dn = rounding * (np.abs(wgx) + np.abs(wgy) + np.abs(ogx) + np.abs(ogy) + vs * (wV + oV)) / gl
+        speed = np.hypot(vwx, vwy) + np.hypot(vox, voy)
+        near = ((np.abs(m_flip) <= np.abs(contrast) * dn + rounding) |
+                (np.abs(m_imp) <= np.hypot(vox - vwx, voy - vwy) * dn + rounding * (speed + 1.0)) |
+                ~np.isfinite(dn) | (dn > 0.1) | (2 * gl * dx * dx <= 1e-6))
+    return contact & near, int(contact.sum())
+
+
+def particles_touching(mask: np.ndarray, x: np.ndarray, y: np.ndarray, cell_size: float) -> np.ndarray:
+    """Whether each particle's 3x3 stencil (base floor(x/dx - 0.5), nodes
+    base + 1 .. base + 3 in node storage) holds a masked node."""
+    bx = np.floor(np.asarray(x, np.float64) / cell_size - 0.5).astype(np.int64)
+    by = np.floor(np.asarray(y, np.float64) / cell_size - 0.5).astype(np.int64)
+    H, W = mask.shape
+    hit = np.zeros(bx.shape, bool)
+    for oy in range(3):
+        for ox in range(3):
+            sx, sy = np.clip(bx + 1 + ox, 0, W - 1), np.clip(by + 1 + oy, 0, H - 1)
+            hit |= mask[sy, sx]
+    return hit
+
+
 def params_f32(cells_x, cells_y, cell_size, particle_mass, particle_volume, dt, n) -> ParamsF32:
     cs = np.float32(cell_size)
     return ParamsF32(cells_x, cells_y, cells_x + 3, cells_y + 3, n, (cells_x + 3) * (cells_y + 3),

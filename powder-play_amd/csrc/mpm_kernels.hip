@@ -1496,16 +1496,19 @@ __device__ __forceinline__ float4 integrate_deformation(float4 F, float4 C, floa
     return r;
 }
 
-// projectLiquidDeformation (shaders.ts:474-488)
+// projectLiquidDeformation (shaders.ts:474-488).  The quotients and square
+// roots are the hardware ones with one quotient correction (quot), within an
+// ulp, where the WGSL allows 2.5 ulp; a non-finite or zero scale still gives
+// a non-finite result, which the caller's finiteness test reports.
 __device__ __forceinline__ float4 project_liquid(float4 m) {
     float nx = m.x + m.w, ny = m.z - m.y;
     float sc = fmaxf(fabsf(nx), fabsf(ny));
-    float ax = nx / sc, ay = ny / sc;
-    float len = sqrtf(ax * ax + ay * ay);
-    float rx = ax / len, ry = ay / len;
+    float ax = quot(nx, sc), ay = quot(ny, sc);
+    float len = __builtin_amdgcn_sqrtf(ax * ax + ay * ay);
+    float rx = quot(ax, len), ry = quot(ay, len);
     float vol = det_comp(m);
     float bv = fminf(fmaxf(vol, MIN_LIQUID_DETERMINANT), MAX_LIQUID_DETERMINANT);
-    float s = sqrtf(bv);
+    float s = __builtin_amdgcn_sqrtf(bv);
     return make_float4(s * rx, s * -ry, s * ry, s * rx);
 }
 
@@ -1515,7 +1518,7 @@ __device__ float wave_speed_bound(const Params& P, uint32_t model, float4 F, flo
                                   float volume, float param) {
     if (model == MODEL_WATER || model == MODEL_OIL) {
         if (!finitef(lambda) || !finitef(density) || lambda <= 0.0f || density <= 0.0f) return FLT_MAX;
-        float ws2 = lambda / density;
+        float ws2 = quot(lambda, density);  // positive finite operands: within an ulp
         if (param > 0.0f) {
             if constexpr (!GENERAL) return FLT_MAX;
             float det = det_comp(F);
@@ -1523,7 +1526,7 @@ __device__ float wave_speed_bound(const Params& P, uint32_t model, float4 F, flo
             ws2 = param * ws2 * (float)tait_pow((double)(1.0f / det), param - 1.0f);
         }
         if (!finitef(ws2) || ws2 < 0.0f) return FLT_MAX;
-        return sqrtf(ws2);
+        return __builtin_amdgcn_sqrtf(ws2);
     }
     if constexpr (!GENERAL || !SOLIDS) return FLT_MAX;
     if (model != MODEL_NEO_HOOKEAN) return FLT_MAX;
@@ -1823,7 +1826,7 @@ __device__ bool g2p_deform(const Params& P, Control* ctl, const Material& mt, ui
 template <bool GENERAL, bool SOLIDS = true>
 __device__ bool g2p_cfl_ok(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 vel, float4 Fn,
                            uint32_t step) {
-    const float speed = sqrtf(vel.x * vel.x + vel.y * vel.y);
+    const float speed = __builtin_amdgcn_sqrtf(vel.x * vel.x + vel.y * vel.y);
     const float ws = wave_speed_bound<GENERAL, SOLIDS>(P, mt.model, Fn, mt.lambda, mt.mu, mt.density, mt.volume,
                                                        mt.param);
     if (!finite2(vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {

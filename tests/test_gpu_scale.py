@@ -6,7 +6,13 @@ both P2G modes, against the oracle.
   the GPU takes the same step.  By microstep 20 the particles carry an
   affine C and a deformed F, so the stress and affine terms of P2G are
   compared at full size.  Bars: relative L2 error of the velocities 1e-4 and
-  of the positions 1e-7 (the stated float32 tolerance of the step).
+  of the positions 1e-7 (the stated float32 tolerance of the step).  The
+  velocity bar covers the particles whose stencil avoids every node where
+  the water/oil contact decides within float32 rounding of a threshold
+  (oracle.contact_near_threshold, from the oracle's own node sums of the
+  same input): there float32 may flip the normal or switch the impulse
+  where FP64 does not, a discrete change of that node's velocities.  Such
+  particles are counted and must stay below 1% of the column.
 * Fixed-point P2G: at microsteps 20, 21 and 22 the 11-word int32 grid of the
   GPU's P2G (44 MB) equals the oracle's integer restatement of the WGSL
   scatter (kernels/shaders.ts:971-1070) word for word; the SHA-256 of both
@@ -32,18 +38,27 @@ def compare_with_oracle_step(pkg, orc, s, mode, k):
     s.check_numerical_status()
     got, _ = s.read_particles(packed=True)
     cfg = orc.make_config(float(np.float32(DT)), CELLS, CELLS, 1.0, 0.02, 1.0, (0.0, 90.0))
-    exp, diag = orc.step_f64(orc.particles_from_packed(before), cfg)
+    ref_in = orc.particles_from_packed(before)
+    exp, diag = orc.step_f64(ref_in, cfg)
+    near, contact_nodes = orc.contact_near_threshold(orc.grid_f64(ref_in, cfg), cfg)
+    excluded = orc.particles_touching(near, ref_in["x"], ref_in["y"], 1.0)
+    keep = ~excluded
     want = np.stack([exp[n] for n in STATE], axis=1)
     have = got.view(np.float32)[:, :12].astype(np.float64)
     inp = before.view(np.float32)[:, 4:12].astype(np.float64)
-    rel_v = float(np.linalg.norm(have[:, 2:4] - want[:, 2:4]) / np.linalg.norm(want[:, 2:4]))
+    dv = np.linalg.norm(have[:, 2:4] - want[:, 2:4], axis=1)
+    rel_v = float(np.linalg.norm(dv[keep]) / np.linalg.norm(want[keep, 2:4]))
+    rel_v_all = float(np.linalg.norm(dv) / np.linalg.norm(want[:, 2:4]))
     rel_x = float(np.linalg.norm(have[:, :2] - want[:, :2]) / np.linalg.norm(want[:, :2]))
     rel_c = float(np.linalg.norm(have[:, 4:8] - want[:, 4:8]) / np.linalg.norm(want[:, 4:8]))
     c_in = float(np.abs(inp[:, 0:4]).max())
     f_dev = float(np.abs(inp[:, 4:8] - np.array([1.0, 0.0, 0.0, 1.0])).max())
     print(f"1M column {mode}, microstep {k}: input max |C| {c_in:.2e}, max |F - I| {f_dev:.2e}; "
-          f"rel L2 v {rel_v:.2e}, x {rel_x:.2e}, C {rel_c:.2e}", flush=True)
+          f"rel L2 v {rel_v:.2e} ({int(excluded.sum())} particles at {int(near.sum())} of {contact_nodes} "
+          f"contact nodes near a threshold excluded; all particles {rel_v_all:.2e}, largest |dv| there "
+          f"{float(dv[excluded].max()) if excluded.any() else 0.0:.2e}), x {rel_x:.2e}, C {rel_c:.2e}", flush=True)
     assert rel_v <= 1e-4 and rel_x <= 1e-7
+    assert excluded.sum() <= 0.01 * len(excluded)
     assert diag.grid_mass == pytest.approx(diag.input_mass, rel=1e-12)
     if k > 1:
         assert c_in > 1e-2 and f_dev > 1e-5  # stress and affine terms are live
