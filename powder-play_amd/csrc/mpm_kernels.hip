@@ -1904,6 +1904,15 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
     __shared__ float2 s_lq[G_TN * G_TN];
     __shared__ uint32_t s_boff[NBK + 1];
     __shared__ Material s_mat[LDS_MATS];
+    // tile place -> (bucket of the tile << 16 | place in the bucket), for
+    // tiles of at most G2P_MAP particles (larger ones search s_boff), in
+    // scenes without the stress cache (measured: the map slows the Tait
+    // variant, whose registers already spill) and where the block stays
+    // within 40 KB of LDS (four blocks a CU)
+    constexpr int PATCH_BYTES = 3 * G_TN * G_TN * (int)sizeof(float2);
+    constexpr uint32_t G2P_MAP =
+        !GENERAL && PATCH_BYTES + 8192 + (int)sizeof(Material) * LDS_MATS <= 40960 ? 2048u : 0u;
+    __shared__ uint32_t s_map[G2P_MAP > 0u ? G2P_MAP : 1u];
     if (block_status_set(ctl)) return;
     const Params& P = *Pp;
     const int tid = threadIdx.x;
@@ -1935,18 +1944,28 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
     uint32_t c_pid = 0u, c_mat = 0u, n_pid = 0u, n_mat = 0u;
     float c_pv = 1.0f, n_pv = 1.0f;
     int c_gbx = 0, c_gby = 0, n_gbx = 0, n_gby = 0;
-#define G2P_FETCH(J, PFX)                                                              \
+    const bool mapped = total <= G2P_MAP;
+#define G2P_FETCH(J, PFX, MAPPED)                                                      \
     do {                                                                               \
         const uint32_t j_ = (J);                                                       \
         if (j_ < total) {                                                              \
-            int lo = 0, hi = NBK - 1;                                                  \
-            while (lo < hi) {                                                          \
-                const int mid = (lo + hi + 1) >> 1;                                    \
-                if (s_boff[mid] <= j_) lo = mid; else hi = mid - 1;                    \
+            int lo = 0;                                                                \
+            uint32_t k_ = 0;                                                           \
+            if (MAPPED) {                                                              \
+                const uint32_t e_ = s_map[j_];                                         \
+                lo = (int)(e_ >> 16);                                                  \
+                k_ = e_ & 0xffffu;                                                     \
+            } else {                                                                   \
+                int hi = NBK - 1;                                                      \
+                while (lo < hi) {                                                      \
+                    const int mid = (lo + hi + 1) >> 1;                                \
+                    if (s_boff[mid] <= j_) lo = mid; else hi = mid - 1;                \
+                }                                                                      \
+                k_ = j_ - s_boff[lo];                                                  \
             }                                                                          \
             PFX##gbx = tbx + lo % G_TB;                                                \
             PFX##gby = tby + lo / G_TB;                                                \
-            const size_t g_ = ((size_t)PFX##gby * bw + (size_t)PFX##gbx) * cap + (j_ - s_boff[lo]); \
+            const size_t g_ = ((size_t)PFX##gby * bw + (size_t)PFX##gbx) * cap + k_;  \
             PFX##p = in.pos[g_];                                                       \
             PFX##F = in.F[g_];                                                         \
             PFX##pid = in.pid[g_];                                                     \
@@ -1958,7 +1977,14 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
     // patch loads of a thread are all issued before any is written to LDS
     // (in groups of up to five nodes), so the staging costs one memory
     // round trip per group instead of one per node.
-    G2P_FETCH((uint32_t)tid, c_);
+    G2P_FETCH((uint32_t)tid, c_, false);
+    if (mapped) {  // each bucket's places by G_THREADS / NBK threads (read after the barrier below)
+        constexpr int TPB = G_THREADS / NBK;
+        static_assert(TPB >= 1 && G_THREADS % NBK == 0, "whole threads a bucket");
+        const int b = tid / TPB;
+        const uint32_t b0 = s_boff[b], n = s_boff[b + 1] - b0;
+        for (uint32_t k = (uint32_t)(tid % TPB); k < n; k += TPB) s_map[b0 + k] = (uint32_t)b << 16 | k;
+    }
     {
         constexpr int NODES = G_TN * G_TN, PER = (NODES + G_THREADS - 1) / G_THREADS, GROUP = PER < 5 ? PER : 5;
         const bool has_s = GENERAL && (scene_flags & SCENE_SOLID), has_w = scene_flags & SCENE_WATER,
@@ -2011,7 +2037,7 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
             // triangles moves the position after the deformation update; in
             // scenes with the stress cache the early reservation would hold
             // more registers than four waves a SIMD allow)
-            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
+            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_, mapped);
             if (ok) {
                 Material mt = material_at(s_mat, mats, c_mat);
                 bool cfl_here = false;
@@ -2095,7 +2121,7 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
             }
             const bool reserve = ok && placed;
             const WaveAlloc wa = wave_alloc_issue(out.cnt, dest, reserve);
-            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_);
+            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_, mapped);
             if (ok) {
                 double ndet;
                 ok = g2p_deform<GENERAL, SOLIDS>(P, ctl, mt, c_pid, c_F, o.C, c_pv, step, o.F, o.plastic, ndet);
