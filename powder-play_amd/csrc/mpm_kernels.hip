@@ -1011,6 +1011,13 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
     // first three words are written
     __shared__ float4 s_r2[G_CAP];
     uint32_t* const s_pid = reinterpret_cast<uint32_t*>(s_r2) + 3;  // stride 4 words
+    // In scenes without the stress cache a region of one chunk maps each
+    // staged place to (region bucket << 16 | place in the bucket) in s_r1,
+    // which (D) overwrites only after (A); elsewhere (A) searches s_boff
+    // (measured: the extra barrier costs more than the search in the sparse
+    // tiles of batched Tait tanks).
+    uint32_t* const s_map = reinterpret_cast<uint32_t*>(s_r1);
+    constexpr bool MAPPED = !GENERAL;
 
     stage_materials(s_mat, mats, nmat);
     if (!p2g_head<T, OCC>(ctl, in, cnt_next, tile, tiles_x, bw, bh, cap, step, in_buf, s_boff)) return;
@@ -1053,7 +1060,15 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
         s_chunk[++n] = T::NB;
         s_nchunks = n;
     }
-    if (chunked) __syncthreads();
+    if (MAPPED && !chunked) {  // the slot map, several threads a region bucket
+        constexpr int TPB = T::THREADS / T::NB;
+        if (tid < TPB * T::NB) {
+            const int b = tid / TPB;
+            const uint32_t b0 = s_boff[b], n = s_boff[b + 1] - b0;
+            for (uint32_t k = (uint32_t)(tid % TPB); k < n; k += TPB) s_map[b0 + k] = (uint32_t)b << 16 | k;
+        }
+    }
+    if (MAPPED || chunked) __syncthreads();
 
     float acc[11];
 #pragma unroll
@@ -1078,9 +1093,18 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
             cell[k] = 0xffffffffu;
             pid[k] = 0u; gi[k] = 0u;
             if (j >= c1) continue;
-            const int rb = region_bucket(s_boff, b0, b1, j);
+            int rb;
+            uint32_t place;
+            if (MAPPED && !chunked) {
+                const uint32_t e = s_map[j];
+                rb = (int)(e >> 16);
+                place = e & 0xffffu;
+            } else {
+                rb = region_bucket(s_boff, b0, b1, j);
+                place = j - s_boff[rb];
+            }
             const int gbx = rbx0 + rb % T::RB, gby = rby0 + rb / T::RB;
-            gi[k] = ((uint32_t)gby * bw + (uint32_t)gbx) * cap + (j - s_boff[rb]);
+            gi[k] = ((uint32_t)gby * bw + (uint32_t)gbx) * cap + place;
             const float2 p = in.pos[gi[k]];
             pid[k] = in.pid[gi[k]];
             Stencil sc;
