@@ -1648,12 +1648,53 @@ __device__ float2 project_out_of_triangles(const Params& P, const AuxData& A, fl
 // caller without bonded bodies reserves the output slot after g2p_motion and
 // runs the other parts while that atomic is in flight.  Each part raises its
 // own flags and returns false when the particle failed.
+// Failure codes of the G2P parts, in the order the reference tests them; a
+// particle reports its first failure (raise_g2p).
+enum : uint32_t {
+    G2P_OK = 0u,
+    G2P_FAIL_LIQUID,       // liquid mass, volume or density not finite and positive: ERR_G2P, with the particle
+    G2P_FAIL_G2P,          // velocity, affine, position or F not finite, or a model this variant lacks: ERR_G2P
+    G2P_FAIL_SUBSTEPS,     // the deformation increment needs more than 64 substeps: ERR_SUBSTEP_LIMIT, with the particle
+    G2P_FAIL_DEFORMATION,  // the updated F loses orientation: deformation record 1, ERR_DEFORMATION
+    G2P_FAIL_PROJECTED,    // the projected F loses orientation: deformation record 2, ERR_PROJECTED_DEFORMATION
+    G2P_FAIL_CFL,          // the CFL guard: deformation record 3, ERR_CFL_BOUND
+    G2P_FAIL_STENCIL,      // a stencil outside the grid or in another bucket: ERR_STENCIL
+};
+// The first failure of a sequence of tests: `code` keeps the earliest.
+__device__ __forceinline__ uint32_t first_failure(uint32_t code, bool failed, uint32_t kind) {
+    return code != G2P_OK ? code : (failed ? kind : G2P_OK);
+}
+// Raise the status bits (and the deformation record) of a G2P failure code.
+__device__ void raise_g2p(Control* ctl, uint32_t code, uint32_t pid, uint32_t step, const float (&rec)[4]) {
+    switch (code) {
+    case G2P_FAIL_LIQUID: raise_step_at(ctl, ERR_G2P, step, pid); break;
+    case G2P_FAIL_G2P: raise_step(ctl, ERR_G2P, step); break;
+    case G2P_FAIL_SUBSTEPS: raise_step_at(ctl, ERR_SUBSTEP_LIMIT, step, pid); break;
+    case G2P_FAIL_DEFORMATION:
+        record_failure(ctl, 1u, pid, rec[0], rec[1], rec[2], rec[3]);
+        raise_step(ctl, ERR_DEFORMATION, step);
+        break;
+    case G2P_FAIL_PROJECTED:
+        record_failure(ctl, 2u, pid, rec[0], rec[1], rec[2], rec[3]);
+        raise_step(ctl, ERR_PROJECTED_DEFORMATION, step);
+        break;
+    case G2P_FAIL_CFL:
+        record_failure(ctl, 3u, pid, rec[0], rec[1], rec[2], rec[3]);
+        raise_step(ctl, ERR_CFL_BOUND, step);
+        break;
+    case G2P_FAIL_STENCIL: raise_step(ctl, ERR_STENCIL, step); break;
+    default: break;
+    }
+}
+
 // The node values come from `lv` (the particle's phase velocity) and `llq`
 // (liquid mass and volume), node (x, y) at (x - x0) + (y - y0) * stride.
+// Returns G2P_OK or the first failure code; on a failure the outputs are
+// not meaningful (every test is evaluated, none branches).
 template <class NodePtr>
-__device__ __forceinline__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p,
-                                           float4 F, const Stencil& s, NodePtr lv, NodePtr llq, int stride, int x0,
-                                           int y0, uint32_t step, float2& vel, float4& Cout, float2& np_out) {
+__device__ __forceinline__ uint32_t g2p_motion_code(const Params& P, const Material& mt, float2 p, float4 F,
+                                                    const Stencil& s, NodePtr lv, NodePtr llq, int stride, int x0,
+                                                    int y0, float2& vel, float4& Cout, float2& np_out) {
     const uint32_t model = mt.model;
     const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
@@ -1689,6 +1730,7 @@ __device__ __forceinline__ bool g2p_motion(const Params& P, Control* ctl, const 
             a3 = fmaf(wgy, dpy[oy], a3);
         }
     }
+    uint32_t code = G2P_OK;
     if (liquid) {
 #pragma unroll
         for (int oy = 0; oy < 3; ++oy) {
@@ -1700,35 +1742,42 @@ __device__ __forceinline__ bool g2p_motion(const Params& P, Control* ctl, const 
                 pvol = fmaf(w, lq.y, pvol);
             }
         }
-    }
-    if (liquid) {
         const float pdet = det_comp(F);
-        if (!finitef(pm) || pm <= 0.0f || !finitef(pvol) || pvol <= 0.0f || !finitef(pdet) || pdet <= 0.0f) {
-            raise_step_at(ctl, ERR_G2P, step, pid);
-            return false;
-        }
+        code = first_failure(code,
+                             !finitef(pm) || pm <= 0.0f || !finitef(pvol) || pvol <= 0.0f || !finitef(pdet) ||
+                                 pdet <= 0.0f,
+                             G2P_FAIL_LIQUID);
         // (quotients of finite positive values: quot, within an ulp)
         const float pd = quot(mt.density, pdet);
         const float ld = quot(pm, pvol);
         const float rel = quot(pd - ld, pd);
         vx = vx + P.dt * (rel * P.gx);
         vy = vy + P.dt * (rel * P.gy);
-        if (!finitef(pd) || pd <= 0.0f || !finitef(ld) || ld <= 0.0f || !finitef(rel) || !finitef(vx) ||
-            !finitef(vy)) {
-            raise_step_at(ctl, ERR_G2P, step, pid);
-            return false;
-        }
+        code = first_failure(code,
+                             !finitef(pd) || pd <= 0.0f || !finitef(ld) || ld <= 0.0f || !finitef(rel) ||
+                                 !finitef(vx) || !finitef(vy),
+                             G2P_FAIL_LIQUID);
     }
     const float gs = P.grad_scale, dt = P.dt;
     const float4 C = make_float4(a0 * gs, a1 * gs, a2 * gs, a3 * gs);
     const float2 np = make_float2(p.x + dt * vx, p.y + dt * vy);
-    if (!finitef(vx) || !finitef(vy) || !finite4(C) || !finite2(np)) {
-        raise_step(ctl, ERR_G2P, step);
-        return false;
-    }
+    code = first_failure(code, !finitef(vx) || !finitef(vy) || !finite4(C) || !finite2(np), G2P_FAIL_G2P);
     vel = make_float2(vx, vy);
     Cout = C;
     np_out = np;
+    return code;
+}
+
+template <class NodePtr>
+__device__ __forceinline__ bool g2p_motion(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 p,
+                                           float4 F, const Stencil& s, NodePtr lv, NodePtr llq, int stride, int x0,
+                                           int y0, uint32_t step, float2& vel, float4& Cout, float2& np_out) {
+    const uint32_t code = g2p_motion_code(P, mt, p, F, s, lv, llq, stride, x0, y0, vel, Cout, np_out);
+    if (code != G2P_OK) {
+        const float rec[4] = {0.f, 0.f, 0.f, 0.f};
+        raise_g2p(ctl, code, pid, step, rec);
+        return false;
+    }
     return true;
 }
 
@@ -1846,19 +1895,73 @@ __device__ bool g2p_deform(const Params& P, Control* ctl, const Material& mt, ui
     return true;
 }
 
-// validateCflParticle (shaders.ts:2423-2454) of the new velocity and F.
+// validateCflParticle (shaders.ts:2423-2454) of the new velocity and F, as a
+// failure code; `rec` receives the deformation record of a failure.
 template <bool GENERAL, bool SOLIDS = true>
-__device__ bool g2p_cfl_ok(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 vel, float4 Fn,
-                           uint32_t step) {
+__device__ __forceinline__ uint32_t g2p_cfl_code(const Params& P, const Material& mt, float2 vel, float4 Fn,
+                                                 float (&rec)[4]) {
     const float speed = __builtin_amdgcn_sqrtf(vel.x * vel.x + vel.y * vel.y);
     const float ws = wave_speed_bound<GENERAL, SOLIDS>(P, mt.model, Fn, mt.lambda, mt.mu, mt.density, mt.volume,
                                                        mt.param);
-    if (!finite2(vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound) {
-        record_failure(ctl, 3u, pid, speed, ws, P.vel_bound, P.wave_bound);
-        raise_step(ctl, ERR_CFL_BOUND, step);
+    rec[0] = speed;
+    rec[1] = ws;
+    rec[2] = P.vel_bound;
+    rec[3] = P.wave_bound;
+    const bool bad = !finite2(vel) || !finitef(speed) || !finitef(ws) || speed > P.vel_bound || ws > P.wave_bound;
+    return bad ? G2P_FAIL_CFL : G2P_OK;
+}
+template <bool GENERAL, bool SOLIDS = true>
+__device__ bool g2p_cfl_ok(const Params& P, Control* ctl, const Material& mt, uint32_t pid, float2 vel, float4 Fn,
+                           uint32_t step) {
+    float rec[4];
+    const uint32_t code = g2p_cfl_code<GENERAL, SOLIDS>(P, mt, vel, Fn, rec);
+    if (code != G2P_OK) {
+        raise_g2p(ctl, code, pid, step, rec);
         return false;
     }
     return true;
+}
+
+// g2p_deform for a liquid particle (integrateDeformation with the doubling
+// retry, then projectLiquidDeformation) as a failure code: the tests are
+// evaluated in the reference's order without branching (the retry loop
+// aside, which only a lane that lost orientation enters), and `rec`
+// receives the deformation record of a failure: (det of the update, the
+// increment, det F) or the projected F.
+__device__ __forceinline__ uint32_t g2p_deform_liquid_code(const Params& P, const Material& mt, float4 F, float4 C,
+                                                           float pv, float4& Fout, float (&rec)[4]) {
+    const float dt = P.dt;
+    const float rowsum = fmaxf(fabsf(C.x) + fabsf(C.y), fabsf(C.z) + fabsf(C.w));
+    const float increment = dt * rowsum;
+    const float required = ceilf(increment * (1.0f / MAX_DEFORMATION_INCREMENT));  // exact: a power of two
+    uint32_t code = first_failure(G2P_OK, !finitef(required) || required > (float)MAX_DEFORMATION_SUBSTEPS,
+                                  G2P_FAIL_SUBSTEPS);
+    uint32_t substeps = (code != G2P_OK || required < 1.0f) ? 1u : (uint32_t)required;
+    float4 trial = integrate_deformation(F, C, dt, substeps);
+    bool finite_trial = finite4(trial);
+    double tdet = det_f64(trial);
+    // A near-singular F can lose orientation to f32 rounding; retry that
+    // update with doubled substeps (shaders.ts:2274-2286).  The sign test is FP64.
+    while (code == G2P_OK && finite_trial && isfinite(tdet) && tdet <= 0.0 && substeps < MAX_DEFORMATION_SUBSTEPS) {
+        substeps = min(substeps * 2u, MAX_DEFORMATION_SUBSTEPS);
+        trial = integrate_deformation(F, C, dt, substeps);
+        finite_trial = finite4(trial);
+        tdet = det_f64(trial);
+    }
+    code = first_failure(code, !finite_trial, G2P_FAIL_G2P);
+    const bool lost = code == G2P_OK && (!isfinite(tdet) || tdet <= 0.0);
+    code = first_failure(code, lost, G2P_FAIL_DEFORMATION);
+    code = first_failure(code, mt.model != MODEL_WATER && mt.model != MODEL_OIL, G2P_FAIL_G2P);
+    const float4 Fn = project_liquid(trial);
+    const double ndet = det_f64(Fn);
+    code = first_failure(code, !finite4(Fn) || !finitef(pv) || pv < 0.6f || pv > 20.0f, G2P_FAIL_G2P);
+    code = first_failure(code, !isfinite(ndet) || ndet <= 0.0, G2P_FAIL_PROJECTED);
+    rec[0] = lost ? (float)tdet : Fn.x;
+    rec[1] = lost ? increment : Fn.y;
+    rec[2] = lost ? (float)det_f64(F) : Fn.z;
+    rec[3] = lost ? 0.0f : Fn.w;
+    Fout = Fn;
+    return code;
 }
 
 // The whole G2P of one particle in the reference's order, for scenes with
@@ -2114,49 +2217,51 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
             // is then never read (rollback copies the committed buffer over
             // it, counts included).  The flags are those of the reference's
             // order: a new position outside the grid is reported only when
-            // the rest of the particle's update succeeded.
+            // the rest of the particle's update succeeded.  The tests of the
+            // particle's update accumulate a first-failure code without
+            // branching; one branch raises it at the end.
             Material mt;
             bool placed = false;
+            uint32_t code = G2P_OK;
+            float rec[4] = {0.f, 0.f, 0.f, 0.f};
             if (ok) {
                 mt = material_at(s_mat, mats, c_mat);
                 Stencil s;
                 if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
                     ((s.by + 1) >> 2) != c_gby) {
-                    raise_step(ctl, ERR_STENCIL, step);
-                    ok = false;
+                    code = G2P_FAIL_STENCIL;
                 } else {
-                    const float2* lv = mt.phase == 1u ? s_vw : (mt.phase == 2u || !GENERAL ? s_vo : s_vs);
+                    const float2* lv = mt.phase == 1u ? s_vw : s_vo;
                     float2 np;
-                    ok = g2p_motion(P, ctl, mt, c_pid, c_p, c_F, s, lv, (const float2*)s_lq, G_TN, nx0, ny0, step, o.vel,
-                                    o.C, np);
-                    if (ok) {
-                        // projectLiquidFromTriangles is the identity without
-                        // bonded bodies; the second confineAxis pass is then
-                        // idempotent (shaders.ts:2376-2408)
-                        const float2 bxv = confine_axis(np.x, o.vel.x, P.dom_w);
-                        const float2 byv = confine_axis(np.y, o.vel.y, P.dom_h);
-                        o.pos = make_float2(bxv.x, byv.x);
-                        o.vel = make_float2(bxv.y, byv.y);
-                        Stencil ns;
-                        placed = make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns);
-                        if (placed) dest = bucket_of(ns, bw);
-                    }
+                    code = g2p_motion_code(P, mt, c_p, c_F, s, lv, (const float2*)s_lq, G_TN, nx0, ny0, o.vel, o.C,
+                                           np);
+                    // projectLiquidFromTriangles is the identity without
+                    // bonded bodies; the second confineAxis pass is then
+                    // idempotent (shaders.ts:2376-2408)
+                    const float2 bxv = confine_axis(np.x, o.vel.x, P.dom_w);
+                    const float2 byv = confine_axis(np.y, o.vel.y, P.dom_h);
+                    o.pos = make_float2(bxv.x, byv.x);
+                    o.vel = make_float2(bxv.y, byv.y);
+                    Stencil ns;
+                    placed = code == G2P_OK && make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns);
+                    dest = placed ? bucket_of(ns, bw) : 0u;
                 }
             }
-            const bool reserve = ok && placed;
+            const bool reserve = ok && code == G2P_OK && placed;
             const WaveAlloc wa = wave_alloc_issue(out.cnt, dest, reserve);
             G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_, mapped);
-            if (ok) {
-                double ndet;
-                ok = g2p_deform<GENERAL, SOLIDS>(P, ctl, mt, c_pid, c_F, o.C, c_pv, step, o.F, o.plastic, ndet);
-                if (ok && P.cfl_on >= 0.5f && P.body_count == 0u)
-                    ok = g2p_cfl_ok<GENERAL, SOLIDS>(P, ctl, mt, c_pid, o.vel, o.F, step);
-                if (ok && !placed) {
-                    raise_step(ctl, ERR_STENCIL, step);
-                    ok = false;
-                }
+            if (ok && code == G2P_OK) {
+                code = g2p_deform_liquid_code(P, mt, c_F, o.C, c_pv, o.F, rec);
+                o.plastic = c_pv;
+                if (P.cfl_on >= 0.5f && P.body_count == 0u && code == G2P_OK)
+                    code = g2p_cfl_code<GENERAL, SOLIDS>(P, mt, o.vel, o.F, rec);
+                code = first_failure(code, !placed, G2P_FAIL_STENCIL);
             }
             k = wave_alloc_finish(wa, reserve);
+            if (ok && code != G2P_OK) {
+                raise_g2p(ctl, code, c_pid, step, rec);
+                ok = false;
+            }
         }
         if (ok) {
             if (k >= cap) {

@@ -8,6 +8,9 @@ each checked through the C-ABI against the oracle or an exact restatement.
 * the deformation retry (kernels/shaders.ts:2274-2286): a particle whose
   one-substep F update loses its orientation to float32 rounding is retried
   with two substeps, and its F equals a float32 restatement of that retry;
+  one that loses it at every count up to 64 raises the deformation flag with
+  the last trial's diagnostics, and one that needs more than 64 substeps the
+  substep-limit flag with its index (kernels/shaders.ts:2253-2296);
 * batches: appending a solid to one scene of a liquid-only batch gives every
   scene the stress cache, bit for bit the scenes stepped alone; fracture
   bonds on a batch scene are refused.
@@ -220,6 +223,74 @@ def test_deformation_retry_doubles_the_substeps(pkg, mode):
     want = np.array(project_liquid32(integrate32(F, C, dt, 2)), np.float32)
     print(f"F out {list(out[8:12])} vs restated {list(want)}")
     assert np.all(np.abs(out[8:12] - want) <= 2 * np.spacing(np.abs(want))), (out[8:12], want)
+
+
+def _lone_water(pkg, F, C):
+    arr = particles_lone(pkg, [16.0], [16.0], F=[float(v) for v in F], C=[float(v) for v in C], model=2, lam=1.0,
+                         mu=0.0, density=1.0, volume=1.0)
+    config = {"grid": {"cellsX": 32, "cellsY": 32, "cellSize": 1.0}, "particleMass": 1.0, "particleVolume": 1.0,
+              "material": {"model": "water", "lambda": 1.0, "mu": 0.0, "density": 1.0}}
+    return arr, config
+
+
+def _step_and_fail(pkg, mode, arr, config, dt):
+    factory = pkg.create_mpm_mls_gpu_solver if mode == "gather" else pkg.create_experimental_mpm_mls_atomic_gpu_solver
+    s = factory(arr, config, device=0)
+    try:
+        before, _ = s.ctx.read_particles()
+        s.step(float(dt), {"x": 0.0, "y": 0.0})
+        with pytest.raises(pkg.MpmMlsGpuNumericalError) as e:
+            s.check_numerical_status()
+        after, _ = s.ctx.read_particles()
+    finally:
+        s.dispose()
+    assert np.array_equal(before, after), "a failed microstep should leave the particles as they were"
+    return e.value
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+def test_deformation_substep_limit_names_the_particle(pkg, mode):
+    """The lone water particle with C = [[200, 0], [0, 0]] and dt 1/8: the
+    deformation increment dt * max row sum is 25, which needs
+    ceil(25 / 0.25) = 100 substeps, above the 64 allowed
+    (kernels/shaders.ts:2253-2262).  The step raises the substep-limit flag
+    (32) with the particle's index and no diagnostics, and is rolled back."""
+    F = [F32(1), F32(0), F32(0), F32(1)]
+    arr, config = _lone_water(pkg, F, [F32(200), F32(0), F32(0), F32(0)])
+    err = _step_and_fail(pkg, mode, arr, config, 0.125)
+    assert err.flags == 32, err
+    assert err.particle_index == 0
+    assert err.diagnostics is None and err.projected_deformation is None and err.cfl_details is None
+
+
+@pytest.mark.parametrize("mode", ["gather", "fixed"])
+def test_deformation_failure_after_every_retry_reports_the_last(pkg, mode):
+    """The retry case's F = [[2^14, 2^14 - 1], [1, 1]] with a C (entries
+    k / 2^16, so the lone particle returns it exactly) whose F update loses
+    its orientation at every substep count 1, 2, ..., 64.  The kernel doubles
+    the substeps up to 64 and then raises the deformation flag (2) with the
+    particle's index and the diagnostics of the last trial
+    (kernels/shaders.ts:2270-2296): its determinant (the FP64 determinant of
+    the 64-substep float32 update, rounded to float32), the increment
+    dt * max row sum, and det F before the step (1)."""
+    a = F32(2 ** 14)
+    F = [a, F32(a - F32(1)), F32(1), F32(1)]
+    dt = F32(0.125)
+    C = [F32(k / 65536.0) for k in (19016, -409, -21789, 15378)]
+    rowsum = max(abs(C[0]) + abs(C[1]), abs(C[2]) + abs(C[3]))
+    assert np.ceil(F32(F32(dt * rowsum) * F32(4.0))) == 1
+    dets = [det64(integrate32(F, C, dt, n)) for n in (1, 2, 4, 8, 16, 32, 64)]
+    assert all(d <= 0 for d in dets), dets
+    arr, config = _lone_water(pkg, F, C)
+    err = _step_and_fail(pkg, mode, arr, config, dt)
+    assert err.flags == 2, err
+    assert err.particle_index == 0
+    d = err.diagnostics
+    assert d is not None, err
+    print(f"deformation failure ({mode}): {d} vs restated determinant {dets[-1]}")
+    assert d["determinant"] == float(F32(dets[-1]))
+    assert d["deformationIncrement"] == float(F32(dt * rowsum))
+    assert d["precedingDeterminant"] == 1.0
 
 
 # ------------------------------------------------------------------ batches
