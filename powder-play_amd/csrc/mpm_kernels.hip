@@ -1232,14 +1232,29 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 const float qx = fmaf(r1.y, ddy, fmaf(r1.x, ddx, r0.z));
                 const float qy = fmaf(r1.w, ddy, fmaf(r1.z, ddx, r0.w));
                 // each product is added with one rounding
-                if (ph == 0u) {
-                    acc[0] = fmaf(w, r2.x, acc[0]); acc[1] = fmaf(w, qx, acc[1]); acc[2] = fmaf(w, qy, acc[2]);
-                } else if (ph == 1u) {
-                    acc[3] = fmaf(w, r2.x, acc[3]); acc[4] = fmaf(w, qx, acc[4]);
-                    acc[5] = fmaf(w, qy, acc[5]); acc[6] = fmaf(w, r2.y, acc[6]);
+                if constexpr (GENERAL) {
+                    if (ph == 0u) {
+                        acc[0] = fmaf(w, r2.x, acc[0]); acc[1] = fmaf(w, qx, acc[1]); acc[2] = fmaf(w, qy, acc[2]);
+                    } else if (ph == 1u) {
+                        acc[3] = fmaf(w, r2.x, acc[3]); acc[4] = fmaf(w, qx, acc[4]);
+                        acc[5] = fmaf(w, qy, acc[5]); acc[6] = fmaf(w, r2.y, acc[6]);
+                    } else {
+                        acc[7] = fmaf(w, r2.x, acc[7]); acc[8] = fmaf(w, qx, acc[8]);
+                        acc[9] = fmaf(w, qy, acc[9]); acc[10] = fmaf(w, r2.y, acc[10]);
+                    }
                 } else {
-                    acc[7] = fmaf(w, r2.x, acc[7]); acc[8] = fmaf(w, qx, acc[8]);
-                    acc[9] = fmaf(w, qy, acc[9]); acc[10] = fmaf(w, r2.y, acc[10]);
+                    // Scenes without the stress cache hold water and oil only
+                    // (and failed particles, staged with zero mass, volume and
+                    // momentum).  Both phases' sums take every pair, the other
+                    // phase's with weight exactly zero: fmaf(0, x, s) == s for
+                    // the finite staged values and any sum that is not -0, which
+                    // a sum starting at +0 never is.  No divergent branch, so
+                    // none of its exec-mask bookkeeping either.
+                    const float ww = ph == 1u ? w : 0.0f, wo = w - ww;
+                    acc[3] = fmaf(ww, r2.x, acc[3]); acc[4] = fmaf(ww, qx, acc[4]);
+                    acc[5] = fmaf(ww, qy, acc[5]); acc[6] = fmaf(ww, r2.y, acc[6]);
+                    acc[7] = fmaf(wo, r2.x, acc[7]); acc[8] = fmaf(wo, qx, acc[8]);
+                    acc[9] = fmaf(wo, qy, acc[9]); acc[10] = fmaf(wo, r2.y, acc[10]);
                 }
             }
         }
