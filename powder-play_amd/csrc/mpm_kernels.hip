@@ -2173,7 +2173,7 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
         float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t aux = NO_AUX;
         uint32_t k;
-        if constexpr (AUX || GENERAL) {
+        if constexpr (AUX || (GENERAL && SOLIDS)) {
             // the reference's order: the slot is reserved once the particle
             // is complete (with bonded bodies the projection out of the
             // triangles moves the position after the deformation update; in
@@ -2219,6 +2219,55 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                             if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
                             atomicOr(&ctl->deferred[out_buf], pf);
                         }
+                    }
+                }
+            }
+            k = wave_alloc(out.cnt, dest, ok);
+        } else if constexpr (GENERAL) {
+            // Liquid scenes with the stress cache (Tait water or oil): the
+            // reference's order with the slot reserved once the particle is
+            // complete (an early reservation would hold more registers than
+            // four waves a SIMD allow), the tests accumulated into a
+            // first-failure code as in liquid-only scenes and raised once.
+            G2P_FETCH(base + G_THREADS + (uint32_t)tid, n_, mapped);
+            if (ok) {
+                const Material mt = material_at(s_mat, mats, c_mat);
+                uint32_t code = G2P_OK;
+                float rec[4] = {0.f, 0.f, 0.f, 0.f};
+                Stencil s;
+                if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, c_p, s) || ((s.bx + 1) >> 2) != c_gbx ||
+                    ((s.by + 1) >> 2) != c_gby) {
+                    code = G2P_FAIL_STENCIL;
+                } else {
+                    const float2* lv = mt.phase == 1u ? s_vw : s_vo;
+                    float2 np;
+                    code = g2p_motion_code(P, mt, c_p, c_F, s, lv, (const float2*)s_lq, G_TN, nx0, ny0, o.vel, o.C,
+                                           np);
+                    if (code == G2P_OK) code = g2p_deform_liquid_code(P, mt, c_F, o.C, c_pv, o.F, rec);
+                    o.plastic = c_pv;
+                    // the second confineAxis pass is idempotent without bodies
+                    const float2 bxv = confine_axis(np.x, o.vel.x, P.dom_w);
+                    const float2 byv = confine_axis(np.y, o.vel.y, P.dom_h);
+                    o.pos = make_float2(bxv.x, byv.x);
+                    o.vel = make_float2(bxv.y, byv.y);
+                    if (P.cfl_on >= 0.5f && P.body_count == 0u && code == G2P_OK)
+                        code = g2p_cfl_code<GENERAL, SOLIDS>(P, mt, o.vel, o.F, rec);
+                    Stencil ns;
+                    const bool placed =
+                        code == G2P_OK && make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, o.pos, ns);
+                    dest = placed ? bucket_of(ns, bw) : 0u;
+                    code = first_failure(code, !placed, G2P_FAIL_STENCIL);
+                }
+                if (code != G2P_OK) {
+                    raise_g2p(ctl, code, c_pid, step, rec);
+                    ok = false;
+                } else {
+                    // stress cache of the new state; failures belong to the
+                    // microstep that will start from it
+                    const uint32_t pf = particle_stress<true, SOLIDS>(mt, o.F, o.plastic, det_comp(o.F), S);
+                    if (pf) {
+                        if (pf & ERR_P2G) atomicMin(&ctl->deferred_index[out_buf], c_pid);
+                        atomicOr(&ctl->deferred[out_buf], pf);
                     }
                 }
             }
