@@ -1585,10 +1585,12 @@ __device__ float wave_speed_bound(const Params& P, uint32_t model, float4 F, flo
     return sqrtf(ws2);
 }
 
-__device__ __forceinline__ float2 confine_axis(float x, float v, float maximum) {  // shaders.ts:522-534
-    if (x < 0.0f || (x == 0.0f && v < 0.0f)) return make_float2(0.0f, 0.0f);
-    if (x > maximum || (x == maximum && v > 0.0f)) return make_float2(maximum, 0.0f);
-    return make_float2(x, v);
+// confineAxis (shaders.ts:522-534) as selects: the tests are combined with
+// bitwise operators, so the compiler forms no divergent branch for them.
+__device__ __forceinline__ float2 confine_axis(float x, float v, float maximum) {
+    const bool low = (x < 0.0f) | ((x == 0.0f) & (v < 0.0f));
+    const bool high = !low & ((x > maximum) | ((x == maximum) & (v > 0.0f)));
+    return make_float2(low ? 0.0f : (high ? maximum : x), (low | high) ? 0.0f : v);
 }
 
 // g2p (shaders.ts:2114-2421) for one particle with the grid patch in LDS,
