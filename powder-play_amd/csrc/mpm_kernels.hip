@@ -315,14 +315,17 @@ struct Stencil { int bx, by; float fx, fy; };
 __device__ __forceinline__ bool make_stencil(float dx, float inv_dx, uint32_t cells_x, uint32_t cells_y, float2 p,
                                              Stencil& s) {
     const float gx = inv_dx != 0.0f ? p.x * inv_dx : p.x / dx, gy = inv_dx != 0.0f ? p.y * inv_dx : p.y / dx;
-    if (!finitef(gx) || !finitef(gy)) return false;
-    s.bx = (int)floorf(gx - 0.5f);
-    s.by = (int)floorf(gy - 0.5f);
+    // the tests combined with bitwise operators, so that they form no
+    // divergent branch; the stencil is meaningful only when this returns
+    // true (a non-finite coordinate is replaced before the conversion)
+    const bool finite = finitef(gx) & finitef(gy);
+    const float cx = finite ? gx : 0.0f, cy = finite ? gy : 0.0f;
+    s.bx = (int)floorf(cx - 0.5f);
+    s.by = (int)floorf(cy - 0.5f);
+    s.fx = cx - (float)s.bx;
+    s.fy = cy - (float)s.by;
     // stencilFitsHalo (shaders.ts:511-516)
-    if (s.bx < -1 || s.by < -1 || s.bx + 2 > (int)cells_x + 1 || s.by + 2 > (int)cells_y + 1) return false;
-    s.fx = gx - (float)s.bx;
-    s.fy = gy - (float)s.by;
-    return true;
+    return finite & (s.bx >= -1) & (s.by >= -1) & (s.bx + 2 <= (int)cells_x + 1) & (s.by + 2 <= (int)cells_y + 1);
 }
 // quadraticWeights (shaders.ts:500-509), one component.
 __device__ __forceinline__ float qweight(float f, int o) {
