@@ -1147,9 +1147,18 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
                 pv = in.plastic[g];
             }
             const uint32_t rc = cell[k] >> 16;
-            const uint32_t q0 = s_cell[rc], q1 = s_cell[rc + 1];
+            const uint32_t q0 = s_cell[rc], q1 = s_cell[rc + 1];  // q1 > q0: this particle is one of them
             uint32_t q = q0;
-            for (uint32_t t = q0; t < q1; ++t) q += s_pid[4u * t] < pid[k] ? 1u : 0u;
+            // the cell's first four places in one batch of reads (an index
+            // past the cell reads its last place and counts nothing), then
+            // the rest of a fuller cell
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                const uint32_t t = q0 + u;
+                const uint32_t other = s_pid[4u * min(t, q1 - 1u)];
+                q += (t < q1) & (other < pid[k]) ? 1u : 0u;
+            }
+            for (uint32_t t = q0 + 4u; t < q1; ++t) q += s_pid[4u * t] < pid[k] ? 1u : 0u;
             Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
             make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, pos, sc);
             const Material mt = material_at(s_mat, mats, mat);
