@@ -597,10 +597,11 @@ __device__ __forceinline__ float div_ref(float a, float b, float r) {
 }
 
 __device__ __forceinline__ uint32_t fixed_round(float value, int32_t& out) {
-    float scaled = rintf(value * FIXED_POINT_SCALE);  // addFixedPoint (shaders.ts:971-985)
-    if (!finitef(scaled) || fabsf(scaled) > FIXED_POINT_LIMIT) { out = 0; return ERR_FIXED_POINT; }
-    out = (int32_t)scaled;
-    return 0u;
+    const float scaled = rintf(value * FIXED_POINT_SCALE);  // addFixedPoint (shaders.ts:971-985)
+    // as selects: a non-finite or out-of-range value gives 0 and the flag
+    const bool ok = fabsf(scaled) <= FIXED_POINT_LIMIT;  // false for NaN and infinities
+    out = (int32_t)(ok ? scaled : 0.0f);
+    return ok ? 0u : ERR_FIXED_POINT;
 }
 
 // Copy the first LDS_MATS material rows into LDS (all threads of the block;
@@ -1451,20 +1452,21 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
 #pragma unroll
             for (int oy = 0; oy < 3; ++oy) {
                 const int ly = ny0 + oy;
-                if (ly < 0 || ly >= T::TC) continue;
+                const bool inside_y = (ly >= 0) & (ly < T::TC);
                 const float dpy = ((float)oy - sc.fy) * dx;
 #pragma unroll
                 for (int ox = 0; ox < 3; ++ox) {
                     const int lx = nx0 + ox;
-                    if (lx < 0 || lx >= T::TC) continue;
                     const float dpx = ((float)ox - sc.fx) * dx;
                     const float w = wx[ox] * wy[oy];
                     const float qx = (mvx + A.x * dpx) + A.y * dpy;
                     const float qy = (mvy + A.z * dpx) + A.w * dpy;
-                    if (!finitef(w) || w < 0.0f || !finitef(qx) || !finitef(qy)) {
-                        flags |= ERR_P2G | ERR_NONFINITE_PARTICLE;
-                        continue;
-                    }
+                    // one predicate a pair: the node lies in the tile, and
+                    // (tested only then) the contribution is finite
+                    const bool inside = inside_y & (lx >= 0) & (lx < T::TC);
+                    const bool bad = !finitef(w) | (w < 0.0f) | !finitef(qx) | !finitef(qy);
+                    flags |= (inside & bad) ? (ERR_P2G | ERR_NONFINITE_PARTICLE) : 0u;
+                    if (!(inside & !bad)) continue;
                     const int node = ly * T::TC + lx;
                     int32_t im, iqx, iqy, iv = 0;
                     flags |= fixed_round(div_ref(w * m, mref, rm), im);
