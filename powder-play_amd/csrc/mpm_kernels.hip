@@ -1157,7 +1157,7 @@ __device__ __forceinline__ void p2g_gather_tile(const SceneDesc* __restrict__ sc
             for (uint32_t u = 0; u < 4u; ++u) {
                 const uint32_t t = q0 + u;
                 const uint32_t other = s_pid[4u * min(t, q1 - 1u)];
-                q += (t < q1) & (other < pid[k]) ? 1u : 0u;
+                q += ((t < q1) & (other < pid[k])) ? 1u : 0u;
             }
             for (uint32_t t = q0 + 4u; t < q1; ++t) q += s_pid[4u * t] < pid[k] ? 1u : 0u;
             Stencil sc;  // recomputed (it fitted in (A)) rather than held across the barriers
