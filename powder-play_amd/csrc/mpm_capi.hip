@@ -252,6 +252,9 @@ int alloc_slots(mpm_ctx* c, uint32_t cap) {
     c->slot_mem.release();
     c->lc.cap = cap;
     const size_t slots = (size_t)c->lc.nb * cap;
+    // the kernels index slots with 32-bit arithmetic
+    if (slots > 0xffffffffull)
+        return fail(c, MPM_EINVAL, "%zu bucket slots exceed the 2^32 - 1 the kernels address", slots);
     const bool general = scene_general(c->lc.scene_flags);
     hipError_t e;
     for (int b = 0; b < 2; ++b) {

@@ -512,7 +512,7 @@ __global__ __launch_bounds__(256) void k_unbucket(SlotBuf in, Home home, uint32_
                                                   uint32_t general) {
     const size_t total = (size_t)nb * cap;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t b = (uint32_t)(g / cap), k = (uint32_t)(g % cap);
+        const uint32_t b = (uint32_t)g / cap, k = (uint32_t)g - b * cap;  // slot counts fit 32 bits
         if (k >= in.cnt[b]) continue;
         const uint32_t i = in.pid[g];
         home.pos[i] = in.pos[g];
@@ -3065,7 +3065,7 @@ __global__ __launch_bounds__(256) void k_fix_rollback(const SceneDesc* __restric
     }
     const size_t total = (size_t)nb * cap;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t b = (uint32_t)(g / cap), k = (uint32_t)(g % cap);
+        const uint32_t b = (uint32_t)g / cap, k = (uint32_t)g - b * cap;  // slot counts fit 32 bits
         const uint32_t c = src.cnt[b];
         if (k == 0) dst.cnt[b] = c;
         if (k >= c) continue;
@@ -3093,12 +3093,21 @@ __global__ __launch_bounds__(256) void k_measure_speed(const SceneDesc* __restri
     const uint32_t nb = S.nb, cap = S.cap;
     __shared__ float s_w[4];
     float m = 0.0f;
-    const size_t total = (size_t)nb * cap;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-        if ((uint32_t)(g % cap) >= buf.cnt[g / cap]) continue;
-        const float2 v = buf.vel[g];
-        const float sp = sqrtf(v.x * v.x + v.y * v.y);
-        m = fmaxf(m, finitef(sp) ? sp : FLT_MAX);
+    // a thread a bucket: its count, then its used slots four at a time
+    // (their loads issued together), so no thread waits on a chain of them
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        const uint32_t c = min(buf.cnt[b], cap);
+        const size_t base = (size_t)b * cap;
+        for (uint32_t k0 = 0; k0 < c; k0 += 4u) {
+            float2 v[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) v[u] = k0 + u < c ? buf.vel[base + k0 + u] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                const float sp = sqrtf(v[u].x * v[u].x + v[u].y * v[u].y);
+                m = fmaxf(m, finitef(sp) ? sp : FLT_MAX);
+            }
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
@@ -3452,7 +3461,8 @@ __global__ __launch_bounds__(256) void k_pick(SlotBuf b, uint32_t nb, uint32_t c
     unsigned long long key = ~0ull;
     const size_t total = (size_t)nb * cap;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-        if ((uint32_t)(g % cap) >= b.cnt[g / cap]) continue;
+        const uint32_t bk = (uint32_t)g / cap;  // slot counts fit 32 bits
+        if ((uint32_t)g - bk * cap >= b.cnt[bk]) continue;
         const float2 p = b.pos[g];
         const float ox = p.x - px, oy = p.y - py;
         const float d2 = ox * ox + oy * oy;
@@ -3478,7 +3488,8 @@ __global__ __launch_bounds__(256) void k_pick_where(SlotBuf b, uint32_t nb, uint
     const uint32_t pid = (uint32_t)key;
     const size_t total = (size_t)nb * cap;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-        if ((uint32_t)(g % cap) >= b.cnt[g / cap]) continue;
+        const uint32_t bk = (uint32_t)g / cap;  // slot counts fit 32 bits
+        if ((uint32_t)g - bk * cap >= b.cnt[bk]) continue;
         if (b.pid[g] == pid) *where = b.pos[g];
     }
 }
