@@ -399,7 +399,9 @@ __device__ __forceinline__ WaveAlloc wave_alloc_issue(uint32_t* cnt, uint32_t b,
     WaveAlloc a{0u, lane, 0ull};
     while (pending) {
         const int first = __ffsll((long long)pending) - 1;
-        const uint32_t fb = __shfl(b, first, 64);
+        // the lane is uniform (from a ballot): a lane read, not a permute
+        // through LDS (which __shfl is), on every particle's chain
+        const uint32_t fb = (uint32_t)__builtin_amdgcn_readlane((int)b, first);
         const unsigned long long same = __ballot(active && b == fb);
         if (active && b == fb) {
             a.group = same;
