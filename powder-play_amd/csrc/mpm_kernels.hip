@@ -318,7 +318,8 @@ __device__ __forceinline__ bool make_stencil(float dx, float inv_dx, uint32_t ce
     // the tests combined with bitwise operators, so that they form no
     // divergent branch; the stencil is meaningful only when this returns
     // true (a non-finite coordinate is replaced before the conversion)
-    const bool finite = finitef(gx) & finitef(gy);
+    const bool finite_x = finitef(gx), finite_y = finitef(gy);
+    const bool finite = finite_x & finite_y;
     const float cx = finite ? gx : 0.0f, cy = finite ? gy : 0.0f;
     s.bx = (int)floorf(cx - 0.5f);
     s.by = (int)floorf(cy - 0.5f);
@@ -1466,7 +1467,8 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
                     // one predicate a pair: the node lies in the tile, and
                     // (tested only then) the contribution is finite
                     const bool inside = inside_y & (lx >= 0) & (lx < T::TC);
-                    const bool bad = !finitef(w) | (w < 0.0f) | !finitef(qx) | !finitef(qy);
+                    const bool fw = finitef(w), fqx = finitef(qx), fqy = finitef(qy);
+                    const bool bad = !fw | (w < 0.0f) | !fqx | !fqy;
                     flags |= (inside & bad) ? (ERR_P2G | ERR_NONFINITE_PARTICLE) : 0u;
                     if (!(inside & !bad)) continue;
                     const int node = ly * T::TC + lx;
