@@ -20,6 +20,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #define MPM_GLOBAL_POINTERS  // scene-descriptor pointers are global in device code
 #include "mpm_internal.h"
@@ -1452,37 +1453,54 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
                 wx[o] = qweight(sc.fx, o);
                 wy[o] = qweight(sc.fy, o);
             }
-#pragma unroll
-            for (int oy = 0; oy < 3; ++oy) {
-                const int ly = ny0 + oy;
-                const bool inside_y = (ly >= 0) & (ly < T::TC);
-                const float dpy = ((float)oy - sc.fy) * dx;
-#pragma unroll
-                for (int ox = 0; ox < 3; ++ox) {
-                    const int lx = nx0 + ox;
-                    const float dpx = ((float)ox - sc.fx) * dx;
-                    const float w = wx[ox] * wy[oy];
-                    const float qx = (mvx + A.x * dpx) + A.y * dpy;
-                    const float qy = (mvy + A.z * dpx) + A.w * dpy;
-                    // one predicate a pair: the node lies in the tile, and
-                    // (tested only then) the contribution is finite
-                    const bool inside = inside_y & (lx >= 0) & (lx < T::TC);
-                    const bool fw = finitef(w), fqx = finitef(qx), fqy = finitef(qy);
-                    const bool bad = !fw | (w < 0.0f) | !fqx | !fqy;
-                    flags |= (inside & bad) ? (ERR_P2G | ERR_NONFINITE_PARTICLE) : 0u;
-                    if (!(inside & !bad)) continue;
-                    const int node = ly * T::TC + lx;
-                    int32_t im, iqx, iqy, iv = 0;
-                    flags |= fixed_round(div_ref(w * m, mref, rm), im);
-                    flags |= fixed_round(div_ref(w * qx, mref, rm), iqx);
-                    flags |= fixed_round(div_ref(w * qy, mref, rm), iqy);
-                    if (ph != 0u) flags |= fixed_round(div_ref(w * vd, vref, rv), iv);
-                    atomicAdd(&s_acc[c0][node], (unsigned long long)(long long)im);
-                    atomicAdd(&s_acc[c0 + 1][node], (unsigned long long)(long long)iqx);
-                    atomicAdd(&s_acc[c0 + 2][node], (unsigned long long)(long long)iqy);
-                    if (ph != 0u) atomicAdd(&s_acc[c0 + 3][node], (unsigned long long)(long long)iv);
+            // The quotients by particleMass and particleVolume: the exact
+            // reciprocal form when the host supplied both reciprocals (one
+            // uniform test a particle instead of one a quotient), else
+            // div_ref, which picks per quotient.
+            auto scatter = [&](auto exact) {
+                constexpr bool EXACT = decltype(exact)::value;
+                auto quot_ref = [](float a, float b, float r) {
+                    if constexpr (EXACT) {
+                        const float q = a * r;
+                        return fmaf(fmaf(-q, b, a), r, q);
+                    } else {
+                        return div_ref(a, b, r);
+                    }
+                };
+    #pragma unroll
+                for (int oy = 0; oy < 3; ++oy) {
+                    const int ly = ny0 + oy;
+                    const bool inside_y = (ly >= 0) & (ly < T::TC);
+                    const float dpy = ((float)oy - sc.fy) * dx;
+    #pragma unroll
+                    for (int ox = 0; ox < 3; ++ox) {
+                        const int lx = nx0 + ox;
+                        const float dpx = ((float)ox - sc.fx) * dx;
+                        const float w = wx[ox] * wy[oy];
+                        const float qx = (mvx + A.x * dpx) + A.y * dpy;
+                        const float qy = (mvy + A.z * dpx) + A.w * dpy;
+                        // one predicate a pair: the node lies in the tile, and
+                        // (tested only then) the contribution is finite
+                        const bool inside = inside_y & (lx >= 0) & (lx < T::TC);
+                        const bool fw = finitef(w), fqx = finitef(qx), fqy = finitef(qy);
+                        const bool bad = !fw | (w < 0.0f) | !fqx | !fqy;
+                        flags |= (inside & bad) ? (ERR_P2G | ERR_NONFINITE_PARTICLE) : 0u;
+                        if (!(inside & !bad)) continue;
+                        const int node = ly * T::TC + lx;
+                        int32_t im, iqx, iqy, iv = 0;
+                        flags |= fixed_round(quot_ref(w * m, mref, rm), im);
+                        flags |= fixed_round(quot_ref(w * qx, mref, rm), iqx);
+                        flags |= fixed_round(quot_ref(w * qy, mref, rm), iqy);
+                        if (ph != 0u) flags |= fixed_round(quot_ref(w * vd, vref, rv), iv);
+                        atomicAdd(&s_acc[c0][node], (unsigned long long)(long long)im);
+                        atomicAdd(&s_acc[c0 + 1][node], (unsigned long long)(long long)iqx);
+                        atomicAdd(&s_acc[c0 + 2][node], (unsigned long long)(long long)iqy);
+                        if (ph != 0u) atomicAdd(&s_acc[c0 + 3][node], (unsigned long long)(long long)iv);
+                    }
                 }
-            }
+            };
+            if (rm != 0.0f && rv != 0.0f) scatter(std::true_type{});
+            else scatter(std::false_type{});
         }
     }
     __syncthreads();
