@@ -1491,11 +1491,13 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
                         flags |= fixed_round(quot_ref(w * m, mref, rm), im);
                         flags |= fixed_round(quot_ref(w * qx, mref, rm), iqx);
                         flags |= fixed_round(quot_ref(w * qy, mref, rm), iqy);
-                        if (ph != 0u) flags |= fixed_round(quot_ref(w * vd, vref, rv), iv);
+                        // (scenes without solids: every particle here is liquid)
+                        if (!SOLIDS || ph != 0u) flags |= fixed_round(quot_ref(w * vd, vref, rv), iv);
                         atomicAdd(&s_acc[c0][node], (unsigned long long)(long long)im);
                         atomicAdd(&s_acc[c0 + 1][node], (unsigned long long)(long long)iqx);
                         atomicAdd(&s_acc[c0 + 2][node], (unsigned long long)(long long)iqy);
-                        if (ph != 0u) atomicAdd(&s_acc[c0 + 3][node], (unsigned long long)(long long)iv);
+                        if (!SOLIDS || ph != 0u)
+                            atomicAdd(&s_acc[c0 + 3][node], (unsigned long long)(long long)iv);
                     }
                 }
             };
