@@ -1746,12 +1746,14 @@ __device__ void raise_g2p(Control* ctl, uint32_t code, uint32_t pid, uint32_t st
 // (liquid mass and volume), node (x, y) at (x - x0) + (y - y0) * stride.
 // Returns G2P_OK or the first failure code; on a failure the outputs are
 // not meaningful (every test is evaluated, none branches).
-template <class NodePtr>
+// LIQUID = true: every particle of the scene is water or oil (the liquid
+// terms then need no test of the model).
+template <class NodePtr, bool LIQUID = false>
 __device__ __forceinline__ uint32_t g2p_motion_code(const Params& P, const Material& mt, float2 p, float4 F,
                                                     const Stencil& s, NodePtr lv, NodePtr llq, int stride, int x0,
                                                     int y0, float2& vel, float4& Cout, float2& np_out) {
     const uint32_t model = mt.model;
-    const bool water = model == MODEL_WATER, oil = model == MODEL_OIL, liquid = water || oil;
+    const bool liquid = LIQUID || model == MODEL_WATER || model == MODEL_OIL;
     float vx = 0.f, vy = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, pm = 0.f, pvol = 0.f;
     const float dx = P.dx;
     // The reference tests every weight, offset and node value for
@@ -2281,8 +2283,8 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                 } else {
                     const float2* lv = mt.phase == 1u ? s_vw : s_vo;
                     float2 np;
-                    code = g2p_motion_code(P, mt, c_p, c_F, s, lv, (const float2*)s_lq, G_TN, nx0, ny0, o.vel, o.C,
-                                           np);
+                    code = g2p_motion_code<const float2*, true>(P, mt, c_p, c_F, s, lv, (const float2*)s_lq, G_TN,
+                                                                 nx0, ny0, o.vel, o.C, np);
                     if (code == G2P_OK) code = g2p_deform_liquid_code(P, mt, c_F, o.C, c_pv, o.F, rec);
                     o.plastic = c_pv;
                     // the second confineAxis pass is idempotent without bodies
@@ -2337,8 +2339,8 @@ __device__ __forceinline__ void g2p_tile(const SceneDesc* __restrict__ scenes, c
                 } else {
                     const float2* lv = mt.phase == 1u ? s_vw : s_vo;
                     float2 np;
-                    code = g2p_motion_code(P, mt, c_p, c_F, s, lv, (const float2*)s_lq, G_TN, nx0, ny0, o.vel, o.C,
-                                           np);
+                    code = g2p_motion_code<const float2*, true>(P, mt, c_p, c_F, s, lv, (const float2*)s_lq, G_TN,
+                                                                 nx0, ny0, o.vel, o.C, np);
                     // projectLiquidFromTriangles is the identity without
                     // bonded bodies; the second confineAxis pass is then
                     // idempotent (shaders.ts:2376-2408)
