@@ -1554,7 +1554,9 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
 
 template <int TI, bool GENERAL, bool SOLIDS, bool AUX = false>
 __global__ __launch_bounds__(P2GTile<TI>::THREADS)
-__attribute__((amdgpu_waves_per_eu((TI == 20 && SOLIDS) ? 6 : 8, 8)))  // LDS: 3 x 8 waves a CU for 20 with solids
+// LDS: 3 x 8 waves a CU for 20 with solids; with the stress cache and no
+// solids, 80 registers at 3 blocks a CU (fewer scatter spills) beat 64 at 4
+__attribute__((amdgpu_waves_per_eu((TI == 20 && (SOLIDS || GENERAL)) ? 6 : 8, 8)))
 void k_p2g_fixed(const SceneDesc* __restrict__ scenes, const uint32_t* __restrict__ tile_scene, uint32_t nscenes,
                  uint32_t step, TileQueue q) {
     run_tiles(q, 0u, step, [&](uint32_t t) { p2g_fixed_tile<TI, GENERAL, SOLIDS, AUX>(scenes, tile_scene, nscenes, step, t); });
