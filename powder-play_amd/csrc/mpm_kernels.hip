@@ -1377,16 +1377,18 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
     const float mref = P.particle_mass, vref = P.particle_volume, dx = P.dx;
     const float rm = P.inv_mass_ref, rv = P.inv_volume_ref;
     const uint32_t total = s_boff[T::NB];
-    for (uint32_t base = 0; base < total; base += 2u * T::THREADS) {
-        // load two particles a thread first, so both loads are in flight
-        uint32_t pid[2], mat[2];
-        int gbx[2], gby[2];
-        float2 p[2], v[2];
-        float4 C[2], F[2], S[2];
-        float pv[2];
-        bool have[2];
+    // one particle a thread a pass: with two (both loads in flight) the
+    // second one's inputs spill to scratch across the first one's scatter
+    constexpr int PER = 1;
+    for (uint32_t base = 0; base < total; base += (uint32_t)PER * T::THREADS) {
+        uint32_t pid[PER], mat[PER];
+        int gbx[PER], gby[PER];
+        float2 p[PER], v[PER];
+        float4 C[PER], F[PER], S[PER];
+        float pv[PER];
+        bool have[PER];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const uint32_t j = base + (uint32_t)tid + (uint32_t)(k * T::THREADS);
             have[k] = j < total;
             pid[k] = 0u; mat[k] = 0u; gbx[k] = 0; gby[k] = 0;
@@ -1410,7 +1412,7 @@ __device__ __forceinline__ void p2g_fixed_tile(const SceneDesc* __restrict__ sce
             }
         }
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < PER; ++k) {
             if (!have[k]) continue;
             Stencil sc;
             if (!make_stencil(P.dx, P.inv_dx, P.cells_x, P.cells_y, p[k], sc) || ((sc.bx + 1) >> 2) != gbx[k] ||
